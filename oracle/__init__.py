@@ -1,0 +1,344 @@
+"""CPU oracle for the ohmnivore solve path — TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper over ``oracle/libohm_oracle.so`` (built from ohm_oracle.c by
+oracle/Makefile). Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg may import this package, and only as the checker or the
+timed CPU baseline. Each function restates the reference algorithm cited in
+ohm_oracle.c / ohm_oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from ohmnivore_amd import circuit as _ir
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libohm_oracle.so")
+_lib = None
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+f64p = ctypes.POINTER(ctypes.c_double)
+i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.oo_last_error.restype = ctypes.c_char_p
+        _lib.oo_from_triplets.restype = ctypes.c_int64
+        _lib.oo_pnjlim.restype = ctypes.c_double
+        _lib.oo_eval_pulse.restype = ctypes.c_double
+        _lib.oo_pnjlim.argtypes = [ctypes.c_double] * 4
+        _lib.oo_eval_pulse.argtypes = [ctypes.c_double] * 8
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise OracleError(rc, lib().oo_last_error().decode())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _csr_args(a: _ir.CsrMatrix):
+    rp = np.ascontiguousarray(a.row_pointers, dtype=np.uint32)
+    cl = np.ascontiguousarray(a.col_indices, dtype=np.uint32)
+    if np.iscomplexobj(a.values):
+        v = np.ascontiguousarray(a.values, dtype=np.complex128).view(np.float64)
+    else:
+        v = np.ascontiguousarray(a.values, dtype=np.float64)
+    return rp, cl, v
+
+
+def from_triplets(nrows: int, ncols: int, triplets) -> _ir.CsrMatrix:
+    """CsrMatrix::from_triplets (src/sparse.rs:52-100)."""
+    t = list(triplets)
+    cplx = any(isinstance(v, complex) for _, _, v in t)
+    rows = np.array([r for r, _, _ in t], dtype=np.int64)
+    cols = np.array([c for _, c, _ in t], dtype=np.int64)
+    vals = np.array([v for _, _, v in t], dtype=np.complex128 if cplx else np.float64)
+    return from_triplet_arrays(nrows, ncols, rows, cols, vals)
+
+
+def from_triplet_arrays(nrows, ncols, rows, cols, vals) -> _ir.CsrMatrix:
+    n = rows.shape[0]
+    cplx = np.iscomplexobj(vals)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    v = np.ascontiguousarray(vals, dtype=np.complex128 if cplx else np.float64)
+    vin = v.view(np.float64)
+    rp = np.zeros(nrows + 1, dtype=np.int64)
+    oc = np.zeros(max(n, 1), dtype=np.int64)
+    ov = np.zeros(max(n, 1) * (2 if cplx else 1), dtype=np.float64)
+    nnz = lib().oo_from_triplets(ctypes.c_int64(nrows), ctypes.c_int64(n), _p(rows, i64p),
+                                 _p(cols, i64p), _p(vin, f64p), int(cplx), _p(rp, i64p),
+                                 _p(oc, i64p), _p(ov, f64p))
+    vals_out = ov.view(np.complex128)[:nnz].copy() if cplx else ov[:nnz].copy()
+    return _ir.CsrMatrix(nrows, ncols, vals_out, oc[:nnz].astype(np.uint32),
+                         rp.astype(np.uint32))
+
+
+def compile(ckt: _ir.Circuit) -> _ir.MnaSystem:
+    """compiler::compile (src/compiler.rs:194-754)."""
+    cc = _ir.CCircuit(ckt)
+    m = _ir.MnaC()
+    L = lib()
+    _check(L.oo_compile(cc.ref(), ctypes.byref(m)))
+    try:
+        return _ir.mna_from_c(m, ckt)
+    finally:
+        L.oo_mna_free(ctypes.byref(m))
+
+
+def spmv(a: _ir.CsrMatrix, x: np.ndarray) -> np.ndarray:
+    rp, cl, v = _csr_args(a)
+    cplx = np.iscomplexobj(a.values)
+    xx = np.ascontiguousarray(x, dtype=np.complex128 if cplx else np.float64)
+    y = np.zeros_like(xx)
+    lib().oo_spmv(ctypes.c_int64(a.nrows), _p(rp, u32p), _p(cl, u32p), _p(v, f64p),
+                  _p(xx.view(np.float64), f64p), _p(y.view(np.float64), f64p), int(cplx))
+    return y
+
+
+def dense_solve(a: _ir.CsrMatrix, b) -> np.ndarray:
+    """CpuSolver::solve_real / solve_complex (src/solver/cpu.rs:28-170)."""
+    rp, cl, v = _csr_args(a)
+    cplx = np.iscomplexobj(a.values) or np.iscomplexobj(b)
+    if cplx and not np.iscomplexobj(a.values):
+        v = np.ascontiguousarray(a.values.astype(np.complex128)).view(np.float64)
+    bb = np.ascontiguousarray(b, dtype=np.complex128 if cplx else np.float64)
+    x = np.zeros(max(a.nrows, 1), dtype=bb.dtype)
+    _check(lib().oo_dense_solve(ctypes.c_int64(a.nrows), ctypes.c_int64(a.ncols),
+                                ctypes.c_int64(bb.shape[0]), _p(rp, u32p), _p(cl, u32p),
+                                _p(v, f64p), _p(bb.view(np.float64), f64p),
+                                _p(x.view(np.float64), f64p), int(cplx)))
+    return x[: a.nrows]
+
+
+def sparse_lu_solve(a: _ir.CsrMatrix, b) -> np.ndarray:
+    """Sparse direct LU standing in for faer sp_lu (src/solver/sparse_direct.rs:12-61)."""
+    rp, cl, v = _csr_args(a)
+    cplx = np.iscomplexobj(a.values)
+    bb = np.ascontiguousarray(b, dtype=np.complex128 if cplx else np.float64)
+    x = np.zeros_like(bb)
+    _check(lib().oo_sparse_lu_solve(ctypes.c_int64(a.nrows), _p(rp, u32p), _p(cl, u32p),
+                                    _p(v, f64p), _p(bb.view(np.float64), f64p),
+                                    _p(x.view(np.float64), f64p), int(cplx)))
+    return x
+
+
+def jacobi_inv_diag(a: _ir.CsrMatrix) -> np.ndarray:
+    rp, cl, v = _csr_args(a)
+    inv = np.zeros(a.nrows)
+    lib().oo_jacobi_inv_diag(ctypes.c_int64(a.nrows), _p(rp, u32p), _p(cl, u32p), _p(v, f64p),
+                             _p(inv, f64p))
+    return inv
+
+
+class _Prec(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("inv_diag", f64p), ("ml_rowptr", u32p),
+                ("ml_col", u32p), ("ml_val", f64p), ("mu_rowptr", u32p), ("mu_col", u32p),
+                ("mu_val", f64p)]
+
+
+def bicgstab(a: _ir.CsrMatrix, b, precond=None, max_iters=10_000, tol=1e-5, shadow="hashed"):
+    """bicgstab_with_params (src/solver/bicgstab.rs:56-189). precond is None,
+    ("jacobi", inv_diag) or ("isai", M_L, M_U). Returns (x, iterations)."""
+    rp, cl, v = _csr_args(a)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(bb)
+    p = _Prec()
+    keep = []
+    if precond is None:
+        p.kind = 0
+    elif precond[0] == "jacobi":
+        p.kind = 1
+        inv = np.ascontiguousarray(precond[1], dtype=np.float64)
+        keep.append(inv)
+        p.inv_diag = _p(inv, f64p)
+    else:
+        p.kind = 2
+        ml, mu = precond[1], precond[2]
+        a1, a2 = _csr_args(ml), _csr_args(mu)
+        keep += [a1, a2]
+        p.ml_rowptr, p.ml_col, p.ml_val = _p(a1[0], u32p), _p(a1[1], u32p), _p(a1[2], f64p)
+        p.mu_rowptr, p.mu_col, p.mu_val = _p(a2[0], u32p), _p(a2[1], u32p), _p(a2[2], f64p)
+    it = ctypes.c_int64(0)
+    _check(lib().oo_bicgstab(ctypes.c_int64(a.nrows), _p(rp, u32p), _p(cl, u32p), _p(v, f64p),
+                             _p(bb, f64p), _p(x, f64p), ctypes.byref(p), ctypes.c_int64(max_iters),
+                             ctypes.c_double(tol), int(shadow == "hashed"), ctypes.byref(it)))
+    return x, it.value
+
+
+def _take_csr(n, nnz, rp, cl, vv):
+    L = lib()
+    out = _ir.CsrMatrix(n, n, np.ctypeslib.as_array(vv, shape=(max(nnz.value, 1),))[: nnz.value].copy(),
+                        np.ctypeslib.as_array(cl, shape=(max(nnz.value, 1),))[: nnz.value].copy(),
+                        np.ctypeslib.as_array(rp, shape=(n + 1,)).copy())
+    for q in (rp, cl, vv):
+        L.oo_free(ctypes.cast(q, ctypes.c_void_p))
+    return out
+
+
+def _mk_out():
+    return ctypes.c_int64(0), u32p(), u32p(), f64p()
+
+
+def lu_reference(a: _ir.CsrMatrix):
+    """ilu0 (src/solver/preconditioner.rs:64-153): returns (L, U, perm)."""
+    rp, cl, v = _csr_args(a)
+    l = _mk_out()
+    u = _mk_out()
+    perm = i64p()
+    _check(lib().oo_lu_reference(ctypes.c_int64(a.nrows), _p(rp, u32p), _p(cl, u32p), _p(v, f64p),
+                                 ctypes.byref(l[0]), ctypes.byref(l[1]), ctypes.byref(l[2]),
+                                 ctypes.byref(l[3]), ctypes.byref(u[0]), ctypes.byref(u[1]),
+                                 ctypes.byref(u[2]), ctypes.byref(u[3]), ctypes.byref(perm)))
+    n = a.nrows
+    pm = np.ctypeslib.as_array(perm, shape=(max(n, 1),))[:n].copy()
+    lib().oo_free(ctypes.cast(perm, ctypes.c_void_p))
+    return _take_csr(n, *l), _take_csr(n, *u), pm
+
+
+def isai_reference(a: _ir.CsrMatrix, level: int = 1):
+    """compute_isai (src/solver/preconditioner.rs:239-277): returns (M_L, M_U, perm)."""
+    rp, cl, v = _csr_args(a)
+    ml = _mk_out()
+    mu = _mk_out()
+    perm = i64p()
+    _check(lib().oo_isai_reference(ctypes.c_int64(a.nrows), _p(rp, u32p), _p(cl, u32p),
+                                   _p(v, f64p), int(level), ctypes.byref(ml[0]),
+                                   ctypes.byref(ml[1]), ctypes.byref(ml[2]), ctypes.byref(ml[3]),
+                                   ctypes.byref(mu[0]), ctypes.byref(mu[1]), ctypes.byref(mu[2]),
+                                   ctypes.byref(mu[3]), ctypes.byref(perm)))
+    n = a.nrows
+    pm = np.ctypeslib.as_array(perm, shape=(max(n, 1),))[:n].copy()
+    lib().oo_free(ctypes.cast(perm, ctypes.c_void_p))
+    return _take_csr(n, *ml), _take_csr(n, *mu), pm
+
+
+def ilu0_isai(a: _ir.CsrMatrix, level: int = 1):
+    """Exact ILU(0)+ISAI(level) — the backend's preconditioner. Returns (L, U, M_L, M_U)."""
+    rp, cl, v = _csr_args(a)
+    outs = [_mk_out() for _ in range(4)]
+    args = []
+    for o in outs:
+        args += [ctypes.byref(o[0]), ctypes.byref(o[1]), ctypes.byref(o[2]), ctypes.byref(o[3])]
+    _check(lib().oo_ilu0_isai(ctypes.c_int64(a.nrows), _p(rp, u32p), _p(cl, u32p), _p(v, f64p),
+                              int(level), *args))
+    return tuple(_take_csr(a.nrows, *o) for o in outs)
+
+
+def _desc_ptr(arr, dtype):
+    arr = np.ascontiguousarray(arr, dtype=dtype) if len(arr) else np.zeros(1, dtype=dtype)
+    return arr, ctypes.c_void_p(arr.ctypes.data)
+
+
+def diode_eval(desc, x):
+    d = np.ascontiguousarray(np.atleast_1d(desc), dtype=_ir.DIODE_DESC_DTYPE)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    i, g = ctypes.c_double(), ctypes.c_double()
+    lib().oo_diode_eval(ctypes.c_void_p(d.ctypes.data), _p(xx, f64p), ctypes.byref(i),
+                        ctypes.byref(g))
+    return i.value, g.value
+
+
+def bjt_eval(desc, x):
+    d = np.ascontiguousarray(np.atleast_1d(desc), dtype=_ir.BJT_DESC_DTYPE)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(6)
+    lib().oo_bjt_eval(ctypes.c_void_p(d.ctypes.data), _p(xx, f64p), _p(out, f64p))
+    return out
+
+
+def mosfet_core(vgs, vds, vto, kp, lam, polarity):
+    i, gm, gds = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().oo_mosfet_core(ctypes.c_double(vgs), ctypes.c_double(vds), ctypes.c_double(vto),
+                         ctypes.c_double(kp), ctypes.c_double(lam), ctypes.c_double(polarity),
+                         ctypes.byref(i), ctypes.byref(gm), ctypes.byref(gds))
+    return i.value, gm.value, gds.value
+
+
+def pnjlim(vnew, vold, vt, vcrit):
+    return lib().oo_pnjlim(vnew, vold, vt, vcrit)
+
+
+def eval_pulse(v1, v2, td, tr, tf, pw, per, t):
+    return lib().oo_eval_pulse(v1, v2, td, tr, tf, pw, per, t)
+
+
+class _NewtonParams(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int64), ("abs_tol", ctypes.c_double),
+                ("linear_mode", ctypes.c_int), ("initial_guess", f64p)]
+
+
+def _newton_arrays(g, diodes, bjts, mosfets):
+    rp, cl, v = _csr_args(g)
+    d, dp = _desc_ptr(diodes, _ir.DIODE_DESC_DTYPE)
+    q, qp = _desc_ptr(bjts, _ir.BJT_DESC_DTYPE)
+    m, mp = _desc_ptr(mosfets, _ir.MOSFET_DESC_DTYPE)
+    return rp, cl, v, (d, dp), (q, qp), (m, mp)
+
+
+def newton_solve(g: _ir.CsrMatrix, b, diodes=(), bjts=(), mosfets=(), max_iterations=50,
+                 abs_tol=1e-4, linear_mode="sparse_lu", initial_guess=None):
+    """newton_solve (src/solver/newton.rs:65-513) with a direct-LU linear step.
+    Returns (x, iterations, max_diff)."""
+    rp, cl, v, dd, qq, mm = _newton_arrays(g, diodes, bjts, mosfets)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(g.nrows)
+    prm = _NewtonParams(max_iterations, abs_tol, 1 if linear_mode == "dense" else 0, None)
+    if initial_guess is not None:
+        ig = np.ascontiguousarray(initial_guess, dtype=np.float64)
+        prm.initial_guess = _p(ig, f64p)
+    it, md = ctypes.c_int64(), ctypes.c_double()
+    _check(lib().oo_newton_solve(
+        ctypes.c_int64(g.nrows), ctypes.c_int64(g.nnz), _p(rp, u32p), _p(cl, u32p), _p(v, f64p),
+        _p(bb, f64p), ctypes.c_int64(len(diodes)), dd[1], ctypes.c_int64(len(bjts)), qq[1],
+        ctypes.c_int64(len(mosfets)), mm[1], ctypes.byref(prm), _p(x, f64p), ctypes.byref(it),
+        ctypes.byref(md)))
+    return x, it.value, md.value
+
+
+def newton_assemble(g: _ir.CsrMatrix, b, x, diodes=(), bjts=(), mosfets=()):
+    rp, cl, v, dd, qq, mm = _newton_arrays(g, diodes, bjts, mosfets)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    vo = np.zeros(g.nnz)
+    bo = np.zeros(g.nrows)
+    lib().oo_newton_assemble(ctypes.c_int64(g.nrows), ctypes.c_int64(g.nnz), _p(v, f64p),
+                             _p(bb, f64p), ctypes.c_int64(len(diodes)), dd[1],
+                             ctypes.c_int64(len(bjts)), qq[1], ctypes.c_int64(len(mosfets)), mm[1],
+                             _p(xx, f64p), _p(vo, f64p), _p(bo, f64p))
+    return vo, bo
+
+
+def newton_limit(n, x_old, x_new, diodes=(), bjts=(), mosfets=()):
+    d, dp = _desc_ptr(diodes, _ir.DIODE_DESC_DTYPE)
+    q, qp = _desc_ptr(bjts, _ir.BJT_DESC_DTYPE)
+    m, mp = _desc_ptr(mosfets, _ir.MOSFET_DESC_DTYPE)
+    xo = np.ascontiguousarray(x_old, dtype=np.float64)
+    xn = np.array(x_new, dtype=np.float64)
+    lib().oo_newton_limit(ctypes.c_int64(n), ctypes.c_int64(len(diodes)), dp,
+                          ctypes.c_int64(len(bjts)), qp, ctypes.c_int64(len(mosfets)), mp,
+                          _p(xo, f64p), _p(xn, f64p))
+    return xn

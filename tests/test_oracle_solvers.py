@@ -1,0 +1,300 @@
+"""Pins for the oracle's ISAI, BiCGSTAB, device models and Newton loop.
+
+Ports src/solver/preconditioner.rs:509-820 and src/solver/gpu_shaders.rs:1495-1656,
+then adds closed forms, finite differences and brute force where the
+reference has no test.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.optimize
+
+import oracle
+from ohmnivore_amd import circuit as ir
+
+VT = 0.02585
+
+
+def trip(n, t):
+    return oracle.from_triplets(n, n, t)
+
+
+# ---------------------------------------------------------------- ISAI ----
+def test_ilu0_small_matrix_is_plu():
+    a = trip(3, [(0, 0, 4.0), (0, 1, -1.0), (1, 0, -1.0), (1, 1, 4.0), (1, 2, -1.0),
+                 (2, 1, -1.0), (2, 2, 4.0)])
+    l, u, perm = oracle.lu_reference(a)
+    assert np.allclose(l.to_dense() @ u.to_dense(), a.to_dense()[perm], atol=1e-10)
+    assert np.allclose(np.diag(l.to_dense()), 1.0)
+
+
+def test_isai0_exact_for_diagonal():
+    a = trip(3, [(0, 0, 2.0), (1, 1, 4.0), (2, 2, 5.0)])
+    ml, mu, perm = oracle.isai_reference(a, 0)
+    assert np.allclose(mu.to_dense() @ ml.to_dense() @ a.to_dense(), np.eye(3), atol=1e-12)
+
+
+def test_isai1_tridiagonal_exact():
+    a = trip(3, [(0, 0, 4.0), (0, 1, -1.0), (1, 0, -1.0), (1, 1, 4.0), (1, 2, -1.0),
+                 (2, 1, -1.0), (2, 2, 4.0)])
+    ml, mu, _ = oracle.isai_reference(a, 1)
+    assert np.allclose(mu.to_dense() @ ml.to_dense() @ a.to_dense(), np.eye(3), atol=1e-10)
+
+
+def test_isai_patterns_level0_level1():
+    a = trip(3, [(0, 0, 4.0), (0, 1, -1.0), (1, 0, -1.0), (1, 1, 4.0), (1, 2, -1.0),
+                 (2, 1, -1.0), (2, 2, 4.0)])
+    l, u, _ = oracle.lu_reference(a)
+    ml0, mu0, _ = oracle.isai_reference(a, 0)
+    assert (ml0.to_dense() != 0).sum() <= l.nnz
+    ml1, mu1, _ = oracle.isai_reference(a, 1)
+    # pattern(L^2) for a bidiagonal L adds the second subdiagonal
+    assert ml1.nnz == 6 and mu1.nnz == 6
+
+
+def test_isai_zero_diagonal_mna_finite():
+    g1, g2 = 0.001, 0.002
+    a = trip(3, [(0, 0, g1 + g2), (0, 1, -g2), (0, 2, 1.0), (1, 0, -g2), (1, 1, g2), (2, 0, 1.0)])
+    ml, mu, _ = oracle.isai_reference(a, 0)
+    assert np.all(np.isfinite(ml.values)) and np.all(np.isfinite(mu.values))
+
+
+def test_isai_level1_better_than_level0():
+    t = [(i, i, 4.0) for i in range(4)] + [(i, i + 1, -1.0) for i in range(3)] + \
+        [(i + 1, i, -1.0) for i in range(3)]
+    a = trip(4, t)
+    l, _, _ = oracle.lu_reference(a)
+    e = []
+    for lev in (0, 1):
+        ml, _, _ = oracle.isai_reference(a, lev)
+        e.append(np.linalg.norm(ml.to_dense() @ l.to_dense() - np.eye(4)))
+    assert e[1] <= e[0] + 1e-14
+
+
+def test_isai1_ill_conditioned_mna():
+    a = trip(4, [(0, 0, 1e-3 + 1e-12), (0, 1, -1e-3), (0, 3, 1.0), (1, 0, -1e-3),
+                 (1, 1, 1e-3 + 1e-12), (2, 2, 1e-12), (2, 3, -1.0), (3, 0, 1.0), (3, 2, -1.0)])
+    err = []
+    for lev in (0, 1):
+        ml, mu, _ = oracle.isai_reference(a, lev)
+        err.append(np.linalg.norm(mu.to_dense() @ ml.to_dense() @ a.to_dense() - np.eye(4)))
+    assert err[1] < err[0]
+
+
+def test_ilu0_isai_exact_on_banded_and_consistent():
+    """ILU(0) of a tridiagonal matrix is its exact LU; ISAI(1) of the bidiagonal
+    factors is exact on columns whose inverse fits pattern(L^2)."""
+    n = 6
+    t = [(i, i, 4.0 + i) for i in range(n)] + [(i, i + 1, -1.0) for i in range(n - 1)] + \
+        [(i + 1, i, -2.0) for i in range(n - 1)]
+    a = trip(n, t)
+    l, u, ml, mu = oracle.ilu0_isai(a, 1)
+    assert np.allclose(l.to_dense() @ u.to_dense(), a.to_dense(), atol=1e-12)
+    Linv = np.linalg.inv(l.to_dense())
+    M = ml.to_dense()
+    mask = M != 0
+    assert np.allclose(M[mask], Linv[mask], atol=1e-12)
+
+
+def test_ilu0_on_mna_pattern_plus_diagonal():
+    """Brute force: ILU(0) factors reproduce A exactly on the factor pattern."""
+    m = oracle.compile(ir.diode_mesh(6, 5, seed=3))
+    a = m.g
+    l, u, _, _ = oracle.ilu0_isai(a, 1)
+    lu = l.to_dense() @ u.to_dense()
+    d = a.to_dense()
+    pat = (d != 0) | np.eye(a.nrows, dtype=bool)
+    pat[np.nonzero(a.to_dense() == 0)] = False
+    rows = np.repeat(np.arange(a.nrows), np.diff(a.row_pointers.astype(int)))
+    sel = (rows, a.col_indices.astype(int))
+    assert np.allclose(lu[sel], d[sel], atol=1e-12)
+
+
+# ----------------------------------------------------------- BiCGSTAB -----
+def test_bicgstab_matches_lu_on_ladder():
+    m = oracle.compile(ir.resistor_ladder(300, seed=9))
+    x_lu = oracle.dense_solve(m.g, m.b_dc)
+    _, _, ml, mu = oracle.ilu0_isai(m.g, 1)
+    x, it = oracle.bicgstab(m.g, m.b_dc, ("isai", ml, mu), tol=1e-12)
+    assert it > 0
+    assert np.max(np.abs(x - x_lu) / np.maximum(np.abs(x_lu), 1e-3)) < 1e-6
+
+
+def test_bicgstab_reference_shadow_breaks_down_on_vsource_rhs():
+    """r_hat = b with b = e_branch (the reference choice) breaks down — the
+    outcome tests/gpu_integration_test.rs:148-188 accepts for V-source circuits."""
+    m = oracle.compile(ir.resistor_ladder(50, seed=1))
+    with pytest.raises(oracle.OracleError, match="BiCGSTAB breakdown"):
+        oracle.bicgstab(m.g, m.b_dc, ("jacobi", oracle.jacobi_inv_diag(m.g)), shadow="b")
+
+
+def test_bicgstab_parameter_errors_and_zero_rhs():
+    a = trip(2, [(0, 0, 2.0), (1, 1, 3.0)])
+    with pytest.raises(oracle.OracleError, match="max_iterations must be > 0"):
+        oracle.bicgstab(a, [1.0, 1.0], max_iters=0)
+    with pytest.raises(oracle.OracleError, match="tolerance must be finite"):
+        oracle.bicgstab(a, [1.0, 1.0], tol=float("nan"))
+    x, it = oracle.bicgstab(a, [0.0, 0.0])
+    assert it == 0 and not x.any()
+
+
+def test_bicgstab_not_converged_message():
+    m = oracle.compile(ir.resistor_ladder(400, seed=2))
+    with pytest.raises(oracle.OracleError, match="did not converge within 3 iterations"):
+        oracle.bicgstab(m.g, m.b_dc, None, max_iters=3, tol=1e-14)
+
+
+# ------------------------------------------------------- device models ----
+def mos(vgs, vds, vto=0.7, kp=1.1e-4, lam=0.04, pol=1.0):
+    return oracle.mosfet_core(vgs, vds, vto, kp, lam, pol)
+
+
+def test_mosfet_deep_saturation_and_linear_closed_forms():
+    i, gm, gds = mos(3.0, 5.0)
+    vov, lam = 2.3, 1 + 0.04 * 5
+    assert abs(i / (1.1e-4 / 2 * vov ** 2 * lam) - 1) < 1e-3
+    assert abs(gm / (1.1e-4 * vov * lam) - 1) < 1e-3
+    assert abs(gds / (1.1e-4 / 2 * vov ** 2 * 0.04) - 1) < 1e-3
+    i, gm, gds = mos(3.0, 0.5)
+    lam = 1 + 0.04 * 0.5
+    assert abs(i / (1.1e-4 * (vov * 0.5 - 0.125) * lam) - 1) < 1e-3
+    assert abs(gm / (1.1e-4 * 0.5 * lam) - 1) < 1e-3
+
+
+def test_mosfet_cutoff_threshold_monotone_pmos():
+    i, gm, gds = mos(0.0, 3.0)
+    assert abs(i) < 1e-10 and abs(gm) < 1e-8 and abs(gds) < 1e-10
+    i, gm, _ = mos(0.7, 3.0)
+    vov_eff = VT * math.log(2.0)
+    assert abs(i / (1.1e-4 / 2 * vov_eff ** 2 * 1.12) - 1) < 0.01 and gm > 0
+    prev = -1e300
+    for k in range(21):
+        i, _, _ = mos(0.6 + 0.01 * k, 3.0)
+        assert i >= prev
+        prev = i
+    i, gm, gds = oracle.mosfet_core(-2.0, -4.0, 0.7, 5.5e-5, 0.04, -1.0)
+    assert i < 0 and gm > 0 and gds > 0
+
+
+@pytest.mark.parametrize("vgs", [0.5, 0.7, 0.9, 1.5, 2.5])
+@pytest.mark.parametrize("vds", [0.05, 0.3, 3.0])
+def test_mosfet_jacobian_central_fd(vgs, vds):
+    h = 1e-6
+    i, gm, gds = mos(vgs, vds)
+    fd_gm = (mos(vgs + h, vds)[0] - mos(vgs - h, vds)[0]) / (2 * h)
+    fd_gds = (mos(vgs, vds + h)[0] - mos(vgs, vds - h)[0]) / (2 * h)
+    assert abs(gm - fd_gm) <= 1e-5 * max(abs(fd_gm), 1e-9)
+    assert abs(gds - fd_gds) <= 1e-5 * max(abs(fd_gds), 1e-9)
+
+
+def test_diode_shockley_and_fd():
+    d = np.zeros(1, ir.DIODE_DESC_DTYPE)
+    d["anode_idx"], d["cathode_idx"] = 0, ir.SENTINEL
+    d["is_val"], d["n_vt"] = 1e-14, VT
+    nvt = float(np.float32(VT))
+    is_ = float(np.float32(1e-14))
+    for v in (-1.0, 0.0, 0.3, 0.65, 0.8):
+        i, g = oracle.diode_eval(d, [v])
+        assert abs(i - is_ * math.expm1(v / nvt)) <= 1e-12 * abs(is_ * math.exp(v / nvt)) + 1e-30
+        h = 1e-7
+        fd = (oracle.diode_eval(d, [v + h])[0] - oracle.diode_eval(d, [v - h])[0]) / (2 * h)
+        assert abs(g - fd) <= 1e-5 * g + 1e-20
+    # exponent clamp at 80 (ds_shaders.rs:96-108)
+    i, g = oracle.diode_eval(d, [10.0])
+    assert g == pytest.approx(is_ * math.exp(80) / nvt, rel=1e-12)
+
+
+def test_bjt_ebers_moll_and_fd():
+    d = np.zeros(1, ir.BJT_DESC_DTYPE)
+    d["collector_idx"], d["base_idx"], d["emitter_idx"] = 0, 1, ir.SENTINEL
+    d["polarity"], d["is_val"], d["bf"], d["br"] = 1.0, 1e-15, 100.0, 1.0
+    d["nf_vt"] = d["nr_vt"] = VT
+    x = np.array([2.0, 0.68])
+    o = oracle.bjt_eval(d, x)
+    is_, nvt = float(np.float32(1e-15)), float(np.float32(VT))
+    i_f = is_ * (math.exp(0.68 / nvt) - 1)
+    i_r = is_ * (math.exp((0.68 - 2.0) / nvt) - 1)
+    assert o[0] == pytest.approx(100 / 101 * i_f - i_r / 2, rel=1e-12)
+    assert o[1] == pytest.approx(i_f / 101 + i_r / 2, rel=1e-12)
+    h = 1e-7
+    # total derivative of Ic wrt Vb equals dIc/dVbe + dIc/dVbc
+    xp, xm = x.copy(), x.copy()
+    xp[1] += h
+    xm[1] -= h
+    fd = (oracle.bjt_eval(d, xp)[0] - oracle.bjt_eval(d, xm)[0]) / (2 * h)
+    assert fd == pytest.approx(o[2] + o[3], rel=1e-5)
+    xp, xm = x.copy(), x.copy()
+    xp[0] += h
+    xm[0] -= h
+    fd = (oracle.bjt_eval(d, xp)[1] - oracle.bjt_eval(d, xm)[1]) / (2 * h)
+    assert fd == pytest.approx(-o[5], rel=1e-5, abs=1e-25)
+
+
+def test_pnjlim_branches():
+    vt, vcrit = VT, VT * math.log(VT / (math.sqrt(2) * 1e-14))
+    assert oracle.pnjlim(0.3, 0.2, vt, vcrit) == 0.3  # below vcrit
+    assert oracle.pnjlim(0.75, 0.745, vt, vcrit) == 0.75  # small step
+    v = oracle.pnjlim(1.0, 0.5, vt, vcrit)
+    assert v == pytest.approx(0.5 + vt * (2 + math.log(1 + 0.5 / vt - 2)))
+    assert oracle.pnjlim(1.0, 0.0, vt, vcrit) == pytest.approx(vt * math.log(1.0 / vt))
+    assert oracle.pnjlim(1.0, 1.5, vt, vcrit) == vcrit  # arg <= 0 branch
+
+
+# -------------------------------------------------------------- Newton ----
+def _diode_resistor(vs=1.0, r=1e3):
+    c = ir.Circuit()
+    c.add_diode_model("D", 1e-14, 1.0)
+    c.vsource("V1", "in", "0", dc=vs)
+    c.resistor("R1", "in", "a", r)
+    c.diode("D1", "a", "0", "D")
+    return c
+
+
+def test_newton_diode_resistor_closed_form():
+    m = oracle.compile(_diode_resistor())
+    x, it, md = oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, abs_tol=1e-12,
+                                    max_iterations=100)
+    is_, nvt = float(np.float32(1e-14)), float(np.float32(VT))
+    gmin = 1e-12
+    f = lambda v: (1.0 - v) / 1e3 - is_ * math.expm1(v / nvt) - gmin * v
+    v = scipy.optimize.brentq(f, 0.0, 1.0, xtol=1e-15)
+    assert abs(x[1] - v) < 1e-10 and it > 1
+
+
+def test_newton_dense_and_sparse_modes_agree_on_mesh():
+    m = oracle.compile(ir.diode_mesh(10, 9, seed=4, diode_every=3))
+    xs, its, _ = oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, abs_tol=1e-12)
+    xd, itd, _ = oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, abs_tol=1e-12,
+                                     linear_mode="dense")
+    assert its == itd
+    assert np.allclose(xs, xd, rtol=1e-10, atol=1e-13)
+
+
+def test_newton_residual_zero_at_solution():
+    """F(x) = A(x) x - b(x) vanishes at the converged point (companion form)."""
+    m = oracle.compile(ir.amplifier_array(8, seed=5))
+    x, it, _ = oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, m.bjt_descriptors,
+                                   m.mosfet_descriptors, abs_tol=1e-12, max_iterations=200)
+    vals, b = oracle.newton_assemble(m.g, m.b_dc, x, (), m.bjt_descriptors, m.mosfet_descriptors)
+    A = ir.CsrMatrix(m.size, m.size, vals, m.g.col_indices, m.g.row_pointers)
+    assert np.max(np.abs(A.spmv(x) - b)) < 1e-12
+
+
+def test_newton_not_converged_message():
+    m = oracle.compile(ir.diode_mesh(6, 6, seed=1, diode_every=2))
+    with pytest.raises(oracle.OracleError, match="did not converge after 2 iterations"):
+        oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, max_iterations=2, abs_tol=1e-15)
+
+
+def test_mosfet_limit_encoded_scale():
+    """Two-pass MOSFET limiting (ds_nonlinear_shaders.rs:1052-1130): a 2 V gate
+    step is scaled to floor(0.5/2*1e6)/1e6 = 0.25; fixed nodes are untouched."""
+    d = np.zeros(1, ir.MOSFET_DESC_DTYPE)
+    d["drain_idx"], d["gate_idx"], d["source_idx"] = 0, 1, ir.SENTINEL
+    d["polarity"], d["vto"], d["kp"] = 1.0, 0.7, 1e-4
+    xn = oracle.newton_limit(2, [0.0, 0.0], [0.2, 2.0], mosfets=d)
+    assert xn[1] == pytest.approx(0.5) and xn[0] == pytest.approx(0.2)
+    d["node_fixed_packed"] = 2
+    xn = oracle.newton_limit(2, [0.0, 0.0], [0.2, 2.0], mosfets=d)
+    assert xn[1] == 2.0
