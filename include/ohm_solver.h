@@ -1,0 +1,216 @@
+/*
+ * ohm_solver.h — C ABI of the MI355X (gfx950) solve backend for ohmnivore.
+ *
+ * The entry points replace the reference's solver traits one for one:
+ *   LinearSolver      src/solver/mod.rs:28-34
+ *   SolverBackend     src/solver/backend.rs:51-96
+ *   NonlinearBackend  src/solver/nonlinear.rs:31-150
+ *   newton_solve      src/solver/newton.rs:65-78
+ *   compile           src/compiler.rs:194
+ * plus the mixed-precision, batched and adjoint forms, which have no
+ * reference counterpart.
+ *
+ * Conventions: every call returns an ohm_status (mirroring OhmnivoreError,
+ * src/error.rs:4-36); on failure ohm_last_error() returns the message
+ * formatted like the reference's Display impl (e.g. "Solve error: BiCGSTAB
+ * breakdown: rho ~ 0"). Handles are opaque and owned by the caller until
+ * their *_free/_destroy call. Every call is complete on return unless its
+ * comment says otherwise (the reference's blocking semantics,
+ * src/solver/backend.rs:322-326). A backend is single-threaded, like the
+ * reference's (Cell counters, src/solver/backend.rs:205-209).
+ */
+#ifndef OHM_SOLVER_H
+#define OHM_SOLVER_H
+
+#include "ohm_circuit.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ohm_backend ohm_backend; /* one HIP device + stream */
+typedef struct ohm_buffer ohm_buffer;   /* device f64 vector        (WgpuBuffer, backend.rs:30-34) */
+typedef struct ohm_matrix ohm_matrix;   /* device CSR f64 (+f32)    (GpuCsrMatrix, backend.rs:20-27) */
+typedef struct ohm_precond ohm_precond; /* Jacobi or ISAI(level)    */
+
+const char *ohm_last_error(void);
+const char *ohm_version(void);
+
+/* ---- MNA compiler: compiler::compile (src/compiler.rs:194-754). The CSR
+ * pattern, ordering and values match the reference bit for bit. */
+ohm_status ohm_compile(const ohm_circuit *circuit, ohm_mna *out);
+void ohm_mna_free(ohm_mna *m);
+
+/* ---- Backend lifecycle: GpuSolver::new / WgpuDsBackend::new
+ * (src/solver/gpu.rs:44-49). Fails with OHM_ERR_NO_DEVICE ("no GPU adapter")
+ * when no gfx950 device is visible. */
+ohm_status ohm_backend_create(int device, ohm_backend **out);
+void ohm_backend_destroy(ohm_backend *b);
+ohm_status ohm_backend_sync(ohm_backend *b);
+/* Dispatch / readback counters (src/solver/ds_backend.rs:182-190). */
+ohm_status ohm_backend_counters(const ohm_backend *b, int64_t *dispatches, int64_t *readbacks);
+
+/* ---- SolverBackend (src/solver/backend.rs:51-96). Vectors are f64: the
+ * reference's DS (hi,lo) pairs become native FP64 on CDNA4. */
+ohm_status ohm_buffer_new(ohm_backend *b, size_t n, ohm_buffer **out);       /* new_buffer  :81 */
+void ohm_buffer_free(ohm_buffer *buf);
+size_t ohm_buffer_len(const ohm_buffer *buf);
+ohm_status ohm_upload_vec(ohm_backend *b, const double *data, size_t n, ohm_buffer *buf); /* :93 */
+ohm_status ohm_download_vec(ohm_backend *b, const ohm_buffer *buf, double *out, size_t n); /* :96 */
+ohm_status ohm_upload_matrix(ohm_backend *b, const double *values, const uint32_t *col_indices,
+                             const uint32_t *row_pointers, size_t n, size_t nnz,
+                             ohm_matrix **out);                              /* :84-90 */
+void ohm_matrix_free(ohm_matrix *m);
+/* Replace the values of an uploaded matrix (same pattern). */
+ohm_status ohm_matrix_set_values(ohm_backend *b, ohm_matrix *m, const double *values);
+ohm_status ohm_spmv(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x, ohm_buffer *y); /* :56 */
+ohm_status ohm_spmv_transpose(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x,
+                              ohm_buffer *y); /* y = A^T x (adjoint; no reference counterpart) */
+/* Measurement helper (no reference counterpart): `reps` back-to-back SpMV
+ * launches on the backend stream bracketed by HIP events; average ms per
+ * launch in *avg_ms. */
+ohm_status ohm_spmv_timed(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x, ohm_buffer *y,
+                          int reps, int precision, double *avg_ms);
+ohm_status ohm_dot(ohm_backend *b, const ohm_buffer *x, const ohm_buffer *y, double *out); /* :59 */
+ohm_status ohm_dot_n(ohm_backend *b, const ohm_buffer *x, const ohm_buffer *y, size_t n,
+                     double *out);                                                   /* :66 */
+ohm_status ohm_axpy(ohm_backend *b, double alpha, const ohm_buffer *x, ohm_buffer *y); /* :72 */
+ohm_status ohm_copy(ohm_backend *b, const ohm_buffer *src, ohm_buffer *dst);           /* :75 */
+ohm_status ohm_scale(ohm_backend *b, double alpha, ohm_buffer *x);                     /* :78 */
+
+/* ---- Preconditioners. Jacobi: inv_diag exactly as src/solver/gpu.rs:178-189,
+ * except that a zero-diagonal row r takes 1/S_rr, S_rr = -sum_j a_rj a_jr / a_jj
+ * (the diagonal of the Schur complement), or 1 if that is zero too.
+ * ISAI(level): the reference's approximate inverses M_L ~ L^-1, M_U ~ U^-1 on
+ * pattern(L^(level+1)), pattern(U^(level+1)) (src/solver/preconditioner.rs:239-277),
+ * with L, U from a GPU ILU(0) of pattern(A) + diagonal. AUTO picks ISAI(1)
+ * when any diagonal is zero, else Jacobi (src/solver/gpu.rs:128-139). */
+enum { OHM_PREC_NONE = 0, OHM_PREC_JACOBI = 1, OHM_PREC_ISAI = 2, OHM_PREC_AUTO = 3 };
+ohm_status ohm_precond_create(ohm_backend *b, const ohm_matrix *a, int kind, int level,
+                              ohm_precond **out);
+/* Recompute the numeric part after ohm_matrix_set_values (pattern reused). */
+ohm_status ohm_precond_update(ohm_backend *b, ohm_precond *p, const ohm_matrix *a);
+ohm_status ohm_precond_apply(ohm_backend *b, const ohm_precond *p, const ohm_buffer *in,
+                             ohm_buffer *out);
+int ohm_precond_kind(const ohm_precond *p);
+/* ISAI: fixed-point ILU(0) sweeps per numeric refresh (default 5); takes
+ * effect at the next ohm_precond_update, which restarts from A. */
+ohm_status ohm_precond_set_ilu_sweeps(ohm_precond *p, int sweeps);
+void ohm_precond_free(ohm_precond *p);
+
+/* ---- BiCGSTAB: bicgstab_with_params (src/solver/bicgstab.rs:56-189). */
+enum { OHM_SHADOW_RHS = 0, OHM_SHADOW_HASHED = 1 };
+enum { OHM_PRECISION_FP64 = 0, OHM_PRECISION_FP32_IR = 1 };
+typedef struct {
+  int64_t max_iterations;  /* default 10000 (bicgstab.rs:10) */
+  double tolerance;        /* default 1e-5  (bicgstab.rs:11); relative to ||b|| */
+  int32_t shadow;          /* OHM_SHADOW_RHS = r_hat = r0 (reference); HASHED (default) */
+  int32_t warm_start;      /* 1: x holds x0 on entry; 0: x0 = 0 (reference) */
+  int32_t precision;       /* FP64, or FP32 inner BiCGSTAB inside FP64 iterative refinement */
+  int32_t check_interval;  /* iterations between host status polls (0 = adaptive) */
+  double inner_tolerance;  /* FP32_IR: inner relative tolerance (default 1e-4) */
+  int64_t max_refinements; /* FP32_IR: outer steps (default 30) */
+  int32_t breakdown;       /* OHM_BREAKDOWN_SCALED (default) or OHM_BREAKDOWN_ABSOLUTE */
+  int32_t tol_mode;        /* OHM_TOL_RHS: stop at tolerance*||b|| (reference);
+                              OHM_TOL_INITIAL: stop at max(tolerance*||r0||, tol_floor*||b||) */
+  double tol_floor;        /* OHM_TOL_INITIAL floor, relative to ||b|| (default 1e-15) */
+} ohm_bicgstab_params;
+enum { OHM_TOL_RHS = 0, OHM_TOL_INITIAL = 1 };
+/* Breakdown tests. ABSOLUTE is the reference's |rho|, |r_hat.v|, t.t, |omega|
+ * < 1e-30 (src/solver/bicgstab.rs:107,125,151,178), which fires on healthy
+ * iterations once MNA residuals (amperes at ~mS scale) fall below ~1e-15.
+ * SCALED applies the same 1e-30 to the normalised quantities
+ * rho/(|r_hat||r|), r_hat.v/(|r_hat||v|), t.t/s.s and omega. */
+enum { OHM_BREAKDOWN_SCALED = 0, OHM_BREAKDOWN_ABSOLUTE = 1 };
+
+typedef struct {
+  int64_t iterations;      /* BiCGSTAB iterations (summed over refinements) */
+  int64_t refinements;     /* FP32_IR outer steps */
+  double residual_norm;    /* FP64 ||b - A x|| / ||b|| at exit */
+  double device_ms;        /* device time of the solve (HIP events) */
+  int64_t dispatches;      /* kernels enqueued */
+  int64_t readbacks;       /* host status polls */
+} ohm_bicgstab_result;
+
+void ohm_bicgstab_default_params(ohm_bicgstab_params *p);
+ohm_status ohm_bicgstab(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *rhs, ohm_buffer *x,
+                        const ohm_precond *p, const ohm_bicgstab_params *params,
+                        ohm_bicgstab_result *result);
+/* Solve A^T x = rhs (adjoint systems; transposed preconditioner). */
+ohm_status ohm_bicgstab_transpose(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *rhs,
+                                  ohm_buffer *x, const ohm_precond *p,
+                                  const ohm_bicgstab_params *params, ohm_bicgstab_result *result);
+
+/* ---- LinearSolver::solve_real (src/solver/mod.rs:30): host slices in and
+ * out, preconditioner chosen by `precond` (OHM_PREC_AUTO = the reference's
+ * rule). params may be NULL for defaults. */
+ohm_status ohm_solve_real(ohm_backend *b, size_t n, size_t nnz, const double *values,
+                          const uint32_t *col_indices, const uint32_t *row_pointers,
+                          const double *rhs, double *x, int precond,
+                          const ohm_bicgstab_params *params, ohm_bicgstab_result *result);
+
+/* ---- Newton-Raphson: NewtonParams (src/solver/newton.rs:34-55) and
+ * newton_solve (:65-513). Device evaluation, stamping, linear solve,
+ * limiting and the convergence test all run on the GPU; the host polls one
+ * status word per Newton iteration. */
+typedef struct {
+  int64_t max_iterations;          /* default 50 */
+  double abs_tol;                  /* default 1e-4 (max |x - x_old|) */
+  int64_t bicgstab_max_iterations; /* default 10000 */
+  double bicgstab_tolerance;       /* default 1e-5, relative to the Newton residual ||F(x_k)|| */
+  int32_t precision;               /* OHM_PRECISION_* for the linear sub-solve */
+  int32_t precond;                 /* OHM_PREC_AUTO (reference rule) / JACOBI / ISAI */
+  const double *initial_guess;     /* NewtonParams::initial_guess, or NULL */
+  double bicgstab_floor;           /* sub-solve floor relative to ||b(x_k)|| (default 1e-15) */
+} ohm_newton_params;
+/* The sub-solve is warm-started from x_k, so its initial residual is the
+ * Newton residual F(x_k) = b(x_k) - A(x_k) x_k and the stopping rule is
+ * max(bicgstab_tolerance*||F||, bicgstab_floor*||b||) (an inexact Newton
+ * step). The reference solves from x = 0 to 1e-5*||b|| (src/solver/newton.rs:407-421),
+ * which caps the accuracy of the converged point at about kappa*1e-5. */
+
+typedef struct {
+  int64_t iterations;         /* Newton iterations (Stats::newton_iterations) */
+  int64_t bicgstab_iterations;/* summed over Newton steps */
+  double max_diff;            /* last max |x - x_old| */
+  double device_ms;
+  int64_t fp64_fallbacks;     /* FP32_IR: Newton steps redone in FP64 after the
+                                 refinement stalled (kappa * 2^-24 >~ 1) */
+} ohm_newton_result;
+
+void ohm_newton_default_params(ohm_newton_params *p);
+ohm_status ohm_newton_solve(ohm_backend *b, const double *base_values,
+                            const uint32_t *col_indices, const uint32_t *row_pointers,
+                            const double *base_b, const ohm_diode_desc *diodes, size_t n_diodes,
+                            const ohm_bjt_desc *bjts, size_t n_bjts, const ohm_mosfet_desc *mosfets,
+                            size_t n_mosfets, size_t system_size, size_t matrix_nnz,
+                            const ohm_newton_params *params, double *x_out,
+                            ohm_newton_result *result, int32_t *bicgstab_iters_per_newton);
+
+/* Device-resident Newton context: uploads once, solves many times (the
+ * timed form: inputs already in HBM). */
+typedef struct ohm_newton_ctx ohm_newton_ctx;
+ohm_status ohm_newton_create(ohm_backend *b, const double *base_values,
+                             const uint32_t *col_indices, const uint32_t *row_pointers,
+                             const double *base_b, const ohm_diode_desc *diodes, size_t n_diodes,
+                             const ohm_bjt_desc *bjts, size_t n_bjts,
+                             const ohm_mosfet_desc *mosfets, size_t n_mosfets,
+                             size_t system_size, size_t matrix_nnz, ohm_newton_ctx **out);
+ohm_status ohm_newton_run(ohm_newton_ctx *ctx, const ohm_newton_params *params,
+                          ohm_newton_result *result, int32_t *bicgstab_iters_per_newton);
+ohm_status ohm_newton_get_x(ohm_newton_ctx *ctx, double *x_out);
+/* The assembled A(x), b(x) of the last iteration (for checks and adjoints). */
+ohm_status ohm_newton_get_system(ohm_newton_ctx *ctx, double *values_out, double *b_out);
+void ohm_newton_free(ohm_newton_ctx *ctx);
+
+/* ---- NonlinearBackend kernels (src/solver/nonlinear.rs:31-150) on a
+ * Newton context's device buffers: evaluate + stamp into A, b at x. */
+ohm_status ohm_newton_assemble_at(ohm_newton_ctx *ctx, const double *x, double *values_out,
+                                  double *b_out);
+/* Limiting of x_new against x_old (BJT, MOSFET, diode passes). */
+ohm_status ohm_newton_limit(ohm_newton_ctx *ctx, const double *x_old, double *x_new);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OHM_SOLVER_H */

@@ -1,0 +1,566 @@
+// ohm_bicgstab.hip — right-preconditioned BiCGSTAB (bicgstab_with_params,
+// src/solver/bicgstab.rs:56-189) as a five-kernel fused pipeline per
+// iteration with every scalar (rho, alpha, omega, beta, status) resident in
+// device memory:
+//
+//   K1  p = r + beta (p - omega v);  p_hat = M^-1 p (Jacobi/none fused)
+//   K2  v = A p_hat  (+ r_hat.v -> alpha, breakdown test)
+//   K3  s = r - alpha v; s_hat = M^-1 s;  (+ ||s|| -> early exit test)
+//   K4  t = A s_hat  (+ t.s, t.t -> omega, breakdown test)
+//   K5  x += alpha p_hat + omega s_hat; r = s - omega t
+//       (+ ||r||, r_hat.r -> NaN/convergence/omega tests, beta for the next
+//        iteration, rho breakdown test)
+//
+// The last workgroup of each reducing kernel applies the reference's scalar
+// update and tests; once the state leaves RUNNING every later kernel exits
+// at its first instruction, so the host enqueues iterations in chunks and
+// polls one status word per chunk instead of syncing on each dot product
+// (the reference syncs six times per iteration, src/solver/backend.rs:443-446).
+// ISAI applies M^-1 = M_U M_L as two extra SpMV launches between K1/K2 and
+// K3/K4. The same kernels run in FP32 (with compensated dot products) as the
+// inner solver of FP64 iterative refinement.
+#include <algorithm>
+#include <cmath>
+
+#include "ohm_precond.h"
+#include "ohm_spmv.h"
+
+namespace ohm {
+
+__global__ void k_state_reset(SolveState *st) {
+  if (threadIdx.x == 0) {
+    SolveState z;
+    z.rho = z.alpha = z.omega = z.beta = 1.0;
+    z.abs_tol = z.bnorm2 = z.rr = z.ss = z.rv = z.ts = z.tt = z.rhrh = 0.0;
+    z.status = ST_RUNNING;
+    z.s_conv = 0;
+    z.iters = 0;
+    z.max_iters = 0;
+    z.counter = 0u;
+    z.scaled = 1;
+    *st = z;
+  }
+}
+
+// Breakdown tests: the reference's absolute thresholds, or the same 1e-30
+// applied to normalised quantities.
+__device__ __forceinline__ bool rho_breakdown(double rho, double hh, double rr, int scaled) {
+  return scaled ? fabs(rho) <= 1e-30 * sqrt(hh * rr) : fabs(rho) < 1e-30;
+}
+
+// r = b - A x0 (warm) or b; r_hat = hashed or r; p = v = 0; x = 0 (cold).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
+                                                 const T *__restrict__ ax0, T *__restrict__ x,
+                                                 T *__restrict__ r, T *__restrict__ rh,
+                                                 T *__restrict__ p, T *__restrict__ v, int shadow,
+                                                 int warm, double tol, double floor_rel,
+                                                 int tol_mode, int max_iters, int scaled,
+                                                 SolveState *st, double *partials) {
+  Acc<T> bb, rr, rhr, hh;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    T bi = b[i];
+    T ri = warm ? bi - ax0[i] : bi;
+    if (!warm) x[i] = T(0);
+    r[i] = ri;
+    T rhi = shadow ? (T)shadow_value((uint64_t)i) : ri;
+    rh[i] = rhi;
+    p[i] = T(0);
+    v[i] = T(0);
+    bb.add_prod(bi, bi);
+    rr.add_prod(ri, ri);
+    rhr.add_prod(rhi, ri);
+    hh.add_prod(rhi, rhi);
+  }
+  double vals[4] = {bb.get(), rr.get(), rhr.get(), hh.get()}, tot[4];
+  if (grid_sum<4>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
+    st->bnorm2 = tot[0];
+    st->rhrh = tot[3];
+    st->scaled = scaled;
+    st->max_iters = max_iters;
+    st->rr = tot[1];
+    double bnorm = sqrt(tot[0]);
+    if (bnorm < 1e-30) {  // b == 0: x = 0 (src/solver/bicgstab.rs:93-97)
+      st->status = ST_CONVERGED;
+      return;
+    }
+    st->abs_tol = tol_mode == OHM_TOL_INITIAL ? fmax(tol * sqrt(tot[1]), floor_rel * bnorm)
+                                              : tol * bnorm;
+    if (warm && sqrt(tot[1]) < st->abs_tol) {
+      st->status = ST_CONVERGED;
+    } else if (rho_breakdown(tot[2], tot[3], tot[1], scaled)) {
+      st->status = ST_BRK_RHO;
+    } else {
+      st->beta = tot[2];
+      st->rho = tot[2];
+    }
+  }
+}
+
+template <typename T, int PK>
+__global__ __launch_bounds__(kBlock) void k_p_update(int n, const T *__restrict__ r,
+                                                     T *__restrict__ p, const T *__restrict__ v,
+                                                     T *__restrict__ ph,
+                                                     const T *__restrict__ inv,
+                                                     const SolveState *st) {
+  if (st->status != ST_RUNNING) return;
+  const T beta = (T)st->beta, omega = (T)st->omega;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    T pi = (p[i] - omega * v[i]) * beta + r[i];
+    p[i] = pi;
+    if (PK == PK_JACOBI) ph[i] = inv[i] * pi;
+    if (PK == PK_NONE) ph[i] = pi;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_alpha(const uint32_t *__restrict__ rowptr,
+                                                       const uint32_t *__restrict__ col,
+                                                       const T *__restrict__ val,
+                                                       const uint32_t *__restrict__ blocks,
+                                                       const T *__restrict__ ph,
+                                                       T *__restrict__ v,
+                                                       const T *__restrict__ rh, SolveState *st,
+                                                       double *partials) {
+  if (st->status != ST_RUNNING) return;
+  Acc<T> acc, vv;
+  spmv_block<T>(rowptr, col, val, blocks, ph, [&](uint32_t r, T y) {
+    v[r] = y;
+    acc.add_prod(rh[r], y);
+    vv.add_prod(y, y);
+  });
+  double vals[2] = {acc.get(), vv.get()}, tot[2];
+  if (grid_sum<2>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
+    st->rv = tot[0];
+    const bool brk = st->scaled ? fabs(tot[0]) <= 1e-30 * sqrt(st->rhrh * tot[1])
+                                : fabs(tot[0]) < 1e-30;
+    if (brk)
+      st->status = ST_BRK_RV;
+    else
+      st->alpha = st->rho / tot[0];
+  }
+}
+
+template <typename T, int PK>
+__global__ __launch_bounds__(kBlock) void k_s_update(int n, const T *__restrict__ r,
+                                                     const T *__restrict__ v, T *__restrict__ s,
+                                                     T *__restrict__ sh,
+                                                     const T *__restrict__ inv, SolveState *st,
+                                                     double *partials) {
+  if (st->status != ST_RUNNING) return;
+  const T alpha = (T)st->alpha;
+  Acc<T> acc;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    T si = r[i] - alpha * v[i];
+    s[i] = si;
+    if (PK == PK_JACOBI) sh[i] = inv[i] * si;
+    if (PK == PK_NONE) sh[i] = si;
+    acc.add_prod(si, si);
+  }
+  double vals[1] = {acc.get()}, tot[1];
+  if (grid_sum<1>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
+    st->ss = tot[0];
+    if (sqrt(tot[0]) < st->abs_tol) st->s_conv = 1;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_omega(const uint32_t *__restrict__ rowptr,
+                                                       const uint32_t *__restrict__ col,
+                                                       const T *__restrict__ val,
+                                                       const uint32_t *__restrict__ blocks,
+                                                       const T *__restrict__ sh,
+                                                       T *__restrict__ t,
+                                                       const T *__restrict__ s, SolveState *st,
+                                                       double *partials) {
+  if (st->status != ST_RUNNING || st->s_conv) return;
+  Acc<T> ts, tt;
+  spmv_block<T>(rowptr, col, val, blocks, sh, [&](uint32_t r, T y) {
+    t[r] = y;
+    ts.add_prod(y, s[r]);
+    tt.add_prod(y, y);
+  });
+  double vals[2] = {ts.get(), tt.get()}, tot[2];
+  if (grid_sum<2>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
+    st->ts = tot[0];
+    st->tt = tot[1];
+    const bool brk = st->scaled ? tot[1] <= 1e-30 * st->ss : fabs(tot[1]) < 1e-30;
+    if (brk)
+      st->status = ST_BRK_TT;
+    else
+      st->omega = tot[0] / tot[1];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
+                                                      T *__restrict__ r,
+                                                      const T *__restrict__ rh,
+                                                      const T *__restrict__ ph,
+                                                      const T *__restrict__ sh,
+                                                      const T *__restrict__ s,
+                                                      const T *__restrict__ t, SolveState *st,
+                                                      double *partials) {
+  if (st->status != ST_RUNNING) return;
+  const int sconv = st->s_conv;
+  const T alpha = (T)st->alpha, omega = (T)st->omega;
+  Acc<T> rr, rhr;
+  if (sconv) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+      x[i] = x[i] + alpha * ph[i];
+  } else {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+      x[i] = (x[i] + alpha * ph[i]) + omega * sh[i];
+      T ri = s[i] - omega * t[i];
+      r[i] = ri;
+      rr.add_prod(ri, ri);
+      rhr.add_prod(rh[i], ri);
+    }
+  }
+  double vals[2] = {rr.get(), rhr.get()}, tot[2];
+  if (grid_sum<2>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
+    st->iters += 1;
+    if (sconv) {
+      st->status = ST_CONVERGED;
+      return;
+    }
+    st->rr = tot[0];
+    double rnorm = sqrt(tot[0]);
+    if (std::isnan(rnorm) || std::isinf(rnorm)) {
+      st->status = ST_NAN;
+    } else if (rnorm < st->abs_tol) {
+      st->status = ST_CONVERGED;
+    } else if (fabs(st->omega) < 1e-30) {
+      st->status = ST_BRK_OMEGA;
+    } else if (st->iters >= st->max_iters) {
+      st->status = ST_MAXITER;
+    } else if (rho_breakdown(tot[1], st->rhrh, tot[0], st->scaled)) {
+      st->status = ST_BRK_RHO;
+    } else {
+      st->beta = (tot[1] / st->rho) * (st->alpha / st->omega);
+      st->rho = tot[1];
+    }
+  }
+}
+
+// ------------------------------------------------------------ workspace --
+template <typename T>
+struct Workspace {
+  int n = 0;
+  T *base = nullptr;
+  SolveState *st = nullptr;
+  BicgVecs<T> v{};
+};
+
+static Workspace<double> g_ws64;  // one cached workspace per type (single-threaded backend)
+static Workspace<float> g_ws32;
+
+template <typename T>
+static ohm_status ws_get(Workspace<T> &w, int n) {
+  if (w.base && w.n >= n) return OHM_OK;
+  if (w.base) OHM_HIP(hipFree(w.base));
+  if (w.st) OHM_HIP(hipFree(w.st));
+  size_t stride = ((size_t)std::max(n, 1) + 63) & ~size_t(63);
+  OHM_HIP(hipMalloc(&w.base, sizeof(T) * stride * 10));
+  OHM_HIP(hipMalloc(&w.st, sizeof(SolveState)));
+  T *q = w.base;
+  w.v.x = q;
+  w.v.r = q + stride;
+  w.v.rh = q + 2 * stride;
+  w.v.p = q + 3 * stride;
+  w.v.v = q + 4 * stride;
+  w.v.s = q + 5 * stride;
+  w.v.t = q + 6 * stride;
+  w.v.ph = q + 7 * stride;
+  w.v.sh = q + 8 * stride;
+  w.v.tmp = q + 9 * stride;
+  w.n = n;
+  return OHM_OK;
+}
+
+template <typename T>
+struct Ops;
+template <>
+struct Ops<double> {
+  static const double *vals(const DevCsr &a) { return a.val; }
+  static const double *inv(const PrecondDev &p) { return p.inv; }
+  static void spmv(ohm_backend *b, const DevCsr &a, const double *x, double *y,
+                   const SolveState *st, int skip) {
+    launch_spmv(b, a, x, y, st, skip);
+  }
+  static void prec(ohm_backend *b, const PrecondDev &p, const double *in, double *out,
+                   double *tmp, const SolveState *st, int skip) {
+    precond_apply(b, p, in, out, tmp, st, skip);
+  }
+};
+template <>
+struct Ops<float> {
+  static const float *vals(const DevCsr &a) { return a.val32; }
+  static const float *inv(const PrecondDev &p) { return p.inv32; }
+  static void spmv(ohm_backend *b, const DevCsr &a, const float *x, float *y,
+                   const SolveState *st, int skip) {
+    launch_spmv_f32(b, a, x, y, st, skip);
+  }
+  static void prec(ohm_backend *b, const PrecondDev &p, const float *in, float *out, float *tmp,
+                   const SolveState *st, int skip) {
+    precond_apply_f32(b, p, in, out, tmp, st, skip);
+  }
+};
+
+template <typename T, int PK>
+static void enqueue_iteration(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
+                              BicgVecs<T> &v, SolveState *st, int n) {
+  const dim3 gv(grid_for(n, 4)), blk(kBlock), ga(a.n_blocks);
+  const T *inv = Ops<T>::inv(p);
+  const T *av = Ops<T>::vals(a);
+  hipLaunchKernelGGL((k_p_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.p, v.v, v.ph, inv, st);
+  if (PK == PK_ISAI) Ops<T>::prec(b, p, v.p, v.ph, v.tmp, st, 1);
+  hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks, v.ph,
+                     v.v, v.rh, st, b->d_partials);
+  hipLaunchKernelGGL((k_s_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.v, v.s, v.sh, inv, st,
+                     b->d_partials);
+  if (PK == PK_ISAI) Ops<T>::prec(b, p, v.s, v.sh, v.tmp, st, 2);
+  hipLaunchKernelGGL(k_spmv_omega<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks, v.sh,
+                     v.t, v.s, st, b->d_partials);
+  hipLaunchKernelGGL(k_xr_update<T>, gv, blk, 0, b->stream, n, v.x, v.r, v.rh, v.ph, v.sh, v.s,
+                     v.t, st, b->d_partials);
+  b->dispatches += 5;
+}
+
+struct StatusWord {
+  int status, s_conv, iters, max_iters;
+};
+
+// Runs BiCGSTAB on the workspace vectors; rhs in ws.r on entry? No: rhs is a
+// separate device pointer; x0 (warm) is read from xin.
+template <typename T>
+static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const T *rhs,
+                            Workspace<T> &w, const BicgOpts &o, BicgOut &out) {
+  const int n = a.n;
+  SolveState *st = w.st;
+  BicgVecs<T> &v = w.v;
+  OHM_TRY(ensure_partials(b, (size_t)std::max(a.n_blocks, kMaxReduceBlocks) * 4));
+  hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, b->stream, st);
+  if (o.warm) Ops<T>::spmv(b, a, v.x, v.t, nullptr, 0);
+  hipLaunchKernelGGL(k_init<T>, dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream, n, rhs, v.t, v.x,
+                     v.r, v.rh, v.p, v.v, o.shadow == OHM_SHADOW_HASHED ? 1 : 0, o.warm ? 1 : 0,
+                     o.tol, o.tol_floor, o.tol_mode,
+                     (int)std::min<int64_t>(o.max_iters, 0x7fffffff), o.scaled_breakdown ? 1 : 0,
+                     st, b->d_partials);
+  b->dispatches += 2;
+  StatusWord *h = reinterpret_cast<StatusWord *>(b->h_pinned);
+  int64_t enqueued = 0;
+  for (;;) {
+    OHM_HIP(hipMemcpyAsync(h, &st->status, sizeof(StatusWord), hipMemcpyDeviceToHost, b->stream));
+    OHM_HIP(hipStreamSynchronize(b->stream));
+    b->readbacks++;
+    out.polls++;
+    if (h->status != ST_RUNNING) break;
+    int64_t done = h->iters;
+    int chunk = o.check_interval > 0
+                    ? o.check_interval
+                    : (int)std::min<int64_t>(32, std::max<int64_t>(4, done / 2));
+    chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, o.max_iters - enqueued));
+    for (int k = 0; k < chunk; k++) {
+      if (p.kind == PK_JACOBI)
+        enqueue_iteration<T, PK_JACOBI>(b, a, p, v, st, n);
+      else if (p.kind == PK_ISAI)
+        enqueue_iteration<T, PK_ISAI>(b, a, p, v, st, n);
+      else
+        enqueue_iteration<T, PK_NONE>(b, a, p, v, st, n);
+    }
+    enqueued += chunk;
+    OHM_HIP(hipGetLastError());
+    // Every enqueued iteration either advances iters or ends the solve, so a
+    // state that is still RUNNING well past max_iters means a lost launch.
+    if (enqueued > o.max_iters + 64)
+      return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB state did not advance (%lld enqueued)",
+                  (long long)enqueued);
+  }
+  out.status = h->status;
+  out.iters = h->iters;
+  OHM_HIP(hipGetLastError());
+  return OHM_OK;
+}
+
+ohm_status status_to_error(int status, int64_t iters, int64_t max_iters) {
+  switch (status) {
+    case ST_CONVERGED:
+      return OHM_OK;
+    case ST_BRK_RHO:
+      return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: rho ~ 0");
+    case ST_BRK_RV:
+      return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: r_hat.v ~ 0");
+    case ST_BRK_TT:
+      return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: ||t|| ~ 0");
+    case ST_NAN:
+      return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB diverged: NaN/Inf in residual");
+    case ST_BRK_OMEGA:
+      return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: omega ~ 0");
+    default:
+      return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB did not converge within %lld iterations",
+                  (long long)max_iters);
+  }
+}
+
+static ohm_status check_params(const BicgOpts &o) {
+  if (o.max_iters <= 0) return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB max_iterations must be > 0");
+  if (!(o.tol > 0.0) || !std::isfinite(o.tol))
+    return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB tolerance must be finite and > 0");
+  return OHM_OK;
+}
+
+ohm_status bicg_solve_f64(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const double *rhs,
+                          double *x, const BicgOpts &o, BicgOut &out) {
+  OHM_TRY(check_params(o));
+  if (a.n == 0) {
+    out.status = ST_CONVERGED;
+    return OHM_OK;
+  }
+  OHM_TRY(ws_get(g_ws64, a.n));
+  if (o.warm) OHM_HIP(hipMemcpyAsync(g_ws64.v.x, x, sizeof(double) * a.n, hipMemcpyDeviceToDevice, b->stream));
+  OHM_TRY(bicg_core<double>(b, a, p, rhs, g_ws64, o, out));
+  OHM_HIP(hipMemcpyAsync(x, g_ws64.v.x, sizeof(double) * a.n, hipMemcpyDeviceToDevice, b->stream));
+  OHM_HIP(hipStreamSynchronize(b->stream));
+  return status_to_error(out.status, out.iters, o.max_iters);
+}
+
+// ------------------------------------------------- iterative refinement --
+// r = b - A x (FP64) with ||r||^2 and ||b||^2 into d_scalars[0..1]; the
+// scaled residual r / ||r|| is written in FP32 by the next kernel.
+__global__ __launch_bounds__(kBlock) void k_residual(const uint32_t *__restrict__ rowptr,
+                                                     const uint32_t *__restrict__ col,
+                                                     const double *__restrict__ val,
+                                                     const uint32_t *__restrict__ blocks,
+                                                     const double *__restrict__ x,
+                                                     const double *__restrict__ rhs,
+                                                     double *__restrict__ r, double *partials,
+                                                     unsigned *counter, double *out) {
+  Acc<double> rr, bb;
+  spmv_block<double>(rowptr, col, val, blocks, x, [&](uint32_t i, double y) {
+    double ri = rhs[i] - y;
+    r[i] = ri;
+    rr.add_prod(ri, ri);
+    bb.add_prod(rhs[i], rhs[i]);
+  });
+  double vals[2] = {rr.get(), bb.get()}, tot[2];
+  if (grid_sum<2>(vals, partials, counter, tot) && threadIdx.x == 0) {
+    out[0] = tot[0];
+    out[1] = tot[1];
+  }
+}
+
+__global__ void k_scale_to_f32(int n, const double *__restrict__ r, const double *__restrict__ nrm2,
+                               float *__restrict__ r32) {
+  const double inv = 1.0 / sqrt(nrm2[0]);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    r32[i] = (float)(r[i] * inv);
+}
+
+__global__ void k_correct(int n, const float *__restrict__ d, const double *__restrict__ nrm2,
+                          double *__restrict__ x) {
+  const double sc = sqrt(nrm2[0]);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    x[i] += sc * (double)d[i];
+}
+
+static double *g_ir_r = nullptr;
+static float *g_ir_r32 = nullptr;
+static int g_ir_n = 0;
+
+ohm_status true_residual(ohm_backend *b, const DevCsr &a, const double *rhs, const double *x,
+                         double &rel) {
+  if (a.n == 0) {
+    rel = 0.0;
+    return OHM_OK;
+  }
+  if (g_ir_n < a.n) {
+    if (g_ir_r) OHM_HIP(hipFree(g_ir_r));
+    if (g_ir_r32) OHM_HIP(hipFree(g_ir_r32));
+    OHM_HIP(hipMalloc(&g_ir_r, sizeof(double) * a.n));
+    OHM_HIP(hipMalloc(&g_ir_r32, sizeof(float) * a.n));
+    g_ir_n = a.n;
+  }
+  OHM_TRY(ensure_partials(b, (size_t)std::max(a.n_blocks, kMaxReduceBlocks) * 4));
+  hipLaunchKernelGGL(k_residual, dim3(a.n_blocks), dim3(kBlock), 0, b->stream, a.rowptr, a.col,
+                     a.val, a.blocks, x, rhs, g_ir_r, b->d_partials, b->d_counter, b->d_scalars);
+  b->dispatches++;
+  OHM_HIP(hipMemcpyAsync(b->h_pinned, b->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                         b->stream));
+  OHM_HIP(hipStreamSynchronize(b->stream));
+  b->readbacks++;
+  double rn = sqrt(b->h_pinned[0]), bn = sqrt(b->h_pinned[1]);
+  rel = bn > 0 ? rn / bn : rn;
+  return OHM_OK;
+}
+
+ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double *rhs, double *x,
+                         const BicgOpts &o, double inner_tol, int64_t max_ref, BicgOut &out,
+                         int64_t &refinements) {
+  OHM_TRY(check_params(o));
+  refinements = 0;
+  const int n = a.n;
+  if (n == 0) {
+    out.status = ST_CONVERGED;
+    return OHM_OK;
+  }
+  OHM_TRY(csr_ensure_f32(b, a));
+  OHM_TRY(precond_ensure_f32(b, p));
+  OHM_TRY(ws_get(g_ws32, n));
+  if (!o.warm) OHM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, b->stream));
+  double rel = 0.0, prev = INFINITY, target = o.tol;
+  int64_t total = 0;
+  int last_status = ST_RUNNING, stalled = 0;
+  constexpr int64_t kInnerMaxIters = 1000;  // per refinement
+  for (int64_t k = 0; k <= max_ref; k++) {
+    OHM_TRY(true_residual(b, a, rhs, x, rel));
+    double bn = sqrt(b->h_pinned[1]);
+    if (k == 0 && o.tol_mode == OHM_TOL_INITIAL) target = std::max(o.tol * rel, o.tol_floor);
+    if (bn < 1e-30 || rel < target) {
+      out.status = ST_CONVERGED;
+      out.iters = total;
+      return OHM_OK;
+    }
+    if (!std::isfinite(rel)) {
+      out.status = ST_NAN;
+      return status_to_error(ST_NAN, total, o.max_iters);
+    }
+    // A refinement that does not halve the FP64 residual means the FP32
+    // inner solver is beyond its attainable accuracy (kappa * 2^-24 >~ 1).
+    if (k > 0 && rel > 0.5 * prev) stalled = 1;
+    if (k == max_ref || total >= o.max_iters || stalled) break;
+    prev = rel;
+    hipLaunchKernelGGL(k_scale_to_f32, dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream, n, g_ir_r,
+                       b->d_scalars, g_ir_r32);
+    BicgOpts io = o;
+    io.warm = false;
+    io.tol_mode = OHM_TOL_RHS;
+    io.tol = std::max(inner_tol, 0.5 * target / std::max(rel, 1e-300));
+    io.tol = std::min(io.tol, 0.5);
+    io.max_iters = std::max<int64_t>(1, std::min<int64_t>(kInnerMaxIters, o.max_iters - total));
+    BicgOut iout;
+    OHM_TRY(bicg_core<float>(b, a, p, g_ir_r32, g_ws32, io, iout));
+    total += iout.iters;
+    out.polls += iout.polls;
+    last_status = iout.status;
+    refinements++;
+    // An FP32 breakdown or iteration cap still leaves a useful correction (the
+    // inner residual stalls at its FP32 attainable accuracy); only a NaN
+    // iterate is discarded. The FP64 residual decides convergence.
+    if (iout.status != ST_NAN && iout.iters > 0) {
+      hipLaunchKernelGGL(k_correct, dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream, n,
+                         g_ws32.v.x, b->d_scalars, x);
+      b->dispatches += 1;
+    }
+    b->dispatches += 1;
+  }
+  out.iters = total;
+  out.status = ST_MAXITER;
+  OHM_HIP(hipStreamSynchronize(b->stream));
+  return fail(OHM_ERR_SOLVE,
+              "Solve error: mixed-precision BiCGSTAB stalled at relative residual %.3e after %lld "
+              "refinements (last inner status %d)",
+              rel, (long long)refinements, last_status);
+}
+
+}  // namespace ohm

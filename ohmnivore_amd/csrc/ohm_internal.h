@@ -1,0 +1,158 @@
+// ohm_internal.h — shared internals of the MI355X solve backend.
+//
+// Layout in HBM: CSR with u32 row pointers / column indices (the reference's
+// GPU upload format, src/solver/gpu.rs:124-125), f64 values plus an optional
+// f32 mirror for the mixed-precision inner solver, and flat f64 / f32
+// vectors. Every solve keeps its scalars (rho, alpha, omega, status) in a
+// device-resident SolveState so no iteration needs a host round trip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ohm_solver.h"
+
+namespace ohm {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const std::string &msg);
+ohm_status fail(ohm_status code, const char *fmt, ...);
+ohm_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define OHM_HIP(call)                                                    \
+  do {                                                                   \
+    hipError_t _e = (call);                                              \
+    if (_e != hipSuccess) return ohm::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define OHM_TRY(call)                        \
+  do {                                       \
+    ohm_status _s = (call);                  \
+    if (_s != OHM_OK) return _s;             \
+  } while (0)
+
+// ------------------------------------------------------ launch constants --
+constexpr int kBlock = 256;              // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kSpmvNnzPerBlock = 2048;   // LDS-staged products per row block
+constexpr int kSpmvRowsPerBlock = kBlock;
+constexpr int kMaxReduceBlocks = 2048;   // cap on grid size of reducing vector kernels
+
+// ----------------------------------------------------------- solve state --
+enum SolveStatus : int {
+  ST_RUNNING = 0,
+  ST_CONVERGED = 1,
+  ST_BRK_RHO = 2,
+  ST_BRK_RV = 3,
+  ST_BRK_TT = 4,
+  ST_NAN = 5,
+  ST_BRK_OMEGA = 6,
+  ST_MAXITER = 7,
+};
+
+struct SolveState {
+  double rho, alpha, omega, beta;
+  double abs_tol;       // tolerance * ||b||
+  double bnorm2;        // ||b||^2
+  double rr;            // last ||r||^2
+  double ss;            // last ||s||^2
+  double rv, ts, tt;
+  double rhrh;          // ||r_hat||^2 (scaled breakdown tests)
+  int status;
+  int s_conv;           // ||s|| test passed: x += alpha*p_hat then stop
+  int iters;
+  int max_iters;
+  unsigned counter;     // last-block-done ticket
+  int scaled;           // OHM_BREAKDOWN_SCALED tests
+};
+
+// ------------------------------------------------------ device objects --
+struct DevCsr {
+  int n = 0;
+  int64_t nnz = 0;
+  uint32_t *rowptr = nullptr, *col = nullptr;
+  double *val = nullptr;
+  float *val32 = nullptr;          // f32 mirror (mixed precision)
+  uint32_t *blocks = nullptr;      // row-block boundaries for SpMV
+  int n_blocks = 0;
+  int32_t *diag_pos = nullptr;     // CSR position of (r, r) or -1
+  // transpose pattern (built lazily for A^T x): CSR of A^T + map to A's slots
+  uint32_t *t_rowptr = nullptr, *t_col = nullptr, *t_src = nullptr, *t_blocks = nullptr;
+  int t_n_blocks = 0;
+  std::vector<uint32_t> h_rowptr, h_col; // host copy of the pattern
+};
+
+}  // namespace ohm
+
+struct ohm_backend {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  double *d_partials = nullptr;      // per-block partial sums for reductions
+  size_t partials_cap = 0;
+  unsigned *d_counter = nullptr;     // ticket for one-shot reductions
+  double *d_scalars = nullptr;       // device results of one-shot dots
+  double *h_pinned = nullptr;        // pinned host staging for scalar readback
+  int64_t dispatches = 0, readbacks = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct ohm_buffer {
+  ohm_backend *owner = nullptr;
+  size_t n = 0;
+  double *d = nullptr;
+};
+
+struct ohm_matrix {
+  ohm_backend *owner = nullptr;
+  ohm::DevCsr csr;
+};
+
+namespace ohm {
+
+// ------------------------------------------------------ kernel launchers --
+// All launches go on the backend's stream; `state` (may be null) lets solver
+// kernels skip work once a solve has finished.
+void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blocks);
+ohm_status csr_upload(ohm_backend *b, DevCsr &m, const double *val, const uint32_t *col,
+                      const uint32_t *rowptr, int n, int64_t nnz);
+void csr_free(DevCsr &m);
+ohm_status csr_ensure_f32(ohm_backend *b, DevCsr &m);
+ohm_status csr_ensure_transpose(ohm_backend *b, DevCsr &m);
+ohm_status ensure_partials(ohm_backend *b, size_t count);
+
+// y = A x (f64), optionally skipping when state->status != RUNNING (skip=1)
+// or also when state->s_conv (skip=2).
+void launch_spmv(ohm_backend *b, const DevCsr &a, const double *x, double *y,
+                 const SolveState *state, int skip);
+void launch_spmv_f32(ohm_backend *b, const DevCsr &a, const float *x, float *y,
+                     const SolveState *state, int skip);
+// y = A^T x through the transpose pattern
+void launch_spmv_t(ohm_backend *b, const DevCsr &a, const double *x, double *y);
+
+// BLAS-1 on the stream (SolverBackend ops)
+void launch_axpy(ohm_backend *b, int n, double alpha, const double *x, double *y);
+void launch_scale(ohm_backend *b, int n, double alpha, double *x);
+void launch_copy(ohm_backend *b, int n, const double *x, double *y);
+void launch_fill(ohm_backend *b, int n, double v, double *x);
+// one-shot deterministic dot into d_scalars[slot]
+void launch_dot(ohm_backend *b, int n, const double *x, const double *y, int slot);
+
+// Jacobi inverse diagonal (with the Schur-complement rule for zero diagonals)
+void launch_jacobi_setup(ohm_backend *b, const DevCsr &a, double *inv, float *inv32);
+void launch_mul(ohm_backend *b, int n, const double *d, const double *x, double *y);
+
+// ---------------------------------------------------------- BiCGSTAB core --
+template <typename T>
+struct BicgVecs {
+  T *x, *r, *rh, *p, *v, *s, *t, *ph, *sh, *tmp;
+};
+
+struct PrecondDev;  // defined in ohm_precond.h
+
+enum { PK_NONE = 0, PK_JACOBI = 1, PK_ISAI = 2 };
+
+}  // namespace ohm

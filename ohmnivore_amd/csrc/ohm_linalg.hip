@@ -1,0 +1,307 @@
+// ohm_linalg.hip — CSR SpMV, BLAS-1 and Jacobi kernels for gfx950.
+//
+// SpMV (the SolverBackend::spmv of src/solver/backend.rs:56 and the
+// spmv_real/spmv_ds kernels of src/solver/gpu_shaders.rs:45,
+// src/solver/ds_shaders.rs:144) is CSR-stream: the host splits the rows
+// into blocks of at most 256 rows and 2048 nonzeros; a workgroup streams its
+// block's values and column indices with fully coalesced loads, multiplies
+// by the gathered x and stages the products in LDS, then each thread sums
+// one row from LDS in column order. A row longer than the block budget gets
+// a workgroup of its own that strides over it and reduces across lanes.
+// Every launch is a bandwidth-bound pass: bytes = nnz*(V+4) + 4(n+1) + 2nV.
+#include <algorithm>
+
+#include "ohm_spmv.h"
+
+namespace ohm {
+
+// --------------------------------------------------------------- SpMV ----
+void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blocks) {
+  blocks.clear();
+  blocks.push_back(0);
+  int r = 0;
+  while (r < n) {
+    uint32_t start = rowptr[r];
+    int rows = 0;
+    if (rowptr[r + 1] - start > (uint32_t)kSpmvNnzPerBlock) {
+      r++;  // long row: own block
+    } else {
+      while (r < n && rows < kSpmvRowsPerBlock &&
+             rowptr[r + 1] - start <= (uint32_t)kSpmvNnzPerBlock) {
+        r++;
+        rows++;
+      }
+    }
+    blocks.push_back((uint32_t)r);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t *__restrict__ rowptr,
+                                                 const uint32_t *__restrict__ col,
+                                                 const T *__restrict__ val,
+                                                 const uint32_t *__restrict__ blocks,
+                                                 const T *__restrict__ x, T *__restrict__ y,
+                                                 const SolveState *st, int skip) {
+  if (solve_stopped(st, skip)) return;
+  spmv_block<T>(rowptr, col, val, blocks, x, [&](uint32_t r, T v) { y[r] = v; });
+}
+
+void launch_spmv(ohm_backend *b, const DevCsr &a, const double *x, double *y,
+                 const SolveState *state, int skip) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_spmv<double>, dim3(a.n_blocks), dim3(kBlock), 0, b->stream, a.rowptr,
+                     a.col, a.val, a.blocks, x, y, state, skip);
+  b->dispatches++;
+}
+
+void launch_spmv_f32(ohm_backend *b, const DevCsr &a, const float *x, float *y,
+                     const SolveState *state, int skip) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_spmv<float>, dim3(a.n_blocks), dim3(kBlock), 0, b->stream, a.rowptr,
+                     a.col, a.val32, a.blocks, x, y, state, skip);
+  b->dispatches++;
+}
+
+// y = A^T x: SpMV over the transpose pattern, values gathered through t_src.
+__global__ __launch_bounds__(kBlock) void k_spmv_t(const uint32_t *__restrict__ rowptr,
+                                                   const uint32_t *__restrict__ col,
+                                                   const uint32_t *__restrict__ src,
+                                                   const double *__restrict__ val,
+                                                   const uint32_t *__restrict__ blocks,
+                                                   const double *__restrict__ x,
+                                                   double *__restrict__ y) {
+  __shared__ double prod[kSpmvNnzPerBlock];
+  const uint32_t r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
+  const uint32_t k0 = rowptr[r0], k1 = rowptr[r1];
+  const int tid = threadIdx.x;
+  if (k1 - k0 > (uint32_t)kSpmvNnzPerBlock) {
+    double acc = 0.0;
+    for (uint32_t k = k0 + tid; k < k1; k += kBlock) acc = fma(val[src[k]], x[col[k]], acc);
+    double v[1] = {acc};
+    block_sum<1>(v);
+    if (tid == 0) y[r0] = v[0];
+  } else {
+    const uint32_t cnt = k1 - k0;
+    for (uint32_t k = tid; k < cnt; k += kBlock) prod[k] = val[src[k0 + k]] * x[col[k0 + k]];
+    __syncthreads();
+    const uint32_t r = r0 + tid;
+    if (r < r1) {
+      const uint32_t a = rowptr[r] - k0, e = rowptr[r + 1] - k0;
+      double s = 0.0;
+      for (uint32_t k = a; k < e; k++) s += prod[k];
+      y[r] = s;
+    }
+  }
+}
+
+void launch_spmv_t(ohm_backend *b, const DevCsr &a, const double *x, double *y) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_spmv_t, dim3(a.t_n_blocks), dim3(kBlock), 0, b->stream, a.t_rowptr,
+                     a.t_col, a.t_src, a.val, a.t_blocks, x, y);
+  b->dispatches++;
+}
+
+// ------------------------------------------------------------- BLAS-1 ----
+__global__ void k_axpy(int n, double alpha, const double *__restrict__ x, double *__restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    y[i] = fma(alpha, x[i], y[i]);
+}
+__global__ void k_scale(int n, double alpha, double *__restrict__ x) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] *= alpha;
+}
+__global__ void k_copy(int n, const double *__restrict__ x, double *__restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = x[i];
+}
+__global__ void k_fill(int n, double v, double *__restrict__ x) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = v;
+}
+__global__ void k_mul(int n, const double *__restrict__ d, const double *__restrict__ x,
+                      double *__restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = d[i] * x[i];
+}
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const double *__restrict__ x,
+                                                const double *__restrict__ y, double *partials,
+                                                unsigned *counter, double *out) {
+  Acc<double> a;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    a.add_prod(x[i], y[i]);
+  double v[1] = {a.get()}, tot[1];
+  if (grid_sum<1>(v, partials, counter, tot) && threadIdx.x == 0) *out = tot[0];
+}
+
+#define GRID(n) dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream
+void launch_axpy(ohm_backend *b, int n, double alpha, const double *x, double *y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy, GRID(n), n, alpha, x, y);
+  b->dispatches++;
+}
+void launch_scale(ohm_backend *b, int n, double alpha, double *x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale, GRID(n), n, alpha, x);
+  b->dispatches++;
+}
+void launch_copy(ohm_backend *b, int n, const double *x, double *y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy, GRID(n), n, x, y);
+  b->dispatches++;
+}
+void launch_fill(ohm_backend *b, int n, double v, double *x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, GRID(n), n, v, x);
+  b->dispatches++;
+}
+void launch_mul(ohm_backend *b, int n, const double *d, const double *x, double *y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mul, GRID(n), n, d, x, y);
+  b->dispatches++;
+}
+void launch_dot(ohm_backend *b, int n, const double *x, const double *y, int slot) {
+  hipLaunchKernelGGL(k_dot, GRID(n), n, x, y, b->d_partials, b->d_counter, b->d_scalars + slot);
+  b->dispatches++;
+}
+#undef GRID
+
+// ------------------------------------------------------------- Jacobi ----
+// inv[r] = 1/a_rr when |a_rr| > 1e-30 (src/solver/gpu.rs:178-189). For a
+// zero-diagonal row (V-source / inductor branch) inv[r] = 1/S_rr with
+// S_rr = -sum_j a_rj a_jr / a_jj the diagonal of the Schur complement over
+// the rows that do have a pivot; 1 when that sum is zero too.
+__global__ void k_jacobi_setup(int n, const uint32_t *__restrict__ rowptr,
+                               const uint32_t *__restrict__ col, const double *__restrict__ val,
+                               const int32_t *__restrict__ diag_pos, double *inv, float *inv32) {
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < n; r += gridDim.x * kBlock) {
+    int dp = diag_pos[r];
+    double d = dp >= 0 ? val[dp] : 0.0;
+    double out;
+    if (fabs(d) > 1e-30) {
+      out = 1.0 / d;
+    } else {
+      double s = 0.0;
+      for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
+        uint32_t j = col[k];
+        if ((int)j == r) continue;
+        int dj = diag_pos[j];
+        double ajj = dj >= 0 ? val[dj] : 0.0;
+        if (fabs(ajj) <= 1e-30) continue;
+        double ajr = 0.0;  // a_jr: search row j
+        for (uint32_t q = rowptr[j]; q < rowptr[j + 1]; q++)
+          if ((int)col[q] == r) {
+            ajr = val[q];
+            break;
+          }
+        s -= val[k] * ajr / ajj;
+      }
+      out = (fabs(s) > 1e-300) ? 1.0 / s : 1.0;
+    }
+    inv[r] = out;
+    if (inv32) inv32[r] = (float)out;
+  }
+}
+
+void launch_jacobi_setup(ohm_backend *b, const DevCsr &a, double *inv, float *inv32) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_jacobi_setup, dim3(grid_for(a.n)), dim3(kBlock), 0, b->stream, a.n,
+                     a.rowptr, a.col, a.val, a.diag_pos, inv, inv32);
+  b->dispatches++;
+}
+
+// ---------------------------------------------------- matrix management --
+__global__ void k_d2f(int64_t n, const double *__restrict__ x, float *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    y[i] = (float)x[i];
+}
+
+ohm_status csr_ensure_f32(ohm_backend *b, DevCsr &m) {
+  if (!m.val32) OHM_HIP(hipMalloc(&m.val32, sizeof(float) * std::max<int64_t>(m.nnz, 1)));
+  if (m.nnz > 0) {
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(m.nnz, 4)), dim3(kBlock), 0, b->stream, m.nnz, m.val,
+                       m.val32);
+    b->dispatches++;
+  }
+  return OHM_OK;
+}
+
+ohm_status csr_upload(ohm_backend *b, DevCsr &m, const double *val, const uint32_t *col,
+                      const uint32_t *rowptr, int n, int64_t nnz) {
+  m.n = n;
+  m.nnz = nnz;
+  m.h_rowptr.assign(rowptr, rowptr + n + 1);
+  m.h_col.assign(col, col + nnz);
+  std::vector<uint32_t> blocks;
+  build_spmv_blocks(rowptr, n, blocks);
+  m.n_blocks = (int)blocks.size() - 1;
+  std::vector<int32_t> diag(n > 0 ? n : 1, -1);
+  for (int r = 0; r < n; r++)
+    for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; k++)
+      if ((int)col[k] == r) {
+        diag[r] = (int32_t)k;
+        break;
+      }
+  size_t nz = (size_t)std::max<int64_t>(nnz, 1);
+  OHM_HIP(hipMalloc(&m.rowptr, sizeof(uint32_t) * (n + 1)));
+  OHM_HIP(hipMalloc(&m.col, sizeof(uint32_t) * nz));
+  OHM_HIP(hipMalloc(&m.val, sizeof(double) * nz));
+  OHM_HIP(hipMalloc(&m.blocks, sizeof(uint32_t) * blocks.size()));
+  OHM_HIP(hipMalloc(&m.diag_pos, sizeof(int32_t) * diag.size()));
+  OHM_HIP(hipMemcpyAsync(m.rowptr, rowptr, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice,
+                         b->stream));
+  if (nnz > 0) {
+    OHM_HIP(hipMemcpyAsync(m.col, col, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, b->stream));
+    if (val)
+      OHM_HIP(hipMemcpyAsync(m.val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, b->stream));
+  }
+  OHM_HIP(hipMemcpyAsync(m.blocks, blocks.data(), sizeof(uint32_t) * blocks.size(),
+                         hipMemcpyHostToDevice, b->stream));
+  OHM_HIP(hipMemcpyAsync(m.diag_pos, diag.data(), sizeof(int32_t) * diag.size(),
+                         hipMemcpyHostToDevice, b->stream));
+  OHM_HIP(hipStreamSynchronize(b->stream));
+  return OHM_OK;
+}
+
+ohm_status csr_ensure_transpose(ohm_backend *b, DevCsr &m) {
+  if (m.t_rowptr) return OHM_OK;
+  int n = m.n;
+  std::vector<uint32_t> trp(n + 1, 0), tcol(std::max<int64_t>(m.nnz, 1)), tsrc(std::max<int64_t>(m.nnz, 1));
+  for (int64_t k = 0; k < m.nnz; k++) trp[m.h_col[k] + 1]++;
+  for (int i = 0; i < n; i++) trp[i + 1] += trp[i];
+  std::vector<uint32_t> pos(trp.begin(), trp.end() - 1);
+  for (int r = 0; r < n; r++)
+    for (uint32_t k = m.h_rowptr[r]; k < m.h_rowptr[r + 1]; k++) {
+      uint32_t c = m.h_col[k], p = pos[c]++;
+      tcol[p] = (uint32_t)r;
+      tsrc[p] = (uint32_t)k;
+    }
+  std::vector<uint32_t> blocks;
+  build_spmv_blocks(trp.data(), n, blocks);
+  m.t_n_blocks = (int)blocks.size() - 1;
+  OHM_HIP(hipMalloc(&m.t_rowptr, sizeof(uint32_t) * (n + 1)));
+  OHM_HIP(hipMalloc(&m.t_col, sizeof(uint32_t) * tcol.size()));
+  OHM_HIP(hipMalloc(&m.t_src, sizeof(uint32_t) * tsrc.size()));
+  OHM_HIP(hipMalloc(&m.t_blocks, sizeof(uint32_t) * blocks.size()));
+  OHM_HIP(hipMemcpy(m.t_rowptr, trp.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+  OHM_HIP(hipMemcpy(m.t_col, tcol.data(), sizeof(uint32_t) * tcol.size(), hipMemcpyHostToDevice));
+  OHM_HIP(hipMemcpy(m.t_src, tsrc.data(), sizeof(uint32_t) * tsrc.size(), hipMemcpyHostToDevice));
+  OHM_HIP(hipMemcpy(m.t_blocks, blocks.data(), sizeof(uint32_t) * blocks.size(),
+                    hipMemcpyHostToDevice));
+  return OHM_OK;
+}
+
+void csr_free(DevCsr &m) {
+  void *ptrs[] = {m.rowptr, m.col,    m.val,   m.val32, m.blocks,  m.diag_pos,
+                  m.t_rowptr, m.t_col, m.t_src, m.t_blocks};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  m = DevCsr();
+}
+
+ohm_status ensure_partials(ohm_backend *b, size_t count) {
+  if (count <= b->partials_cap) return OHM_OK;
+  if (b->d_partials) OHM_HIP(hipFree(b->d_partials));
+  OHM_HIP(hipMalloc(&b->d_partials, sizeof(double) * count));
+  b->partials_cap = count;
+  return OHM_OK;
+}
+
+}  // namespace ohm

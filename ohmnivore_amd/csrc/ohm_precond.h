@@ -1,0 +1,65 @@
+// ohm_precond.h — device-side preconditioner objects.
+#pragma once
+
+#include "ohm_internal.h"
+
+namespace ohm {
+
+struct IsaiPlan;  // symbolic ILU(0)+ISAI data, defined in ohm_isai.hip
+
+struct PrecondDev {
+  int kind = PK_NONE;
+  int level = 1;
+  int n = 0;
+  double *inv = nullptr;   // Jacobi inverse diagonal
+  float *inv32 = nullptr;
+  DevCsr ml, mu;           // ISAI: z = M_U (M_L r)
+  IsaiPlan *plan = nullptr;
+  int ilu_sweeps = 5;      // fixed-point ILU(0) sweeps per numeric refresh
+  bool has_f32 = false;
+};
+
+ohm_status precond_build(ohm_backend *b, const DevCsr &a, int kind, int level, PrecondDev &p);
+ohm_status precond_refresh(ohm_backend *b, const DevCsr &a, PrecondDev &p);
+ohm_status precond_ensure_f32(ohm_backend *b, PrecondDev &p);
+void precond_free(PrecondDev &p);
+// forget the ILU(0) factors so the next refresh restarts from A
+void isai_reset(PrecondDev &p);
+// out = M^-1 in (f64), skipping per the solve state like the SpMV kernels
+void precond_apply(ohm_backend *b, const PrecondDev &p, const double *in, double *out, double *tmp,
+                   const SolveState *st, int skip);
+void precond_apply_f32(ohm_backend *b, const PrecondDev &p, const float *in, float *out,
+                       float *tmp, const SolveState *st, int skip);
+bool matrix_has_zero_diag(ohm_backend *b, const DevCsr &a, const double *h_values);
+
+// ---- BiCGSTAB drivers (ohm_bicgstab.hip) --------------------------------
+struct BicgOpts {
+  int64_t max_iters = 10000;
+  double tol = 1e-5;
+  int shadow = OHM_SHADOW_HASHED;
+  bool warm = false;
+  int check_interval = 0;
+  bool transpose = false;  // solve A^T x = b (FP64 only)
+  bool scaled_breakdown = true;
+  int tol_mode = OHM_TOL_RHS;
+  double tol_floor = 1e-15;
+};
+struct BicgOut {
+  int status = ST_RUNNING;
+  int64_t iters = 0;
+  int64_t polls = 0;
+};
+
+// FP64 solve on device vectors (rhs, x of length a.n).
+ohm_status bicg_solve_f64(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const double *rhs,
+                          double *x, const BicgOpts &o, BicgOut &out);
+// FP32 inner BiCGSTAB inside FP64 iterative refinement.
+ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double *rhs, double *x,
+                         const BicgOpts &o, double inner_tol, int64_t max_ref, BicgOut &out,
+                         int64_t &refinements);
+// ||b - A x|| / ||b|| in FP64 (synchronous)
+ohm_status true_residual(ohm_backend *b, const DevCsr &a, const double *rhs, const double *x,
+                         double &rel);
+ohm_status status_to_error(int status, int64_t iters, int64_t max_iters);
+
+}  // namespace ohm

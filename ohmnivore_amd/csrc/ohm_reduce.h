@@ -1,0 +1,124 @@
+// ohm_reduce.h — wave64 / workgroup reductions and the deterministic
+// "last workgroup finishes the reduction" pattern used by every fused
+// kernel. Partial sums are combined in a fixed order, so results are
+// bitwise reproducible run to run.
+#pragma once
+
+#include "ohm_internal.h"
+
+namespace ohm {
+
+// Compensated accumulation: products via TwoProd (fma), sums via TwoSum
+// (Ogita, Rump & Oishi, "Accurate sum and dot product", SIAM J. Sci. Comput.
+// 26(6), 2005, Algorithm Dot2). FP64 accumulates directly.
+template <typename T>
+struct Acc;
+
+template <>
+struct Acc<double> {
+  double s = 0.0;
+  __device__ __forceinline__ void add_prod(double a, double b) { s = fma(a, b, s); }
+  __device__ __forceinline__ double get() const { return s; }
+};
+
+template <>
+struct Acc<float> {
+  float hi = 0.f, lo = 0.f;
+  __device__ __forceinline__ void add_prod(float a, float b) {
+    float p = a * b;
+    float pe = fmaf(a, b, -p);
+    float s = hi + p;
+    float bb = s - hi;
+    float se = (hi - (s - bb)) + (p - bb);
+    hi = s;
+    lo += pe + se;
+  }
+  __device__ __forceinline__ double get() const { return (double)hi + (double)lo; }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum K values over the workgroup; totals land in thread 0's v[].
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K]) {
+  __shared__ double sm[K][kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) sm[k][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < kWaves; i++) s += sm[k][i];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Each workgroup deposits its K partials; the last one to arrive (ticket ==
+// gridDim.x-1) sums all partials in workgroup order and returns true with
+// the grand totals in thread 0's tot[]. The ticket counter is reset for the
+// next kernel by that last workgroup.
+template <int K>
+__device__ __forceinline__ bool grid_sum(double (&v)[K], double *partials, unsigned *counter,
+                                         double (&tot)[K]) {
+  block_sum<K>(v);
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      __hip_atomic_store(&partials[(size_t)blockIdx.x * K + k], v[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      acc[k] += __hip_atomic_load(&partials[(size_t)i * K + k], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum<K>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] = acc[k];
+    *counter = 0u;
+  }
+  return true;
+}
+
+// Dense shadow residual value in [-1, 1): SplitMix64 finaliser of (i+1)*gamma.
+__host__ __device__ __forceinline__ double shadow_value(uint64_t i) {
+  uint64_t z = (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+__host__ __device__ __forceinline__ int grid_for(int64_t n, int per_thread = 1) {
+  int64_t g = (n + (int64_t)kBlock * per_thread - 1) / ((int64_t)kBlock * per_thread);
+  if (g < 1) g = 1;
+  if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
+  return (int)g;
+}
+
+}  // namespace ohm

@@ -160,7 +160,8 @@ class _Prec(ctypes.Structure):
                 ("mu_val", f64p)]
 
 
-def bicgstab(a: _ir.CsrMatrix, b, precond=None, max_iters=10_000, tol=1e-5, shadow="hashed"):
+def bicgstab(a: _ir.CsrMatrix, b, precond=None, max_iters=10_000, tol=1e-5, shadow="hashed",
+             breakdown="scaled"):
     """bicgstab_with_params (src/solver/bicgstab.rs:56-189). precond is None,
     ("jacobi", inv_diag) or ("isai", M_L, M_U). Returns (x, iterations)."""
     rp, cl, v = _csr_args(a)
@@ -185,7 +186,8 @@ def bicgstab(a: _ir.CsrMatrix, b, precond=None, max_iters=10_000, tol=1e-5, shad
     it = ctypes.c_int64(0)
     _check(lib().oo_bicgstab(ctypes.c_int64(a.nrows), _p(rp, u32p), _p(cl, u32p), _p(v, f64p),
                              _p(bb, f64p), _p(x, f64p), ctypes.byref(p), ctypes.c_int64(max_iters),
-                             ctypes.c_double(tol), int(shadow == "hashed"), ctypes.byref(it)))
+                             ctypes.c_double(tol), int(shadow == "hashed"),
+                             int(breakdown == "scaled"), ctypes.byref(it)))
     return x, it.value
 
 

@@ -977,7 +977,7 @@ double oo_shadow_value(int64_t i) {
 
 int oo_bicgstab(int64_t n, const uint32_t *rowptr, const uint32_t *col, const double *val,
                 const double *b, double *x, const oo_precond *p, int64_t max_iters, double tol,
-                int shadow, int64_t *iters) {
+                int shadow, int scaled, int64_t *iters) {
   *iters = 0;
   if (max_iters == 0) return fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB max_iterations must be > 0");
   if (!(tol > 0.0) || !isfinite(tol))
@@ -995,9 +995,10 @@ int oo_bicgstab(int64_t n, const uint32_t *rowptr, const uint32_t *col, const do
   int rc = OHM_OK;
   if (bnorm < 1e-30) goto done;
   double abs_tol = tol * bnorm, rho = 1.0, alpha = 1.0, omega = 1.0;
+  double hh = dot(n, rh, rh);
   for (int64_t it = 0; it < max_iters; it++) {
     double rho_new = dot(n, rh, r);
-    if (fabs(rho_new) < 1e-30) {
+    if (scaled ? fabs(rho_new) <= 1e-30 * sqrt(hh * dot(n, r, r)) : fabs(rho_new) < 1e-30) {
       rc = fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: rho ~ 0");
       goto done;
     }
@@ -1009,7 +1010,7 @@ int oo_bicgstab(int64_t n, const uint32_t *rowptr, const uint32_t *col, const do
     prec_apply(n, p, pp, ph, tmp);
     oo_spmv(n, rowptr, col, val, ph, v, 0);
     double rv = dot(n, rh, v);
-    if (fabs(rv) < 1e-30) {
+    if (scaled ? fabs(rv) <= 1e-30 * sqrt(hh * dot(n, v, v)) : fabs(rv) < 1e-30) {
       rc = fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: r_hat.v ~ 0");
       goto done;
     }
@@ -1025,7 +1026,7 @@ int oo_bicgstab(int64_t n, const uint32_t *rowptr, const uint32_t *col, const do
     prec_apply(n, p, s, sh, tmp);
     oo_spmv(n, rowptr, col, val, sh, t, 0);
     double ts = dot(n, t, s), tt = dot(n, t, t);
-    if (fabs(tt) < 1e-30) {
+    if (scaled ? tt <= 1e-30 * snorm * snorm : fabs(tt) < 1e-30) {
       rc = fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: ||t|| ~ 0");
       goto done;
     }

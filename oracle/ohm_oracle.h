@@ -61,11 +61,13 @@ typedef struct {
 
 /* bicgstab_with_params (src/solver/bicgstab.rs:56-189), right-preconditioned,
  * x0 = 0. shadow = 0 keeps the reference's r_hat = b (:88-90); shadow = 1 uses
- * the dense hashed vector oo_shadow_value. Returns OHM_OK and the iteration
+ * the dense hashed vector oo_shadow_value. scaled = 0 keeps the reference's
+ * absolute 1e-30 breakdown tests; 1 applies them to normalised quantities.
+ * Returns OHM_OK and the iteration
  * count in *iters. */
 int oo_bicgstab(int64_t n, const uint32_t *rowptr, const uint32_t *col, const double *val,
                 const double *b, double *x, const oo_precond *p, int64_t max_iters, double tol,
-                int shadow, int64_t *iters);
+                int shadow, int scaled, int64_t *iters);
 double oo_shadow_value(int64_t i);
 
 /* Jacobi inverse diagonal exactly as src/solver/gpu.rs:178-189 (0 for |d|<=1e-30). */

@@ -1,0 +1,126 @@
+"""CPU tests of the product library: it loads, exports every declared entry
+point, and its native MNA compiler reproduces the oracle's CSR bit for bit."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from ohmnivore_amd import _native as N
+from ohmnivore_amd import circuit as ir
+from ohmnivore_amd import solver
+
+
+def test_library_exports_every_declared_symbol():
+    L = N.lib()
+    missing = [s for s in N.declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    assert len(N.declared_symbols()) > 40
+
+
+def test_descriptor_sizes_match_reference():
+    # src/compiler.rs:1545,1575,1584
+    assert ir.DIODE_DESC_DTYPE.itemsize == 48
+    assert ir.BJT_DESC_DTYPE.itemsize == 96
+    assert ir.MOSFET_DESC_DTYPE.itemsize == 64
+
+
+def assert_same_mna(a: ir.MnaSystem, b: ir.MnaSystem):
+    assert a.size == b.size and a.n_nodes == b.n_nodes
+    assert a.node_names == b.node_names and a.branch_names == b.branch_names
+    for m1, m2 in ((a.g, b.g), (a.c, b.c)):
+        assert np.array_equal(m1.row_pointers, m2.row_pointers)
+        assert np.array_equal(m1.col_indices, m2.col_indices)
+        assert m1.values.tobytes() == m2.values.tobytes()  # bit for bit
+    assert a.b_dc.tobytes() == b.b_dc.tobytes()
+    assert a.b_ac.tobytes() == b.b_ac.tobytes()
+    for d1, d2 in ((a.diode_descriptors, b.diode_descriptors),
+                   (a.bjt_descriptors, b.bjt_descriptors),
+                   (a.mosfet_descriptors, b.mosfet_descriptors)):
+        assert d1.tobytes() == d2.tobytes()
+
+
+def _kitchen_sink():
+    c = ir.Circuit()
+    c.add_diode_model("DM", 1e-14, 1.2)
+    c.add_bjt_model("QN", is_=1e-15, bf=120.0, br=2.0)
+    c.add_bjt_model("QP", is_=2e-15, bf=80.0, npn=False)
+    c.add_mosfet_model("NM", vto=0.7, kp=1.1e-4, lam=0.04)
+    c.add_mosfet_model("PM", vto=-0.8, kp=5e-5, lam=0.02, nmos=False)
+    c.vsource("V1", "vdd", "0", dc=5.0, ac=(1.0, 30.0))
+    c.vsource("V2", "in", "gnd", dc=1.0)
+    c.resistor("R1", "vdd", "a", 1e3)
+    c.resistor("R1b", "vdd", "a", 2e3)  # parallel duplicate stamps
+    c.capacitor("C1", "a", "0", 1e-9)
+    c.capacitor("C2", "a", "b", 2e-9)
+    c.inductor("L1", "b", "c", 1e-6)
+    c.isource("I1", "c", "0", dc=1e-3, ac=(0.5, -45.0))
+    c.diode("D1", "a", "b", "DM")
+    c.diode("D2", "0", "c", "DM")
+    c.bjt("Q1", "c", "in", "e", "QN")
+    c.bjt("Q2", "e", "b", "vdd", "QP")
+    c.mosfet("M1", "out", "in", "0", "NM")
+    c.mosfet("M2", "out", "in", "vdd", "PM")
+    c.resistor("R2", "out", "GND", 10e3)
+    c.resistor("R3", "e", "0", 100.0)
+    return c
+
+
+@pytest.mark.parametrize("make", [
+    _kitchen_sink,
+    lambda: ir.resistor_ladder(500, seed=1),
+    lambda: ir.diode_mesh(20, 17, seed=2),
+    lambda: ir.rc_grid(16, 16, n_sinks=20, seed=3),
+    lambda: ir.rlc_sweep(3, 10, 60, seed=4)[0][1],
+    lambda: ir.amplifier_array(40, seed=5),
+])
+def test_compile_bit_parity_with_oracle(make):
+    ckt = make()
+    assert_same_mna(solver.compile(ckt), oracle.compile(ckt))
+
+
+@pytest.mark.slow
+def test_compile_bit_parity_full_size_configs():
+    """Config-size pattern parity: 10k ladder, 256^2 diode mesh, 4,096 stages."""
+    for ckt in (ir.resistor_ladder(), ir.diode_mesh(), ir.amplifier_array()):
+        assert_same_mna(solver.compile(ckt), oracle.compile(ckt))
+
+
+def test_config_sizes():
+    m = solver.compile(ir.diode_mesh())
+    assert m.size == 65_540 and m.g.nnz == 326_664 and len(m.diode_descriptors) == 8192
+    m = solver.compile(ir.resistor_ladder())
+    assert m.size == 10_002
+
+
+def test_compile_errors_match_reference_messages():
+    c = ir.Circuit()
+    c.resistor("R1", "1", "0", 0.0)
+    with pytest.raises(N.OhmnivoreError, match="Compile error: Resistor R1 has zero resistance"):
+        solver.compile(c)
+    c = ir.Circuit()
+    c.capacitor("C9", "1", "0", -1.0)
+    with pytest.raises(N.OhmnivoreError, match="negative capacitance"):
+        solver.compile(c)
+    for build in (lambda c: c.diode("D1", "1", "0", "NOPE"),
+                  lambda c: c.bjt("Q1", "1", "2", "0", "NOPE"),
+                  lambda c: c.mosfet("M1", "1", "2", "0", "NOPE")):
+        c = ir.Circuit()
+        build(c)
+        with pytest.raises(N.OhmnivoreError, match="references undefined model 'NOPE'") as e:
+            solver.compile(c)
+        assert e.value.kind == "Compile"
+
+
+def test_empty_circuit_compiles():
+    m = solver.compile(ir.Circuit())
+    assert m.size == 0 and m.g.nnz == 0 and list(m.g.row_pointers) == [0]
+
+
+def test_backend_without_gpu_reports_no_adapter():
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(N.OhmnivoreError, match="no GPU adapter") as e:
+        solver.HipBackend()
+    assert e.value.kind == "NoDevice"

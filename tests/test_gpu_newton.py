@@ -1,0 +1,111 @@
+"""GPU parity of the Newton path: device evaluation + stamping, limiting, and
+full Newton solves against the oracle's FP64 LU Newton (1e-6 rel / 1e-9 abs)."""
+import numpy as np
+import pytest
+
+import oracle
+from ohmnivore_amd import _native as N
+from ohmnivore_amd import circuit as ir
+from ohmnivore_amd import solver
+from tests.test_capi_cpu import _kitchen_sink
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def be():
+    return solver.HipBackend()
+
+
+def ctx_for(be, m):
+    return solver.NewtonContext(be, m.g, m.b_dc, m.diode_descriptors, m.bjt_descriptors,
+                                m.mosfet_descriptors)
+
+
+def test_assembly_matches_oracle(be):
+    m = solver.compile(_kitchen_sink())
+    ctx = ctx_for(be, m)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        x = rng.uniform(-1.0, 2.0, m.size)
+        v, b = ctx.assemble_at(x)
+        rv, rb = oracle.newton_assemble(m.g, m.b_dc, x, m.diode_descriptors, m.bjt_descriptors,
+                                        m.mosfet_descriptors)
+        assert np.allclose(v, rv, rtol=1e-13, atol=1e-300)
+        assert np.allclose(b, rb, rtol=1e-13, atol=1e-18)
+
+
+def test_limiting_matches_oracle(be):
+    m = solver.compile(_kitchen_sink())
+    ctx = ctx_for(be, m)
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        xo = rng.uniform(-0.5, 1.0, m.size)
+        xn = xo + rng.uniform(-3.0, 3.0, m.size)
+        got = ctx.limit(xo, xn)
+        ref = oracle.newton_limit(m.size, xo, xn, m.diode_descriptors, m.bjt_descriptors,
+                                  m.mosfet_descriptors)
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-15)
+
+
+def newton_ref(m, **kw):
+    x, it, md = oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, m.bjt_descriptors,
+                                    m.mosfet_descriptors, **kw)
+    return x, it
+
+
+def assert_close(x, ref, rel=1e-6, abs_=1e-9):
+    err = np.abs(x - ref)
+    bad = err > np.maximum(rel * np.abs(ref), abs_)
+    assert not bad.any(), (err.max(), np.nonzero(bad)[0][:5])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_newton_diode_mesh_matches_lu_newton(be, precision):
+    m = solver.compile(ir.diode_mesh(40, 36, seed=11))
+    ref, it_ref = newton_ref(m, abs_tol=1e-10, max_iterations=100)
+    ctx = ctx_for(be, m)
+    st = ctx.run(abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-8,
+                 precision=precision)
+    assert_close(ctx.x(), ref)
+    assert st.newton_iterations >= 2 and len(st.bicgstab_iters_per_newton) == st.newton_iterations
+
+
+def test_newton_amplifier_array_matches_lu_newton(be):
+    m = solver.compile(ir.amplifier_array(64, seed=5))
+    ref, _ = newton_ref(m, abs_tol=1e-10, max_iterations=200)
+    ctx = ctx_for(be, m)
+    ctx.run(abs_tol=1e-10, max_iterations=200, bicgstab_tolerance=1e-8)
+    assert_close(ctx.x(), ref)
+
+
+def test_newton_repeat_is_deterministic(be):
+    m = solver.compile(ir.diode_mesh(24, 24, seed=2))
+    ctx = ctx_for(be, m)
+    s1 = ctx.run(abs_tol=1e-9, bicgstab_tolerance=1e-8)
+    x1 = ctx.x()
+    s2 = ctx.run(abs_tol=1e-9, bicgstab_tolerance=1e-8)
+    assert ctx.x().tobytes() == x1.tobytes()
+    assert s1.bicgstab_iters_per_newton == s2.bicgstab_iters_per_newton
+
+
+def test_newton_not_converged_error(be):
+    m = solver.compile(ir.diode_mesh(12, 12, seed=1, diode_every=2))
+    ctx = ctx_for(be, m)
+    with pytest.raises(N.OhmnivoreError, match="did not converge after 2 iterations") as e:
+        ctx.run(max_iterations=2, abs_tol=1e-15)
+    assert e.value.kind == "NewtonNotConverged"
+
+
+def test_newton_full_size_diode_mesh_residual(be):
+    """256x256 mesh: size-independent check — the converged point satisfies
+    its own companion system A(x) x = b(x) (the Newton fixed point)."""
+    m = solver.compile(ir.diode_mesh())
+    ctx = ctx_for(be, m)
+    st = ctx.run(abs_tol=1e-9, bicgstab_tolerance=1e-8)
+    x = ctx.x()
+    v, b = ctx.assemble_at(x)
+    A = ir.CsrMatrix(m.size, m.size, v, m.g.col_indices, m.g.row_pointers)
+    r = A.spmv(x) - b
+    assert np.abs(r).max() <= 1e-9 * np.abs(b).max()
+    assert st.newton_iterations < 50

@@ -129,6 +129,18 @@ def test_bicgstab_reference_shadow_breaks_down_on_vsource_rhs():
         oracle.bicgstab(m.g, m.b_dc, ("jacobi", oracle.jacobi_inv_diag(m.g)), shadow="b")
 
 
+def test_absolute_breakdown_thresholds_misfire_at_tight_tolerance():
+    """The reference's absolute t.t < 1e-30 test (src/solver/bicgstab.rs:151)
+    stops a healthy solve near 1e-12 on the ladder; the scaled test reaches it
+    and meets 1e-6/1e-9 parity with LU."""
+    m = oracle.compile(ir.resistor_ladder(10_000, seed=1))
+    ref = oracle.sparse_lu_solve(m.g, m.b_dc)
+    with pytest.raises(oracle.OracleError, match=r"\|\|t\|\| ~ 0"):
+        oracle.bicgstab(m.g, m.b_dc, None, tol=1e-12, breakdown="absolute")
+    x, _ = oracle.bicgstab(m.g, m.b_dc, None, tol=1e-12, breakdown="scaled")
+    assert np.all(np.abs(x - ref) <= np.maximum(1e-6 * np.abs(ref), 1e-9))
+
+
 def test_bicgstab_parameter_errors_and_zero_rhs():
     a = trip(2, [(0, 0, 2.0), (1, 1, 3.0)])
     with pytest.raises(oracle.OracleError, match="max_iterations must be > 0"):
