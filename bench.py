@@ -54,7 +54,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile", action="store_true", help="skip CPU work (for rocprofv3 runs)")
+    ap.add_argument("--profile", action="store_true",
+                    help="rocprofv3 runs: skip CPU work and launch kernels without hipGraphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -69,6 +70,8 @@ def main() -> int:
     ckt = ir.diode_mesh(256, 256, seed=2)
     m = solver.compile(ckt)
     be = solver.HipBackend()
+    if args.profile:
+        be.set_graphs(False)
     ctx = solver.NewtonContext(be, m.g, m.b_dc, m.diode_descriptors)
     newton_kw = dict(abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-6,
                      bicgstab_max_iterations=20_000)

@@ -47,6 +47,9 @@ void ohm_mna_free(ohm_mna *m);
 ohm_status ohm_backend_create(int device, ohm_backend **out);
 void ohm_backend_destroy(ohm_backend *b);
 ohm_status ohm_backend_sync(ohm_backend *b);
+/* Replay solver iterations from captured hipGraphs (default on) or launch
+ * every kernel from the host. Results are bitwise identical either way. */
+ohm_status ohm_backend_set_graphs(ohm_backend *b, int enable);
 /* Dispatch / readback counters (src/solver/ds_backend.rs:182-190). */
 ohm_status ohm_backend_counters(const ohm_backend *b, int64_t *dispatches, int64_t *readbacks);
 
@@ -130,6 +133,8 @@ typedef struct {
   double device_ms;        /* device time of the solve (HIP events) */
   int64_t dispatches;      /* kernels enqueued */
   int64_t readbacks;       /* host status polls */
+  int64_t fp64_fallback;   /* FP32_IR: 1 when refinement stalled and the solve
+                              finished in FP64 (kappa * 2^-24 >~ 1) */
 } ohm_bicgstab_result;
 
 void ohm_bicgstab_default_params(ohm_bicgstab_params *p);

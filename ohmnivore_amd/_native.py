@@ -54,7 +54,8 @@ class BicgstabParams(ctypes.Structure):
 class BicgstabResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("refinements", ctypes.c_int64),
                 ("residual_norm", ctypes.c_double), ("device_ms", ctypes.c_double),
-                ("dispatches", ctypes.c_int64), ("readbacks", ctypes.c_int64)]
+                ("dispatches", ctypes.c_int64), ("readbacks", ctypes.c_int64),
+                ("fp64_fallback", ctypes.c_int64)]
 
 
 class NewtonParams(ctypes.Structure):

@@ -327,6 +327,61 @@ static void enqueue_iteration(ohm_backend *b, const DevCsr &a, const PrecondDev 
   b->dispatches += 5;
 }
 
+template <typename T>
+static void enqueue_any(ohm_backend *b, const DevCsr &a, const PrecondDev &p, BicgVecs<T> &v,
+                        SolveState *st, int n) {
+  if (p.kind == PK_JACOBI)
+    enqueue_iteration<T, PK_JACOBI>(b, a, p, v, st, n);
+  else if (p.kind == PK_ISAI)
+    enqueue_iteration<T, PK_ISAI>(b, a, p, v, st, n);
+  else
+    enqueue_iteration<T, PK_NONE>(b, a, p, v, st, n);
+}
+
+constexpr int kGraphIters = 8;
+
+void graph_cache_clear(ohm_backend *b) {
+  for (auto &g : b->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  b->graphs.clear();
+}
+
+// Replays `reps` launches of a captured kGraphIters-iteration chunk. The key
+// holds every pointer and launch size baked into the captured kernels.
+template <typename T>
+static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
+                               BicgVecs<T> &v, SolveState *st, int n, int reps) {
+  std::vector<uintptr_t> key = {
+      sizeof(T), (uintptr_t)p.kind, (uintptr_t)n, (uintptr_t)a.rowptr, (uintptr_t)a.col,
+      (uintptr_t)Ops<T>::vals(a), (uintptr_t)a.blocks, (uintptr_t)a.n_blocks,
+      (uintptr_t)Ops<T>::inv(p), (uintptr_t)p.ml.rowptr, (uintptr_t)Ops<T>::vals(p.ml),
+      (uintptr_t)p.ml.blocks, (uintptr_t)p.ml.n_blocks, (uintptr_t)p.mu.rowptr,
+      (uintptr_t)Ops<T>::vals(p.mu), (uintptr_t)p.mu.blocks, (uintptr_t)p.mu.n_blocks,
+      (uintptr_t)v.x, (uintptr_t)v.tmp, (uintptr_t)st, (uintptr_t)b->d_partials, kGraphIters};
+  hipGraphExec_t exec = nullptr;
+  for (auto &g : b->graphs)
+    if (g.key == key) exec = g.exec;
+  const int64_t per_iter = 5 + (p.kind == PK_ISAI ? 4 : 0);
+  if (!exec) {
+    int64_t saved = b->dispatches;
+    GraphEntry e;
+    e.key = key;
+    OHM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    for (int k = 0; k < kGraphIters; k++) enqueue_any<T>(b, a, p, v, st, n);
+    OHM_HIP(hipStreamEndCapture(b->stream, &e.graph));
+    OHM_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+    b->dispatches = saved;
+    if (b->graphs.size() >= 32) graph_cache_clear(b);
+    b->graphs.push_back(e);
+    exec = e.exec;
+  }
+  for (int r = 0; r < reps; r++) OHM_HIP(hipGraphLaunch(exec, b->stream));
+  b->dispatches += per_iter * kGraphIters * reps;
+  return OHM_OK;
+}
+
 struct StatusWord {
   int status, s_conv, iters, max_iters;
 };
@@ -361,13 +416,14 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
                     ? o.check_interval
                     : (int)std::min<int64_t>(32, std::max<int64_t>(4, done / 2));
     chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, o.max_iters - enqueued));
-    for (int k = 0; k < chunk; k++) {
-      if (p.kind == PK_JACOBI)
-        enqueue_iteration<T, PK_JACOBI>(b, a, p, v, st, n);
-      else if (p.kind == PK_ISAI)
-        enqueue_iteration<T, PK_ISAI>(b, a, p, v, st, n);
-      else
-        enqueue_iteration<T, PK_NONE>(b, a, p, v, st, n);
+    if (b->use_graphs) {
+      // replay a captured 8-iteration chunk; surplus iterations are no-ops
+      // once the state leaves RUNNING
+      int reps = (chunk + kGraphIters - 1) / kGraphIters;
+      OHM_TRY(replay_chunk<T>(b, a, p, v, st, n, reps));
+      chunk = reps * kGraphIters;
+    } else {
+      for (int k = 0; k < chunk; k++) enqueue_any<T>(b, a, p, v, st, n);
     }
     enqueued += chunk;
     OHM_HIP(hipGetLastError());
@@ -554,13 +610,23 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
     }
     b->dispatches += 1;
   }
-  out.iters = total;
-  out.status = ST_MAXITER;
+  // FP32 refinement stalled: the system is beyond FP32's reach (for MNA this
+  // is a node grounded only through GMIN or an off junction). Finish in FP64
+  // from the current iterate, to the same target.
   OHM_HIP(hipStreamSynchronize(b->stream));
-  return fail(OHM_ERR_SOLVE,
-              "Solve error: mixed-precision BiCGSTAB stalled at relative residual %.3e after %lld "
-              "refinements (last inner status %d)",
-              rel, (long long)refinements, last_status);
+  BicgOpts fo = o;
+  fo.warm = true;
+  fo.tol_mode = OHM_TOL_RHS;
+  fo.tol = target;
+  fo.max_iters = std::max<int64_t>(1, o.max_iters - total);
+  BicgOut f64;
+  ohm_status s = bicg_solve_f64(b, a, p, rhs, x, fo, f64);
+  out.iters = total + f64.iters;
+  out.polls += f64.polls;
+  out.status = f64.status;
+  out.fp64_fallback = true;
+  (void)last_status;
+  return s;
 }
 
 }  // namespace ohm

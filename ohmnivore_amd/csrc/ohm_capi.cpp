@@ -15,6 +15,13 @@
 namespace ohm {
 
 static thread_local std::string g_last_error;
+static std::vector<const ohm_backend *> g_live_backends;
+
+bool backend_alive(const ohm_backend *b) {
+  for (const ohm_backend *q : g_live_backends)
+    if (q == b) return true;
+  return false;
+}
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
@@ -78,12 +85,19 @@ ohm_status ohm_backend_create(int device, ohm_backend **out) {
   ohm_status s = ensure_partials(b, (size_t)kMaxReduceBlocks * 4);
   if (s != OHM_OK) return s;
   *out = b;
+  g_live_backends.push_back(b);
   return OHM_OK;
 }
 
 void ohm_backend_destroy(ohm_backend *b) {
   if (!b) return;
+  for (size_t i = 0; i < g_live_backends.size(); i++)
+    if (g_live_backends[i] == b) {
+      g_live_backends.erase(g_live_backends.begin() + i);
+      break;
+    }
   (void)hipStreamSynchronize(b->stream);
+  graph_cache_clear(b);
   if (b->d_partials) (void)hipFree(b->d_partials);
   if (b->d_counter) (void)hipFree(b->d_counter);
   if (b->d_scalars) (void)hipFree(b->d_scalars);
@@ -97,6 +111,13 @@ void ohm_backend_destroy(ohm_backend *b) {
 ohm_status ohm_backend_sync(ohm_backend *b) {
   if (!b) return fail(OHM_ERR_INVALID, "null backend");
   OHM_HIP(hipStreamSynchronize(b->stream));
+  return OHM_OK;
+}
+
+ohm_status ohm_backend_set_graphs(ohm_backend *b, int enable) {
+  if (!b) return fail(OHM_ERR_INVALID, "null backend");
+  b->use_graphs = enable != 0;
+  if (!b->use_graphs) graph_cache_clear(b);
   return OHM_OK;
 }
 
@@ -192,6 +213,7 @@ ohm_status ohm_upload_matrix(ohm_backend *b, const double *values, const uint32_
 
 void ohm_matrix_free(ohm_matrix *m) {
   if (!m) return;
+  if (backend_alive(m->owner)) graph_cache_clear(m->owner);
   csr_free(m->csr);
   delete m;
 }
@@ -375,6 +397,7 @@ ohm_status ohm_precond_set_ilu_sweeps(ohm_precond *p, int sweeps) {
 
 void ohm_precond_free(ohm_precond *p) {
   if (!p) return;
+  if (backend_alive(p->owner)) graph_cache_clear(p->owner);
   precond_free(p->dev);
   delete p;
 }
@@ -430,6 +453,7 @@ static ohm_status run_bicgstab(ohm_backend *b, DevCsr &a, PrecondDev &pd, const 
     (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
     result->iterations = out.iters;
     result->refinements = refinements;
+    result->fp64_fallback = out.fp64_fallback ? 1 : 0;
     result->device_ms = ms;
     result->dispatches = b->dispatches - d0;
     result->readbacks = out.polls;

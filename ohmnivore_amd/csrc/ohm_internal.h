@@ -87,7 +87,19 @@ struct DevCsr {
 
 }  // namespace ohm
 
+namespace ohm {
+// A captured chunk of solver iterations, replayable while every pointer and
+// launch size in `key` is unchanged.
+struct GraphEntry {
+  std::vector<uintptr_t> key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+}  // namespace ohm
+
 struct ohm_backend {
+  std::vector<ohm::GraphEntry> graphs;
+  bool use_graphs = true;
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cu = 0;
@@ -123,6 +135,10 @@ void csr_free(DevCsr &m);
 ohm_status csr_ensure_f32(ohm_backend *b, DevCsr &m);
 ohm_status csr_ensure_transpose(ohm_backend *b, DevCsr &m);
 ohm_status ensure_partials(ohm_backend *b, size_t count);
+// Drop every cached graph (called whenever device objects are freed).
+void graph_cache_clear(ohm_backend *b);
+// Backends not yet destroyed; objects whose owner is gone must not touch it.
+bool backend_alive(const ohm_backend *b);
 
 // y = A x (f64), optionally skipping when state->status != RUNNING (skip=1)
 // or also when state->s_conv (skip=2).

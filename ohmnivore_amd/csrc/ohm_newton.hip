@@ -311,30 +311,36 @@ __global__ __launch_bounds__(kBlock) void k_converge(int n, const double *__rest
     v[0] = mm;
     v[1] = sbad ? 1.0 : 0.0;
   }
-  // grid-wide: maxima of non-negative values combine by max, store as partials
+  // grid-wide: the same ticket protocol as grid_sum, combining by max
   __shared__ int s_last;
   if (threadIdx.x == 0) {
     __hip_atomic_store(&partials[2 * blockIdx.x], v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&partials[2 * blockIdx.x + 1], v[1], __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    s_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+    __builtin_amdgcn_s_waitcnt(0);
+    s_last = (__hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+              gridDim.x - 1);
   }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
+  double mm = 0.0, bb = 0.0;
+  for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock) {
+    mm = fmax(mm, __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    bb += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  mm = wave_max(mm);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mm;
+  double bsum[1] = {bb};
+  block_sum<1>(bsum);  // contains a barrier, so sm[] is complete below
   if (threadIdx.x == 0) {
-    double mm = 0.0, bb = 0.0;
-    for (unsigned i = 0; i < gridDim.x; i++) {
-      mm = fmax(mm, __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      bb += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    for (int i = 0; i < kWaves; i++) mm = fmax(mm, sm[i]);
     tot[0] = mm;
-    tot[1] = bb;
+    tot[1] = bsum[0];
     w->max_diff = tot[0];
     w->nan = (tot[1] > 0.0 || (asm_nan && *asm_nan)) ? 1 : 0;
     w->done = 1;
-    *counter = 0u;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -483,6 +489,7 @@ void ohm_newton_default_params(ohm_newton_params *p) {
 
 void ohm_newton_free(ohm_newton_ctx *c) {
   if (!c) return;
+  if (backend_alive(c->b)) graph_cache_clear(c->b);
   csr_free(c->A);
   precond_free(c->P);
   void *ptrs[] = {c->base_val, c->base_b, c->rhs, c->x,  c->x_old, c->S,  c->dd, c->qd, c->md,
@@ -676,17 +683,10 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
     ohm_status ls;
     if (prm.precision == OHM_PRECISION_FP32_IR) {
       int64_t refs = 0;
+      // FP32 refinement hands over to FP64 when kappa * 2^-24 >~ 1 (diodes
+      // still off, mesh grounded only through GMIN)
       ls = bicg_solve_ir(b, c->A, c->P, c->rhs, c->x, o, 1e-4, 30, bo, refs);
-      if (ls != OHM_OK && bo.status == ST_MAXITER) {
-        // FP32 refinement stalled (kappa * 2^-24 >~ 1, e.g. diodes still off
-        // and the mesh grounded only through GMIN): redo this step in FP64
-        // from x_k.
-        total_bicg += bo.iters;
-        fallbacks++;
-        OHM_HIP(hipMemcpyAsync(c->x, c->x_old, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        bo = BicgOut();
-        ls = bicg_solve_f64(b, c->A, c->P, c->rhs, c->x, o, bo);
-      }
+      if (bo.fp64_fallback) fallbacks++;
     } else {
       ls = bicg_solve_f64(b, c->A, c->P, c->rhs, c->x, o, bo);
     }

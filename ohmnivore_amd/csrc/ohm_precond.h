@@ -45,6 +45,7 @@ struct BicgOpts {
   double tol_floor = 1e-15;
 };
 struct BicgOut {
+  bool fp64_fallback = false;  // FP32_IR finished in FP64
   int status = ST_RUNNING;
   int64_t iters = 0;
   int64_t polls = 0;

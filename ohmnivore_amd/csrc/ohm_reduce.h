@@ -80,13 +80,15 @@ __device__ __forceinline__ bool grid_sum(double (&v)[K], double *partials, unsig
     for (int k = 0; k < K; k++)
       __hip_atomic_store(&partials[(size_t)blockIdx.x * K + k], v[k], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
+    // The partials are agent-coherent atomic stores; waiting for them to
+    // complete orders them before the ticket without an agent-scope fence
+    // (which would write back the whole L2 of this XCD).
+    __builtin_amdgcn_s_waitcnt(0);
+    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (!s_last) return false;
-  __threadfence();
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; k++) acc[k] = 0.0;
@@ -100,7 +102,7 @@ __device__ __forceinline__ bool grid_sum(double (&v)[K], double *partials, unsig
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) tot[k] = acc[k];
-    *counter = 0u;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return true;
 }
@@ -114,8 +116,13 @@ __host__ __device__ __forceinline__ double shadow_value(uint64_t i) {
   return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
 }
 
+// Workgroups for an n-element vector kernel: one element per thread (so a
+// 65k-entry vector still spreads over every CU) up to kMaxReduceBlocks
+// workgroups, grid-striding beyond that. `per_thread` is accepted for call
+// sites that state their unroll but does not shrink small grids.
 __host__ __device__ __forceinline__ int grid_for(int64_t n, int per_thread = 1) {
-  int64_t g = (n + (int64_t)kBlock * per_thread - 1) / ((int64_t)kBlock * per_thread);
+  (void)per_thread;
+  int64_t g = (n + (int64_t)kBlock - 1) / (int64_t)kBlock;
   if (g < 1) g = 1;
   if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
   return (int)g;

@@ -10,6 +10,7 @@ Every call runs on the MI355X through libohm_hip.so; there is no CPU path.
 from __future__ import annotations
 
 import ctypes
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -42,7 +43,7 @@ class Buffer:
         check(lib().ohm_buffer_new(backend.h, ctypes.c_size_t(n), ctypes.byref(self.h)))
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value:
+        if getattr(self, "h", None) and self.h.value and not sys.is_finalizing():
             lib().ohm_buffer_free(self.h)
             self.h = None
 
@@ -67,7 +68,7 @@ class Matrix:
         check(lib().ohm_matrix_set_values(self.backend.h, self.h, ptr(v, N.c_f64p)))
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value:
+        if getattr(self, "h", None) and self.h.value and not sys.is_finalizing():
             lib().ohm_matrix_free(self.h)
             self.h = None
 
@@ -92,7 +93,7 @@ class Preconditioner:
         check(lib().ohm_precond_apply(self.backend.h, self.h, inp.h, out.h))
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value:
+        if getattr(self, "h", None) and self.h.value and not sys.is_finalizing():
             lib().ohm_precond_free(self.h)
             self.h = None
 
@@ -126,11 +127,12 @@ class SolveStats:
     device_ms: float = 0.0
     dispatches: int = 0
     readbacks: int = 0
+    fp64_fallback: bool = False
 
     @classmethod
     def from_c(cls, r: N.BicgstabResult) -> "SolveStats":
         return cls(r.iterations, r.refinements, r.residual_norm, r.device_ms, r.dispatches,
-                   r.readbacks)
+                   r.readbacks, bool(r.fp64_fallback))
 
 
 class HipBackend:
@@ -141,7 +143,7 @@ class HipBackend:
         check(lib().ohm_backend_create(int(device), ctypes.byref(self.h)))
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value:
+        if getattr(self, "h", None) and self.h.value and not sys.is_finalizing():
             lib().ohm_backend_destroy(self.h)
             self.h = None
 
@@ -200,6 +202,10 @@ class HipBackend:
 
     def sync(self):
         check(lib().ohm_backend_sync(self.h))
+
+    def set_graphs(self, enable: bool):
+        """Replay iteration chunks from captured hipGraphs (default) or not."""
+        check(lib().ohm_backend_set_graphs(self.h, int(bool(enable))))
 
     def counters(self) -> tuple[int, int]:
         d, r = ctypes.c_int64(), ctypes.c_int64()
@@ -336,7 +342,7 @@ class NewtonContext:
         return xn
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value:
+        if getattr(self, "h", None) and self.h.value and not sys.is_finalizing():
             lib().ohm_newton_free(self.h)
             self.h = None
 

@@ -191,6 +191,28 @@ def test_zero_rhs_empty_and_dimension_errors(be):
         gs.solve_real(m.g, np.zeros(m.size + 1))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_graph_replay_matches_eager(be, precision):
+    """Graph-captured iteration chunks, replayed many times with eager solves
+    and host syncs in between, give bitwise the eager result every time."""
+    m = solver.compile(ir.diode_mesh(40, 40, seed=9))
+    a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
+    p = be.preconditioner(a, N.PREC_ISAI, 1)
+    b, x = be.new_buffer(m.size), be.new_buffer(m.size)
+    be.upload_vec(m.b_dc, b)
+    prm = solver.bicgstab_params(tolerance=1e-11, precision=precision)
+    results = []
+    for mode in (False, True, False, True, True):
+        be.set_graphs(mode)
+        be.upload_vec(np.zeros(m.size), x)
+        s = be.bicgstab(a, b, x, p, prm)
+        be.sync()
+        results.append((be.download_vec(x).tobytes(), s.iterations))
+    be.set_graphs(True)
+    assert all(r == results[0] for r in results)
+    assert results[0][1] > 8  # several replays per solve
+
+
 def test_warm_start_and_determinism(be):
     m = solver.compile(ir.diode_mesh(32, 32, seed=3))
     a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
