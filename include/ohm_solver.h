@@ -154,6 +154,125 @@ ohm_status ohm_solve_real(ohm_backend *b, size_t n, size_t nnz, const double *va
                           const double *rhs, double *x, int precond,
                           const ohm_bicgstab_params *params, ohm_bicgstab_result *result);
 
+/* ---- LinearSolver::solve_complex (src/solver/mod.rs:33; GPU version
+ * src/solver/gpu.rs:210-735): complex Jacobi-BiCGSTAB with conjugated dot
+ * products. `values`, `rhs` and `x` hold interleaved (re, im) pairs
+ * (Complex64 layout). precision OHM_PRECISION_FP32_IR runs complex64 inner
+ * solves inside complex128 refinement. Unlike the reference, which returns
+ * its iterate as Ok after 10 000 iterations without converging
+ * (gpu.rs:729-734), a solve that does not reach the tolerance returns an
+ * error (x still holds the last iterate). */
+ohm_status ohm_solve_complex(ohm_backend *b, size_t n, size_t nnz, const double *values,
+                             const uint32_t *col_indices, const uint32_t *row_pointers,
+                             const double *rhs, double *x, int precond,
+                             const ohm_bicgstab_params *params, ohm_bicgstab_result *result);
+
+/* ---- Batched systems (no counterpart in the reference, which solves one
+ * system per call): K independent systems (G_k + sigma_k C_k) x_k = b_k,
+ * real or complex, solved together. Vectors of all systems are concatenated
+ * in system order (complex as (re, im) pairs). Systems may share patterns
+ * and value arrays: identical host pointers are uploaded once. Each system
+ * converges, breaks down or stops on its own (per-system status). */
+typedef struct ohm_batch ohm_batch;
+typedef struct {
+  int64_t n;                     /* rows (= columns) */
+  const uint32_t *rowptr, *col;  /* CSR pattern: n+1 / rowptr[n] entries, columns < n */
+  const double *g_val;           /* values on the pattern */
+  const double *c_val;           /* second value array on the same pattern, or NULL */
+  double sigma_re, sigma_im;     /* A = G + sigma C (complex batches use both parts) */
+} ohm_batch_system;
+typedef struct {
+  int64_t iterations;            /* BiCGSTAB iterations (inner ones under FP32_IR) */
+  int64_t refinements;           /* FP32_IR refinement steps */
+  double residual_norm;          /* ||b - A x|| / ||b|| in FP64 */
+  int32_t status;                /* OHM_OK or the error this system's solve ended with */
+  int32_t reason;                /* 1 converged, 2-6 breakdown kinds / NaN, 7 max iterations */
+} ohm_batch_result;
+ohm_status ohm_batch_create(ohm_backend *b, int32_t is_complex, int64_t k,
+                            const ohm_batch_system *systems, ohm_batch **out);
+void ohm_batch_free(ohm_batch *bt);
+int64_t ohm_batch_rows(const ohm_batch *bt); /* total rows over all systems */
+/* Replace every sigma_k (2k doubles: re, im) without re-uploading values. */
+ohm_status ohm_batch_set_sigma(ohm_batch *bt, const double *sigma);
+/* y = A x for every system (host vectors, ohm_batch_rows entries). */
+ohm_status ohm_batch_spmv(ohm_batch *bt, const double *x, double *y);
+/* Solve all systems. x is the warm start when params->warm_start, and
+ * receives every system's final iterate. results: k entries. Returns OHM_OK
+ * when every system converged, else the first failing system's error. */
+ohm_status ohm_batch_solve(ohm_batch *bt, const double *rhs, double *x,
+                           const ohm_bicgstab_params *params, int32_t precond,
+                           ohm_batch_result *results);
+
+/* ---- AC sweep (src/analysis/ac.rs:25-77): for every frequency f_i solve
+ * (G + j 2 pi f_i C) x_i = b_ac, with A formed on the fly from G and C
+ * (form_complex_matrix, src/sparse.rs:149-178) and the frequency points
+ * batched `batch` at a time (0 = all). C's pattern may differ from G's (the
+ * union is used). b_ac: 2n doubles; x: n_freqs * 2n; results: n_freqs. */
+ohm_status ohm_ac_sweep(ohm_backend *b, int64_t n, const uint32_t *g_rowptr,
+                        const uint32_t *g_col, const double *g_val, const uint32_t *c_rowptr,
+                        const uint32_t *c_col, const double *c_val, const double *b_ac,
+                        int64_t n_freqs, const double *freqs, int64_t batch,
+                        const ohm_bicgstab_params *params, int32_t precond, double *x,
+                        ohm_batch_result *results);
+
+/* generate_frequencies (src/analysis/ac.rs:80-109). Writes up to `cap`
+ * points and returns how many the sweep has. */
+enum { OHM_SWEEP_DEC = 0, OHM_SWEEP_OCT = 1, OHM_SWEEP_LIN = 2 };
+int64_t ohm_generate_frequencies(int32_t sweep, int64_t n_points, double f_start, double f_stop,
+                                 double *out, int64_t cap);
+
+/* ---- Transient (src/analysis/transient.rs:25-199). One or more instances
+ * of a linear circuit (Monte Carlo: same pattern and sources, own G and C
+ * values) stepped together on the GPU: A = G + (alpha/h) C formed on the
+ * fly, companion right-hand side and source values evaluated by one kernel
+ * per step, warm-started solve, probe values gathered on the device.
+ * Instances are ohm_batch_system entries whose c_val holds C on G's
+ * pattern (the compiler zero-fills C positions into G, src/compiler.rs:577-579);
+ * sigma is ignored. */
+enum { OHM_SRC_DC = 0, OHM_SRC_PULSE = 1, OHM_SRC_SIN = 2, OHM_SRC_EXP = 3, OHM_SRC_PWL = 4 };
+/* SourceEntry (transient.rs:200-262): b[idx[j]] += sign[j] * value(t).
+ * param: PULSE v1 v2 td tr tf pw per | SIN vo va freq td theta |
+ * EXP v1 v2 td1 tau1 td2 tau2 (transient_source.rs:39-116); PWL uses
+ * pwl_count (t, v) pairs from the shared pair table at pwl_offset. */
+typedef struct {
+  int32_t kind;
+  int32_t n_idx;
+  uint32_t idx[2];
+  double sign[2];
+  double dc;
+  double param[7];
+  int64_t pwl_offset, pwl_count;
+} ohm_tran_source;
+enum { OHM_TRAN_FIXED_BE = 0, OHM_TRAN_ADAPTIVE = 1 };
+typedef struct {
+  double tstep, tstop, tstart;
+  int32_t method;    /* FIXED_BE: backward Euler at h = tstep; ADAPTIVE: the
+                        reference's TRAP/BE stepping with LTE control (one instance) */
+  int32_t precond;   /* OHM_PREC_JACOBI or OHM_PREC_NONE */
+  ohm_bicgstab_params solver;  /* per-step solves (always warm-started) */
+} ohm_tran_params;
+typedef struct {
+  int64_t steps_accepted, steps_rejected;
+  int64_t linear_solves, bicgstab_iterations;
+  int64_t records;
+  double t_end;
+  double device_ms;
+} ohm_tran_stats;
+typedef struct ohm_transient ohm_transient;
+void ohm_tran_default_params(ohm_tran_params *p);
+ohm_status ohm_transient_create(ohm_backend *b, int64_t k, const ohm_batch_system *instances,
+                                int64_t n_sources, const ohm_tran_source *sources,
+                                int64_t n_pwl, const double *pwl_pairs, ohm_transient **out);
+void ohm_transient_free(ohm_transient *tr);
+/* x0: k*n initial state (NULL = zero, i.e. UIC with zero initial
+ * conditions). Records t = 0 (when tstart <= 0) and every accepted step
+ * with t >= tstart, up to max_records: times[r], waves[r][k][probe].
+ * x_final: k*n. */
+ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran_params *params,
+                             int64_t n_probes, const uint32_t *probes, int64_t max_records,
+                             double *times, double *waves, double *x_final,
+                             ohm_tran_stats *stats);
+
 /* ---- Newton-Raphson: NewtonParams (src/solver/newton.rs:34-55) and
  * newton_solve (:65-513). Device evaluation, stamping, linear solve,
  * limiting and the convergence test all run on the GPU; the host polls one

@@ -71,6 +71,20 @@ class NewtonResult(ctypes.Structure):
                 ("fp64_fallbacks", ctypes.c_int64)]
 
 
+class BatchSystem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("rowptr", c_u32p), ("col", c_u32p),
+                ("g_val", c_f64p), ("c_val", c_f64p), ("sigma_re", ctypes.c_double),
+                ("sigma_im", ctypes.c_double)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("refinements", ctypes.c_int64),
+                ("residual_norm", ctypes.c_double), ("status", ctypes.c_int32),
+                ("reason", ctypes.c_int32)]
+
+
+SWEEP_DEC, SWEEP_OCT, SWEEP_LIN = 0, 1, 2
+
 _lib = None
 
 
@@ -86,9 +100,14 @@ def lib() -> ctypes.CDLL:
         L.ohm_version.restype = ctypes.c_char_p
         L.ohm_buffer_len.restype = ctypes.c_size_t
         L.ohm_precond_kind.restype = ctypes.c_int
+        L.ohm_batch_rows.restype = ctypes.c_int64
+        L.ohm_generate_frequencies.restype = ctypes.c_int64
+        L.ohm_generate_frequencies.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_double,
+                                               ctypes.c_double, c_f64p, ctypes.c_int64]
         for name in ("ohm_backend_destroy", "ohm_buffer_free", "ohm_matrix_free",
                      "ohm_precond_free", "ohm_newton_free", "ohm_mna_free",
-                     "ohm_bicgstab_default_params", "ohm_newton_default_params"):
+                     "ohm_bicgstab_default_params", "ohm_newton_default_params",
+                     "ohm_batch_free", "ohm_transient_free", "ohm_tran_default_params"):
             getattr(L, name).restype = None
         _lib = L
     return _lib

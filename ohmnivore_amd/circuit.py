@@ -65,6 +65,13 @@ assert MOSFET_DESC_DTYPE.itemsize == 64
 
 SENTINEL = 0xFFFFFFFF
 
+# ohm_tran_source (include/ohm_solver.h): SourceEntry, src/analysis/transient.rs:200-208
+TRAN_SOURCE_DTYPE = np.dtype(
+    [("kind", "<i4"), ("n_idx", "<i4"), ("idx", "<u4", (2,)), ("sign", "<f8", (2,)),
+     ("dc", "<f8"), ("param", "<f8", (7,)), ("pwl_offset", "<i8"), ("pwl_count", "<i8")])
+assert TRAN_SOURCE_DTYPE.itemsize == 112
+SRC_DC, SRC_PULSE, SRC_SIN, SRC_EXP, SRC_PWL = range(5)
+
 
 @dataclass
 class Pulse:

@@ -4,6 +4,9 @@
 - ``HipBackend``     SolverBackend                (src/solver/backend.rs:51-96)
 - ``GpuSolver``      LinearSolver                 (src/solver/mod.rs:28-34)
 - ``newton_solve``   newton_solve / NewtonParams  (src/solver/newton.rs:34-78)
+- ``Batch``          batched (G_k + sigma_k C_k) x_k = b_k solves (no reference
+                     counterpart: AC points, Monte Carlo, uneven circuits)
+- ``ac_sweep``       ac::run / generate_frequencies (src/analysis/ac.rs:25-109)
 
 Every call runs on the MI355X through libohm_hip.so; there is no CPU path.
 """
@@ -243,6 +246,196 @@ class GpuSolver:
                                    ctypes.byref(res)))
         self.last_stats = SolveStats.from_c(res)
         return x
+
+    def solve_complex(self, a: ir.CsrMatrix, b) -> np.ndarray:
+        """LinearSolver::solve_complex (src/solver/mod.rs:33; gpu.rs:210-735)."""
+        n = a.nrows
+        bb = np.ascontiguousarray(b, dtype=np.complex128)
+        if a.ncols != n or bb.shape[0] != n:
+            raise OhmnivoreError(3, f"Solve error: dimension mismatch: matrix is {a.nrows}x"
+                                    f"{a.ncols}, rhs length is {bb.shape[0]}")
+        x = np.zeros(n, dtype=np.complex128)
+        if n == 0:
+            return x
+        v = np.ascontiguousarray(a.values, dtype=np.complex128)
+        c = np.ascontiguousarray(a.col_indices, dtype=np.uint32)
+        r = np.ascontiguousarray(a.row_pointers, dtype=np.uint32)
+        res = N.BicgstabResult()
+        prec = N.PREC_JACOBI if self.precond == N.PREC_AUTO else self.precond
+        check(lib().ohm_solve_complex(
+            self.backend.h, ctypes.c_size_t(n), ctypes.c_size_t(v.shape[0]),
+            v.ctypes.data_as(N.c_f64p), ptr(c, N.c_u32p), ptr(r, N.c_u32p),
+            bb.ctypes.data_as(N.c_f64p), x.ctypes.data_as(N.c_f64p), int(prec),
+            ctypes.byref(self.params) if self.params is not None else None, ctypes.byref(res)))
+        self.last_stats = SolveStats.from_c(res)
+        return x
+
+
+# ---------------------------------------------------------------- batches --
+@dataclass
+class BatchStats:
+    iterations: int
+    refinements: int
+    residual_norm: float
+    status: int
+    reason: int
+
+    @property
+    def converged(self) -> bool:
+        return self.status == 0
+
+
+class Batch:
+    """K independent systems (G_k + sigma_k C_k) x_k = b_k solved together.
+
+    ``systems`` is a list of ``(rowptr, col, g, c_or_None, sigma)``; arrays
+    that are the same numpy object are uploaded once. Vectors are the
+    systems' vectors concatenated (complex dtype for complex batches)."""
+
+    def __init__(self, backend: HipBackend, systems, complex_: bool):
+        self.backend = backend
+        self.complex = bool(complex_)
+        self._keep = []
+        cache = {}
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            key = id(a)
+            if key not in cache:
+                cache[key] = np.ascontiguousarray(a, dtype=dt)
+                self._keep.append((a, cache[key]))
+            return cache[key]
+
+        k = len(systems)
+        cs = (N.BatchSystem * k)()
+        self.sizes = []
+        for i, (rp, cl, g, c, sig) in enumerate(systems):
+            rp_, cl_, g_, c_ = (arr(rp, np.uint32), arr(cl, np.uint32), arr(g, np.float64),
+                                arr(c, np.float64))
+            n = int(rp_.shape[0] - 1)
+            sig = complex(sig)
+            cs[i] = N.BatchSystem(n, rp_.ctypes.data_as(N.c_u32p), cl_.ctypes.data_as(N.c_u32p),
+                                  g_.ctypes.data_as(N.c_f64p),
+                                  c_.ctypes.data_as(N.c_f64p) if c_ is not None else None,
+                                  sig.real, sig.imag)
+            self.sizes.append(n)
+        self.k = k
+        self.h = ctypes.c_void_p()
+        check(lib().ohm_batch_create(backend.h, int(self.complex), ctypes.c_int64(k), cs,
+                                     ctypes.byref(self.h)))
+        self.rows = int(lib().ohm_batch_rows(self.h))
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.last_results: list[BatchStats] = []
+
+    @property
+    def dtype(self):
+        return np.complex128 if self.complex else np.float64
+
+    def set_sigma(self, sigma):
+        s = np.asarray(sigma, dtype=np.complex128)
+        buf = np.ascontiguousarray(np.stack([s.real, s.imag], axis=1).ravel())
+        check(lib().ohm_batch_set_sigma(self.h, buf.ctypes.data_as(N.c_f64p)))
+
+    def spmv(self, x) -> np.ndarray:
+        xx = np.ascontiguousarray(x, dtype=self.dtype)
+        y = np.zeros(self.rows, dtype=self.dtype)
+        check(lib().ohm_batch_spmv(self.h, xx.ctypes.data_as(N.c_f64p), y.ctypes.data_as(N.c_f64p)))
+        return y
+
+    def solve(self, rhs, x0=None, params: N.BicgstabParams | None = None,
+              precond: int = N.PREC_JACOBI, raise_on_failure: bool = True) -> np.ndarray:
+        b = np.ascontiguousarray(rhs, dtype=self.dtype)
+        if b.shape[0] != self.rows:
+            raise OhmnivoreError(3, f"Solve error: dimension mismatch: batch has {self.rows} "
+                                    f"rows, rhs length is {b.shape[0]}")
+        prm = params if params is not None else bicgstab_params()
+        x = (np.ascontiguousarray(x0, dtype=self.dtype).copy() if x0 is not None
+             else np.zeros(self.rows, dtype=self.dtype))
+        if x0 is not None:
+            prm = _with(prm, warm_start=1)
+        res = (N.BatchResult * self.k)()
+        rc = lib().ohm_batch_solve(self.h, b.ctypes.data_as(N.c_f64p), x.ctypes.data_as(N.c_f64p),
+                                   ctypes.byref(prm), int(precond), res)
+        self.last_results = [BatchStats(r.iterations, r.refinements, r.residual_norm, r.status,
+                                        r.reason) for r in res]
+        if rc != 0 and raise_on_failure:
+            check(rc)
+        return x
+
+    def split(self, v) -> list:
+        return [v[self.offsets[i]:self.offsets[i + 1]] for i in range(self.k)]
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and not sys.is_finalizing():
+            lib().ohm_batch_free(self.h)
+            self.h = None
+
+
+def _with(p: N.BicgstabParams, **kw) -> N.BicgstabParams:
+    q = N.BicgstabParams()
+    ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(p))
+    for k, v in kw.items():
+        setattr(q, k, v)
+    return q
+
+
+# --------------------------------------------------------------------- AC --
+SWEEPS = {"dec": N.SWEEP_DEC, "oct": N.SWEEP_OCT, "lin": N.SWEEP_LIN}
+
+
+def generate_frequencies(sweep: str, n_points: int, f_start: float, f_stop: float) -> np.ndarray:
+    """generate_frequencies (src/analysis/ac.rs:80-109), native."""
+    t = SWEEPS[sweep.lower()]
+    cnt = lib().ohm_generate_frequencies(t, n_points, f_start, f_stop, None, 0)
+    out = np.zeros(max(cnt, 0), dtype=np.float64)
+    if cnt > 0:
+        lib().ohm_generate_frequencies(t, n_points, f_start, f_stop,
+                                       out.ctypes.data_as(N.c_f64p), cnt)
+    return out
+
+
+@dataclass
+class AcResult:
+    """AcResult (src/analysis/mod.rs): per-node / per-branch complex phasors."""
+    frequencies: np.ndarray
+    node_voltages: list
+    branch_currents: list
+    x: np.ndarray                    # [n_freqs, n] complex
+    stats: list = field(default_factory=list)
+
+
+def ac_sweep(backend: HipBackend, mna: ir.MnaSystem, freqs, batch: int = 0,
+             params: N.BicgstabParams | None = None, precond: int = N.PREC_JACOBI,
+             raise_on_failure: bool = True) -> AcResult:
+    """ac::run (src/analysis/ac.rs:25-77): every frequency point solved on
+    the GPU, ``batch`` points per launch sequence (0 = all at once)."""
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    n = mna.size
+    g, c = mna.g, mna.c
+    gv = np.ascontiguousarray(g.values, dtype=np.float64)
+    gc = np.ascontiguousarray(g.col_indices, dtype=np.uint32)
+    gr = np.ascontiguousarray(g.row_pointers, dtype=np.uint32)
+    cv = np.ascontiguousarray(c.values, dtype=np.float64)
+    cc = np.ascontiguousarray(c.col_indices, dtype=np.uint32)
+    cr = np.ascontiguousarray(c.row_pointers, dtype=np.uint32)
+    bac = np.ascontiguousarray(mna.b_ac, dtype=np.complex128)
+    x = np.zeros((f.shape[0], n), dtype=np.complex128)
+    res = (N.BatchResult * max(f.shape[0], 1))()
+    prm = params if params is not None else bicgstab_params()
+    rc = lib().ohm_ac_sweep(backend.h, ctypes.c_int64(n), ptr(gr, N.c_u32p), ptr(gc, N.c_u32p),
+                            ptr(gv, N.c_f64p), ptr(cr, N.c_u32p), ptr(cc, N.c_u32p),
+                            ptr(cv, N.c_f64p), bac.ctypes.data_as(N.c_f64p),
+                            ctypes.c_int64(f.shape[0]), ptr(f, N.c_f64p), ctypes.c_int64(batch),
+                            ctypes.byref(prm), int(precond), x.ctypes.data_as(N.c_f64p), res)
+    stats = [BatchStats(r.iterations, r.refinements, r.residual_norm, r.status, r.reason)
+             for r in res[:f.shape[0]]]
+    if rc != 0 and raise_on_failure:
+        check(rc)
+    nn = mna.n_nodes
+    nodes = [(name, x[:, i].copy()) for i, name in enumerate(mna.node_names)]
+    branches = [(name, x[:, nn + i].copy()) for i, name in enumerate(mna.branch_names)]
+    return AcResult(f, nodes, branches, x, stats)
 
 
 # ----------------------------------------------------------------- Newton --

@@ -45,6 +45,7 @@ def lib():
         _lib.oo_from_triplets.restype = ctypes.c_int64
         _lib.oo_pnjlim.restype = ctypes.c_double
         _lib.oo_eval_pulse.restype = ctypes.c_double
+        _lib.oo_source_value.restype = ctypes.c_double
         _lib.oo_pnjlim.argtypes = [ctypes.c_double] * 4
         _lib.oo_eval_pulse.argtypes = [ctypes.c_double] * 8
     return _lib
@@ -344,3 +345,38 @@ def newton_limit(n, x_old, x_new, diodes=(), bjts=(), mosfets=()):
                           ctypes.c_int64(len(bjts)), qp, ctypes.c_int64(len(mosfets)), mp,
                           _p(xo, f64p), _p(xn, f64p))
     return xn
+
+
+def transient(mna: _ir.MnaSystem, sources, pwl=None, method="be", tstep=1e-12, tstop=1e-10,
+              tstart=0.0, x0=None, probes=None, max_records=None):
+    """transient::run (src/analysis/transient.rs:25-199) with sparse LU solves.
+    Returns (times, waves[records, probes], x_final, accepted, rejected)."""
+    n = mna.size
+    grp, gcl, gv = _csr_args(mna.g)
+    crp, ccl, cv = _csr_args(mna.c)
+    src = np.ascontiguousarray(sources, dtype=_ir.TRAN_SOURCE_DTYPE)
+    pw = np.ascontiguousarray(pwl if pwl is not None else np.zeros((0, 2)), dtype=np.float64)
+    pr = np.ascontiguousarray(np.arange(n) if probes is None else probes, dtype=np.uint32)
+    if max_records is None:
+        max_records = int(np.ceil(tstop / tstep * (1 if method == "be" else 4))) + 2
+    times = np.zeros(max_records)
+    waves = np.zeros((max_records, pr.shape[0]))
+    xf = np.zeros(max(n, 1))
+    x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    nrec, acc, rej = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().oo_transient(
+        ctypes.c_int64(n), _p(grp, u32p), _p(gcl, u32p), _p(gv, f64p), _p(crp, u32p),
+        _p(ccl, u32p), _p(cv, f64p), ctypes.c_int64(src.shape[0]), src.ctypes.data_as(ctypes.c_void_p),
+        _p(pw, f64p), 0 if method == "be" else 1, ctypes.c_double(tstep), ctypes.c_double(tstop),
+        ctypes.c_double(tstart), _p(x0a, f64p) if x0a is not None else None,
+        ctypes.c_int64(pr.shape[0]), _p(pr, u32p), ctypes.c_int64(max_records), _p(times, f64p),
+        _p(waves, f64p), ctypes.byref(nrec), _p(xf, f64p), ctypes.byref(acc), ctypes.byref(rej)))
+    r = nrec.value
+    return times[:r], waves[:r], xf[:n], acc.value, rej.value
+
+
+def source_value(src_entry, t: float, pwl=None) -> float:
+    """evaluate (src/analysis/transient_source.rs:8-37) for one ohm_tran_source."""
+    e = np.ascontiguousarray(np.array([src_entry], dtype=_ir.TRAN_SOURCE_DTYPE))
+    pw = np.ascontiguousarray(pwl if pwl is not None else np.zeros((0, 2)), dtype=np.float64)
+    return lib().oo_source_value(e.ctypes.data_as(ctypes.c_void_p), _p(pw, f64p), ctypes.c_double(t))

@@ -1859,3 +1859,159 @@ double oo_eval_pulse(double v1, double v2, double td, double tr, double tf, doub
   if (t_rel < tr + pw + tf) return tf > 0.0 ? v2 + (v1 - v2) * (t_rel - tr - pw) / tf : v1;
   return v1;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Transient — src/analysis/transient.rs:25-362, transient_source.rs:8-116.  */
+/* ------------------------------------------------------------------------ */
+double oo_source_value(const ohm_tran_source *s, const double *pwl, double t) {
+  const double *p = s->param;
+  switch (s->kind) {
+    case OHM_SRC_PULSE:
+      return oo_eval_pulse(p[0], p[1], p[2], p[3], p[4], p[5], p[6], t);
+    case OHM_SRC_SIN: { /* eval_sin :73-83 */
+      if (t < p[3]) return p[0];
+      double dt = t - p[3];
+      double env = p[4] != 0.0 ? exp(-dt * p[4]) : 1.0;
+      return p[0] + p[1] * sin(2.0 * 3.14159265358979323846 * p[2] * dt) * env;
+    }
+    case OHM_SRC_EXP: { /* eval_exp :106-116 */
+      if (t < p[2]) return p[0];
+      double rise = (p[1] - p[0]) * (1.0 - exp(-(t - p[2]) / p[3]));
+      if (t < p[4]) return p[0] + rise;
+      double fall = (p[0] - p[1]) * (1.0 - exp(-(t - p[4]) / p[5]));
+      return p[0] + rise + fall;
+    }
+    case OHM_SRC_PWL: { /* eval_pwl :85-104 */
+      const double *q = pwl + 2 * s->pwl_offset;
+      int64_t m = s->pwl_count;
+      if (m == 0) return 0.0;
+      if (t <= q[0]) return q[1];
+      if (t >= q[2 * (m - 1)]) return q[2 * (m - 1) + 1];
+      for (int64_t i = 1; i < m; i++)
+        if (t <= q[2 * i]) {
+          double frac = (t - q[2 * i - 2]) / (q[2 * i] - q[2 * i - 2]);
+          return q[2 * i - 1] + (q[2 * i + 1] - q[2 * i - 1]) * frac;
+        }
+      return q[2 * (m - 1) + 1];
+    }
+    default:
+      return s->dc;
+  }
+}
+
+/* evaluate_b_at_time :265-280 — starts from zeros, sources in order */
+static void tr_b_at(int64_t n, int64_t n_src, const ohm_tran_source *src, const double *pwl,
+                    double t, double *b) {
+  for (int64_t i = 0; i < n; i++) b[i] = 0.0;
+  for (int64_t k = 0; k < n_src; k++) {
+    double v = oo_source_value(&src[k], pwl, t);
+    for (int j = 0; j < src[k].n_idx; j++) b[src[k].idx[j]] += src[k].sign[j] * v;
+  }
+}
+
+/* form_companion_matrix :285-323 (A on G's pattern) */
+static void tr_companion(int64_t n, const uint32_t *grp, const uint32_t *gcl, const double *gv,
+                         const uint32_t *crp, const uint32_t *ccl, const double *cv, double h,
+                         double alpha, double *a) {
+  double factor = alpha / h;
+  for (uint32_t k = 0; k < grp[n]; k++) a[k] = gv[k];
+  for (int64_t r = 0; r < n; r++)
+    for (uint32_t q = crp[r]; q < crp[r + 1]; q++) {
+      if (cv[q] == 0.0) continue;
+      for (uint32_t k = grp[r]; k < grp[r + 1]; k++)
+        if (gcl[k] == ccl[q]) {
+          a[k] += factor * cv[q];
+          break;
+        }
+    }
+}
+
+int oo_transient(int64_t n, const uint32_t *grp, const uint32_t *gcl, const double *gv,
+                 const uint32_t *crp, const uint32_t *ccl, const double *cv, int64_t n_src,
+                 const ohm_tran_source *src, const double *pwl, int method, double tstep,
+                 double tstop, double tstart, const double *x0, int64_t n_probes,
+                 const uint32_t *probes, int64_t max_rec, double *times, double *waves,
+                 int64_t *n_rec, double *x_final, int64_t *accepted, int64_t *rejected) {
+  const double h_min = tstep / 10000.0, h_max = tstep, lte_tol = 1e-3;
+  const int max_fail = 10;
+  double *x = (double *)xcalloc(n > 0 ? n : 1, sizeof(double));
+  double *b_next = (double *)xcalloc(n > 0 ? n : 1, sizeof(double));
+  double *b_prev = (double *)xcalloc(n > 0 ? n : 1, sizeof(double));
+  double *rhs = (double *)xcalloc(n > 0 ? n : 1, sizeof(double));
+  double *cx = (double *)xcalloc(n > 0 ? n : 1, sizeof(double));
+  double *gx = (double *)xcalloc(n > 0 ? n : 1, sizeof(double));
+  double *xt = (double *)xcalloc(n > 0 ? n : 1, sizeof(double));
+  double *xb = (double *)xcalloc(n > 0 ? n : 1, sizeof(double));
+  double *a = (double *)xcalloc(grp[n] > 0 ? grp[n] : 1, sizeof(double));
+  if (x0)
+    for (int64_t i = 0; i < n; i++) x[i] = x0[i];
+  int64_t rec = 0, acc = 0, rej = 0;
+  int rc = OHM_OK;
+#define RECORD(tt)                                                          \
+  do {                                                                      \
+    if (rec < max_rec) {                                                    \
+      times[rec] = (tt);                                                    \
+      for (int64_t j = 0; j < n_probes; j++) waves[rec * n_probes + j] = x[probes[j]]; \
+      rec++;                                                                \
+    }                                                                       \
+  } while (0)
+  if (tstart <= 0.0) RECORD(0.0);
+  double t = 0.0, h = tstep < h_max ? tstep : h_max;
+  tr_b_at(n, n_src, src, pwl, 0.0, b_prev);
+  int use_be = 1, fails = 0;
+  while (t < tstop) {
+    if (t + h > tstop) h = tstop - t;
+    if (h < h_min * 0.5) break;
+    double t_next = t + h;
+    tr_b_at(n, n_src, src, pwl, t_next, b_next);
+    oo_spmv(n, crp, ccl, cv, x, cx, 0);
+    if (use_be || method == OHM_TRAN_FIXED_BE) {
+      tr_companion(n, grp, gcl, gv, crp, ccl, cv, h, 1.0, a);
+      for (int64_t i = 0; i < n; i++) rhs[i] = b_next[i] + (1.0 / h) * cx[i];
+      if ((rc = oo_sparse_lu_solve(n, grp, gcl, a, rhs, xb, 0)) != OHM_OK) break;
+      for (int64_t i = 0; i < n; i++) xt[i] = xb[i];
+    } else {
+      oo_spmv(n, grp, gcl, gv, x, gx, 0);
+      tr_companion(n, grp, gcl, gv, crp, ccl, cv, h, 2.0, a);
+      double f = 2.0 / h;
+      for (int64_t i = 0; i < n; i++) rhs[i] = b_next[i] + f * cx[i] - gx[i] + b_prev[i];
+      if ((rc = oo_sparse_lu_solve(n, grp, gcl, a, rhs, xt, 0)) != OHM_OK) break;
+      tr_companion(n, grp, gcl, gv, crp, ccl, cv, h, 1.0, a);
+      for (int64_t i = 0; i < n; i++) rhs[i] = b_next[i] + (1.0 / h) * cx[i];
+      if ((rc = oo_sparse_lu_solve(n, grp, gcl, a, rhs, xb, 0)) != OHM_OK) break;
+      double md = 0.0;
+      for (int64_t i = 0; i < n; i++) md = fmax(md, fabs(xt[i] - xb[i]));
+      double lte = (2.0 / 3.0) * md;
+      double hn = h * fmin(2.0, fmax(0.5, 0.9 * sqrt(lte_tol / lte)));
+      if (lte > lte_tol) {
+        h = fmax(hn, h_min);
+        fails++;
+        if (fails >= max_fail && h <= h_min * 1.01) {
+          rc = fail(OHM_ERR_TIMESTEP, "Timestep error: simulation failed: %d consecutive failures", fails);
+          break;
+        }
+        rej++;
+        use_be = 1;
+        continue;
+      }
+      h = fmin(h_max, fmax(h_min, hn));
+    }
+    if (use_be && method == OHM_TRAN_ADAPTIVE) use_be = 0;
+    fails = 0;
+    acc++;
+    for (int64_t i = 0; i < n; i++) {
+      x[i] = xt[i];
+      b_prev[i] = b_next[i];
+    }
+    t = t_next;
+    if (t >= tstart) RECORD(t);
+  }
+#undef RECORD
+  if (x_final)
+    for (int64_t i = 0; i < n; i++) x_final[i] = x[i];
+  *n_rec = rec;
+  if (accepted) *accepted = acc;
+  if (rejected) *rejected = rej;
+  free(x); free(b_next); free(b_prev); free(rhs); free(cx); free(gx); free(xt); free(xb); free(a);
+  return rc;
+}

@@ -13,6 +13,7 @@
 #define OHM_ORACLE_H
 
 #include "../include/ohm_circuit.h"
+#include "../include/ohm_solver.h" /* ohm_tran_source layout only */
 
 #ifdef __cplusplus
 extern "C" {
@@ -140,6 +141,22 @@ void oo_newton_limit(int64_t n, int64_t n_diodes, const ohm_diode_desc *diodes, 
 /* PULSE source value at time t (src/analysis/transient_source.rs:39-71). */
 double oo_eval_pulse(double v1, double v2, double td, double tr, double tf, double pw, double per,
                      double t);
+
+/* Source value at time t: evaluate (src/analysis/transient_source.rs:8-37). */
+double oo_source_value(const ohm_tran_source *s, const double *pwl_pairs, double t);
+
+/* transient::run (src/analysis/transient.rs:25-199) with every linear
+ * solve done by oo_sparse_lu_solve: method OHM_TRAN_FIXED_BE (backward
+ * Euler at h = tstep, the same stepping rule without LTE control) or
+ * OHM_TRAN_ADAPTIVE (the reference's TRAP/BE with LTE). Records t = 0 when
+ * tstart <= 0 and every accepted step with t >= tstart: times[r],
+ * waves[r * n_probes + j]; *n_rec receives the record count. */
+int oo_transient(int64_t n, const uint32_t *g_rowptr, const uint32_t *g_col, const double *g_val,
+                 const uint32_t *c_rowptr, const uint32_t *c_col, const double *c_val,
+                 int64_t n_src, const ohm_tran_source *src, const double *pwl_pairs, int method,
+                 double tstep, double tstop, double tstart, const double *x0, int64_t n_probes,
+                 const uint32_t *probes, int64_t max_rec, double *times, double *waves,
+                 int64_t *n_rec, double *x_final, int64_t *accepted, int64_t *rejected);
 
 #ifdef __cplusplus
 }

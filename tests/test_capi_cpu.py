@@ -124,3 +124,18 @@ def test_backend_without_gpu_reports_no_adapter():
     with pytest.raises(N.OhmnivoreError, match="no GPU adapter") as e:
         solver.HipBackend()
     assert e.value.kind == "NoDevice"
+
+
+# ------------------------------------------------- generate_frequencies --
+def test_generate_frequencies_reference_cases():
+    """tests/integration_test.rs:182-221 (LIN count / endpoints, OCT count)
+    and the DEC rule of src/analysis/ac.rs:85-90."""
+    f = solver.generate_frequencies("lin", 100, 1.0, 1000.0)
+    assert f.shape == (100,) and abs(f[0] - 1.0) < 1e-9 and abs(f[-1] - 1000.0) < 1e-9
+    f = solver.generate_frequencies("oct", 5, 100.0, 800.0)
+    assert f.shape == (int(np.ceil(5 * np.log2(8.0))),) and abs(f[0] - 100.0) < 1e-9
+    f = solver.generate_frequencies("dec", 10, 1.0, 1e6)
+    assert f.shape == (60,) and f[0] == 1.0          # the stop point itself is excluded
+    assert np.allclose(f, 10.0 ** (np.arange(60) / 10.0), rtol=1e-15)
+    assert solver.generate_frequencies("lin", 1, 5.0, 9.0).tolist() == [5.0]
+    assert solver.generate_frequencies("dec", 10, 10.0, 1.0).shape == (0,)  # ceil < 0 -> 0 points

@@ -1,0 +1,242 @@
+"""GPU parity of the batched and complex solves: LinearSolver::solve_complex
+(src/solver/gpu.rs:210-735), batches of real / complex systems with
+per-system sigma (AC points, companion matrices, Monte Carlo, uneven
+circuits) and the AC sweep (src/analysis/ac.rs:25-77), all against the
+oracle's FP64 / complex128 LU. Tolerance 1e-6 relative / 1e-9 absolute."""
+import numpy as np
+import pytest
+
+import oracle
+from ohmnivore_amd import _native as N
+from ohmnivore_amd import circuit as ir
+from ohmnivore_amd import solver
+
+pytestmark = pytest.mark.gpu
+
+REL, ABS = 1e-6, 1e-9
+
+
+@pytest.fixture(scope="module")
+def be():
+    return solver.HipBackend()
+
+
+def close(x, ref, rel=REL, abs_=ABS):
+    err = np.abs(x - ref)
+    return bool((err <= np.maximum(rel * np.abs(ref), abs_)).all()), float(err.max(initial=0.0))
+
+
+def csr_of(a):
+    return (np.asarray(a.row_pointers, np.uint32), np.asarray(a.col_indices, np.uint32),
+            np.asarray(a.values))
+
+
+def dd_complex(n, per_row, rng, band=40):
+    """Diagonally dominant complex sparse matrix with columns within `band`
+    of the diagonal (BiCGSTAB converges; the oracle's banded LU stays cheap)."""
+    rows, cols, vals = [], [], []
+    for r in range(n):
+        near = np.clip(r + rng.integers(-band, band + 1, size=per_row), 0, n - 1)
+        c = np.unique(np.concatenate([[r], near]))
+        off = rng.standard_normal(len(c)) + 1j * rng.standard_normal(len(c))
+        off[c == r] = 0
+        d = np.abs(off).sum() + 1.0 + rng.random()
+        off[c == r] = d * np.exp(1j * rng.uniform(-0.5, 0.5))
+        rows += [r] * len(c)
+        cols += list(c)
+        vals += list(off)
+    return oracle.from_triplet_arrays(n, n, np.array(rows), np.array(cols), np.array(vals))
+
+
+# ---------------------------------------------------------- solve_complex --
+def test_complex_identity_2x2(be):
+    """gpu_solve_complex_identity (src/solver/gpu.rs:830-843)."""
+    a = oracle.from_triplets(2, 2, [(0, 0, 1 + 0j), (1, 1, 1 + 0j)])
+    x = solver.GpuSolver(be, params=solver.bicgstab_params(tolerance=1e-12)).solve_complex(
+        a, [3 + 1j, 7 - 2j])
+    assert np.allclose(x, [3 + 1j, 7 - 2j], atol=1e-12)
+
+
+def test_complex_known_2x2(be):
+    """gpu_solve_complex_known_2x2 (src/solver/gpu.rs:846-866): x = [1, 2]."""
+    a = oracle.from_triplets(2, 2, [(0, 0, 1 + 1j), (0, 1, 2 + 0j), (1, 1, 1 - 1j)])
+    x = solver.GpuSolver(be, params=solver.bicgstab_params(tolerance=1e-12)).solve_complex(
+        a, [5 + 1j, 2 - 2j])
+    assert np.allclose(x, [1, 2], atol=1e-10)
+
+
+@pytest.mark.parametrize("n,precision", [(300, "fp64"), (3000, "fp64"), (20000, "fp64"),
+                                         (3000, "fp32_ir"), (20000, "fp32_ir")])
+def test_complex_random_vs_lu(be, n, precision):
+    rng = np.random.default_rng(n)
+    a = dd_complex(n, 4, rng)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = oracle.sparse_lu_solve(a, b)
+    s = solver.GpuSolver(be, precond=N.PREC_JACOBI,
+                         params=solver.bicgstab_params(tolerance=1e-12, precision=precision))
+    x = s.solve_complex(a, b)
+    ok, err = close(x, ref)
+    assert ok, (err, s.last_stats)
+    if precision == "fp32_ir":
+        assert s.last_stats.refinements >= 1
+
+
+def test_complex_empty(be):
+    a = oracle.from_triplets(0, 0, [])
+    a.values = a.values.astype(np.complex128)
+    assert solver.GpuSolver(be).solve_complex(a, np.zeros(0, complex)).shape == (0,)
+
+
+# ------------------------------------------------------------------ batch --
+def ladder_mna(sections, seed):
+    return solver.compile(ir.resistor_ladder(sections, seed))
+
+
+@pytest.mark.parametrize("sizes", [(50, 50, 50), (5, 2000, 300), (40000, 7, 9000)])
+def test_batch_uneven_real_vs_lu(be, sizes):
+    """Uneven-size circuits in one batch (persistent path when every system
+    is small, multi-kernel path otherwise); each against its own LU."""
+    mnas = [ladder_mna(s, 10 + i) for i, s in enumerate(sizes)]
+    systems = [(m.g.row_pointers, m.g.col_indices, m.g.values, None, 0.0) for m in mnas]
+    bt = solver.Batch(be, systems, complex_=False)
+    rhs = np.concatenate([m.b_dc for m in mnas])
+    x = bt.solve(rhs, params=solver.bicgstab_params(tolerance=1e-13, max_iterations=200000))
+    for m, xi, st in zip(mnas, bt.split(x), bt.last_results):
+        ref = oracle.sparse_lu_solve(m.g, m.b_dc)
+        ok, err = close(xi, ref)
+        assert ok and st.converged, (err, st)
+        assert st.residual_norm < 1e-12
+
+
+def test_batch_monte_carlo_shared_pattern(be):
+    """Monte Carlo: one pattern, per-instance values (RC companion A = G + C/h)."""
+    base = solver.compile(ir.rc_grid(24, 24, 8, seed=3))
+    rng = np.random.default_rng(7)
+    k, h = 12, 1e-12
+    systems, refs, rhs = [], [], []
+    for i in range(k):
+        g = base.g.values * (1 + 0.1 * rng.standard_normal(base.g.values.shape[0]))
+        c = np.zeros_like(base.g.values)
+        # C on G's pattern (the compiler zero-fills C positions into G)
+        for r in range(base.size):
+            for q in range(base.c.row_pointers[r], base.c.row_pointers[r + 1]):
+                c[base.g.value_index(r, int(base.c.col_indices[q]))] += base.c.values[q]
+        systems.append((base.g.row_pointers, base.g.col_indices, g, c, 1.0 / h))
+        a = ir.CsrMatrix(base.size, base.size, g + c / h, base.g.col_indices, base.g.row_pointers)
+        b = rng.standard_normal(base.size)
+        refs.append(oracle.sparse_lu_solve(a, b))
+        rhs.append(b)
+    bt = solver.Batch(be, systems, complex_=False)
+    x = bt.solve(np.concatenate(rhs), params=solver.bicgstab_params(tolerance=1e-13))
+    for xi, ref, st in zip(bt.split(x), refs, bt.last_results):
+        ok, err = close(xi, ref)
+        assert ok, (err, st)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_batch_complex_sigma_vs_lu(be, precision):
+    """Same G, C for every system, sigma_k = j*omega_k: the AC batch form,
+    on systems BiCGSTAB can solve (diagonally dominant G, small C)."""
+    rng = np.random.default_rng(11)
+    n = 6000
+    a = dd_complex(n, 3, rng)
+    rows = np.repeat(np.arange(n), np.diff(a.row_pointers.astype(np.int64)))
+    g = a.values.real.copy()
+    g[rows == a.col_indices] *= 4.0       # stays diagonally dominant at every omega
+    c = np.abs(a.values.imag) * 1e-10
+    omegas = 2 * np.pi * np.logspace(0, 9, 9)
+    systems = [(a.row_pointers, a.col_indices, g, c, 1j * w) for w in omegas]
+    bt = solver.Batch(be, systems, complex_=True)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x = bt.solve(np.tile(b, len(omegas)),
+                 params=solver.bicgstab_params(tolerance=1e-12, precision=precision))
+    for w, xi, st in zip(omegas, bt.split(x), bt.last_results):
+        aw = ir.CsrMatrix(n, n, g + 1j * w * c, a.col_indices, a.row_pointers)
+        ok, err = close(xi, oracle.sparse_lu_solve(aw, b))
+        assert ok, (w, err, st)
+
+
+def test_batch_spmv(be):
+    rng = np.random.default_rng(5)
+    a = dd_complex(700, 5, rng)
+    g, c = a.values.real.copy(), a.values.imag.copy()
+    sig = [0.5 + 2j, -1j, 3.0]
+    bt = solver.Batch(be, [(a.row_pointers, a.col_indices, g, c, s) for s in sig], complex_=True)
+    x = rng.standard_normal(bt.rows) + 1j * rng.standard_normal(bt.rows)
+    y = bt.spmv(x)
+    for s, xi, yi in zip(sig, bt.split(x), bt.split(y)):
+        vals = (g + s * c).astype(np.complex128)
+        ref = oracle.spmv(ir.CsrMatrix(700, 700, vals, a.col_indices, a.row_pointers), xi)
+        assert np.allclose(yi, ref, rtol=1e-13, atol=1e-12)
+
+
+def test_batch_masks_and_statuses(be):
+    """Per-system status: a zero RHS converges at once with x = 0, a capped
+    system reports max-iterations, the others still converge."""
+    m1, m2 = ladder_mna(3000, 1), ladder_mna(2000, 2)
+    systems = [(m.g.row_pointers, m.g.col_indices, m.g.values, None, 0.0) for m in (m1, m2, m1)]
+    bt = solver.Batch(be, systems, complex_=False)
+    rhs = np.concatenate([m1.b_dc, np.zeros(m2.size), m1.b_dc])
+    x = bt.solve(rhs, params=solver.bicgstab_params(tolerance=1e-13, max_iterations=100000))
+    r = bt.last_results
+    assert all(s.converged for s in r)
+    assert np.all(bt.split(x)[1] == 0) and r[1].iterations == 0
+    a, b_, c = bt.split(x)
+    assert np.array_equal(a, c)  # identical systems give identical bits
+    with pytest.raises(N.OhmnivoreError, match="did not converge"):
+        bt.solve(rhs, params=solver.bicgstab_params(tolerance=1e-13, max_iterations=3))
+    assert [s.reason for s in bt.last_results] == [7, 1, 7]
+
+
+def test_batch_deterministic(be):
+    mnas = [ladder_mna(30000, 3), ladder_mna(12000, 4)]
+    systems = [(m.g.row_pointers, m.g.col_indices, m.g.values, None, 0.0) for m in mnas]
+    bt = solver.Batch(be, systems, complex_=False)
+    rhs = np.concatenate([m.b_dc for m in mnas])
+    prm = solver.bicgstab_params(tolerance=1e-12)
+    assert np.array_equal(bt.solve(rhs, params=prm), bt.solve(rhs, params=prm))
+
+
+# --------------------------------------------------------------------- AC --
+def rc_lowpass():
+    ckt = ir.Circuit()
+    ckt.vsource("V1", "1", "0", ac=(1.0, 0.0))
+    ckt.resistor("R1", "1", "2", 1e3)
+    ckt.capacitor("C1", "2", "0", 1e-6)
+    return solver.compile(ckt)
+
+
+def test_ac_rc_lowpass(be):
+    """test_rc_lowpass_ac_sweep (tests/integration_test.rs:131-176)."""
+    mna = rc_lowpass()
+    f = solver.generate_frequencies("dec", 10, 1.0, 1e6)
+    res = solver.ac_sweep(be, mna, f, params=solver.bicgstab_params(tolerance=1e-12))
+    v2 = dict(res.node_voltages)["2"]
+    assert abs(abs(v2[0]) - 1.0) < 0.01
+    fc = 1.0 / (2 * np.pi * 1e3 * 1e-6)
+    i = int(np.argmin(np.abs(f - fc)))
+    assert abs(abs(v2[i]) - 1 / np.sqrt(2)) < 0.05
+    assert abs(v2[-1]) < 0.01
+    # exact divider: V2 = 1 / (1 + j w R C)
+    assert np.allclose(v2, 1 / (1 + 2j * np.pi * f * 1e-3), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("batch", [0, 7])
+def test_ac_short_rlc_ladders_vs_lu(be, batch):
+    """101-point sweep over short RLC ladders (the AC workload's circuit
+    family at sizes where Jacobi-BiCGSTAB converges at every frequency)."""
+    ckts, freqs = ir.rlc_sweep(n_circuits=3, max_sections=5)
+    for ckt in ckts:
+        mna = solver.compile(ckt)
+        res = solver.ac_sweep(be, mna, freqs, batch=batch,
+                              params=solver.bicgstab_params(tolerance=1e-13, max_iterations=20000))
+        g = mna.g
+        c_on_g = np.zeros(g.values.shape[0])
+        for r in range(mna.size):
+            for q in range(mna.c.row_pointers[r], mna.c.row_pointers[r + 1]):
+                c_on_g[g.value_index(r, int(mna.c.col_indices[q]))] += mna.c.values[q]
+        for i, fr in enumerate(freqs):
+            a = ir.CsrMatrix(mna.size, mna.size, g.values + 2j * np.pi * fr * c_on_g,
+                             g.col_indices, g.row_pointers)
+            ok, err = close(res.x[i], oracle.sparse_lu_solve(a, mna.b_ac))
+            assert ok, (fr, err, res.stats[i])
