@@ -334,6 +334,20 @@ ohm_status ohm_newton_assemble_at(ohm_newton_ctx *ctx, const double *x, double *
 /* Limiting of x_new against x_old (BJT, MOSFET, diode passes). */
 ohm_status ohm_newton_limit(ohm_newton_ctx *ctx, const double *x_old, double *x_new);
 
+/* ---- Adjoint DC sensitivity (no counterpart in the reference). At the
+ * operating point left in ctx by ohm_newton_run, with F(x, p) = 0 the MNA
+ * residual and J = dF/dx the assembled Newton matrix there:
+ *   solve J^T lambda = dJdx (transposed BiCGSTAB, transposed preconditioner)
+ *   grad_g[r]  = dObj/dg_r  = -lambda^T dF/dg_r  for resistor conductances
+ *                g_r between res_nodes[2r], res_nodes[2r+1] (OHM_SENTINEL = ground)
+ *   grad_kp[m] = dObj/dkp_m = -lambda^T dF/dkp_m for every MOSFET's kp (= KP W/L)
+ * dJdx: n doubles (dObjective/dx). grad_kp (n_mosfets) and lambda_out (n)
+ * may be NULL. params NULL = defaults with tolerance 1e-12. */
+ohm_status ohm_adjoint_dc(ohm_newton_ctx *ctx, const double *dJdx,
+                          const ohm_bicgstab_params *params, int64_t n_res,
+                          const uint32_t *res_nodes, double *grad_g, double *grad_kp,
+                          double *lambda_out, ohm_bicgstab_result *result);
+
 #ifdef __cplusplus
 }
 #endif

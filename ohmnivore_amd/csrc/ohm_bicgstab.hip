@@ -481,6 +481,27 @@ ohm_status bicg_solve_f64(ohm_backend *b, const DevCsr &a, const PrecondDev &p, 
   return status_to_error(out.status, out.iters, o.max_iters);
 }
 
+ohm_status bicg_solve_transpose(ohm_backend *b, DevCsr &a, PrecondDev &p, const double *rhs,
+                                double *x, const BicgOpts &o, BicgOut &out) {
+  DevCsr at;
+  OHM_TRY(csr_transpose_view(b, a, at));
+  PrecondDev pt;  // view: shares every device array of p
+  pt.kind = p.kind;
+  pt.level = p.level;
+  pt.n = p.n;
+  pt.inv = p.inv;  // diag(A^T) = diag(A); the zero-diagonal rule is symmetric too
+  if (p.kind == PK_ISAI) {
+    DevCsr mlt, mut;
+    OHM_TRY(csr_transpose_view(b, p.ml, mlt));
+    OHM_TRY(csr_transpose_view(b, p.mu, mut));
+    pt.ml = mut;  // applied first: M_U^T
+    pt.mu = mlt;  // then M_L^T
+  }
+  BicgOpts ot = o;
+  ot.transpose = false;
+  return bicg_solve_f64(b, at, pt, rhs, x, ot, out);
+}
+
 // ------------------------------------------------- iterative refinement --
 // r = b - A x (FP64) with ||r||^2 and ||b||^2 into d_scalars[0..1]; the
 // scaled residual r / ||r|| is written in FP32 by the next kernel.

@@ -441,7 +441,9 @@ static ohm_status run_bicgstab(ohm_backend *b, DevCsr &a, PrecondDev &pd, const 
   int64_t d0 = b->dispatches;
   OHM_HIP(hipEventRecord(b->ev0, b->stream));
   ohm_status s;
-  if (prm.precision == OHM_PRECISION_FP32_IR)
+  if (transpose)  // adjoint systems run in FP64
+    s = bicg_solve_transpose(b, a, pd, rhs, x, o, out);
+  else if (prm.precision == OHM_PRECISION_FP32_IR)
     s = bicg_solve_ir(b, a, pd, rhs, x, o, prm.inner_tolerance, prm.max_refinements, out,
                       refinements);
   else
@@ -460,7 +462,10 @@ static ohm_status run_bicgstab(ohm_backend *b, DevCsr &a, PrecondDev &pd, const 
     double rel = NAN;
     if (s == OHM_OK || s == OHM_ERR_SOLVE) {
       std::string keep = g_last_error;
-      if (true_residual(b, a, rhs, x, rel) != OHM_OK) rel = NAN;
+      DevCsr at;
+      const DevCsr *ra = &a;
+      if (transpose && csr_transpose_view(b, a, at) == OHM_OK) ra = &at;
+      if (true_residual(b, *ra, rhs, x, rel) != OHM_OK) rel = NAN;
       g_last_error = keep;
     }
     result->residual_norm = rel;
@@ -484,14 +489,13 @@ ohm_status ohm_bicgstab(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *r
 ohm_status ohm_bicgstab_transpose(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *rhs,
                                   ohm_buffer *x, const ohm_precond *p,
                                   const ohm_bicgstab_params *params, ohm_bicgstab_result *result) {
-  (void)b;
-  (void)a;
-  (void)rhs;
-  (void)x;
-  (void)p;
-  (void)params;
-  (void)result;
-  return fail(OHM_ERR_SOLVE, "Solve error: transposed BiCGSTAB is not built in this library");
+  if (!b || !a || !rhs || !x) return fail(OHM_ERR_INVALID, "null handle");
+  if (a->csr.n != (int)rhs->n || a->csr.n != (int)x->n)
+    return fail(OHM_ERR_SOLVE, "Solve error: dimension mismatch");
+  PrecondDev none;
+  none.n = a->csr.n;
+  PrecondDev &pd = p ? const_cast<ohm_precond *>(p)->dev : none;
+  return run_bicgstab(b, const_cast<ohm_matrix *>(a)->csr, pd, rhs->d, x->d, params, result, true);
 }
 
 ohm_status ohm_solve_real(ohm_backend *b, size_t n, size_t nnz, const double *values,

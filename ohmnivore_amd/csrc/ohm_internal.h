@@ -82,6 +82,7 @@ struct DevCsr {
   // transpose pattern (built lazily for A^T x): CSR of A^T + map to A's slots
   uint32_t *t_rowptr = nullptr, *t_col = nullptr, *t_src = nullptr, *t_blocks = nullptr;
   int t_n_blocks = 0;
+  double *t_val = nullptr;         // values of A^T in its CSR order (gathered via t_src)
   std::vector<uint32_t> h_rowptr, h_col; // host copy of the pattern
 };
 
@@ -134,6 +135,9 @@ ohm_status csr_upload(ohm_backend *b, DevCsr &m, const double *val, const uint32
 void csr_free(DevCsr &m);
 ohm_status csr_ensure_f32(ohm_backend *b, DevCsr &m);
 ohm_status csr_ensure_transpose(ohm_backend *b, DevCsr &m);
+// Refresh t_val from the current values and return A^T as a DevCsr view
+// (shares the pattern arrays of m; never freed on its own).
+ohm_status csr_transpose_view(ohm_backend *b, DevCsr &m, DevCsr &view);
 ohm_status ensure_partials(ohm_backend *b, size_t count);
 // Drop every cached graph (called whenever device objects are freed).
 void graph_cache_clear(ohm_backend *b);

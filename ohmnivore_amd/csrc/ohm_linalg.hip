@@ -288,9 +288,35 @@ ohm_status csr_ensure_transpose(ohm_backend *b, DevCsr &m) {
   return OHM_OK;
 }
 
+__global__ void k_gather_vals(int64_t nnz, const uint32_t *__restrict__ src,
+                              const double *__restrict__ val, double *__restrict__ out) {
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * kBlock)
+    out[k] = val[src[k]];
+}
+
+ohm_status csr_transpose_view(ohm_backend *b, DevCsr &m, DevCsr &view) {
+  OHM_TRY(csr_ensure_transpose(b, m));
+  if (!m.t_val) OHM_HIP(hipMalloc(&m.t_val, sizeof(double) * std::max<int64_t>(m.nnz, 1)));
+  if (m.nnz > 0) {
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(m.nnz)), dim3(kBlock), 0, b->stream, m.nnz,
+                       m.t_src, m.val, m.t_val);
+    b->dispatches++;
+  }
+  view = DevCsr();
+  view.n = m.n;
+  view.nnz = m.nnz;
+  view.rowptr = m.t_rowptr;
+  view.col = m.t_col;
+  view.val = m.t_val;
+  view.blocks = m.t_blocks;
+  view.n_blocks = m.t_n_blocks;
+  return OHM_OK;
+}
+
 void csr_free(DevCsr &m) {
   void *ptrs[] = {m.rowptr, m.col,    m.val,   m.val32, m.blocks,  m.diag_pos,
-                  m.t_rowptr, m.t_col, m.t_src, m.t_blocks};
+                  m.t_rowptr, m.t_col, m.t_src, m.t_blocks, m.t_val};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   m = DevCsr();

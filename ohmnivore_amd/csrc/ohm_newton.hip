@@ -352,6 +352,36 @@ __global__ void k_copy_d(int n, const double *__restrict__ x, double *__restrict
 
 using namespace ohm;
 
+// ------------------------------------------------- adjoint scatter ----
+// dJ/dp = -lambda^T dF/dp at the operating point x, with F the MNA residual
+// G x + i_dev(x) - b (the assembled Newton matrix is dF/dx).
+// Resistor conductance g between a and b adds g (x_a - x_b) to F_a and
+// subtracts it from F_b.
+__global__ void k_adj_conductance(int nr, const uint32_t *__restrict__ nodes,
+                                  const double *__restrict__ x, const double *__restrict__ lam,
+                                  double *__restrict__ grad) {
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < nr; r += gridDim.x * kBlock) {
+    const uint32_t a = nodes[2 * r], c = nodes[2 * r + 1];
+    const double v = xv(x, a) - xv(x, c);
+    grad[r] = -(xv(lam, a) - xv(lam, c)) * v;
+  }
+}
+
+// MOSFET transconductance parameter kp (= KP W/L): the drain current is
+// linear in kp, i_d = kp * i_unit, entering F at the drain stamp row and
+// leaving at the source stamp row (b_idx, the rows the Norton currents go to).
+__global__ void k_adj_mosfet_kp(int nm, const ohm_mosfet_desc *__restrict__ d,
+                                const double *__restrict__ x, const double *__restrict__ lam,
+                                double *__restrict__ grad) {
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < nm; k += gridDim.x * kBlock) {
+    const ohm_mosfet_desc &q = d[k];
+    const double vd = xv(x, q.drain_idx), vg = xv(x, q.gate_idx), vs = xv(x, q.source_idx);
+    double iu, gm, gds;
+    mosfet_core(vg - vs, vd - vs, q.vto, 1.0, q.lambda, q.polarity, iu, gm, gds);
+    grad[k] = -(xv(lam, q.b_idx[0]) - xv(lam, q.b_idx[1])) * iu;
+  }
+}
+
 struct ohm_newton_ctx {
   ohm_backend *b = nullptr;
   int n = 0;
@@ -374,6 +404,8 @@ struct ohm_newton_ctx {
   PrecondDev P;
   int prec_kind = -1;
   int prec_level = 1;
+  double *adj = nullptr;  // adjoint scratch: rhs, lambda, gradients (grown on demand)
+  size_t adj_cap = 0;
 };
 
 namespace {
@@ -488,6 +520,7 @@ void ohm_newton_default_params(ohm_newton_params *p) {
 }
 
 void ohm_newton_free(ohm_newton_ctx *c) {
+  if (c && c->adj && backend_alive(c->b)) (void)hipFree(c->adj);
   if (!c) return;
   if (backend_alive(c->b)) graph_cache_clear(c->b);
   csr_free(c->A);
@@ -798,6 +831,93 @@ ohm_status ohm_newton_solve(ohm_backend *b, const double *base_values,
   ohm_status s = ohm_newton_run(c, params, result, iters_per_newton);
   if (s == OHM_OK && x_out) s = ohm_newton_get_x(c, x_out);
   ohm_newton_free(c);
+  return s;
+}
+
+ohm_status ohm_adjoint_dc(ohm_newton_ctx *c, const double *dJdx, const ohm_bicgstab_params *params,
+                          int64_t n_res, const uint32_t *res_nodes, double *grad_g,
+                          double *grad_kp, double *lambda_out, ohm_bicgstab_result *result) {
+  if (!c || !dJdx) return fail(OHM_ERR_INVALID, "null argument");
+  if (n_res < 0 || (n_res > 0 && (!res_nodes || !grad_g)))
+    return fail(OHM_ERR_INVALID, "bad resistor parameter list");
+  for (int64_t r = 0; r < 2 * n_res; r++)
+    if (res_nodes[r] != OHM_SENTINEL && res_nodes[r] >= (uint32_t)c->n)
+      return fail(OHM_ERR_INVALID, "resistor node index out of range");
+  ohm_backend *b = c->b;
+  hipStream_t st = b->stream;
+  const int n = c->n;
+  const size_t need = 2 * (size_t)n + (size_t)n_res + (size_t)c->nm + 2;
+  if (need > c->adj_cap) {
+    if (c->adj) OHM_HIP(hipFree(c->adj));
+    OHM_HIP(hipMalloc(&c->adj, sizeof(double) * need));
+    c->adj_cap = need;
+  }
+  double *rhs = c->adj, *lam = rhs + n, *gg = lam + n, *gk = gg + n_res;
+  uint32_t *d_nodes = nullptr;
+  if (n_res > 0) OHM_HIP(hipMalloc(&d_nodes, sizeof(uint32_t) * 2 * n_res));
+  struct Guard {
+    uint32_t *p;
+    ~Guard() {
+      if (p) (void)hipFree(p);
+    }
+  } guard{d_nodes};
+  if (n_res > 0)
+    OHM_HIP(hipMemcpyAsync(d_nodes, res_nodes, sizeof(uint32_t) * 2 * n_res, hipMemcpyHostToDevice, st));
+  // dF/dx at the operating point in c->x
+  OHM_HIP(hipMemcpyAsync(c->A.val, c->base_val, sizeof(double) * c->nnz, hipMemcpyDeviceToDevice, st));
+  OHM_HIP(hipMemcpyAsync(c->rhs, c->base_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  enqueue_eval_assemble(c, c->x);
+  if (c->prec_kind < 0) {
+    int kind = matrix_has_zero_diag(b, c->A, nullptr) ? OHM_PREC_ISAI : OHM_PREC_JACOBI;
+    OHM_TRY(newton_prepare_precond(c, kind));
+  }
+  OHM_TRY(precond_refresh(b, c->A, c->P));
+  OHM_HIP(hipMemcpyAsync(rhs, dJdx, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  OHM_HIP(hipMemsetAsync(lam, 0, sizeof(double) * n, st));
+  ohm_bicgstab_params prm;
+  if (params)
+    prm = *params;
+  else {
+    ohm_bicgstab_default_params(&prm);
+    prm.tolerance = 1e-12;
+  }
+  BicgOpts o;
+  o.max_iters = prm.max_iterations;
+  o.tol = prm.tolerance;
+  o.shadow = prm.shadow;
+  o.check_interval = prm.check_interval;
+  o.scaled_breakdown = prm.breakdown != OHM_BREAKDOWN_ABSOLUTE;
+  BicgOut out;
+  OHM_HIP(hipEventRecord(b->ev0, st));
+  ohm_status s = bicg_solve_transpose(b, c->A, c->P, rhs, lam, o, out);
+  if (s == OHM_OK) {
+    if (n_res > 0)
+      hipLaunchKernelGGL(k_adj_conductance, dim3(grid_for(n_res)), dim3(kBlock), 0, st, (int)n_res,
+                         d_nodes, c->x, lam, gg);
+    if (c->nm > 0)
+      hipLaunchKernelGGL(k_adj_mosfet_kp, dim3(grid_for(c->nm)), dim3(kBlock), 0, st, c->nm, c->md,
+                         c->x, lam, gk);
+    b->dispatches += 2;
+    if (n_res > 0)
+      OHM_HIP(hipMemcpyAsync(grad_g, gg, sizeof(double) * n_res, hipMemcpyDeviceToHost, st));
+    if (grad_kp && c->nm > 0)
+      OHM_HIP(hipMemcpyAsync(grad_kp, gk, sizeof(double) * c->nm, hipMemcpyDeviceToHost, st));
+    if (lambda_out)
+      OHM_HIP(hipMemcpyAsync(lambda_out, lam, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  }
+  OHM_HIP(hipEventRecord(b->ev1, st));
+  OHM_HIP(hipEventSynchronize(b->ev1));
+  if (result) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    result->iterations = out.iters;
+    result->refinements = 0;
+    result->fp64_fallback = 0;
+    result->device_ms = ms;
+    result->dispatches = 0;
+    result->readbacks = out.polls;
+    result->residual_norm = NAN;
+  }
   return s;
 }
 

@@ -54,6 +54,12 @@ struct BicgOut {
 // FP64 solve on device vectors (rhs, x of length a.n).
 ohm_status bicg_solve_f64(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const double *rhs,
                           double *x, const BicgOpts &o, BicgOut &out);
+// FP64 solve of A^T x = rhs: explicit transpose of A (gather through the
+// transpose map) and the transposed preconditioner (Jacobi: the same
+// inverse diagonal; ISAI: (M_U M_L)^T = M_L^T M_U^T from transposes of the
+// two factors).
+ohm_status bicg_solve_transpose(ohm_backend *b, DevCsr &a, PrecondDev &p, const double *rhs,
+                                double *x, const BicgOpts &o, BicgOut &out);
 // FP32 inner BiCGSTAB inside FP64 iterative refinement.
 ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double *rhs, double *x,
                          const BicgOpts &o, double inner_tol, int64_t max_ref, BicgOut &out,

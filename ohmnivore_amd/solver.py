@@ -203,6 +203,16 @@ class HipBackend:
                                  ctypes.byref(res)))
         return SolveStats.from_c(res)
 
+    def bicgstab_transpose(self, a: Matrix, b: Buffer, x: Buffer,
+                           precond: Preconditioner | None = None,
+                           params: N.BicgstabParams | None = None) -> SolveStats:
+        """A^T x = b (adjoint systems) with the transposed preconditioner."""
+        res = N.BicgstabResult()
+        check(lib().ohm_bicgstab_transpose(self.h, a.h, b.h, x.h, precond.h if precond else None,
+                                           ctypes.byref(params) if params is not None else None,
+                                           ctypes.byref(res)))
+        return SolveStats.from_c(res)
+
     def sync(self):
         check(lib().ohm_backend_sync(self.h))
 
@@ -492,6 +502,8 @@ class NewtonContext:
         d, nd = _descs(diodes, ir.DIODE_DESC_DTYPE)
         q, nq = _descs(bjts, ir.BJT_DESC_DTYPE)
         m, nm = _descs(mosfets, ir.MOSFET_DESC_DTYPE)
+        self.n_mosfets = nm
+        self._descs_keep = (d, q, m)
         self.h = ctypes.c_void_p()
         check(lib().ohm_newton_create(
             backend.h, ptr(self._v, N.c_f64p), ptr(self._c, N.c_u32p), ptr(self._r, N.c_u32p),
@@ -527,6 +539,24 @@ class NewtonContext:
         check(lib().ohm_newton_assemble_at(self.h, ptr(xx, N.c_f64p), ptr(v, N.c_f64p),
                                            ptr(b, N.c_f64p)))
         return v, b
+
+    def adjoint(self, dJdx, res_nodes=None, params: N.BicgstabParams | None = None):
+        """ohm_adjoint_dc at the operating point of the last run: returns
+        (dObj/dg per resistor, dObj/dkp per MOSFET, lambda, SolveStats)."""
+        g = np.ascontiguousarray(dJdx, dtype=np.float64)
+        rn = (np.zeros((0, 2), np.uint32) if res_nodes is None
+              else np.ascontiguousarray(res_nodes, dtype=np.uint32).reshape(-1, 2))
+        gg = np.zeros(max(rn.shape[0], 1))
+        nm = int(getattr(self, "n_mosfets", 0))
+        gk = np.zeros(max(nm, 1))
+        lam = np.zeros(self.n)
+        res = N.BicgstabResult()
+        check(lib().ohm_adjoint_dc(self.h, ptr(g, N.c_f64p),
+                                   ctypes.byref(params) if params is not None else None,
+                                   ctypes.c_int64(rn.shape[0]), ptr(rn, N.c_u32p),
+                                   ptr(gg, N.c_f64p), ptr(gk, N.c_f64p), ptr(lam, N.c_f64p),
+                                   ctypes.byref(res)))
+        return gg[: rn.shape[0]], gk[:nm], lam, SolveStats.from_c(res)
 
     def limit(self, x_old, x_new):
         xo = np.ascontiguousarray(x_old, dtype=np.float64)
