@@ -215,6 +215,33 @@ ohm_status ohm_ac_sweep(ohm_backend *b, int64_t n, const uint32_t *g_rowptr,
                         const ohm_bicgstab_params *params, int32_t precond, double *x,
                         ohm_batch_result *results);
 
+/* ---- AC sweep over many circuits (the reference runs ac::run once per
+ * circuit). method OHM_AC_DIRECT: every (circuit, frequency) system by a
+ * banded LU with partial pivoting after a reverse Cuthill-McKee ordering
+ * (circuits whose half bandwidth is at most 4: ladders, chains), one system
+ * per GPU lane; OHM_AC_ITERATIVE: ohm_ac_sweep per circuit; OHM_AC_AUTO:
+ * direct where the band allows, iterative elsewhere. Each circuit's x
+ * (n_freqs * 2n doubles, may be NULL) receives its solutions; results has
+ * n_circuits * n_freqs entries (circuit-major) with the FP64 residual
+ * ||b - A x|| / ||b|| of every system. device_ms (may be NULL): kernel time
+ * of the direct part. OHM_AC_DIRECT_BUDGET_GB bounds its working memory
+ * (default 64). */
+enum { OHM_AC_AUTO = 0, OHM_AC_ITERATIVE = 1, OHM_AC_DIRECT = 2 };
+typedef struct {
+  int64_t n;
+  const uint32_t *g_rowptr, *g_col;
+  const double *g_val;
+  const uint32_t *c_rowptr, *c_col; /* may differ from G's pattern; NULL c_val = no C */
+  const double *c_val;
+  const double *b_ac;               /* 2n doubles */
+  double *x;                        /* out: n_freqs * 2n doubles, or NULL */
+} ohm_ac_circuit;
+ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits,
+                                 const ohm_ac_circuit *circuits, int64_t n_freqs,
+                                 const double *freqs, int32_t method,
+                                 const ohm_bicgstab_params *params, int32_t precond,
+                                 ohm_batch_result *results, double *device_ms);
+
 /* generate_frequencies (src/analysis/ac.rs:80-109). Writes up to `cap`
  * points and returns how many the sweep has. */
 enum { OHM_SWEEP_DEC = 0, OHM_SWEEP_OCT = 1, OHM_SWEEP_LIN = 2 };

@@ -84,6 +84,13 @@ class BatchResult(ctypes.Structure):
 
 
 SWEEP_DEC, SWEEP_OCT, SWEEP_LIN = 0, 1, 2
+AC_AUTO, AC_ITERATIVE, AC_DIRECT = 0, 1, 2
+
+
+class AcCircuit(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("g_rowptr", c_u32p), ("g_col", c_u32p),
+                ("g_val", c_f64p), ("c_rowptr", c_u32p), ("c_col", c_u32p), ("c_val", c_f64p),
+                ("b_ac", c_f64p), ("x", c_f64p)]
 
 _lib = None
 

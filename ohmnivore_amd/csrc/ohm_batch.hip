@@ -1597,8 +1597,10 @@ ohm_status ohm_solve_complex(ohm_backend *b, size_t n, size_t nnz, const double 
   return OHM_OK;
 }
 
+}  // extern "C"
+
 // Union of two CSR patterns with G and C values scattered onto it.
-static ohm_status union_pattern(int64_t n, const uint32_t *grp, const uint32_t *gcl,
+ohm_status ohm::csr_union(int64_t n, const uint32_t *grp, const uint32_t *gcl,
                                 const double *gv, const uint32_t *crp, const uint32_t *ccl,
                                 const double *cv, std::vector<uint32_t> &rp,
                                 std::vector<uint32_t> &cl, std::vector<double> &g,
@@ -1633,6 +1635,8 @@ static ohm_status union_pattern(int64_t n, const uint32_t *grp, const uint32_t *
   return OHM_OK;
 }
 
+extern "C" {
+
 ohm_status ohm_ac_sweep(ohm_backend *b, int64_t n, const uint32_t *g_rowptr,
                         const uint32_t *g_col, const double *g_val, const uint32_t *c_rowptr,
                         const uint32_t *c_col, const double *c_val, const double *b_ac,
@@ -1646,8 +1650,8 @@ ohm_status ohm_ac_sweep(ohm_backend *b, int64_t n, const uint32_t *g_rowptr,
     return fail(OHM_ERR_INVALID, "null argument");
   std::vector<uint32_t> rp, cl;
   std::vector<double> g, c;
-  OHM_TRY(union_pattern(n, g_rowptr, g_col, g_val, c_val ? c_rowptr : nullptr, c_col, c_val, rp,
-                        cl, g, c));
+  OHM_TRY(csr_union(n, g_rowptr, g_col, g_val, c_val ? c_rowptr : nullptr, c_col, c_val, rp,
+                    cl, g, c));
   const int64_t per = batch > 0 ? std::min(batch, n_freqs) : n_freqs;
   ohm_status first = OHM_OK;
   std::string msg;

@@ -139,6 +139,12 @@ ohm_status csr_ensure_transpose(ohm_backend *b, DevCsr &m);
 // (shares the pattern arrays of m; never freed on its own).
 ohm_status csr_transpose_view(ohm_backend *b, DevCsr &m, DevCsr &view);
 ohm_status ensure_partials(ohm_backend *b, size_t count);
+// Union of the patterns of G and C (rows sorted by column, duplicates
+// summed) with both value arrays scattered onto it.
+ohm_status csr_union(int64_t n, const uint32_t *grp, const uint32_t *gcl, const double *gv,
+                     const uint32_t *crp, const uint32_t *ccl, const double *cv,
+                     std::vector<uint32_t> &rp, std::vector<uint32_t> &cl, std::vector<double> &g,
+                     std::vector<double> &c);
 // Drop every cached graph (called whenever device objects are freed).
 void graph_cache_clear(ohm_backend *b);
 // Backends not yet destroyed; objects whose owner is gone must not touch it.

@@ -448,6 +448,54 @@ def ac_sweep(backend: HipBackend, mna: ir.MnaSystem, freqs, batch: int = 0,
     return AcResult(f, nodes, branches, x, stats)
 
 
+AC_METHODS = {"auto": N.AC_AUTO, "iterative": N.AC_ITERATIVE, "direct": N.AC_DIRECT}
+
+
+@dataclass
+class AcSweepResult:
+    x: list                      # per circuit: [n_freqs, n] complex (None when not kept)
+    stats: list                  # per circuit: list of BatchStats per frequency
+    device_ms: float
+
+
+def ac_sweep_circuits(backend: HipBackend, mnas, freqs, method: str = "auto",
+                      params: N.BicgstabParams | None = None, precond: int = N.PREC_JACOBI,
+                      keep_x: bool = True, raise_on_failure: bool = True) -> AcSweepResult:
+    """ac::run over many circuits at once (ohm_ac_sweep_circuits): banded
+    direct solves for chain-like circuits, batched BiCGSTAB for the rest."""
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    nf = f.shape[0]
+    k = len(mnas)
+    cs = (N.AcCircuit * max(k, 1))()
+    keep, xs = [], []
+    for i, m in enumerate(mnas):
+        arrs = [np.ascontiguousarray(m.g.row_pointers, np.uint32),
+                np.ascontiguousarray(m.g.col_indices, np.uint32),
+                np.ascontiguousarray(m.g.values, np.float64),
+                np.ascontiguousarray(m.c.row_pointers, np.uint32),
+                np.ascontiguousarray(m.c.col_indices, np.uint32),
+                np.ascontiguousarray(m.c.values, np.float64),
+                np.ascontiguousarray(m.b_ac, np.complex128)]
+        keep.append(arrs)
+        x = np.zeros((nf, m.size), dtype=np.complex128) if keep_x else None
+        xs.append(x)
+        cs[i] = N.AcCircuit(m.size, ptr(arrs[0], N.c_u32p), ptr(arrs[1], N.c_u32p),
+                            ptr(arrs[2], N.c_f64p), ptr(arrs[3], N.c_u32p), ptr(arrs[4], N.c_u32p),
+                            ptr(arrs[5], N.c_f64p), arrs[6].ctypes.data_as(N.c_f64p),
+                            x.ctypes.data_as(N.c_f64p) if x is not None else None)
+    res = (N.BatchResult * max(k * nf, 1))()
+    ms = ctypes.c_double(0.0)
+    prm = params if params is not None else bicgstab_params()
+    rc = lib().ohm_ac_sweep_circuits(backend.h, ctypes.c_int64(k), cs, ctypes.c_int64(nf),
+                                     ptr(f, N.c_f64p), AC_METHODS[method], ctypes.byref(prm),
+                                     int(precond), res, ctypes.byref(ms))
+    stats = [[BatchStats(r.iterations, r.refinements, r.residual_norm, r.status, r.reason)
+              for r in res[i * nf:(i + 1) * nf]] for i in range(k)]
+    if rc != 0 and raise_on_failure:
+        check(rc)
+    return AcSweepResult(xs, stats, ms.value)
+
+
 # ----------------------------------------------------------------- Newton --
 @dataclass
 class NewtonStats:

@@ -240,3 +240,77 @@ def test_ac_short_rlc_ladders_vs_lu(be, batch):
                              g.col_indices, g.row_pointers)
             ok, err = close(res.x[i], oracle.sparse_lu_solve(a, mna.b_ac))
             assert ok, (fr, err, res.stats[i])
+
+
+# ------------------------------------------------ AC: banded direct sweep --
+def c_on_g_matrix(mna):
+    g = mna.g
+    c_on_g = np.zeros(g.values.shape[0])
+    for r in range(mna.size):
+        for q in range(mna.c.row_pointers[r], mna.c.row_pointers[r + 1]):
+            c_on_g[g.value_index(r, int(mna.c.col_indices[q]))] += mna.c.values[q]
+    return c_on_g
+
+
+def test_ac_direct_rlc_ladders_vs_lu(be):
+    """The AC workload's circuit family (uneven RLC ladders, 101 points
+    1 Hz - 1 GHz) at reduced section counts: every system against the
+    oracle's complex LU, where Jacobi-BiCGSTAB cannot converge."""
+    ckts, freqs = ir.rlc_sweep(n_circuits=5, lo=20, hi=3000, seed=4)
+    mnas = [solver.compile(c) for c in ckts]
+    res = solver.ac_sweep_circuits(be, mnas, freqs, method="direct")
+    for m, x, st in zip(mnas, res.x, res.stats):
+        cg = c_on_g_matrix(m)
+        assert all(s.converged and s.residual_norm < 1e-12 for s in st)
+        for i in range(0, 101, 4):
+            a = ir.CsrMatrix(m.size, m.size, m.g.values + 2j * np.pi * freqs[i] * cg,
+                             m.g.col_indices, m.g.row_pointers)
+            ok, err = close(x[i], oracle.sparse_lu_solve(a, m.b_ac))
+            assert ok, (m.size, freqs[i], err)
+
+
+def test_ac_auto_mixes_direct_and_iterative(be):
+    """AUTO: banded circuits go direct, a mesh (wide band) goes to BiCGSTAB."""
+    ckts, freqs = ir.rlc_sweep(n_circuits=2, lo=10, hi=40, seed=4)
+    mesh = ir.Circuit()
+    mesh.vsource("V1", "1", "0", ac=(1.0, 0.0))
+    for r in range(12):
+        for c in range(12):
+            a = f"n{r}_{c}" if r or c else "1"
+            if c < 11:
+                mesh.resistor(f"Rh{r}_{c}", a, f"n{r}_{c + 1}", 100.0 + r + c)
+            if r < 11:
+                mesh.resistor(f"Rv{r}_{c}", a, f"n{r + 1}_{c}", 120.0 + r * c)
+            mesh.capacitor(f"C{r}_{c}", a, "0", 1e-12)
+    mnas = [solver.compile(c) for c in ckts] + [solver.compile(mesh)]
+    f = freqs[::10]
+    res = solver.ac_sweep_circuits(be, mnas, f, method="auto",
+                                   params=solver.bicgstab_params(tolerance=1e-13,
+                                                                 max_iterations=20000))
+    for m, x in zip(mnas, res.x):
+        cg = c_on_g_matrix(m)
+        for i, fr in enumerate(f):
+            a = ir.CsrMatrix(m.size, m.size, m.g.values + 2j * np.pi * fr * cg, m.g.col_indices,
+                             m.g.row_pointers)
+            ok, err = close(x[i], oracle.sparse_lu_solve(a, m.b_ac))
+            assert ok, (m.size, fr, err)
+    assert res.stats[-1][0].iterations > 0 and res.stats[0][0].iterations == 0
+
+
+def test_ac_direct_full_size_ladders(be):
+    """Full-size ladders of the AC workload (the longest of the 256): every
+    one of the 101 points has FP64 residual < 1e-12, and sampled points
+    match the oracle's LU."""
+    ckts, freqs = ir.rlc_sweep(n_circuits=256, seed=4)
+    sizes = [c.n_components for c in ckts]
+    pick = [int(np.argmax(sizes)), int(np.argmin(sizes))]
+    mnas = [solver.compile(ckts[i]) for i in pick]
+    res = solver.ac_sweep_circuits(be, mnas, freqs, method="direct")
+    for m, x, st in zip(mnas, res.x, res.stats):
+        assert all(s.converged and s.residual_norm < 1e-12 for s in st)
+        cg = c_on_g_matrix(m)
+        for i in (0, 37, 100):
+            a = ir.CsrMatrix(m.size, m.size, m.g.values + 2j * np.pi * freqs[i] * cg,
+                             m.g.col_indices, m.g.row_pointers)
+            ok, err = close(x[i], oracle.sparse_lu_solve(a, m.b_ac))
+            assert ok, (m.size, freqs[i], err)
