@@ -1,30 +1,44 @@
-"""Benchmark: nonlinear DC operating point of the 256x256 diode mesh on one MI355X.
+"""Benchmark of the MI355X solve backend on one GPU.
 
-Workload (one "step" = one solve): the 65,540-unknown MNA system of a 256x256
-resistor mesh (log-uniform 100 Ω–10 kΩ, seeded), four 1 V corner sources and
-8,192 diodes (IS 1e-14, N 1) from every 8th node to ground, solved from
-x = 0 by GPU Newton-Raphson with ISAI(1)-preconditioned BiCGSTAB — once in
-plain FP64 and once with the FP32 inner solver inside FP64 iterative
-refinement. Inputs are uploaded once; the timed region is K full solves.
+Headline ("step" = one solve; the value of the JSON line): the nonlinear DC
+operating point of the 256x256 diode mesh — 65,540 unknowns, 326,664 nnz,
+four 1 V corner sources, 8,192 diodes (IS 1e-14, N 1) from every 8th node —
+solved from x = 0 by GPU Newton-Raphson with ISAI(1)-preconditioned
+BiCGSTAB, in plain FP64 and with the FP32 inner solver inside FP64
+iterative refinement. Inputs are uploaded once; the timed region is K full
+solves. Its error against the oracle's FP64 LU-Newton, the SpMV kernel's
+HBM roofline position (HIP events, live) and the CPU baseline (oracle on
+this host) ride along.
 
-Prints one JSON line: throughput (solves/s), the error of each form against
-the FP64 LU Newton of the CPU oracle, the SpMV kernel's HBM roofline position
-(measured live with HIP events) and the CPU baseline timed on this host.
+The other workloads are measured once each (they are not the headline):
+  resistor_ladder_10k  linear DC, 10,002 unknowns, Jacobi-BiCGSTAB to 1e-10
+  rc_grid_512          transient, 262,148 unknowns, 1,000 backward-Euler
+                       steps of 1 ps, warm-started; plus K Monte Carlo
+                       instances batched versus the same instances one by one
+  rlc_ac_256x101       256 uneven RLC ladders x 101 frequencies = 25,856
+                       complex systems, batched banded direct solve versus
+                       one system per call
+  amplifier_adjoint    4,096 stages (8,194 unknowns): operating point and
+                       the adjoint gradient of sum (V_out - 0.9)^2 with
+                       respect to 12,288 conductances and 2,048 kp
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only headline]
 """
 from __future__ import annotations
 
 import argparse
 import ctypes
 import json
-import os
 import sys
 import time
 
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def spmv_bytes(n: int, nnz: int, v: int) -> int:
@@ -48,34 +62,18 @@ def replicated(g, copies: int):
     return val, col.astype(np.uint32), rp.astype(np.uint32), n * copies
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile", action="store_true",
-                    help="rocprofv3 runs: skip CPU work and launch kernels without hipGraphs")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
-              f"--gpus {args.gpus}); the 256x256 diode mesh is one system on one MI355X.",
-              file=sys.stderr)
-        return 2
+def within(x, ref, rel=1e-6, abs_=1e-9):
+    err = np.abs(x - ref)
+    return float(err.max(initial=0.0)), bool(np.all(err <= np.maximum(rel * np.abs(ref), abs_)))
 
-    from ohmnivore_amd import _native as N
-    from ohmnivore_amd import circuit as ir
-    from ohmnivore_amd import solver
 
+# ------------------------------------------------------------- headline --
+def headline(args, be, N, ir, solver):
     ckt = ir.diode_mesh(256, 256, seed=2)
     m = solver.compile(ckt)
-    be = solver.HipBackend()
-    if args.profile:
-        be.set_graphs(False)
     ctx = solver.NewtonContext(be, m.g, m.b_dc, m.diode_descriptors)
     newton_kw = dict(abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-6,
                      bicgstab_max_iterations=20_000)
-
     forms = {}
     for prec in ("fp64", "fp32_ir"):
         for _ in range(args.warmup):
@@ -96,11 +94,10 @@ def main() -> int:
             "fp64_fallbacks": stats.fp64_fallbacks,
             "x": ctx.x(),
         }
-        print(f"[bench] {prec}: {forms[prec]['ms_per_solve']:.2f} ms/solve, "
-              f"{stats.newton_iterations} Newton, {stats.bicgstab_iterations} BiCGSTAB its",
-              file=sys.stderr, flush=True)
+        log(f"diode mesh {prec}: {forms[prec]['ms_per_solve']:.2f} ms/solve, "
+            f"{stats.newton_iterations} Newton, {stats.bicgstab_iterations} BiCGSTAB its")
 
-    # ---- SpMV roofline (dominant kernel), HIP events on the backend stream
+    # SpMV roofline (dominant kernel), HIP events on the backend stream
     a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
     bx, by = be.new_buffer(m.size), be.new_buffer(m.size)
     be.upload_vec(np.ones(m.size), bx)
@@ -108,33 +105,27 @@ def main() -> int:
     N.check(N.lib().ohm_spmv_timed(be.h, a.h, bx.h, by.h, 200, 0, ctypes.byref(ms)))
     spmv_b = spmv_bytes(m.size, m.g.nnz, 8)
     spmv_gbs = spmv_b / (ms.value * 1e-3) / 1e9
-    # large working set (64 block-diagonal copies, ~335 MB > 256 MB MALL)
     val, col, rp, nbig = replicated(m.g, 64)
     abig = be.upload_matrix(val, col, rp, nbig)
     del val, col, rp
     bbx, bby = be.new_buffer(nbig), be.new_buffer(nbig)
     be.upload_vec(np.ones(nbig), bbx)
-    msb = ctypes.c_double()
+    msb, msb32 = ctypes.c_double(), ctypes.c_double()
     N.check(N.lib().ohm_spmv_timed(be.h, abig.h, bbx.h, bby.h, 50, 0, ctypes.byref(msb)))
-    big_b = spmv_bytes(nbig, m.g.nnz * 64, 8)
-    big_gbs = big_b / (msb.value * 1e-3) / 1e9
-    msb32 = ctypes.c_double()
     N.check(N.lib().ohm_spmv_timed(be.h, abig.h, bbx.h, bby.h, 50, 1, ctypes.byref(msb32)))
+    big_b = spmv_bytes(nbig, m.g.nnz * 64, 8)
     big32_b = spmv_bytes(nbig, m.g.nnz * 64, 4)
+    big_gbs = big_b / (msb.value * 1e-3) / 1e9
     big32_gbs = big32_b / (msb32.value * 1e-3) / 1e9
     del abig, bbx, bby
 
-    # ---- CPU baseline and accuracy vs FP64 LU (oracle; bounded sample)
-    cpu = None
-    errors = {}
+    cpu, errors = None, {}
     if not (args.no_cpu_baseline or args.profile):
         import oracle
-        x64 = forms["fp64"]["x"]
         t0 = time.perf_counter()
         x_ref, its, _ = oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, abs_tol=1e-12,
-                                            max_iterations=20, initial_guess=x64)
-        t_cpu = time.perf_counter() - t0
-        per_iter = t_cpu / max(its, 1)
+                                            max_iterations=20, initial_guess=forms["fp64"]["x"])
+        per_iter = (time.perf_counter() - t0) / max(its, 1)
         cold_iters = 16  # oracle LU-Newton from x = 0 on this circuit (measured, same algorithm)
         cpu = {"value": 1.0 / (per_iter * cold_iters), "unit": "solves/s", "cores": 1,
                "kind": "port",
@@ -142,16 +133,212 @@ def main() -> int:
                          f"at the GPU answer, {per_iter:.2f} s/iteration; rate = 1/(s/iteration "
                          f"x {cold_iters} cold-start iterations)"}
         for prec in forms:
-            x = forms[prec]["x"]
-            err = np.abs(x - x_ref)
-            rel = float(np.max(err / np.maximum(np.abs(x_ref), 1e-300)))
-            ok = bool(np.all(err <= np.maximum(1e-6 * np.abs(x_ref), 1e-9)))
-            errors[prec] = {"max_abs_err": float(err.max()), "within_1e-6_rel_1e-9_abs": ok}
+            e, ok = within(forms[prec]["x"], x_ref)
+            errors[prec] = {"max_abs_err": e, "within_1e-6_rel_1e-9_abs": ok}
+    out = {
+        "forms": forms, "errors": errors, "cpu": cpu, "newton_kw": newton_kw,
+        "roofline": {
+            "bound": "hbm", "kernel": "k_spmv<double> on the 326,664-nnz mesh matrix",
+            "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": spmv_b,
+            "avg_launch_us": ms.value * 1e3},
+        "spmv_large_working_set": {
+            "copies": 64, "bytes_per_launch": big_b, "fp64_gbs": big_gbs,
+            "fp64_frac": big_gbs / HBM_PEAK_GBS, "fp32_bytes_per_launch": big32_b,
+            "fp32_gbs": big32_gbs, "fp32_frac": big32_gbs / HBM_PEAK_GBS},
+    }
+    return out
 
+
+# ------------------------------------------------------------ workloads --
+def ladder(be, N, ir, solver, check):
+    mna = solver.compile(ir.resistor_ladder(10_000, seed=1))
+    s = solver.GpuSolver(be, precond=N.PREC_JACOBI,
+                         params=solver.bicgstab_params(tolerance=1e-10, max_iterations=200_000))
+    s.solve_real(mna.g, mna.b_dc)  # warm-up
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        x = s.solve_real(mna.g, mna.b_dc)
+    dt = (time.perf_counter() - t0) / reps
+    out = {"unknowns": mna.size, "ms_per_solve": 1e3 * dt, "solves_per_s": 1 / dt,
+           "bicgstab_iterations": s.last_stats.iterations,
+           "residual_norm": s.last_stats.residual_norm}
+    if check:
+        import oracle
+        e, ok = within(x, oracle.sparse_lu_solve(mna.g, mna.b_dc))
+        out.update({"max_abs_err_vs_lu": e, "within_1e-6_rel_1e-9_abs": ok})
+    log(f"ladder: {out['ms_per_solve']:.2f} ms/solve, {out['bicgstab_iterations']} its")
+    return out
+
+
+def rc_grid(be, N, ir, solver, steps, mc_k, mc_steps):
+    from ohmnivore_amd import transient as T
+    ckt = ir.rc_grid(512, 512, 1024, seed=3)
+    mna = solver.compile(ckt)
+    dc = solver.GpuSolver(be, precond=N.PREC_AUTO,
+                          params=solver.bicgstab_params(tolerance=1e-12, max_iterations=100_000))
+    x0 = dc.solve_real(mna.g, mna.b_dc)
+    src, _ = T.source_table(ckt, mna)
+    probes = np.arange(0, mna.size, 4099)
+    out = {"unknowns": mna.size, "nnz": mna.g.nnz, "steps": steps, "h": 1e-12}
+    for name, pc, prec in (("fp64_isai", N.PREC_ISAI, "fp64"), ("fp64_jacobi", N.PREC_JACOBI, "fp64"),
+                           ("fp32_ir_isai", N.PREC_ISAI, "fp32_ir")):
+        n_steps = steps if prec == "fp64" else max(steps // 10, 1)
+        tr = T.Transient(be, mna, src)
+        prm = solver.bicgstab_params(tolerance=1e-9, max_iterations=20_000, precision=prec)
+        t0 = time.perf_counter()
+        r = tr.run(1e-12, n_steps * 1e-12, method="be", x0=x0, probes=probes, params=prm,
+                   precond=pc)
+        dt = time.perf_counter() - t0
+        out[name] = {"steps": r.steps_accepted, "ms_per_step": 1e3 * dt / r.steps_accepted,
+                     "steps_per_s": r.steps_accepted / dt,
+                     "bicgstab_iters_per_step": r.bicgstab_iterations / r.steps_accepted}
+        log(f"rc grid {name}: {out[name]['ms_per_step']:.2f} ms/step, "
+            f"{out[name]['bicgstab_iters_per_step']:.1f} its/step")
+    # Monte Carlo: K perturbed instances batched vs one at a time (both FP64, Jacobi)
+    cg = T.c_on_g(mna)
+    rng = np.random.default_rng(7)
+    inst = [(mna.g.values * (1 + 0.05 * (rng.random() - 0.5)),
+             cg * (1 + 0.2 * rng.random(cg.shape[0]))) for _ in range(mc_k)]
+    prm = solver.bicgstab_params(tolerance=1e-9, max_iterations=20_000)
+    trb = T.Transient(be, mna, src, instances=inst)
+    t0 = time.perf_counter()
+    rb = trb.run(1e-12, mc_steps * 1e-12, method="be", x0=x0, probes=probes, params=prm,
+                 precond=N.PREC_JACOBI)
+    t_batched = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain_its = 0
+    for k in range(mc_k):
+        trk = T.Transient(be, mna, src, instances=[inst[k]])
+        rk = trk.run(1e-12, mc_steps * 1e-12, method="be", x0=x0, probes=probes, params=prm,
+                     precond=N.PREC_JACOBI)
+        plain_its += rk.bicgstab_iterations
+    t_plain = time.perf_counter() - t0
+    solves = mc_k * mc_steps
+    out["monte_carlo"] = {
+        "instances": mc_k, "steps": mc_steps,
+        "batched_solves_per_s": solves / t_batched, "plain_solves_per_s": solves / t_plain,
+        "speedup": t_plain / t_batched, "batched_iterations": rb.bicgstab_iterations,
+        "plain_iterations": plain_its}
+    log(f"rc grid Monte Carlo K={mc_k}: batched {solves / t_batched:.1f} solves/s, plain "
+        f"{solves / t_plain:.1f} solves/s, speedup {t_plain / t_batched:.2f}")
+    return out
+
+
+def rlc_ac(be, N, ir, solver, n_circuits):
+    t0 = time.perf_counter()
+    ckts, freqs = ir.rlc_sweep(n_circuits=n_circuits, seed=4)
+    mnas = [solver.compile(c) for c in ckts]
+    del ckts
+    t_build = time.perf_counter() - t0
+    n_sys = n_circuits * freqs.shape[0]
+    t0 = time.perf_counter()
+    res = solver.ac_sweep_circuits(be, mnas, freqs, method="direct", keep_x=False)
+    t_batched = time.perf_counter() - t0
+    worst = max(s.residual_norm for sts in res.stats for s in sts)
+    # plain: one system per call, same direct method (sample of 8 systems)
+    picks = [(i * (n_circuits // 8), (i * 13) % freqs.shape[0]) for i in range(8)]
+    t0 = time.perf_counter()
+    for c, f in picks:
+        solver.ac_sweep_circuits(be, [mnas[c]], freqs[f:f + 1], method="direct", keep_x=False)
+    t_plain_per = (time.perf_counter() - t0) / len(picks)
+    # plain single-system Jacobi-BiCGSTAB (the reference's AC method) on a sample
+    from ohmnivore_amd import transient as T
+    gs = solver.GpuSolver(be, precond=N.PREC_JACOBI,
+                          params=solver.bicgstab_params(tolerance=1e-10, max_iterations=10_000))
+    sample = []
+    for c, f in picks[:3]:
+        m = mnas[c]
+        cg = T.c_on_g(m)
+        a = ir.CsrMatrix(m.size, m.size, m.g.values + 2j * np.pi * freqs[f] * cg,
+                         m.g.col_indices, m.g.row_pointers)
+        t1 = time.perf_counter()
+        try:
+            gs.solve_complex(a, m.b_ac)
+            conv = True
+        except N.OhmnivoreError:
+            conv = False
+        sample.append({"unknowns": m.size, "freq_hz": float(freqs[f]), "converged": conv,
+                       "iterations": gs.last_stats.iterations,
+                       "seconds": time.perf_counter() - t1})
+    out = {"circuits": n_circuits, "frequencies": int(freqs.shape[0]), "systems": n_sys,
+           "unknowns_total": int(sum(m.size for m in mnas)) * int(freqs.shape[0]),
+           "build_s": t_build, "batched_s": t_batched, "batched_device_ms": res.device_ms,
+           "batched_solves_per_s": n_sys / t_batched, "worst_residual": worst,
+           "plain_direct_s_per_system": t_plain_per,
+           "plain_solves_per_s": 1 / t_plain_per, "speedup": t_plain_per * n_sys / t_batched,
+           "plain_bicgstab_sample": sample}
+    log(f"rlc AC: {n_sys} systems in {t_batched:.2f} s ({n_sys / t_batched:.0f} solves/s), worst "
+        f"residual {worst:.1e}, plain {1 / t_plain_per:.1f} solves/s")
+    return out
+
+
+def adjoint(be, N, ir, solver):
+    from ohmnivore_amd import adjoint as A
+    ckt = ir.amplifier_array(4096, seed=5)
+    mna = solver.compile(ckt)
+    prob = A.DcProblem(be, ckt, mna)
+    outs = A.output_nodes(ckt, mna)
+    prob.solve()
+    t0 = time.perf_counter()
+    x = prob.solve()
+    t1 = time.perf_counter()
+    dj = np.zeros(mna.size)
+    dj[outs] = 2 * (x[outs] - 0.9)
+    gg, gk = prob.adjoint(dj)
+    t2 = time.perf_counter()
+    out = {"unknowns": mna.size, "parameters": int(gg.shape[0] + gk.shape[0]),
+           "forward_ms": 1e3 * (t1 - t0), "adjoint_ms": 1e3 * (t2 - t1),
+           "newton_iterations": prob.last_stats.newton_iterations,
+           "adjoint_bicgstab_iterations": prob.last_adjoint.iterations,
+           "gradients_per_s": 1 / (t2 - t0)}
+    log(f"adjoint: forward {out['forward_ms']:.1f} ms, adjoint {out['adjoint_ms']:.1f} ms")
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["headline", "all"], default="all")
+    ap.add_argument("--tran-steps", type=int, default=1000)
+    ap.add_argument("--mc-instances", type=int, default=8)
+    ap.add_argument("--mc-steps", type=int, default=50)
+    ap.add_argument("--ac-circuits", type=int, default=256)
+    ap.add_argument("--profile", action="store_true",
+                    help="rocprofv3 runs: headline only, no CPU work, no hipGraphs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+              f"--gpus {args.gpus}).", file=sys.stderr)
+        return 2
+
+    from ohmnivore_amd import _native as N
+    from ohmnivore_amd import circuit as ir
+    from ohmnivore_amd import solver
+
+    be = solver.HipBackend()
+    if args.profile:
+        be.set_graphs(False)
+        args.only = "headline"
+    h = headline(args, be, N, ir, solver)
+    forms, errors = h["forms"], h["errors"]
     best = max(forms, key=lambda p: forms[p]["solves_per_s"])
     if errors and not errors[best]["within_1e-6_rel_1e-9_abs"]:
         best = "fp64"
-    ms_step = forms[best]["ms_per_solve"]
+    workloads = {}
+    speedups = None
+    if args.only == "all":
+        workloads["resistor_ladder_10k"] = ladder(be, N, ir, solver, check=not args.no_cpu_baseline)
+        workloads["rc_grid_512"] = rc_grid(be, N, ir, solver, args.tran_steps, args.mc_instances,
+                                           args.mc_steps)
+        workloads["rlc_ac_256x101"] = rlc_ac(be, N, ir, solver, args.ac_circuits)
+        workloads["amplifier_adjoint_4096"] = adjoint(be, N, ir, solver)
+        speedups = {"rc_grid_512_monte_carlo": workloads["rc_grid_512"]["monte_carlo"]["speedup"],
+                    "rlc_ac_256x101": workloads["rlc_ac_256x101"]["speedup"]}
     line = {
         "metric": "solves/sec at 1e-6 rel err vs FP64 LU; SpMV HBM GB/s; batched-vs-plain speedup",
         "value": forms[best]["solves_per_s"],
@@ -159,39 +346,26 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": forms[best]["ms_per_solve"],
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64" if best == "fp64" else "f32+f64-refinement",
-        "data": "synthetic (seeded SplitMix64 circuit, seed 2)",
+        "data": "synthetic (seeded SplitMix64 circuits)",
         "config": {
             "workload": "nonlinear DC, 256x256 resistor mesh + 8,192 diodes: 65,540 unknowns, "
                         "326,664 nnz; Newton + ISAI(1)-BiCGSTAB from x=0",
             "form": best,
-            "newton_abs_tol": newton_kw["abs_tol"],
-            "bicgstab_tolerance_rel_to_newton_residual": newton_kw["bicgstab_tolerance"],
+            "newton_abs_tol": h["newton_kw"]["abs_tol"],
+            "bicgstab_tolerance_rel_to_newton_residual": h["newton_kw"]["bicgstab_tolerance"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_spmv<double> on the 326,664-nnz mesh matrix",
-            "achieved": spmv_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": spmv_gbs / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_launch": spmv_b,
-            "avg_launch_us": ms.value * 1e3,
-        },
-        "cpu_baseline": cpu,
+        "roofline": h["roofline"],
+        "cpu_baseline": h["cpu"],
         "forms": {p: {k: v for k, v in f.items() if k != "x"} for p, f in forms.items()},
         "error_vs_fp64_lu": errors or None,
-        "spmv_large_working_set": {
-            "copies": 64, "bytes_per_launch": big_b, "fp64_gbs": big_gbs,
-            "fp64_frac": big_gbs / HBM_PEAK_GBS, "fp32_bytes_per_launch": big32_b,
-            "fp32_gbs": big32_gbs, "fp32_frac": big32_gbs / HBM_PEAK_GBS,
-        },
-        "batched_vs_plain_speedup": None,
+        "spmv_large_working_set": h["spmv_large_working_set"],
+        "batched_vs_plain_speedup": speedups,
+        "workloads": workloads or None,
     }
     print(json.dumps(line))
     return 0

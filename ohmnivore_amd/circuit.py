@@ -135,6 +135,15 @@ class Circuit:
     def nodes(self, names) -> np.ndarray:
         return np.fromiter((self.node(n) for n in names), dtype=np.int32)
 
+    def fresh_nodes(self, names: list) -> np.ndarray:
+        """Register names that are known to be new, in order (bulk form of node())."""
+        start = len(self.node_names)
+        self.node_names.extend(names)
+        self._node_ids.update(zip(names, range(start, start + len(names))))
+        if len(self._node_ids) != len(self.node_names):
+            raise ValueError("fresh_nodes: a name was already registered")
+        return np.arange(start, start + len(names), dtype=np.int32)
+
     def add_diode_model(self, name: str, is_: float = 1e-14, n: float = 1.0) -> None:
         self.diode_models.append((name, is_, n))
 
@@ -288,8 +297,8 @@ class CCircuit:
         n = len(self.comps)
         self.node_names = (ctypes.c_char_p * max(len(ckt.node_names), 1))(
             *[s.encode() for s in ckt.node_names])
-        # Names are only read by error messages; skip them for very large circuits.
-        if with_names and n <= 200_000:
+        # Names are only read by error messages; skip them for large circuits.
+        if with_names and n <= 50_000:
             self.comp_names = (ctypes.c_char_p * max(n, 1))(
                 *[s.encode() for s in ckt.component_names()])
         else:
@@ -527,9 +536,10 @@ def rlc_ladder(sections: int, rng: SplitMix64, tag: str = "") -> Circuit:
     """One AC ladder: V in 0 AC 1; per section R a_s b_s, L b_s a_{s+1}, C a_{s+1} 0."""
     ckt = Circuit()
     ckt.vsource("V1", f"a{tag}1", "0", ac=(1.0, 0.0))
+    a1 = ckt.node(f"a{tag}1")
     u = rng.uniform(3 * sections).reshape(sections, 3)
-    a = ckt.nodes(f"a{tag}{s}" for s in range(1, sections + 2))
-    b = ckt.nodes(f"b{tag}{s}" for s in range(1, sections + 1))
+    a = np.concatenate([[a1], ckt.fresh_nodes([f"a{tag}{s}" for s in range(2, sections + 2)])])
+    b = ckt.fresh_nodes([f"b{tag}{s}" for s in range(1, sections + 1)])
     gnd = ckt.node("0")
     comps = np.zeros(3 * sections, dtype=COMPONENT_DTYPE)
     comps["kind"][0::3], comps["kind"][1::3], comps["kind"][2::3] = R, L, C

@@ -1841,6 +1841,14 @@ __global__ void k_maxdiff(int64_t n, const double *__restrict__ a, const double 
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// a = g + sigma c (the single-instance companion matrix values)
+__global__ void k_axpby_vals(int64_t nnz, const double *__restrict__ g,
+                             const double *__restrict__ c, double sigma, double *__restrict__ a) {
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * kBlock)
+    a[k] = c ? g[k] + sigma * c[k] : g[k];
+}
+
 }  // namespace ohm
 
 struct ohm_transient {
@@ -1857,6 +1865,14 @@ struct ohm_transient {
   int64_t rec_cap = 0;
   double *d_save = nullptr, *d_trap = nullptr;  // adaptive mode: x_n and x_trap
   unsigned long long *d_max = nullptr;
+  // one instance: the companion matrix as a CSR of its own, solved by the
+  // single-system BiCGSTAB (ISAI / Jacobi, hipGraph chunks, FP32-IR); its
+  // preconditioner is built once per step size
+  std::vector<uint32_t> h_rowptr, h_col;
+  DevCsr A1;
+  PrecondDev P1;
+  int p1_kind = -1;
+  double a1_sigma = NAN;
 };
 
 namespace {
@@ -1874,7 +1890,46 @@ ohm_status tran_rhs(ohm_transient *tr, double t_next, double t_prev, double sig_
   return OHM_OK;
 }
 
+// One instance: refresh A1 = G + sig C (and its preconditioner) when the
+// step size changes, then solve A1 x = b_eff warm-started from x_n.
+ohm_status tran_single_solve(ohm_transient *tr, double sig, int precond, const BicgOpts &o,
+                             int precision, double inner_tol, int64_t max_ref, int64_t &iters) {
+  ohm_backend *b = tr->owner;
+  ohm_batch *bt = tr->bt;
+  const BGeo &q = bt->h_geo[0];
+  const int64_t nnz = bt->h_nnz[0];
+  if (!tr->A1.rowptr)
+    OHM_TRY(csr_upload(b, tr->A1, nullptr, tr->h_col.data(), tr->h_rowptr.data(), (int)tr->n, nnz));
+  if (!(tr->a1_sigma == sig)) {
+    if (nnz > 0)
+      hipLaunchKernelGGL(k_axpby_vals, dim3(grid_for(nnz)), dim3(kBlock), 0, b->stream, nnz, q.g,
+                         q.c, sig, tr->A1.val);
+    if (tr->p1_kind < 0) {
+      OHM_TRY(precond_build(b, tr->A1, precond, 1, tr->P1));
+      tr->p1_kind = precond;
+    } else {
+      OHM_TRY(precond_refresh(b, tr->A1, tr->P1));
+    }
+    if (precision == OHM_PRECISION_FP32_IR) OHM_TRY(csr_ensure_f32(b, tr->A1));
+    tr->a1_sigma = sig;
+  }
+  const int64_t stride = (bt->N + 63) & ~int64_t(63);
+  double *x = reinterpret_cast<double *>(bt->ws64);
+  double *rhs = x + 10 * stride;
+  BicgOut out;
+  ohm_status s;
+  if (precision == OHM_PRECISION_FP32_IR) {
+    int64_t refs = 0;
+    s = bicg_solve_ir(b, tr->A1, tr->P1, rhs, x, o, inner_tol, max_ref, out, refs);
+  } else {
+    s = bicg_solve_f64(b, tr->A1, tr->P1, rhs, x, o, out);
+  }
+  iters += out.iters;
+  return s;
+}
+
 ohm_status tran_set_sigma(ohm_transient *tr, double sig) {
+  if (tr->K == 1 && tr->n > 0) return OHM_OK;  // the single path forms A1 itself
   std::vector<double> s(2 * tr->K, 0.0);
   for (int k = 0; k < tr->K; k++) s[2 * k] = sig;
   bool same = true;
@@ -1907,6 +1962,10 @@ ohm_status ohm_transient_create(ohm_backend *b, int64_t k, const ohm_batch_syste
   };
   ohm_status s = ohm_batch_create(b, 0, k, sys.data(), &tr->bt);
   if (s != OHM_OK) return bail(s);
+  if (k == 1 && n > 0) {
+    tr->h_rowptr.assign(instances[0].rowptr, instances[0].rowptr + n + 1);
+    tr->h_col.assign(instances[0].col, instances[0].col + tr->h_rowptr[n]);
+  }
   // per-row source gather lists in source order
   std::vector<std::vector<std::pair<int, int>>> rows(n);
   std::vector<TSrc> src(n_sources);
@@ -1968,6 +2027,9 @@ void ohm_transient_free(ohm_transient *tr) {
     (void)hipStreamSynchronize(tr->owner->stream);
     for (void *p : ds)
       if (p) (void)hipFree(p);
+    graph_cache_clear(tr->owner);
+    precond_free(tr->P1);
+    csr_free(tr->A1);
   }
   ohm_batch_free(tr->bt);
   delete tr;
@@ -2040,12 +2102,29 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
   ohm_bicgstab_params sp = prm->solver;
   sp.warm_start = 1;
   BatchOpts o;
-  OHM_TRY(batch_opts(&sp, prm->precond, o));
+  const bool single_inst = tr->K == 1 && tr->n > 0;
+  OHM_TRY(batch_opts(&sp, single_inst ? OHM_PREC_JACOBI : prm->precond, o));
   const double itol = sp.inner_tolerance > 0 ? sp.inner_tolerance : 1e-4;
   const int64_t mref = sp.max_refinements > 0 ? sp.max_refinements : 20;
+  const bool single = K == 1 && tr->n > 0;
+  if (!single && (prm->precond == OHM_PREC_ISAI))
+    return fail(OHM_ERR_INVALID, "ISAI is not available for several instances (use Jacobi or none)");
+  BicgOpts so;
+  so.max_iters = sp.max_iterations;
+  so.tol = sp.tolerance;
+  so.shadow = sp.shadow;
+  so.warm = true;
+  so.check_interval = sp.check_interval;
+  so.scaled_breakdown = sp.breakdown != OHM_BREAKDOWN_ABSOLUTE;
+  so.tol_mode = sp.tol_mode;
+  so.tol_floor = sp.tol_floor > 0 ? sp.tol_floor : 1e-15;
+  double cur_sigma = 0.0;
   auto solve = [&]() -> ohm_status {
-    ohm_status s = batch_solve_t<double, float>(bt, o, sp.precision, itol, mref, res.data(), false);
     st.linear_solves += K;
+    if (single)
+      return tran_single_solve(tr, cur_sigma, prm->precond, so, sp.precision, itol, mref,
+                               st.bicgstab_iterations);
+    ohm_status s = batch_solve_t<double, float>(bt, o, sp.precision, itol, mref, res.data(), false);
     for (int k = 0; k < K; k++) st.bicgstab_iterations += res[k].iterations;
     return s;
   };
@@ -2062,6 +2141,7 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
       if (h < h_min * 0.5) break;
       const double t_next = t + h;
       OHM_TRY(tran_set_sigma(tr, 1.0 / h));
+      cur_sigma = 1.0 / h;
       OHM_TRY(tran_rhs(tr, t_next, t, 1.0 / h, false));
       rc = solve();
       if (rc != OHM_OK) break;
@@ -2082,15 +2162,18 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
       OHM_HIP(hipMemcpyAsync(tr->d_save, wx, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
       if (use_be) {
         OHM_TRY(tran_set_sigma(tr, 1.0 / h));
+        cur_sigma = 1.0 / h;
         OHM_TRY(tran_rhs(tr, t_next, t, 1.0 / h, false));
         if ((rc = solve()) != OHM_OK) break;
       } else {
         OHM_TRY(tran_set_sigma(tr, 2.0 / h));
+        cur_sigma = 2.0 / h;
         OHM_TRY(tran_rhs(tr, t_next, t, 2.0 / h, true));
         if ((rc = solve()) != OHM_OK) break;
         OHM_HIP(hipMemcpyAsync(tr->d_trap, wx, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
         OHM_HIP(hipMemcpyAsync(wx, tr->d_save, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
         OHM_TRY(tran_set_sigma(tr, 1.0 / h));
+        cur_sigma = 1.0 / h;
         OHM_TRY(tran_rhs(tr, t_next, t, 1.0 / h, false));
         if ((rc = solve()) != OHM_OK) break;
         OHM_HIP(hipMemsetAsync(tr->d_max, 0, sizeof(unsigned long long), stream));

@@ -249,12 +249,13 @@ class GpuSolver:
         c = np.ascontiguousarray(a.col_indices, dtype=np.uint32)
         r = np.ascontiguousarray(a.row_pointers, dtype=np.uint32)
         res = N.BicgstabResult()
-        check(lib().ohm_solve_real(self.backend.h, ctypes.c_size_t(n), ctypes.c_size_t(v.shape[0]),
-                                   ptr(v, N.c_f64p), ptr(c, N.c_u32p), ptr(r, N.c_u32p),
-                                   ptr(bb, N.c_f64p), ptr(x, N.c_f64p), int(self.precond),
-                                   ctypes.byref(self.params) if self.params is not None else None,
-                                   ctypes.byref(res)))
+        rc = lib().ohm_solve_real(self.backend.h, ctypes.c_size_t(n), ctypes.c_size_t(v.shape[0]),
+                                  ptr(v, N.c_f64p), ptr(c, N.c_u32p), ptr(r, N.c_u32p),
+                                  ptr(bb, N.c_f64p), ptr(x, N.c_f64p), int(self.precond),
+                                  ctypes.byref(self.params) if self.params is not None else None,
+                                  ctypes.byref(res))
         self.last_stats = SolveStats.from_c(res)
+        check(rc)
         return x
 
     def solve_complex(self, a: ir.CsrMatrix, b) -> np.ndarray:
@@ -272,12 +273,13 @@ class GpuSolver:
         r = np.ascontiguousarray(a.row_pointers, dtype=np.uint32)
         res = N.BicgstabResult()
         prec = N.PREC_JACOBI if self.precond == N.PREC_AUTO else self.precond
-        check(lib().ohm_solve_complex(
+        rc = lib().ohm_solve_complex(
             self.backend.h, ctypes.c_size_t(n), ctypes.c_size_t(v.shape[0]),
             v.ctypes.data_as(N.c_f64p), ptr(c, N.c_u32p), ptr(r, N.c_u32p),
             bb.ctypes.data_as(N.c_f64p), x.ctypes.data_as(N.c_f64p), int(prec),
-            ctypes.byref(self.params) if self.params is not None else None, ctypes.byref(res)))
+            ctypes.byref(self.params) if self.params is not None else None, ctypes.byref(res))
         self.last_stats = SolveStats.from_c(res)
+        check(rc)
         return x
 
 
