@@ -313,7 +313,15 @@ typedef struct {
   int32_t precond;                 /* OHM_PREC_AUTO (reference rule) / JACOBI / ISAI */
   const double *initial_guess;     /* NewtonParams::initial_guess, or NULL */
   double bicgstab_floor;           /* sub-solve floor relative to ||b(x_k)|| (default 1e-15) */
+  int32_t forcing;                 /* OHM_FORCING_FIXED (default): every sub-solve to
+                                      bicgstab_tolerance; OHM_FORCING_EW: Eisenstat-Walker
+                                      choice 2, eta_k = 0.9 (||F_k|| / ||F_k-1||)^2 with the
+                                      0.9 eta_k-1^2 safeguard, clamped to
+                                      [bicgstab_tolerance, forcing_max] */
+  int32_t _pad;
+  double forcing_max;              /* OHM_FORCING_EW: largest eta (default 0.1) */
 } ohm_newton_params;
+enum { OHM_FORCING_FIXED = 0, OHM_FORCING_EW = 1 };
 /* The sub-solve is warm-started from x_k, so its initial residual is the
  * Newton residual F(x_k) = b(x_k) - A(x_k) x_k and the stopping rule is
  * max(bicgstab_tolerance*||F||, bicgstab_floor*||b||) (an inexact Newton

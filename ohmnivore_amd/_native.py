@@ -62,7 +62,9 @@ class NewtonParams(ctypes.Structure):
     _fields_ = [("max_iterations", ctypes.c_int64), ("abs_tol", ctypes.c_double),
                 ("bicgstab_max_iterations", ctypes.c_int64), ("bicgstab_tolerance", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("precond", ctypes.c_int32),
-                ("initial_guess", c_f64p), ("bicgstab_floor", ctypes.c_double)]
+                ("initial_guess", c_f64p), ("bicgstab_floor", ctypes.c_double),
+                ("forcing", ctypes.c_int32), ("_pad", ctypes.c_int32),
+                ("forcing_max", ctypes.c_double)]
 
 
 class NewtonResult(ctypes.Structure):

@@ -20,6 +20,7 @@
 // K3/K4. The same kernels run in FP32 (with compensated dot products) as the
 // inner solver of FP64 iterative refinement.
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 
 #include "ohm_precond.h"
@@ -38,6 +39,7 @@ __global__ void k_state_reset(SolveState *st) {
     z.max_iters = 0;
     z.counter = 0u;
     z.scaled = 1;
+    z.r0 = 0.0;
     *st = z;
   }
 }
@@ -79,6 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
     st->scaled = scaled;
     st->max_iters = max_iters;
     st->rr = tot[1];
+    st->r0 = sqrt(tot[1]);
     double bnorm = sqrt(tot[0]);
     if (bnorm < 1e-30) {  // b == 0: x = 0 (src/solver/bicgstab.rs:93-97)
       st->status = ST_CONVERGED;
@@ -382,9 +385,15 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
   return OHM_OK;
 }
 
-struct StatusWord {
+struct StatusWord {  // SolveState from `status` on
   int status, s_conv, iters, max_iters;
+  unsigned counter;
+  int scaled;
+  double r0;
 };
+static_assert(offsetof(SolveState, r0) - offsetof(SolveState, status) ==
+                  offsetof(StatusWord, r0),
+              "StatusWord mirrors the tail of SolveState");
 
 // Runs BiCGSTAB on the workspace vectors; rhs in ws.r on entry? No: rhs is a
 // separate device pointer; x0 (warm) is read from xin.
@@ -435,6 +444,7 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   }
   out.status = h->status;
   out.iters = h->iters;
+  out.r0 = h->r0;
   OHM_HIP(hipGetLastError());
   return OHM_OK;
 }
@@ -592,6 +602,7 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
   for (int64_t k = 0; k <= max_ref; k++) {
     OHM_TRY(true_residual(b, a, rhs, x, rel));
     double bn = sqrt(b->h_pinned[1]);
+    if (k == 0) out.r0 = rel * bn;
     if (k == 0 && o.tol_mode == OHM_TOL_INITIAL) target = std::max(o.tol * rel, o.tol_floor);
     if (bn < 1e-30 || rel < target) {
       out.status = ST_CONVERGED;

@@ -67,6 +67,7 @@ struct SolveState {
   int max_iters;
   unsigned counter;     // last-block-done ticket
   int scaled;           // OHM_BREAKDOWN_SCALED tests
+  double r0;            // ||b - A x0|| of the solve (read with the status word)
 };
 
 // ------------------------------------------------------ device objects --

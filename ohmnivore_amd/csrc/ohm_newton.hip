@@ -517,6 +517,9 @@ void ohm_newton_default_params(ohm_newton_params *p) {
   p->precond = OHM_PREC_AUTO;
   p->initial_guess = nullptr;
   p->bicgstab_floor = 1e-15;
+  p->forcing = OHM_FORCING_FIXED;
+  p->_pad = 0;
+  p->forcing_max = 0.1;
 }
 
 void ohm_newton_free(ohm_newton_ctx *c) {
@@ -708,10 +711,18 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
   ohm_status rc = OHM_OK;
   int64_t it = 0;
   bool converged = false;
+  // Eisenstat-Walker forcing (choice 2, gamma 0.9, alpha 2; Eisenstat &
+  // Walker, SIAM J. Sci. Comput. 17(1), 1996): loose sub-solves while Newton
+  // is far from the solution, tight ones as ||F|| falls.
+  const bool ew = prm.forcing == OHM_FORCING_EW;
+  const double eta_max = prm.forcing_max > 0 ? std::max(prm.forcing_max, prm.bicgstab_tolerance)
+                                             : 0.1;
+  double eta = ew ? eta_max : prm.bicgstab_tolerance, f_prev = -1.0;
   for (it = 0; it < prm.max_iterations; it++) {
     hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n, 4)), dim3(kBlock), 0, st, n, c->x, c->x_old);
     enqueue_eval_assemble(c, c->x);
     OHM_TRY(precond_refresh(b, c->A, c->P));
+    o.tol = eta;
     BicgOut bo;
     ohm_status ls;
     if (prm.precision == OHM_PRECISION_FP32_IR) {
@@ -725,6 +736,16 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
     }
     total_bicg += bo.iters;
     if (iters_per_newton) iters_per_newton[it] = (int32_t)bo.iters;
+    if (ew) {
+      const double f = bo.r0;  // ||F(x_k)||
+      if (f_prev > 0.0 && std::isfinite(f)) {
+        double e = 0.9 * (f / f_prev) * (f / f_prev);
+        const double safe = 0.9 * eta * eta;
+        if (safe > 0.1) e = std::max(e, safe);
+        eta = std::min(eta_max, std::max(prm.bicgstab_tolerance, e));
+      }
+      f_prev = f;
+    }
     // the assembled-value NaN flag (src/solver/newton.rs:258-263)
     OHM_HIP(hipMemcpyAsync(&h->nan, c->asm_nan, sizeof(int), hipMemcpyDeviceToHost, st));
     OHM_HIP(hipStreamSynchronize(st));

@@ -49,6 +49,7 @@ struct BicgOut {
   int status = ST_RUNNING;
   int64_t iters = 0;
   int64_t polls = 0;
+  double r0 = 0.0;  // ||b - A x0|| (the Newton residual when x0 = x_k)
 };
 
 // FP64 solve on device vectors (rhs, x of length a.n).

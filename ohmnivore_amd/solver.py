@@ -512,9 +512,10 @@ class NewtonStats:
 
 def newton_params(max_iterations=50, abs_tol=1e-4, bicgstab_max_iterations=10_000,
                   bicgstab_tolerance=1e-5, precision="fp64", precond=N.PREC_AUTO,
-                  initial_guess=None, bicgstab_floor=1e-15):
+                  initial_guess=None, bicgstab_floor=1e-15, forcing="fixed", forcing_max=0.1):
     """NewtonParams (src/solver/newton.rs:34-55). bicgstab_tolerance is
-    relative to the Newton residual ||F(x_k)|| (inexact Newton)."""
+    relative to the Newton residual ||F(x_k)|| (inexact Newton); forcing
+    "ew" adapts it per iteration (Eisenstat-Walker) down to that value."""
     p = N.NewtonParams()
     lib().ohm_newton_default_params(ctypes.byref(p))
     p.max_iterations = max_iterations
@@ -524,6 +525,8 @@ def newton_params(max_iterations=50, abs_tol=1e-4, bicgstab_max_iterations=10_00
     p.bicgstab_floor = bicgstab_floor
     p.precision = N.FP32_IR if precision in ("fp32_ir", "mixed") else N.FP64
     p.precond = precond
+    p.forcing = 1 if forcing == "ew" else 0
+    p.forcing_max = forcing_max
     keep = None
     if initial_guess is not None:
         keep = np.ascontiguousarray(initial_guess, dtype=np.float64)
