@@ -74,6 +74,15 @@ ohm_status ohm_spmv_transpose(ohm_backend *b, const ohm_matrix *a, const ohm_buf
  * launch in *avg_ms. */
 ohm_status ohm_spmv_timed(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x, ohm_buffer *y,
                           int reps, int precision, double *avg_ms);
+/* SpMV autotuner (no reference counterpart): times the matrix's SpMV under
+ * each candidate row-block shape (rows per workgroup 32-256 x staged
+ * products per workgroup 512-2048) and keeps the fastest; the chosen shape
+ * is returned. Results do not depend on the shape. Matrices the library
+ * builds itself (Newton, ISAI, transient) are tuned automatically unless
+ * ohm_backend_set_autotune(b, 0). */
+ohm_status ohm_matrix_autotune(ohm_backend *b, ohm_matrix *m, int32_t *rows_per_block,
+                               int32_t *nnz_per_block);
+ohm_status ohm_backend_set_autotune(ohm_backend *b, int enable);
 ohm_status ohm_dot(ohm_backend *b, const ohm_buffer *x, const ohm_buffer *y, double *out); /* :59 */
 ohm_status ohm_dot_n(ohm_backend *b, const ohm_buffer *x, const ohm_buffer *y, size_t n,
                      double *out);                                                   /* :66 */

@@ -362,11 +362,13 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
       (uintptr_t)Ops<T>::inv(p), (uintptr_t)p.ml.rowptr, (uintptr_t)Ops<T>::vals(p.ml),
       (uintptr_t)p.ml.blocks, (uintptr_t)p.ml.n_blocks, (uintptr_t)p.mu.rowptr,
       (uintptr_t)Ops<T>::vals(p.mu), (uintptr_t)p.mu.blocks, (uintptr_t)p.mu.n_blocks,
+      (uintptr_t)p.mm.rowptr, (uintptr_t)Ops<T>::vals(p.mm), (uintptr_t)p.mm.blocks,
+      (uintptr_t)p.mm.n_blocks,
       (uintptr_t)v.x, (uintptr_t)v.tmp, (uintptr_t)st, (uintptr_t)b->d_partials, kGraphIters};
   hipGraphExec_t exec = nullptr;
   for (auto &g : b->graphs)
     if (g.key == key) exec = g.exec;
-  const int64_t per_iter = 5 + (p.kind == PK_ISAI ? 4 : 0);
+  const int64_t per_iter = 5 + (p.kind == PK_ISAI ? (p.mm.rowptr ? 2 : 4) : 0);
   if (!exec) {
     int64_t saved = b->dispatches;
     GraphEntry e;

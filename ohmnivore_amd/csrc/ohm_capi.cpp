@@ -246,6 +246,22 @@ ohm_status ohm_spmv(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x, oh
   return OHM_OK;
 }
 
+ohm_status ohm_backend_set_autotune(ohm_backend *b, int enable) {
+  if (!b) return fail(OHM_ERR_INVALID, "null backend");
+  b->autotune = enable != 0;
+  return OHM_OK;
+}
+
+ohm_status ohm_matrix_autotune(ohm_backend *b, ohm_matrix *m, int32_t *rows_per_block,
+                               int32_t *nnz_per_block) {
+  if (!b || !m) return fail(OHM_ERR_INVALID, "null handle");
+  int r = 0, z = 0;
+  OHM_TRY(csr_autotune(b, m->csr, &r, &z));
+  if (rows_per_block) *rows_per_block = r;
+  if (nnz_per_block) *nnz_per_block = z;
+  return OHM_OK;
+}
+
 ohm_status ohm_spmv_timed(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x, ohm_buffer *y,
                           int reps, int precision, double *avg_ms) {
   if (!b || !a || !avg_ms || reps <= 0) return fail(OHM_ERR_INVALID, "bad argument");

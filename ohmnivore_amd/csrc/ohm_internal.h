@@ -84,6 +84,7 @@ struct DevCsr {
   uint32_t *t_rowptr = nullptr, *t_col = nullptr, *t_src = nullptr, *t_blocks = nullptr;
   int t_n_blocks = 0;
   double *t_val = nullptr;         // values of A^T in its CSR order (gathered via t_src)
+  int tuned_rows = kSpmvRowsPerBlock, tuned_nnz = kSpmvNnzPerBlock;  // row-block shape
   std::vector<uint32_t> h_rowptr, h_col; // host copy of the pattern
 };
 
@@ -102,6 +103,7 @@ struct GraphEntry {
 struct ohm_backend {
   std::vector<ohm::GraphEntry> graphs;
   bool use_graphs = true;
+  bool autotune = true;              // tune row blocks of solver-internal matrices
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cu = 0;
@@ -130,7 +132,13 @@ namespace ohm {
 // ------------------------------------------------------ kernel launchers --
 // All launches go on the backend's stream; `state` (may be null) lets solver
 // kernels skip work once a solve has finished.
-void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blocks);
+// Row blocks for the CSR-stream SpMV: at most rows_cap rows (<= 256, one
+// thread per row) and nnz_cap LDS-staged products (<= 2048) per workgroup; a
+// longer row gets a workgroup of its own.
+void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blocks,
+                       int rows_cap = kSpmvRowsPerBlock, int nnz_cap = kSpmvNnzPerBlock);
+// Pick the fastest (rows_cap, nnz_cap) for m by timing each candidate.
+ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out);
 ohm_status csr_upload(ohm_backend *b, DevCsr &m, const double *val, const uint32_t *col,
                       const uint32_t *rowptr, int n, int64_t nnz);
 void csr_free(DevCsr &m);

@@ -17,7 +17,12 @@
 //     U), each local triangular solve done by level-synchronous substitution
 //     sweeps over precomputed dependency lists, all columns in parallel.
 //
-// Apply is z = M_U (M_L r): two CSR-stream SpMVs.
+// Apply is z = M r with the product M = M_U M_L formed on the device after
+// every numeric refresh (pattern and the (M_U, M_L) entry pairs of each
+// product entry found once on the host, products summed in a fixed order):
+// one CSR-stream SpMV per apply instead of two dependent launches. When the
+// product would be much denser than the two factors the apply stays
+// z = M_U (M_L r).
 #include <algorithm>
 #include <cmath>
 #include <unordered_map>
@@ -49,7 +54,24 @@ struct IsaiPlan {
     double *m0 = nullptr, *m1 = nullptr;
     int sweeps = 1;
   } side[2];
+  // product M = M_U M_L: per product entry, (M_U position, M_L position) pairs
+  uint32_t *prod_ptr = nullptr, *prod_pairs = nullptr;
+  int64_t prod_nnz = 0;
 };
+
+// m[e] = sum over the entry's pairs of mu[q] * ml[t], in pair order
+__global__ void k_isai_product(int64_t nm, const uint32_t *__restrict__ ptr,
+                               const uint32_t *__restrict__ pairs, const double *__restrict__ mu,
+                               const double *__restrict__ ml, double *__restrict__ m,
+                               float *__restrict__ m32) {
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < nm;
+       e += (int64_t)gridDim.x * kBlock) {
+    double acc = 0.0;
+    for (uint32_t k = ptr[e]; k < ptr[e + 1]; k++) acc = fma(mu[pairs[2 * k]], ml[pairs[2 * k + 1]], acc);
+    m[e] = acc;
+    if (m32) m32[e] = (float)acc;
+  }
+}
 
 // ------------------------------------------------------------- kernels ----
 __global__ void k_ilu_init(int64_t nf, const int32_t *__restrict__ from_a,
@@ -283,6 +305,53 @@ static ohm_status build_side(int n, int level, bool lower, const std::vector<uin
   return OHM_OK;
 }
 
+// Pattern of M_U M_L and its entry pairs; skipped (two-SpMV apply) when the
+// product would hold more than 3x the nonzeros of the two factors.
+static ohm_status build_product(ohm_backend *b, PrecondDev &p) {
+  IsaiPlan *pl = p.plan;
+  const int n = p.n;
+  const DevCsr &U = p.mu, &L = p.ml;
+  if (U.h_rowptr.empty() || L.h_rowptr.empty()) return OHM_OK;
+  std::vector<uint32_t> rp(n + 1, 0), cl, ptr(1, 0), pairs;
+  std::vector<int64_t> mark(n, -1);
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> acc;
+  std::vector<uint32_t> cols;
+  const int64_t cap = 3 * (U.nnz + L.nnz);
+  for (int i = 0; i < n; i++) {
+    cols.clear();
+    for (uint32_t q = U.h_rowptr[i]; q < U.h_rowptr[i + 1]; q++) {
+      const uint32_t k = U.h_col[q];
+      for (uint32_t t = L.h_rowptr[k]; t < L.h_rowptr[k + 1]; t++) {
+        const uint32_t j = L.h_col[t];
+        if (mark[j] < 0) {
+          mark[j] = (int64_t)acc.size();
+          acc.emplace_back();
+          cols.push_back(j);
+        }
+        acc[mark[j]].push_back({q, t});
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    for (uint32_t j : cols) {
+      cl.push_back(j);
+      for (auto &pr : acc[mark[j]]) {
+        pairs.push_back(pr.first);
+        pairs.push_back(pr.second);
+      }
+      ptr.push_back((uint32_t)(pairs.size() / 2));
+      mark[j] = -1;
+    }
+    acc.clear();
+    rp[i + 1] = (uint32_t)cl.size();
+    if ((int64_t)cl.size() > cap) return OHM_OK;  // too dense: keep the two-SpMV apply
+  }
+  pl->prod_nnz = (int64_t)cl.size();
+  OHM_TRY(csr_upload(b, p.mm, nullptr, cl.data(), rp.data(), n, pl->prod_nnz));
+  OHM_TRY(upload(&pl->prod_ptr, ptr));
+  OHM_TRY(upload(&pl->prod_pairs, pairs));
+  return OHM_OK;
+}
+
 static ohm_status isai_plan_build(ohm_backend *b, const DevCsr &a, int level, PrecondDev &p) {
   const int n = a.n;
   IsaiPlan *pl = new IsaiPlan();
@@ -345,6 +414,7 @@ static ohm_status isai_plan_build(ohm_backend *b, const DevCsr &a, int level, Pr
   OHM_HIP(hipMalloc(&pl->f_tmp, sizeof(double) * std::max<int64_t>(nf, 1)));
   OHM_TRY(build_side(n, level, true, frp, fcl, fdiag, pl->side[0], p.ml, b));
   OHM_TRY(build_side(n, level, false, frp, fcl, fdiag, pl->side[1], p.mu, b));
+  OHM_TRY(build_product(b, p));
   return OHM_OK;
 }
 
@@ -379,6 +449,12 @@ static ohm_status isai_numeric(ohm_backend *b, const DevCsr &a, PrecondDev &p) {
     }
     hipLaunchKernelGGL(k_scatter, gm, blk, 0, b->stream, sd.m, sd.out_pos, src, M.val,
                        p.has_f32 ? M.val32 : nullptr);
+    b->dispatches++;
+  }
+  if (p.mm.rowptr && pl->prod_nnz > 0) {
+    hipLaunchKernelGGL(k_isai_product, dim3(grid_for(pl->prod_nnz)), blk, 0, b->stream,
+                       pl->prod_nnz, pl->prod_ptr, pl->prod_pairs, p.mu.val, p.ml.val, p.mm.val,
+                       p.has_f32 ? p.mm.val32 : nullptr);
     b->dispatches++;
   }
   OHM_HIP(hipGetLastError());
@@ -417,7 +493,16 @@ ohm_status precond_build(ohm_backend *b, const DevCsr &a, int kind, int level, P
   } else {
     return fail(OHM_ERR_INVALID, "unknown preconditioner kind %d", kind);
   }
-  return precond_refresh(b, a, p);
+  OHM_TRY(precond_refresh(b, a, p));
+  if (b->autotune && p.kind == PK_ISAI) {
+    if (p.mm.rowptr) {
+      OHM_TRY(csr_autotune(b, p.mm, nullptr, nullptr));
+    } else {
+      OHM_TRY(csr_autotune(b, p.ml, nullptr, nullptr));
+      OHM_TRY(csr_autotune(b, p.mu, nullptr, nullptr));
+    }
+  }
+  return OHM_OK;
 }
 
 ohm_status precond_refresh(ohm_backend *b, const DevCsr &a, PrecondDev &p) {
@@ -445,6 +530,7 @@ ohm_status precond_ensure_f32(ohm_backend *b, PrecondDev &p) {
   } else if (p.kind == PK_ISAI) {
     OHM_TRY(csr_ensure_f32(b, p.ml));
     OHM_TRY(csr_ensure_f32(b, p.mu));
+    if (p.mm.rowptr) OHM_TRY(csr_ensure_f32(b, p.mm));
   }
   p.has_f32 = true;
   return OHM_OK;
@@ -457,6 +543,8 @@ void precond_apply(ohm_backend *b, const PrecondDev &p, const double *in, double
     hipLaunchKernelGGL(k_mul_f64, dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream, n, p.inv, in,
                        out, st, skip);
     b->dispatches++;
+  } else if (p.kind == PK_ISAI && p.mm.rowptr) {
+    launch_spmv(b, p.mm, in, out, st, skip);
   } else if (p.kind == PK_ISAI) {
     launch_spmv(b, p.ml, in, tmp, st, skip);
     launch_spmv(b, p.mu, tmp, out, st, skip);
@@ -474,6 +562,8 @@ void precond_apply_f32(ohm_backend *b, const PrecondDev &p, const float *in, flo
     hipLaunchKernelGGL(k_mul_f32, dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream, n, p.inv32, in,
                        out, st, skip);
     b->dispatches++;
+  } else if (p.kind == PK_ISAI && p.mm.rowptr) {
+    launch_spmv_f32(b, p.mm, in, out, st, skip);
   } else if (p.kind == PK_ISAI) {
     launch_spmv_f32(b, p.ml, in, tmp, st, skip);
     launch_spmv_f32(b, p.mu, tmp, out, st, skip);
@@ -493,10 +583,11 @@ void precond_free(PrecondDev &p) {
   if (p.inv32) (void)hipFree(p.inv32);
   csr_free(p.ml);
   csr_free(p.mu);
+  csr_free(p.mm);
   if (p.plan) {
     IsaiPlan *pl = p.plan;
     void *ptrs[] = {pl->f_rowptr, pl->f_col, pl->f_from_a, pl->f_div, pl->cp_ptr,
-                    pl->cp_pairs, pl->f_val, pl->f_tmp};
+                    pl->cp_pairs, pl->f_val, pl->f_tmp, pl->prod_ptr, pl->prod_pairs};
     for (void *q : ptrs)
       if (q) (void)hipFree(q);
     for (auto &sd : pl->side) {

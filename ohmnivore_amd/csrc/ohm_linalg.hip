@@ -16,18 +16,20 @@
 namespace ohm {
 
 // --------------------------------------------------------------- SpMV ----
-void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blocks) {
+void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blocks, int rows_cap,
+                       int nnz_cap) {
+  rows_cap = std::max(1, std::min(rows_cap, kSpmvRowsPerBlock));
+  nnz_cap = std::max(1, std::min(nnz_cap, kSpmvNnzPerBlock));
   blocks.clear();
   blocks.push_back(0);
   int r = 0;
   while (r < n) {
     uint32_t start = rowptr[r];
     int rows = 0;
-    if (rowptr[r + 1] - start > (uint32_t)kSpmvNnzPerBlock) {
+    if (rowptr[r + 1] - start > (uint32_t)nnz_cap) {
       r++;  // long row: own block
     } else {
-      while (r < n && rows < kSpmvRowsPerBlock &&
-             rowptr[r + 1] - start <= (uint32_t)kSpmvNnzPerBlock) {
+      while (r < n && rows < rows_cap && rowptr[r + 1] - start <= (uint32_t)nnz_cap) {
         r++;
         rows++;
       }
@@ -312,6 +314,90 @@ ohm_status csr_transpose_view(ohm_backend *b, DevCsr &m, DevCsr &view) {
   view.blocks = m.t_blocks;
   view.n_blocks = m.t_n_blocks;
   return OHM_OK;
+}
+
+// Row-block autotuner: every candidate shape (rows per workgroup x LDS
+// products per workgroup) is a valid block table for the same kernel, so
+// the choice only changes speed. Each candidate is timed with HIP events over
+// back-to-back launches on the backend stream and the fastest kept.
+ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) {
+  if (m.n < 4096 || m.h_rowptr.empty()) {
+    if (rows_out) *rows_out = m.tuned_rows;
+    if (nnz_out) *nnz_out = m.tuned_nnz;
+    return OHM_OK;
+  }
+  static const int kRows[] = {32, 64, 128, 256};
+  static const int kNnz[] = {512, 1024, 2048};
+  double *x = nullptr, *y = nullptr;
+  OHM_HIP(hipMalloc(&x, sizeof(double) * m.n));
+  OHM_HIP(hipMalloc(&y, sizeof(double) * m.n));
+  OHM_HIP(hipMemsetAsync(x, 0, sizeof(double) * m.n, b->stream));
+  std::vector<uint32_t> blocks;
+  float best = 1e30f;
+  int best_r = m.tuned_rows, best_z = m.tuned_nnz;
+  std::vector<uint32_t> best_blocks;
+  uint32_t *d_blk = nullptr;
+  size_t d_cap = 0;
+  ohm_status rc = OHM_OK;
+  for (int rows : kRows)
+    for (int nz : kNnz) {
+      build_spmv_blocks(m.h_rowptr.data(), m.n, blocks, rows, nz);
+      if (blocks.size() > d_cap) {
+        if (d_blk) (void)hipFree(d_blk);
+        d_cap = blocks.size();
+        if (hipMalloc(&d_blk, sizeof(uint32_t) * d_cap) != hipSuccess) {
+          rc = fail(OHM_ERR_SOLVE, "Solve error: out of device memory in the autotuner");
+          goto done;
+        }
+      }
+      (void)hipMemcpyAsync(d_blk, blocks.data(), sizeof(uint32_t) * blocks.size(),
+                           hipMemcpyHostToDevice, b->stream);
+      {
+        DevCsr v = m;  // shallow view with the candidate table
+        v.blocks = d_blk;
+        v.n_blocks = (int)blocks.size() - 1;
+        v.h_rowptr.clear();
+        v.h_col.clear();
+        for (int w = 0; w < 3; w++) launch_spmv(b, v, x, y, nullptr, 0);
+        (void)hipEventRecord(b->ev0, b->stream);
+        for (int k = 0; k < 20; k++) launch_spmv(b, v, x, y, nullptr, 0);
+        (void)hipEventRecord(b->ev1, b->stream);
+        (void)hipEventSynchronize(b->ev1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+        if (ms < best) {
+          best = ms;
+          best_r = rows;
+          best_z = nz;
+          best_blocks = blocks;
+        }
+      }
+    }
+  if (!best_blocks.empty()) {
+    uint32_t *nb = nullptr;
+    if (hipMalloc(&nb, sizeof(uint32_t) * best_blocks.size()) != hipSuccess) {
+      rc = fail(OHM_ERR_SOLVE, "Solve error: out of device memory in the autotuner");
+      goto done;
+    }
+    (void)hipMemcpyAsync(nb, best_blocks.data(), sizeof(uint32_t) * best_blocks.size(),
+                         hipMemcpyHostToDevice, b->stream);
+    (void)hipStreamSynchronize(b->stream);
+    if (m.blocks) (void)hipFree(m.blocks);
+    m.blocks = nb;
+    m.n_blocks = (int)best_blocks.size() - 1;
+    m.tuned_rows = best_r;
+    m.tuned_nnz = best_z;
+    graph_cache_clear(b);
+  }
+done:
+  (void)hipStreamSynchronize(b->stream);
+  if (d_blk) (void)hipFree(d_blk);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  b->dispatches += 12 * 23;
+  if (rows_out) *rows_out = m.tuned_rows;
+  if (nnz_out) *nnz_out = m.tuned_nnz;
+  return rc;
 }
 
 void csr_free(DevCsr &m) {

@@ -14,6 +14,7 @@ struct PrecondDev {
   double *inv = nullptr;   // Jacobi inverse diagonal
   float *inv32 = nullptr;
   DevCsr ml, mu;           // ISAI: z = M_U (M_L r)
+  DevCsr mm;               // ISAI: the product M_U M_L when formed (one SpMV per apply)
   IsaiPlan *plan = nullptr;
   int ilu_sweeps = 5;      // fixed-point ILU(0) sweeps per numeric refresh
   bool has_f32 = false;

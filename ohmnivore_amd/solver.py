@@ -66,6 +66,12 @@ class Matrix:
                                       ptr(r, N.c_u32p), ctypes.c_size_t(n),
                                       ctypes.c_size_t(self.nnz), ctypes.byref(self.h)))
 
+    def autotune(self) -> tuple[int, int]:
+        """ohm_matrix_autotune: fastest (rows per workgroup, staged products)."""
+        r, z = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().ohm_matrix_autotune(self.backend.h, self.h, ctypes.byref(r), ctypes.byref(z)))
+        return r.value, z.value
+
     def set_values(self, values):
         v = np.ascontiguousarray(values, dtype=np.float64)
         check(lib().ohm_matrix_set_values(self.backend.h, self.h, ptr(v, N.c_f64p)))
@@ -215,6 +221,9 @@ class HipBackend:
 
     def sync(self):
         check(lib().ohm_backend_sync(self.h))
+
+    def set_autotune(self, enable: bool):
+        check(lib().ohm_backend_set_autotune(self.h, int(bool(enable))))
 
     def set_graphs(self, enable: bool):
         """Replay iteration chunks from captured hipGraphs (default) or not."""

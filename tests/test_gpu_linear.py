@@ -228,3 +228,30 @@ def test_warm_start_and_determinism(be):
     warm = solver.bicgstab_params(tolerance=1e-10, warm_start=True)
     s3 = be.bicgstab(a, b, x, p, warm)
     assert s3.iterations == 0  # already converged from x1
+
+
+def test_autotune_keeps_results(be):
+    """The autotuner only changes the row-block shape: SpMV and a BiCGSTAB
+    solve agree with the untuned matrix."""
+    mna = solver.compile(ir.diode_mesh(96, 96, seed=2))
+    a = mna.g
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(a.nrows)
+    m0 = be.upload_matrix(a.values, a.col_indices, a.row_pointers, a.nrows)
+    m1 = be.upload_matrix(a.values, a.col_indices, a.row_pointers, a.nrows)
+    rows, nnz = m1.autotune()
+    assert rows in (32, 64, 128, 256) and nnz in (512, 1024, 2048)
+    bx, y0, y1 = be.new_buffer(a.nrows), be.new_buffer(a.nrows), be.new_buffer(a.nrows)
+    be.upload_vec(x, bx)
+    be.spmv(m0, bx, y0)
+    be.spmv(m1, bx, y1)
+    assert np.allclose(be.download_vec(y1), be.download_vec(y0), rtol=1e-14, atol=1e-14)
+    b = rng.standard_normal(a.nrows)
+    ref = oracle.sparse_lu_solve(a, b)
+    for m in (m0, m1):
+        p = be.preconditioner(m, N.PREC_ISAI)
+        bb, xx = be.new_buffer(a.nrows), be.new_buffer(a.nrows)
+        be.upload_vec(b, bb)
+        be.bicgstab(m, bb, xx, p, solver.bicgstab_params(tolerance=1e-13, max_iterations=20000))
+        ok, err = close(be.download_vec(xx), ref)
+        assert ok, err
