@@ -73,16 +73,19 @@ def headline(args, be, N, ir, solver):
     m = solver.compile(ckt)
     ctx = solver.NewtonContext(be, m.g, m.b_dc, m.diode_descriptors)
     newton_kw = dict(abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-6,
-                     bicgstab_max_iterations=20_000)
+                     bicgstab_max_iterations=20_000, forcing="ew", forcing_max=0.1)
     forms = {}
-    for prec in ("fp64", "fp32_ir"):
+    for prec in ("fp64", "fp32_ir", "fp64_fixed_tol"):
+        kw = dict(newton_kw, precision=prec)
+        if prec == "fp64_fixed_tol":  # every sub-solve to 1e-6 (no forcing)
+            kw.update(precision="fp64", forcing="fixed")
         for _ in range(args.warmup):
-            ctx.run(precision=prec, **newton_kw)
+            ctx.run(**kw)
         be.sync()
         t0 = time.perf_counter()
         stats = None
         for _ in range(args.steps):
-            stats = ctx.run(precision=prec, **newton_kw)
+            stats = ctx.run(**kw)
         be.sync()
         dt = time.perf_counter() - t0
         forms[prec] = {
@@ -306,6 +309,27 @@ def adjoint(be, N, ir, solver):
     return out
 
 
+def roofline_only(be, N, ir, solver) -> int:
+    """The dominant kernel alone (k_spmv<double> on the diode-mesh matrix)
+    and a plain copy kernel of known size, for rocprofv3 counter runs."""
+    m = solver.compile(ir.diode_mesh(256, 256, seed=2))
+    a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
+    bx, by = be.new_buffer(m.size), be.new_buffer(m.size)
+    be.upload_vec(np.ones(m.size), bx)
+    ms = ctypes.c_double()
+    N.check(N.lib().ohm_spmv_timed(be.h, a.h, bx.h, by.h, 200, 0, ctypes.byref(ms)))
+    nc = 8 * 1024 * 1024  # 64 MiB of doubles
+    cx, cy = be.new_buffer(nc), be.new_buffer(nc)
+    for _ in range(20):
+        be.copy(cx, cy)
+    be.sync()
+    b = spmv_bytes(m.size, m.g.nnz, 8)
+    print(json.dumps({"kernel": "k_spmv<double>", "bytes_per_launch": b,
+                      "avg_launch_us": ms.value * 1e3, "achieved_gbs": b / (ms.value * 1e-3) / 1e9,
+                      "copy_kernel": "k_copy", "copy_bytes_per_launch": 2 * 8 * nc}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +343,9 @@ def main() -> int:
     ap.add_argument("--ac-circuits", type=int, default=256)
     ap.add_argument("--profile", action="store_true",
                     help="rocprofv3 runs: headline only, no CPU work, no hipGraphs")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline kernel: 200 SpMVs on the mesh matrix and, for "
+                         "counter calibration, 20 copies of a 64 MiB vector")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -330,6 +357,8 @@ def main() -> int:
     from ohmnivore_amd import solver
 
     be = solver.HipBackend()
+    if args.roofline_only:
+        return roofline_only(be, N, ir, solver)
     if args.profile:
         be.set_graphs(False)
         args.only = "headline"
@@ -359,7 +388,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "f32+f64-refinement",
+        "dtype": "f32+f64-refinement" if best == "fp32_ir" else "f64",
         "data": "synthetic (seeded SplitMix64 circuits)",
         "config": {
             "workload": "nonlinear DC, 256x256 resistor mesh + 8,192 diodes: 65,540 unknowns, "
@@ -367,6 +396,7 @@ def main() -> int:
             "form": best,
             "newton_abs_tol": h["newton_kw"]["abs_tol"],
             "bicgstab_tolerance_rel_to_newton_residual": h["newton_kw"]["bicgstab_tolerance"],
+            "forcing": "Eisenstat-Walker, eta in [1e-6, 0.1]",
         },
         "roofline": h["roofline"],
         "cpu_baseline": h["cpu"],

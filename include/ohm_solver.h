@@ -77,9 +77,11 @@ ohm_status ohm_spmv_timed(ohm_backend *b, const ohm_matrix *a, const ohm_buffer 
 /* SpMV autotuner (no reference counterpart): times the matrix's SpMV under
  * each candidate row-block shape (rows per workgroup 32-256 x staged
  * products per workgroup 512-2048) and keeps the fastest; the chosen shape
- * is returned. Results do not depend on the shape. Matrices the library
- * builds itself (Newton, ISAI, transient) are tuned automatically unless
- * ohm_backend_set_autotune(b, 0). */
+ * is returned. Results do not depend on the shape. The preconditioner
+ * matrices the library builds itself (ISAI) are tuned automatically unless
+ * ohm_backend_set_autotune(b, 0); system matrices are not, because their
+ * block table also sets the grid of the fused SpMV + reduction kernels,
+ * whose cost grows with the number of workgroups. */
 ohm_status ohm_matrix_autotune(ohm_backend *b, ohm_matrix *m, int32_t *rows_per_block,
                                int32_t *nnz_per_block);
 ohm_status ohm_backend_set_autotune(ohm_backend *b, int enable);

@@ -1911,7 +1911,6 @@ ohm_status tran_single_solve(ohm_transient *tr, double sig, int precond, const B
       OHM_TRY(precond_refresh(b, tr->A1, tr->P1));
     }
     if (precision == OHM_PRECISION_FP32_IR) OHM_TRY(csr_ensure_f32(b, tr->A1));
-    if (std::isnan(tr->a1_sigma) && b->autotune) OHM_TRY(csr_autotune(b, tr->A1, nullptr, nullptr));
     tr->a1_sigma = sig;
   }
   const int64_t stride = (bt->N + 63) & ~int64_t(63);

@@ -587,7 +587,6 @@ ohm_status ohm_newton_create(ohm_backend *b, const double *base_values,
     return s;
   };
   ohm_status s = csr_upload(b, c->A, base_values, col_indices, row_pointers, n, c->nnz);
-  if (s == OHM_OK && b->autotune) s = csr_autotune(b, c->A, nullptr, nullptr);
   if (s != OHM_OK) return cleanup(s);
   size_t ns = (size_t)kDiodeS * c->nd + (size_t)kBjtS * c->nq + (size_t)kMosS * c->nm;
 #define ALLOC(p, T, cnt)                                                      \
