@@ -36,6 +36,13 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# HBM traffic of one k_spmv<double> launch on the mesh matrix, from the two
+# rocprofv3 counter passes over `bench.py --roofline-only` (FETCH_SIZE, then
+# WRITE_SIZE, KiB per dispatch), each scaled by what the counter reads for a
+# copy kernel of known size (65,536 KiB each way): FETCH x 1.9994, WRITE x 1.0.
+# Summary: profiles/spmv_diode_mesh_256_counters.md.
+SPMV_TRAFFIC_BYTES = round((2911.07 * 1.9994 + 512.03 * 1.0) * 1024)
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -143,7 +150,9 @@ def headline(args, be, N, ir, solver):
         "roofline": {
             "bound": "hbm", "kernel": "k_spmv<double> on the 326,664-nnz mesh matrix",
             "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": spmv_b,
+            "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": SPMV_TRAFFIC_BYTES,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "bytes_per_launch": spmv_b,
             "avg_launch_us": ms.value * 1e3},
         "spmv_large_working_set": {
             "copies": 64, "bytes_per_launch": big_b, "fp64_gbs": big_gbs,

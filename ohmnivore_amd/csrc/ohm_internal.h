@@ -104,6 +104,8 @@ struct ohm_backend {
   std::vector<ohm::GraphEntry> graphs;
   bool use_graphs = true;
   bool autotune = true;              // tune row blocks of solver-internal matrices
+  // tuned row-block shapes by pattern (n, nnz, row-pointer hash)
+  std::vector<std::pair<std::vector<uint64_t>, std::pair<int, int>>> tune_cache;
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cu = 0;

@@ -326,6 +326,26 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
     if (nnz_out) *nnz_out = m.tuned_nnz;
     return OHM_OK;
   }
+  // a pattern tuned before keeps its shape (no timing)
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the row pointers
+  for (uint32_t v : m.h_rowptr) h = (h ^ v) * 1099511628211ull;
+  const std::vector<uint64_t> key = {(uint64_t)m.n, (uint64_t)m.nnz, h};
+  for (auto &e : b->tune_cache)
+    if (e.first == key) {
+      std::vector<uint32_t> blocks;
+      build_spmv_blocks(m.h_rowptr.data(), m.n, blocks, e.second.first, e.second.second);
+      uint32_t *nb = nullptr;
+      OHM_HIP(hipMalloc(&nb, sizeof(uint32_t) * blocks.size()));
+      OHM_HIP(hipMemcpy(nb, blocks.data(), sizeof(uint32_t) * blocks.size(), hipMemcpyHostToDevice));
+      if (m.blocks) OHM_HIP(hipFree(m.blocks));
+      m.blocks = nb;
+      m.n_blocks = (int)blocks.size() - 1;
+      m.tuned_rows = e.second.first;
+      m.tuned_nnz = e.second.second;
+      if (rows_out) *rows_out = m.tuned_rows;
+      if (nnz_out) *nnz_out = m.tuned_nnz;
+      return OHM_OK;
+    }
   static const int kRows[] = {32, 64, 128, 256};
   static const int kNnz[] = {512, 1024, 2048};
   double *x = nullptr, *y = nullptr;
@@ -388,6 +408,8 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
     m.tuned_rows = best_r;
     m.tuned_nnz = best_z;
     graph_cache_clear(b);
+    if (b->tune_cache.size() >= 64) b->tune_cache.erase(b->tune_cache.begin());
+    b->tune_cache.push_back({key, {best_r, best_z}});
   }
 done:
   (void)hipStreamSynchronize(b->stream);
