@@ -368,6 +368,17 @@ ohm_status ohm_newton_create(ohm_backend *b, const double *base_values,
                              size_t system_size, size_t matrix_nnz, ohm_newton_ctx **out);
 ohm_status ohm_newton_run(ohm_newton_ctx *ctx, const ohm_newton_params *params,
                           ohm_newton_result *result, int32_t *bicgstab_iters_per_newton);
+/* New base values (matrix_nnz), base RHS (system_size) and device
+ * parameters for the context's circuit (no counterpart in the reference,
+ * which recompiles per analysis; for loops that re-solve one topology with
+ * new parameter values). Any argument may be NULL to keep the current one;
+ * descriptor arrays have the counts given at create and must keep every node
+ * index and stamp slot (only the device parameters may differ), else
+ * OHM_ERR_INVALID. The symbolic preconditioner plan, tuned SpMV shapes and
+ * captured graphs of the context stay valid. */
+ohm_status ohm_newton_update(ohm_newton_ctx *ctx, const double *base_values,
+                             const double *base_b, const ohm_diode_desc *diodes,
+                             const ohm_bjt_desc *bjts, const ohm_mosfet_desc *mosfets);
 ohm_status ohm_newton_get_x(ohm_newton_ctx *ctx, double *x_out);
 /* The assembled A(x), b(x) of the last iteration (for checks and adjoints). */
 ohm_status ohm_newton_get_system(ohm_newton_ctx *ctx, double *values_out, double *b_out);

@@ -75,14 +75,20 @@ class DcProblem:
         return vals
 
     def solve(self, g_res=None, kp=None) -> np.ndarray:
-        """Operating point for the given conductances and kp values."""
+        """Operating point for the given conductances and kp values. The
+        first call builds the device context; later calls update its values
+        and parameters in place (same topology, so the preconditioner plan is
+        reused) and run Newton again from x = 0."""
         g = self.g_values(self.g0 if g_res is None else g_res)
         m = self.mosfets.copy()
         if kp is not None:
             m["kp"] = np.asarray(kp, dtype=np.float64).astype(np.float32)
-        gm = ir.CsrMatrix(self.n, self.n, g, self.mna.g.col_indices, self.mna.g.row_pointers)
-        self.ctx = NewtonContext(self.backend, gm, self.mna.b_dc, self.mna.diode_descriptors,
-                                 self.mna.bjt_descriptors, m)
+        if self.ctx is None:
+            gm = ir.CsrMatrix(self.n, self.n, g, self.mna.g.col_indices, self.mna.g.row_pointers)
+            self.ctx = NewtonContext(self.backend, gm, self.mna.b_dc, self.mna.diode_descriptors,
+                                     self.mna.bjt_descriptors, m)
+        else:
+            self.ctx.update(values=g, mosfets=m if len(m) else None)
         self.last_stats = self.ctx.run(**self.newton_kw)
         return self.ctx.x()
 

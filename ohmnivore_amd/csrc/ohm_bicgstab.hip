@@ -1,24 +1,30 @@
 // ohm_bicgstab.hip — right-preconditioned BiCGSTAB (bicgstab_with_params,
 // src/solver/bicgstab.rs:56-189) as a five-kernel fused pipeline per
-// iteration with every scalar (rho, alpha, omega, beta, status) resident in
-// device memory:
+// iteration with every scalar (rho, alpha, omega, status) resident in device
+// memory:
 //
-//   K1  p = r + beta (p - omega v);  p_hat = M^-1 p (Jacobi/none fused)
-//   K2  v = A p_hat  (+ r_hat.v -> alpha, breakdown test)
-//   K3  s = r - alpha v; s_hat = M^-1 s;  (+ ||s|| -> early exit test)
-//   K4  t = A s_hat  (+ t.s, t.t -> omega, breakdown test)
-//   K5  x += alpha p_hat + omega s_hat; r = s - omega t
-//       (+ ||r||, r_hat.r -> NaN/convergence/omega tests, beta for the next
-//        iteration, rho breakdown test)
+//   K1  [r.r, r_hat.r -> NaN/convergence/omega/rho tests, beta]
+//       p = r + beta (p - omega v);  p_hat = M^-1 p
+//   K2  v = A p_hat                         (partials r_hat.v, v.v)
+//   K3  [r_hat.v -> alpha, breakdown test]
+//       s = r - alpha v;  s_hat = M^-1 s    (partials s.s)
+//   K4  [||s|| early-exit test]  t = A s_hat  (partials t.s, t.t)
+//   K5  [t.t breakdown test, omega]
+//       x += alpha p_hat + omega s_hat; r = s - omega t  (partials r.r, r_hat.r)
 //
-// The last workgroup of each reducing kernel applies the reference's scalar
-// update and tests; once the state leaves RUNNING every later kernel exits
-// at its first instruction, so the host enqueues iterations in chunks and
-// polls one status word per chunk instead of syncing on each dot product
-// (the reference syncs six times per iteration, src/solver/backend.rs:443-446).
-// ISAI applies M^-1 = M_U M_L as two extra SpMV launches between K1/K2 and
-// K3/K4. The same kernels run in FP32 (with compensated dot products) as the
-// inner solver of FP64 iterative refinement.
+// Reductions are deferred: a producing kernel leaves one partial per
+// workgroup and every workgroup of the consuming kernel re-sums them in a
+// fixed order (the bracketed heads), so no kernel waits on an atomic ticket
+// and every workgroup derives the same scalars; workgroup 0 publishes them.
+// Once the state leaves RUNNING every later kernel exits at its first
+// instruction, so the host enqueues iterations in chunks closed by a
+// one-workgroup check and polls one status word per chunk instead of syncing
+// on each dot product (the reference syncs six times per iteration,
+// src/solver/backend.rs:443-446). Jacobi fuses into K1/K3 elementwise; ISAI
+// fuses the p and s updates into the first preconditioner SpMV (M = M_U M_L
+// when the product is formed, else M_L followed by an M_U launch). The same
+// kernels run in FP32 (with compensated dot products) as the inner solver of
+// FP64 iterative refinement.
 #include <algorithm>
 #include <cstddef>
 #include <cmath>
@@ -31,7 +37,7 @@ namespace ohm {
 __global__ void k_state_reset(SolveState *st) {
   if (threadIdx.x == 0) {
     SolveState z;
-    z.rho = z.alpha = z.omega = z.beta = 1.0;
+    z.rho = z.alpha = z.omega = z.beta = z.rho_new = 1.0;
     z.abs_tol = z.bnorm2 = z.rr = z.ss = z.rv = z.ts = z.tt = z.rhrh = 0.0;
     z.status = ST_RUNNING;
     z.s_conv = 0;
@@ -50,7 +56,85 @@ __device__ __forceinline__ bool rho_breakdown(double rho, double hh, double rr, 
   return scaled ? fabs(rho) <= 1e-30 * sqrt(hh * rr) : fabs(rho) < 1e-30;
 }
 
+// Partial-sum regions of one solve (doubles, `stride` workgroups each).
+struct Parts {
+  double *init;  // k_init's ticket scratch, 4 per workgroup
+  double *rv;    // K2: r_hat.v, v.v
+  double *ss;    // K3: s.s
+  double *ts;    // K4: t.s, t.t
+  double *rr;    // K5 (and k_init): r.r, r_hat.r
+  int n_rv, n_ss, n_ts, n_rr;  // partials each consumer sums (1 = ticket-folded)
+  bool t_rv, t_ss, t_ts, t_rr; // producer folds with the ticket
+};
+
+// A workgroup-uniform read of the status word. Workgroup 0 of a consuming
+// kernel may publish a stop while later workgroups start; every workgroup
+// takes either that status or the same decision from the same partials, so
+// reading it once per workgroup keeps all of its waves on one path.
+__device__ __forceinline__ bool block_running(const SolveState *st) {
+  __shared__ int s_run;
+  if (threadIdx.x == 0) s_run = st->status == ST_RUNNING;
+  __syncthreads();
+  return s_run != 0;
+}
+
+// End-of-iteration tests on ||r||^2, r_hat.r (src/solver/bicgstab.rs:158-185
+// in the reference's order), taken by the first kernel of the next iteration
+// and by the chunk's closing check. Returns the stop status or ST_RUNNING
+// with beta for the p update.
+__device__ __forceinline__ int decide_rho(const SolveState *st, double rr, double rhr,
+                                          double &beta) {
+  if (st->s_conv) return ST_CONVERGED;
+  const double rnorm = sqrt(rr);
+  if (std::isnan(rnorm) || std::isinf(rnorm)) return ST_NAN;
+  if (rnorm < st->abs_tol) return ST_CONVERGED;
+  if (fabs(st->omega) < 1e-30) return ST_BRK_OMEGA;
+  if (st->iters >= st->max_iters) return ST_MAXITER;
+  if (rho_breakdown(rhr, st->rhrh, rr, st->scaled)) return ST_BRK_RHO;
+  beta = (rhr / st->rho) * (st->alpha / st->omega);
+  return ST_RUNNING;
+}
+
+// p-update head: every workgroup forms beta from K5's partials; workgroup 0
+// publishes rho_new (or the stop status). False means: exit.
+__device__ __forceinline__ bool head_rho(SolveState *st, const Parts &pp, double &beta) {
+  if (!block_running(st)) return false;
+  double tot[2];
+  gather_sum<2>(pp.rr, pp.n_rr, tot);
+  const int stat = decide_rho(st, tot[0], tot[1], beta);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rr = tot[0];
+    if (stat != ST_RUNNING)
+      st->status = stat;
+    else
+      st->rho_new = tot[1];
+  }
+  return stat == ST_RUNNING;
+}
+
+// s-update head: alpha = rho_new / r_hat.v with the r_hat.v breakdown test.
+__device__ __forceinline__ bool head_alpha(SolveState *st, const Parts &pp, double &alpha) {
+  if (!block_running(st)) return false;
+  double tot[2];
+  gather_sum<2>(pp.rv, pp.n_rv, tot);
+  const bool brk = st->scaled ? fabs(tot[0]) <= 1e-30 * sqrt(st->rhrh * tot[1])
+                              : fabs(tot[0]) < 1e-30;
+  const double rho = st->rho_new;
+  alpha = rho / tot[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rv = tot[0];
+    if (brk) {
+      st->status = ST_BRK_RV;
+    } else {
+      st->alpha = alpha;
+      st->rho = rho;
+    }
+  }
+  return !brk;
+}
+
 // r = b - A x0 (warm) or b; r_hat = hashed or r; p = v = 0; x = 0 (cold).
+// Also leaves the r.r / r_hat.r partials where the first p update reads them.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
                                                  const T *__restrict__ ax0, T *__restrict__ x,
@@ -58,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
                                                  T *__restrict__ p, T *__restrict__ v, int shadow,
                                                  int warm, double tol, double floor_rel,
                                                  int tol_mode, int max_iters, int scaled,
-                                                 SolveState *st, double *partials) {
+                                                 SolveState *st, Parts pp) {
   Acc<T> bb, rr, rhr, hh;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     T bi = b[i];
@@ -75,7 +159,17 @@ __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
     hh.add_prod(rhi, rhi);
   }
   double vals[4] = {bb.get(), rr.get(), rhr.get(), hh.get()}, tot[4];
-  if (grid_sum<4>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
+  const bool last = grid_sum<4>(vals, pp.init, &st->counter, tot);
+  // grid_sum leaves this workgroup's sums in thread 0's vals[]
+  if (!pp.t_rr && threadIdx.x == 0) {
+    pp.rr[(size_t)blockIdx.x * 2] = vals[1];
+    pp.rr[(size_t)blockIdx.x * 2 + 1] = vals[2];
+  }
+  if (last && threadIdx.x == 0) {
+    if (pp.t_rr) {
+      pp.rr[0] = tot[1];
+      pp.rr[1] = tot[2];
+    }
     st->bnorm2 = tot[0];
     st->rhrh = tot[3];
     st->scaled = scaled;
@@ -93,29 +187,60 @@ __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
       st->status = ST_CONVERGED;
     } else if (rho_breakdown(tot[2], tot[3], tot[1], scaled)) {
       st->status = ST_BRK_RHO;
-    } else {
-      st->beta = tot[2];
-      st->rho = tot[2];
     }
+    // rho = alpha = omega = 1 from k_state_reset: the first beta multiplies
+    // p - omega v = 0, so p = r as in the reference's first iteration
   }
 }
 
+// p_j = r_j + beta (p_j - omega v_j), one rounding sequence for every use.
+template <typename T>
+__device__ __forceinline__ T p_next(T r, T p, T v, T beta, T omega) {
+  return fmac<T>(fmac<T>(-omega, v, p), beta, r);
+}
+
+// K1 (Jacobi / none): p update, p_hat = inv * p (or p).
 template <typename T, int PK>
 __global__ __launch_bounds__(kBlock) void k_p_update(int n, const T *__restrict__ r,
                                                      T *__restrict__ p, const T *__restrict__ v,
                                                      T *__restrict__ ph,
-                                                     const T *__restrict__ inv,
-                                                     const SolveState *st) {
-  if (st->status != ST_RUNNING) return;
-  const T beta = (T)st->beta, omega = (T)st->omega;
+                                                     const T *__restrict__ inv, SolveState *st,
+                                                     Parts pp) {
+  double beta_d;
+  if (!head_rho(st, pp, beta_d)) return;
+  const T beta = (T)beta_d, omega = (T)st->omega;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    T pi = (p[i] - omega * v[i]) * beta + r[i];
+    T pi = p_next<T>(r[i], p[i], v[i], beta, omega);
     p[i] = pi;
     if (PK == PK_JACOBI) ph[i] = inv[i] * pi;
     if (PK == PK_NONE) ph[i] = pi;
   }
 }
 
+// K1 (ISAI): the p update fused into the first preconditioner SpMV:
+// y = M1 p_new with p_new formed on the fly from p_old, so p is double
+// buffered (p_old is read at neighbour rows while p_new is written).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_p_spmv(const uint32_t *__restrict__ rowptr,
+                                                   const uint32_t *__restrict__ col,
+                                                   const T *__restrict__ val,
+                                                   const uint32_t *__restrict__ blocks,
+                                                   const T *__restrict__ r,
+                                                   const T *__restrict__ p_old,
+                                                   const T *__restrict__ v,
+                                                   T *__restrict__ p_new, T *__restrict__ y,
+                                                   SolveState *st, Parts pp) {
+  double beta_d;
+  if (!head_rho(st, pp, beta_d)) return;
+  const T beta = (T)beta_d, omega = (T)st->omega;
+  auto pf = [&](uint32_t j) { return p_next<T>(r[j], p_old[j], v[j], beta, omega); };
+  spmv_block_g<T>(rowptr, col, val, blocks, pf, [&](uint32_t i, T yi) {
+    y[i] = yi;
+    p_new[i] = pf(i);
+  });
+}
+
+// K2: v = A p_hat; partials r_hat.v, v.v.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv_alpha(const uint32_t *__restrict__ rowptr,
                                                        const uint32_t *__restrict__ col,
@@ -124,49 +249,66 @@ __global__ __launch_bounds__(kBlock) void k_spmv_alpha(const uint32_t *__restric
                                                        const T *__restrict__ ph,
                                                        T *__restrict__ v,
                                                        const T *__restrict__ rh, SolveState *st,
-                                                       double *partials) {
-  if (st->status != ST_RUNNING) return;
+                                                       Parts pp) {
+  if (!block_running(st)) return;
   Acc<T> acc, vv;
   spmv_block<T>(rowptr, col, val, blocks, ph, [&](uint32_t r, T y) {
     v[r] = y;
     acc.add_prod(rh[r], y);
     vv.add_prod(y, y);
   });
-  double vals[2] = {acc.get(), vv.get()}, tot[2];
-  if (grid_sum<2>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
-    st->rv = tot[0];
-    const bool brk = st->scaled ? fabs(tot[0]) <= 1e-30 * sqrt(st->rhrh * tot[1])
-                                : fabs(tot[0]) < 1e-30;
-    if (brk)
-      st->status = ST_BRK_RV;
-    else
-      st->alpha = st->rho / tot[0];
-  }
+  double vals[2] = {acc.get(), vv.get()};
+  deposit<2>(vals, pp.rv, &st->counter, pp.t_rv);
 }
 
+// K3 (Jacobi / none): alpha, s = r - alpha v, s_hat = inv * s; partial s.s.
 template <typename T, int PK>
 __global__ __launch_bounds__(kBlock) void k_s_update(int n, const T *__restrict__ r,
                                                      const T *__restrict__ v, T *__restrict__ s,
                                                      T *__restrict__ sh,
                                                      const T *__restrict__ inv, SolveState *st,
-                                                     double *partials) {
-  if (st->status != ST_RUNNING) return;
-  const T alpha = (T)st->alpha;
+                                                     Parts pp) {
+  double alpha_d;
+  if (!head_alpha(st, pp, alpha_d)) return;
+  const T alpha = (T)alpha_d;
   Acc<T> acc;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    T si = r[i] - alpha * v[i];
+    T si = fmac<T>(-alpha, v[i], r[i]);
     s[i] = si;
     if (PK == PK_JACOBI) sh[i] = inv[i] * si;
     if (PK == PK_NONE) sh[i] = si;
     acc.add_prod(si, si);
   }
-  double vals[1] = {acc.get()}, tot[1];
-  if (grid_sum<1>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
-    st->ss = tot[0];
-    if (sqrt(tot[0]) < st->abs_tol) st->s_conv = 1;
-  }
+  double vals[1] = {acc.get()};
+  deposit<1>(vals, pp.ss, &st->counter, pp.t_ss);
 }
 
+// K3 (ISAI): the s update fused into the first preconditioner SpMV
+// (y = M1 s with s_j = r_j - alpha v_j formed on the fly); partial s.s.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_s_spmv(const uint32_t *__restrict__ rowptr,
+                                                   const uint32_t *__restrict__ col,
+                                                   const T *__restrict__ val,
+                                                   const uint32_t *__restrict__ blocks,
+                                                   const T *__restrict__ r,
+                                                   const T *__restrict__ v, T *__restrict__ s,
+                                                   T *__restrict__ y, SolveState *st, Parts pp) {
+  double alpha_d;
+  if (!head_alpha(st, pp, alpha_d)) return;
+  const T alpha = (T)alpha_d;
+  auto sf = [&](uint32_t j) { return fmac<T>(-alpha, v[j], r[j]); };
+  Acc<T> acc;
+  spmv_block_g<T>(rowptr, col, val, blocks, sf, [&](uint32_t i, T yi) {
+    y[i] = yi;
+    T si = sf(i);
+    s[i] = si;
+    acc.add_prod(si, si);
+  });
+  double vals[1] = {acc.get()};
+  deposit<1>(vals, pp.ss, &st->counter, pp.t_ss);
+}
+
+// K4: the ||s|| early-exit test, then t = A s_hat; partials t.s, t.t.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv_omega(const uint32_t *__restrict__ rowptr,
                                                        const uint32_t *__restrict__ col,
@@ -175,26 +317,30 @@ __global__ __launch_bounds__(kBlock) void k_spmv_omega(const uint32_t *__restric
                                                        const T *__restrict__ sh,
                                                        T *__restrict__ t,
                                                        const T *__restrict__ s, SolveState *st,
-                                                       double *partials) {
-  if (st->status != ST_RUNNING || st->s_conv) return;
+                                                       Parts pp) {
+  if (!block_running(st)) return;
+  double ss[1];
+  gather_sum<1>(pp.ss, pp.n_ss, ss);
+  const bool sconv = sqrt(ss[0]) < st->abs_tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->ss = ss[0];
+    if (sconv) st->s_conv = 1;
+  }
+  if (sconv) return;
   Acc<T> ts, tt;
   spmv_block<T>(rowptr, col, val, blocks, sh, [&](uint32_t r, T y) {
     t[r] = y;
     ts.add_prod(y, s[r]);
     tt.add_prod(y, y);
   });
-  double vals[2] = {ts.get(), tt.get()}, tot[2];
-  if (grid_sum<2>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
-    st->ts = tot[0];
-    st->tt = tot[1];
-    const bool brk = st->scaled ? tot[1] <= 1e-30 * st->ss : fabs(tot[1]) < 1e-30;
-    if (brk)
-      st->status = ST_BRK_TT;
-    else
-      st->omega = tot[0] / tot[1];
-  }
+  double vals[2] = {ts.get(), tt.get()};
+  deposit<2>(vals, pp.ts, &st->counter, pp.t_ts);
 }
 
+// K5: omega = t.s / t.t with the t.t breakdown test; x += alpha p_hat +
+// omega s_hat, r = s - omega t; partials r.r, r_hat.r for the next p update.
+// After the ||s|| exit only x += alpha p_hat runs (src/solver/bicgstab.rs:
+// 128-135); the next kernel turns s_conv into CONVERGED.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
                                                       T *__restrict__ r,
@@ -203,47 +349,51 @@ __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
                                                       const T *__restrict__ sh,
                                                       const T *__restrict__ s,
                                                       const T *__restrict__ t, SolveState *st,
-                                                      double *partials) {
-  if (st->status != ST_RUNNING) return;
-  const int sconv = st->s_conv;
-  const T alpha = (T)st->alpha, omega = (T)st->omega;
-  Acc<T> rr, rhr;
-  if (sconv) {
+                                                      Parts pp) {
+  if (!block_running(st)) return;
+  const T alpha = (T)st->alpha;
+  if (st->s_conv) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
       x[i] = x[i] + alpha * ph[i];
-  } else {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-      x[i] = (x[i] + alpha * ph[i]) + omega * sh[i];
-      T ri = s[i] - omega * t[i];
-      r[i] = ri;
-      rr.add_prod(ri, ri);
-      rhr.add_prod(rh[i], ri);
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->iters += 1;
+    return;
   }
-  double vals[2] = {rr.get(), rhr.get()}, tot[2];
-  if (grid_sum<2>(vals, partials, &st->counter, tot) && threadIdx.x == 0) {
-    st->iters += 1;
-    if (sconv) {
-      st->status = ST_CONVERGED;
-      return;
-    }
-    st->rr = tot[0];
-    double rnorm = sqrt(tot[0]);
-    if (std::isnan(rnorm) || std::isinf(rnorm)) {
-      st->status = ST_NAN;
-    } else if (rnorm < st->abs_tol) {
-      st->status = ST_CONVERGED;
-    } else if (fabs(st->omega) < 1e-30) {
-      st->status = ST_BRK_OMEGA;
-    } else if (st->iters >= st->max_iters) {
-      st->status = ST_MAXITER;
-    } else if (rho_breakdown(tot[1], st->rhrh, tot[0], st->scaled)) {
-      st->status = ST_BRK_RHO;
+  double tot[2];
+  gather_sum<2>(pp.ts, pp.n_ts, tot);
+  const bool brk = st->scaled ? tot[1] <= 1e-30 * st->ss : fabs(tot[1]) < 1e-30;
+  const double omega_d = tot[0] / tot[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->ts = tot[0];
+    st->tt = tot[1];
+    if (brk) {
+      st->status = ST_BRK_TT;
     } else {
-      st->beta = (tot[1] / st->rho) * (st->alpha / st->omega);
-      st->rho = tot[1];
+      st->omega = omega_d;
+      st->iters += 1;
     }
   }
+  if (brk) return;
+  const T omega = (T)omega_d;
+  Acc<T> rr, rhr;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] = (x[i] + alpha * ph[i]) + omega * sh[i];
+    T ri = s[i] - omega * t[i];
+    r[i] = ri;
+    rr.add_prod(ri, ri);
+    rhr.add_prod(rh[i], ri);
+  }
+  double vals[2] = {rr.get(), rhr.get()};
+  deposit<2>(vals, pp.rr, &st->counter, pp.t_rr);
+}
+
+// Closing test of an enqueued chunk (one workgroup): publishes the stop
+// status of the chunk's last iteration so the host's poll sees it.
+__global__ __launch_bounds__(kBlock) void k_check(SolveState *st, Parts pp) {
+  if (!block_running(st)) return;
+  double tot[2], beta;
+  gather_sum<2>(pp.rr, pp.n_rr, tot);
+  const int stat = decide_rho(st, tot[0], tot[1], beta);
+  if (threadIdx.x == 0 && stat != ST_RUNNING) st->status = stat;
 }
 
 // ------------------------------------------------------------ workspace --
@@ -264,7 +414,7 @@ static ohm_status ws_get(Workspace<T> &w, int n) {
   if (w.base) OHM_HIP(hipFree(w.base));
   if (w.st) OHM_HIP(hipFree(w.st));
   size_t stride = ((size_t)std::max(n, 1) + 63) & ~size_t(63);
-  OHM_HIP(hipMalloc(&w.base, sizeof(T) * stride * 10));
+  OHM_HIP(hipMalloc(&w.base, sizeof(T) * stride * 11));
   OHM_HIP(hipMalloc(&w.st, sizeof(SolveState)));
   T *q = w.base;
   w.v.x = q;
@@ -277,6 +427,7 @@ static ohm_status ws_get(Workspace<T> &w, int n) {
   w.v.ph = q + 7 * stride;
   w.v.sh = q + 8 * stride;
   w.v.tmp = q + 9 * stride;
+  w.v.p2 = q + 10 * stride;
   w.n = n;
   return OHM_OK;
 }
@@ -310,38 +461,100 @@ struct Ops<float> {
   }
 };
 
+// First SpMV of the ISAI apply: the product M when formed, else M_L (or the
+// transpose view's first factor).
+static const DevCsr &isai_first(const PrecondDev &p) { return p.mm.rowptr ? p.mm : p.ml; }
+
+// Partial regions and consumer widths for one solve; sizes the backend's
+// partials buffer. A producer defers (plain per-workgroup partials) unless
+// its consumer would read too many of them: consumer workgroups x producer
+// workgroups beyond kDeferMaxReads, where the ticket's fixed tail is cheaper
+// than every consumer workgroup re-reading the partials.
+static ohm_status parts_for(ohm_backend *b, const DevCsr &a, const PrecondDev &p, int n,
+                            Parts &pp) {
+  const int64_t gv = grid_for(n, 4), ga = std::max(a.n_blocks, 1);
+  const bool isai = p.kind == PK_ISAI;
+  const int64_t gm = isai ? std::max(isai_first(p).n_blocks, 1) : gv;
+  const int64_t g3 = gm;  // K3 runs on the first ISAI SpMV's grid, else gv
+  auto ticket = [](int64_t producer, int64_t consumer) {
+    return producer > kDeferMaxBlocks || producer * consumer > kDeferMaxReads;
+  };
+  const size_t S = (size_t)std::max(std::max(gv, ga), g3);
+  OHM_TRY(ensure_partials(b, std::max<size_t>(S * 11, (size_t)kMaxReduceBlocks * 4)));
+  double *q = b->d_partials;
+  pp.init = q;
+  pp.rv = q + 4 * S;
+  pp.ss = q + 6 * S;
+  pp.ts = q + 7 * S;
+  pp.rr = q + 9 * S;
+  pp.t_rr = ticket(gv, gm);  // K5 -> K1 (on gm workgroups: gv, or the ISAI SpMV's)
+  pp.t_rv = ticket(ga, gm);  // K2 -> K3
+  pp.t_ss = ticket(g3, ga);  // K3 -> K4
+  pp.t_ts = ticket(ga, gv);  // K4 -> K5
+  pp.n_rv = pp.t_rv ? 1 : (int)ga;
+  pp.n_ss = pp.t_ss ? 1 : (int)g3;
+  pp.n_ts = pp.t_ts ? 1 : (int)ga;
+  pp.n_rr = pp.t_rr ? 1 : (int)gv;
+  return OHM_OK;
+}
+
+// One iteration: K1 .. K5 (ISAI: the p and s updates ride on the first
+// preconditioner SpMV; a two-factor apply adds its second SpMV after each).
+// `parity` selects which p buffer is current for the fused ISAI p update.
 template <typename T, int PK>
 static void enqueue_iteration(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
-                              BicgVecs<T> &v, SolveState *st, int n) {
+                              BicgVecs<T> &v, SolveState *st, int n, const Parts &pp,
+                              int parity) {
   const dim3 gv(grid_for(n, 4)), blk(kBlock), ga(a.n_blocks);
   const T *inv = Ops<T>::inv(p);
   const T *av = Ops<T>::vals(a);
-  hipLaunchKernelGGL((k_p_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.p, v.v, v.ph, inv, st);
-  if (PK == PK_ISAI) Ops<T>::prec(b, p, v.p, v.ph, v.tmp, st, 1);
-  hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks, v.ph,
-                     v.v, v.rh, st, b->d_partials);
-  hipLaunchKernelGGL((k_s_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.v, v.s, v.sh, inv, st,
-                     b->d_partials);
-  if (PK == PK_ISAI) Ops<T>::prec(b, p, v.s, v.sh, v.tmp, st, 2);
+  if (PK == PK_ISAI) {
+    const DevCsr &m1 = isai_first(p);
+    const bool two = !p.mm.rowptr;
+    T *p_old = parity ? v.p2 : v.p, *p_new = parity ? v.p : v.p2;
+    T *y1 = two ? v.tmp : v.ph;
+    hipLaunchKernelGGL(k_p_spmv<T>, dim3(m1.n_blocks), blk, 0, b->stream, m1.rowptr, m1.col,
+                       Ops<T>::vals(m1), m1.blocks, v.r, p_old, v.v, p_new, y1, st, pp);
+    if (two) Ops<T>::spmv(b, p.mu, v.tmp, v.ph, st, 1);
+    hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks,
+                       v.ph, v.v, v.rh, st, pp);
+    T *y3 = two ? v.tmp : v.sh;
+    hipLaunchKernelGGL(k_s_spmv<T>, dim3(m1.n_blocks), blk, 0, b->stream, m1.rowptr, m1.col,
+                       Ops<T>::vals(m1), m1.blocks, v.r, v.v, v.s, y3, st, pp);
+    if (two) Ops<T>::spmv(b, p.mu, v.tmp, v.sh, st, 1);
+  } else {
+    hipLaunchKernelGGL((k_p_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.p, v.v, v.ph, inv,
+                       st, pp);
+    hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks,
+                       v.ph, v.v, v.rh, st, pp);
+    hipLaunchKernelGGL((k_s_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.v, v.s, v.sh, inv,
+                       st, pp);
+  }
   hipLaunchKernelGGL(k_spmv_omega<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks, v.sh,
-                     v.t, v.s, st, b->d_partials);
+                     v.t, v.s, st, pp);
   hipLaunchKernelGGL(k_xr_update<T>, gv, blk, 0, b->stream, n, v.x, v.r, v.rh, v.ph, v.sh, v.s,
-                     v.t, st, b->d_partials);
+                     v.t, st, pp);
   b->dispatches += 5;
 }
 
 template <typename T>
 static void enqueue_any(ohm_backend *b, const DevCsr &a, const PrecondDev &p, BicgVecs<T> &v,
-                        SolveState *st, int n) {
+                        SolveState *st, int n, const Parts &pp, int parity) {
   if (p.kind == PK_JACOBI)
-    enqueue_iteration<T, PK_JACOBI>(b, a, p, v, st, n);
+    enqueue_iteration<T, PK_JACOBI>(b, a, p, v, st, n, pp, parity);
   else if (p.kind == PK_ISAI)
-    enqueue_iteration<T, PK_ISAI>(b, a, p, v, st, n);
+    enqueue_iteration<T, PK_ISAI>(b, a, p, v, st, n, pp, parity);
   else
-    enqueue_iteration<T, PK_NONE>(b, a, p, v, st, n);
+    enqueue_iteration<T, PK_NONE>(b, a, p, v, st, n, pp, parity);
+}
+
+static void enqueue_check(ohm_backend *b, SolveState *st, const Parts &pp) {
+  hipLaunchKernelGGL(k_check, dim3(1), dim3(kBlock), 0, b->stream, st, pp);
+  b->dispatches++;
 }
 
 constexpr int kGraphIters = 8;
+static_assert(kGraphIters % 2 == 0, "each replay must start on p buffer 0");
 
 void graph_cache_clear(ohm_backend *b) {
   for (auto &g : b->graphs) {
@@ -355,7 +568,8 @@ void graph_cache_clear(ohm_backend *b) {
 // holds every pointer and launch size baked into the captured kernels.
 template <typename T>
 static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
-                               BicgVecs<T> &v, SolveState *st, int n, int reps) {
+                               BicgVecs<T> &v, SolveState *st, int n, const Parts &pp,
+                               int reps) {
   std::vector<uintptr_t> key = {
       sizeof(T), (uintptr_t)p.kind, (uintptr_t)n, (uintptr_t)a.rowptr, (uintptr_t)a.col,
       (uintptr_t)Ops<T>::vals(a), (uintptr_t)a.blocks, (uintptr_t)a.n_blocks,
@@ -368,13 +582,14 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
   hipGraphExec_t exec = nullptr;
   for (auto &g : b->graphs)
     if (g.key == key) exec = g.exec;
-  const int64_t per_iter = 5 + (p.kind == PK_ISAI ? (p.mm.rowptr ? 2 : 4) : 0);
+  const int64_t per_iter = 5 + (p.kind == PK_ISAI && !p.mm.rowptr ? 2 : 0);
   if (!exec) {
     int64_t saved = b->dispatches;
     GraphEntry e;
     e.key = key;
     OHM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-    for (int k = 0; k < kGraphIters; k++) enqueue_any<T>(b, a, p, v, st, n);
+    for (int k = 0; k < kGraphIters; k++) enqueue_any<T>(b, a, p, v, st, n, pp, k & 1);
+    enqueue_check(b, st, pp);
     OHM_HIP(hipStreamEndCapture(b->stream, &e.graph));
     OHM_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
     b->dispatches = saved;
@@ -383,7 +598,7 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
     exec = e.exec;
   }
   for (int r = 0; r < reps; r++) OHM_HIP(hipGraphLaunch(exec, b->stream));
-  b->dispatches += per_iter * kGraphIters * reps;
+  b->dispatches += (per_iter * kGraphIters + 1) * reps;
   return OHM_OK;
 }
 
@@ -405,14 +620,15 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   const int n = a.n;
   SolveState *st = w.st;
   BicgVecs<T> &v = w.v;
-  OHM_TRY(ensure_partials(b, (size_t)std::max(a.n_blocks, kMaxReduceBlocks) * 4));
+  Parts pp;
+  OHM_TRY(parts_for(b, a, p, n, pp));
   hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, b->stream, st);
   if (o.warm) Ops<T>::spmv(b, a, v.x, v.t, nullptr, 0);
   hipLaunchKernelGGL(k_init<T>, dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream, n, rhs, v.t, v.x,
                      v.r, v.rh, v.p, v.v, o.shadow == OHM_SHADOW_HASHED ? 1 : 0, o.warm ? 1 : 0,
                      o.tol, o.tol_floor, o.tol_mode,
                      (int)std::min<int64_t>(o.max_iters, 0x7fffffff), o.scaled_breakdown ? 1 : 0,
-                     st, b->d_partials);
+                     st, pp);
   b->dispatches += 2;
   StatusWord *h = reinterpret_cast<StatusWord *>(b->h_pinned);
   int64_t enqueued = 0;
@@ -431,10 +647,12 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
       // replay a captured 8-iteration chunk; surplus iterations are no-ops
       // once the state leaves RUNNING
       int reps = (chunk + kGraphIters - 1) / kGraphIters;
-      OHM_TRY(replay_chunk<T>(b, a, p, v, st, n, reps));
+      OHM_TRY(replay_chunk<T>(b, a, p, v, st, n, pp, reps));
       chunk = reps * kGraphIters;
     } else {
-      for (int k = 0; k < chunk; k++) enqueue_any<T>(b, a, p, v, st, n);
+      for (int k = 0; k < chunk; k++)
+        enqueue_any<T>(b, a, p, v, st, n, pp, (int)((enqueued + k) & 1));
+      enqueue_check(b, st, pp);
     }
     enqueued += chunk;
     OHM_HIP(hipGetLastError());

@@ -40,6 +40,8 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kSpmvNnzPerBlock = 2048;   // LDS-staged products per row block
 constexpr int kSpmvRowsPerBlock = kBlock;
 constexpr int kMaxReduceBlocks = 2048;   // cap on grid size of reducing vector kernels
+constexpr int kDeferMaxBlocks = 2048;    // widest producer grid whose partials consumers re-sum
+constexpr int64_t kDeferMaxReads = int64_t(1) << 21;  // consumer x producer workgroups
 
 // ----------------------------------------------------------- solve state --
 enum SolveStatus : int {
@@ -55,6 +57,7 @@ enum SolveStatus : int {
 
 struct SolveState {
   double rho, alpha, omega, beta;
+  double rho_new;       // r_hat.r of the current iteration (rho once alpha is formed)
   double abs_tol;       // tolerance * ||b||
   double bnorm2;        // ||b||^2
   double rr;            // last ||r||^2
@@ -186,6 +189,7 @@ void launch_mul(ohm_backend *b, int n, const double *d, const double *x, double 
 template <typename T>
 struct BicgVecs {
   T *x, *r, *rh, *p, *v, *s, *t, *ph, *sh, *tmp;
+  T *p2;  // second p buffer of the fused ISAI p update
 };
 
 struct PrecondDev;  // defined in ohm_precond.h

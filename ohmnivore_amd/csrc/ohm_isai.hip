@@ -57,19 +57,46 @@ struct IsaiPlan {
   // product M = M_U M_L: per product entry, (M_U position, M_L position) pairs
   uint32_t *prod_ptr = nullptr, *prod_pairs = nullptr;
   int64_t prod_nnz = 0;
+  uint32_t *prod_long = nullptr;  // entries with more than kProdPairsPerThread pairs
+  int64_t n_long = 0;
 };
 
-// m[e] = sum over the entry's pairs of mu[q] * ml[t], in pair order
+// A product entry whose pair list is longer than this (a dense row of M_U
+// meeting a dense column of M_L, e.g. a supply node that touches every
+// stage) is summed by a whole workgroup instead of one thread.
+constexpr uint32_t kProdPairsPerThread = 64;
+
+// m[e] = sum over the entry's pairs of mu[q] * ml[t], in pair order, for
+// entries with at most kProdPairsPerThread pairs (one thread each).
 __global__ void k_isai_product(int64_t nm, const uint32_t *__restrict__ ptr,
                                const uint32_t *__restrict__ pairs, const double *__restrict__ mu,
                                const double *__restrict__ ml, double *__restrict__ m,
                                float *__restrict__ m32) {
   for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < nm;
        e += (int64_t)gridDim.x * kBlock) {
+    const uint32_t k0 = ptr[e], k1 = ptr[e + 1];
+    if (k1 - k0 > kProdPairsPerThread) continue;
     double acc = 0.0;
-    for (uint32_t k = ptr[e]; k < ptr[e + 1]; k++) acc = fma(mu[pairs[2 * k]], ml[pairs[2 * k + 1]], acc);
+    for (uint32_t k = k0; k < k1; k++) acc = fma(mu[pairs[2 * k]], ml[pairs[2 * k + 1]], acc);
     m[e] = acc;
     if (m32) m32[e] = (float)acc;
+  }
+}
+
+// The long entries: one workgroup each, pairs strided over the threads and
+// combined by the fixed-order workgroup sum.
+__global__ __launch_bounds__(kBlock) void k_isai_product_long(
+    const uint32_t *__restrict__ longs, const uint32_t *__restrict__ ptr,
+    const uint32_t *__restrict__ pairs, const double *__restrict__ mu,
+    const double *__restrict__ ml, double *__restrict__ m, float *__restrict__ m32) {
+  const uint32_t e = longs[blockIdx.x];
+  double acc[1] = {0.0};
+  for (uint32_t k = ptr[e] + threadIdx.x; k < ptr[e + 1]; k += kBlock)
+    acc[0] = fma(mu[pairs[2 * k]], ml[pairs[2 * k + 1]], acc[0]);
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) {
+    m[e] = acc[0];
+    if (m32) m32[e] = (float)acc[0];
   }
 }
 
@@ -346,9 +373,14 @@ static ohm_status build_product(ohm_backend *b, PrecondDev &p) {
     if ((int64_t)cl.size() > cap) return OHM_OK;  // too dense: keep the two-SpMV apply
   }
   pl->prod_nnz = (int64_t)cl.size();
+  std::vector<uint32_t> longs;
+  for (size_t e = 0; e + 1 < ptr.size(); e++)
+    if (ptr[e + 1] - ptr[e] > kProdPairsPerThread) longs.push_back((uint32_t)e);
+  pl->n_long = (int64_t)longs.size();
   OHM_TRY(csr_upload(b, p.mm, nullptr, cl.data(), rp.data(), n, pl->prod_nnz));
   OHM_TRY(upload(&pl->prod_ptr, ptr));
   OHM_TRY(upload(&pl->prod_pairs, pairs));
+  if (!longs.empty()) OHM_TRY(upload(&pl->prod_long, longs));
   return OHM_OK;
 }
 
@@ -456,6 +488,12 @@ static ohm_status isai_numeric(ohm_backend *b, const DevCsr &a, PrecondDev &p) {
                        pl->prod_nnz, pl->prod_ptr, pl->prod_pairs, p.mu.val, p.ml.val, p.mm.val,
                        p.has_f32 ? p.mm.val32 : nullptr);
     b->dispatches++;
+    if (pl->n_long > 0) {
+      hipLaunchKernelGGL(k_isai_product_long, dim3((unsigned)pl->n_long), blk, 0, b->stream,
+                         pl->prod_long, pl->prod_ptr, pl->prod_pairs, p.mu.val, p.ml.val, p.mm.val,
+                         p.has_f32 ? p.mm.val32 : nullptr);
+      b->dispatches++;
+    }
   }
   OHM_HIP(hipGetLastError());
   return OHM_OK;
@@ -587,7 +625,8 @@ void precond_free(PrecondDev &p) {
   if (p.plan) {
     IsaiPlan *pl = p.plan;
     void *ptrs[] = {pl->f_rowptr, pl->f_col, pl->f_from_a, pl->f_div, pl->cp_ptr,
-                    pl->cp_pairs, pl->f_val, pl->f_tmp, pl->prod_ptr, pl->prod_pairs};
+                    pl->cp_pairs, pl->f_val, pl->f_tmp, pl->prod_ptr, pl->prod_pairs,
+                    pl->prod_long};
     for (void *q : ptrs)
       if (q) (void)hipFree(q);
     for (auto &sd : pl->side) {

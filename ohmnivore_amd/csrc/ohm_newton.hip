@@ -406,6 +406,11 @@ struct ohm_newton_ctx {
   int prec_level = 1;
   double *adj = nullptr;  // adjoint scratch: rhs, lambda, gradients (grown on demand)
   size_t adj_cap = 0;
+  int zero_diag = -1;     // base matrix has a zero diagonal (-1: not yet checked)
+  // host copies of the descriptors: ohm_newton_update keeps their topology
+  std::vector<ohm_diode_desc> h_dd;
+  std::vector<ohm_bjt_desc> h_qd;
+  std::vector<ohm_mosfet_desc> h_md;
 };
 
 namespace {
@@ -503,6 +508,37 @@ void enqueue_limit(ohm_newton_ctx *c, const double *xo, double *xn) {
   b->dispatches += 5;
 }
 
+bool same_topology(const ohm_diode_desc &a, const ohm_diode_desc &b) {
+  return a.anode_idx == b.anode_idx && a.cathode_idx == b.cathode_idx &&
+         !std::memcmp(a.g_row_col, b.g_row_col, sizeof a.g_row_col) &&
+         !std::memcmp(a.b_idx, b.b_idx, sizeof a.b_idx);
+}
+bool same_topology(const ohm_bjt_desc &a, const ohm_bjt_desc &b) {
+  return a.collector_idx == b.collector_idx && a.base_idx == b.base_idx &&
+         a.emitter_idx == b.emitter_idx &&
+         !std::memcmp(a.node_fixed, b.node_fixed, sizeof a.node_fixed) &&
+         !std::memcmp(a.g_row_col, b.g_row_col, sizeof a.g_row_col) &&
+         !std::memcmp(a.b_idx, b.b_idx, sizeof a.b_idx);
+}
+bool same_topology(const ohm_mosfet_desc &a, const ohm_mosfet_desc &b) {
+  return a.drain_idx == b.drain_idx && a.gate_idx == b.gate_idx &&
+         a.source_idx == b.source_idx && a.node_fixed_packed == b.node_fixed_packed &&
+         !std::memcmp(a.g_row_col, b.g_row_col, sizeof a.g_row_col) &&
+         !std::memcmp(a.b_idx, b.b_idx, sizeof a.b_idx);
+}
+template <typename D>
+ohm_status replace_desc(std::vector<D> &host, D *dev, const D *in, const char *what,
+                        hipStream_t st) {
+  if (!in) return OHM_OK;
+  for (size_t k = 0; k < host.size(); k++)
+    if (!same_topology(host[k], in[k]))
+      return fail(OHM_ERR_INVALID,
+                  "%s descriptor %zu changes nodes or stamp slots: create a new context", what, k);
+  host.assign(in, in + host.size());
+  if (!host.empty())
+    OHM_HIP(hipMemcpyAsync(dev, host.data(), sizeof(D) * host.size(), hipMemcpyHostToDevice, st));
+  return OHM_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -659,9 +695,38 @@ ohm_status ohm_newton_create(ohm_backend *b, const double *base_values,
     o[2] = bjts[k].emitter_idx;
     return 3;
   });
+  c->h_dd.assign(diodes, diodes + n_diodes);
+  c->h_qd.assign(bjts, bjts + n_bjts);
+  c->h_md.assign(mosfets, mosfets + n_mosfets);
   hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) return cleanup(hip_fail(e, "newton upload", __FILE__, __LINE__));
   *out = c;
+  return OHM_OK;
+}
+
+ohm_status ohm_newton_update(ohm_newton_ctx *c, const double *base_values, const double *base_b,
+                             const ohm_diode_desc *diodes, const ohm_bjt_desc *bjts,
+                             const ohm_mosfet_desc *mosfets) {
+  if (!c) return fail(OHM_ERR_INVALID, "null context");
+  hipStream_t st = c->b->stream;
+  if (base_values) {
+    for (int64_t k = 0; k < c->nnz; k++)
+      if (!std::isfinite(base_values[k]))
+        return fail(OHM_ERR_NEWTON_NUMERICAL,
+                    "Newton solver encountered numerical error at iteration 0");
+  }
+  OHM_TRY(replace_desc(c->h_dd, c->dd, diodes, "diode", st));
+  OHM_TRY(replace_desc(c->h_qd, c->qd, bjts, "BJT", st));
+  OHM_TRY(replace_desc(c->h_md, c->md, mosfets, "MOSFET", st));
+  if (base_values && c->nnz) {
+    OHM_HIP(hipMemcpyAsync(c->base_val, base_values, sizeof(double) * c->nnz,
+                           hipMemcpyHostToDevice, st));
+    c->zero_diag = -1;
+  }
+  if (base_b && c->n)
+    OHM_HIP(hipMemcpyAsync(c->base_b, base_b, sizeof(double) * c->n, hipMemcpyHostToDevice, st));
+  // the host arrays are the caller's: finish the copies before returning
+  OHM_HIP(hipStreamSynchronize(st));
   return OHM_OK;
 }
 
@@ -693,9 +758,12 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
   // the assembled matrix has a zero diagonal, else Jacobi. Device stamps only
   // add to node diagonals, so the base matrix decides.
   int kind = prm.precond;
-  if (kind == OHM_PREC_AUTO) kind = matrix_has_zero_diag(b, c->A, nullptr) ? OHM_PREC_ISAI : OHM_PREC_JACOBI;
   // the assembled values start from base
   OHM_HIP(hipMemcpyAsync(c->A.val, c->base_val, sizeof(double) * c->nnz, hipMemcpyDeviceToDevice, st));
+  if (kind == OHM_PREC_AUTO) {
+    if (c->zero_diag < 0) c->zero_diag = matrix_has_zero_diag(b, c->A, nullptr) ? 1 : 0;
+    kind = c->zero_diag ? OHM_PREC_ISAI : OHM_PREC_JACOBI;
+  }
   OHM_TRY(newton_prepare_precond(c, kind));
   isai_reset(c->P);  // restart the ILU fixed point from the matrix for a fresh solve
   OHM_HIP(hipEventRecord(b->ev0, st));

@@ -1,7 +1,7 @@
-// ohm_reduce.h — wave64 / workgroup reductions and the deterministic
-// "last workgroup finishes the reduction" pattern used by every fused
-// kernel. Partial sums are combined in a fixed order, so results are
-// bitwise reproducible run to run.
+// ohm_reduce.h — wave64 / workgroup reductions, the deterministic "last
+// workgroup finishes the reduction" pattern, and deferred reductions whose
+// consumer re-sums the producer's partials. Partial sums are combined in a
+// fixed order, so results are bitwise reproducible run to run.
 #pragma once
 
 #include "ohm_internal.h"
@@ -105,6 +105,54 @@ __device__ __forceinline__ bool grid_sum(double (&v)[K], double *partials, unsig
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return true;
+}
+
+// Producer side of a deferred reduction. Normally each workgroup stores its
+// K block sums at partials[blockIdx.x * K ...] with plain stores and the
+// kernel boundary publishes them; the consuming kernel sums them itself
+// (gather_sum). Where every consumer workgroup would read too many partials
+// (wide producer and consumer grids) the ticket pattern (grid_sum) folds
+// them in the producer instead and the totals land in partials[0..K): the
+// consumer then gathers a single partial.
+template <int K>
+__device__ __forceinline__ void deposit(double (&v)[K], double *partials, unsigned *counter,
+                                        bool ticket) {
+  if (ticket) {
+    double tot[K];
+    if (grid_sum<K>(v, partials, counter, tot) && threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) partials[k] = tot[k];
+    }
+  } else {
+    block_sum<K>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) partials[(size_t)blockIdx.x * K + k] = v[k];
+    }
+  }
+}
+
+// Consumer side: sums np partials in the same order grid_sum uses, so every
+// workgroup of the consuming kernel holds bitwise-identical totals (in every
+// thread) and takes the same scalar decisions.
+template <int K>
+__device__ __forceinline__ void gather_sum(const double *__restrict__ partials, int np,
+                                           double (&tot)[K]) {
+  __shared__ double bc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) tot[k] = 0.0;
+  for (int i = threadIdx.x; i < np; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] += partials[(size_t)i * K + k];
+  }
+  block_sum<K>(tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) bc[k] = tot[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) tot[k] = bc[k];
 }
 
 // Dense shadow residual value in [-1, 1): SplitMix64 finaliser of (i+1)*gamma.

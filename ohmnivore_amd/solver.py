@@ -565,6 +565,7 @@ class NewtonContext:
         q, nq = _descs(bjts, ir.BJT_DESC_DTYPE)
         m, nm = _descs(mosfets, ir.MOSFET_DESC_DTYPE)
         self.n_mosfets = nm
+        self._counts = (nd, nq, nm)
         self._descs_keep = (d, q, m)
         self.h = ctypes.c_void_p()
         check(lib().ohm_newton_create(
@@ -572,6 +573,34 @@ class NewtonContext:
             ptr(self._b, N.c_f64p), vp(d), ctypes.c_size_t(nd), vp(q), ctypes.c_size_t(nq),
             vp(m), ctypes.c_size_t(nm), ctypes.c_size_t(self.n), ctypes.c_size_t(self.nnz),
             ctypes.byref(self.h)))
+
+    def update(self, values=None, base_b=None, diodes=None, bjts=None, mosfets=None):
+        """ohm_newton_update: new base matrix values / RHS / device
+        parameters on the same topology (None keeps the current ones); the
+        preconditioner plan built by the first run is reused."""
+        def arr(a, dtype):
+            return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+        v, b = arr(values, np.float64), arr(base_b, np.float64)
+        if v is not None and v.shape[0] != self.nnz:
+            raise ValueError(f"values: expected {self.nnz} entries, got {v.shape[0]}")
+        if b is not None and b.shape[0] != self.n:
+            raise ValueError(f"base_b: expected {self.n} entries, got {b.shape[0]}")
+        descs = []
+        for a, dtype, count in zip((diodes, bjts, mosfets), (ir.DIODE_DESC_DTYPE, ir.BJT_DESC_DTYPE,
+                                                             ir.MOSFET_DESC_DTYPE), self._counts):
+            d = arr(a, dtype)
+            if d is not None and len(d) != count:
+                raise ValueError(f"descriptor arrays must keep the counts given at create "
+                                 f"({count}, got {len(d)})")
+            descs.append(d)
+        check(lib().ohm_newton_update(
+            self.h, ptr(v, N.c_f64p) if v is not None else None,
+            ptr(b, N.c_f64p) if b is not None else None,
+            *[vp(d) if d is not None and len(d) else None for d in descs]))
+        if v is not None:
+            self._v = v
+        if b is not None:
+            self._b = b
 
     def run(self, **kw) -> NewtonStats:
         p, keep = newton_params(**kw)

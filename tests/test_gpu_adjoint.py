@@ -146,3 +146,27 @@ def test_autograd_function_gradcheck(be):
     # kp is stored in f32 descriptors: eps = 1e-3 keeps its quantisation
     # (6e-8 relative) well below the check's tolerance
     assert torch.autograd.gradcheck(f, (ug, uk), eps=1e-3, atol=1e-6, rtol=2e-3)
+
+
+def test_newton_update_matches_fresh_context(be):
+    """ohm_newton_update (new conductances and kp on one topology) gives the
+    operating point a freshly built context gives, and rejects descriptors
+    whose nodes or stamp slots differ."""
+    ckt = ir.amplifier_array(256, seed=11)
+    mna = solver.compile(ckt)
+    prob = A.DcProblem(be, ckt, mna, newton_kw=dict(abs_tol=1e-13, bicgstab_tolerance=1e-14))
+    x0 = prob.solve()
+    rng = np.random.default_rng(5)
+    g = prob.g0 * (1 + 0.1 * rng.random(prob.g0.shape[0]))
+    kp = prob.kp0 * (1 + 0.2 * rng.random(prob.kp0.shape[0]))
+    x_upd = prob.solve(g, kp)
+    fresh = A.DcProblem(be, ckt, mna, newton_kw=prob.newton_kw)
+    x_new = fresh.solve(g, kp)
+    assert np.allclose(x_upd, x_new, rtol=1e-9, atol=1e-12), np.abs(x_upd - x_new).max()
+    assert not np.allclose(x_upd, x0, rtol=1e-6, atol=1e-9)  # the update changed the answer
+    bad = prob.mosfets.copy()
+    bad["drain_idx"][0] = bad["gate_idx"][0]
+    with pytest.raises(N.OhmnivoreError, match="create a new context"):
+        prob.ctx.update(mosfets=bad)
+    with pytest.raises(ValueError):
+        prob.ctx.update(values=np.zeros(3))

@@ -255,3 +255,28 @@ def test_autotune_keeps_results(be):
         be.bicgstab(m, bb, xx, p, solver.bicgstab_params(tolerance=1e-13, max_iterations=20000))
         ok, err = close(be.download_vec(xx), ref)
         assert ok, err
+
+
+@pytest.mark.parametrize("precond", [N.PREC_JACOBI, N.PREC_ISAI])
+def test_wide_grid_ticket_reductions(be, precond):
+    """600,000 rows: the SpMV kernels run more than 2,048 workgroups, so their
+    dot products fold in the producer (ticket) instead of being re-summed by
+    every consumer workgroup. The solve still matches the banded LU."""
+    n = 600_000
+    rng = np.random.default_rng(21)
+    lo, up = -rng.uniform(0.5, 1.0, n - 1), -rng.uniform(0.5, 1.0, n - 1)
+    diag = 2.5 + rng.uniform(0.0, 1.0, n)
+    rows = np.concatenate([np.arange(1, n), np.arange(n), np.arange(n - 1)])
+    cols = np.concatenate([np.arange(n - 1), np.arange(n), np.arange(1, n)])
+    vals = np.concatenate([lo, diag, up])
+    a = oracle.from_triplet_arrays(n, n, rows, cols, vals)
+    b = rng.standard_normal(n)
+    ref = oracle.sparse_lu_solve(a, b)
+    m = be.upload_matrix(a.values, a.col_indices, a.row_pointers, n)
+    p = be.preconditioner(m, precond)
+    bb, xx = be.new_buffer(n), be.new_buffer(n)
+    be.upload_vec(b, bb)
+    s = be.bicgstab(m, bb, xx, p, solver.bicgstab_params(tolerance=1e-13, max_iterations=5000))
+    ok, err = close(be.download_vec(xx), ref)
+    assert ok, (err, s.iterations)
+    assert s.iterations > 3
