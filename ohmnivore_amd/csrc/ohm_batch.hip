@@ -18,8 +18,9 @@
 // tagged with its system; every kernel of an iteration runs over that table,
 // so SpMV and vector updates of all systems go out in one launch. Dot
 // products reduce per system: each block writes its partials to a fixed
-// slot and the last block of a system (per-system ticket) sums that system's
-// slots in block order, so results are bitwise reproducible. Every system
+// slot and a fold kernel with one workgroup per system sums that system's
+// slots in block order and applies the scalar update, so results are
+// bitwise reproducible and no block waits on a contended counter. Every system
 // keeps its own scalars and status (its mask); blocks of finished systems
 // exit at their first instruction, and the host drops them from the table
 // when it polls (compaction), so converged frequency points stop costing
@@ -249,6 +250,31 @@ __device__ __forceinline__ bool sys_sum(double (&v)[K], const BBlock &bk, const 
   return true;
 }
 
+// Producer side: this block's K sums go to its partial slot (plain stores;
+// the kernel boundary publishes them to the fold kernel).
+template <int K>
+__device__ __forceinline__ void bstore(double (&v)[K], const BBlock &bk, double *partials) {
+  block_sum<K>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partials[(size_t)bk.slot * kBatchPartials + k] = v[k];
+  }
+}
+
+// Fold side: one workgroup per system sums the system's slots in block
+// order (the order sys_sum's last block uses); totals in thread 0's tot[].
+template <int K>
+__device__ __forceinline__ void fold_sum(const BGeo &q, const double *__restrict__ partials,
+                                         double (&tot)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; k++) tot[k] = 0.0;
+  for (int i = q.first_block + threadIdx.x; i < q.first_block + q.n_blocks; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] += partials[(size_t)i * kBatchPartials + k];
+  }
+  block_sum<K>(tot);
+}
+
 template <typename S>
 __device__ __forceinline__ S wg_sum(S v) {
   double t[2] = {re_of(v), im_of(v)};
@@ -414,8 +440,23 @@ __global__ __launch_bounds__(kBlock) void k_binit(const BBlock *__restrict__ blo
     hh.add(hi);
     hr.add(hi, ri);
   }
-  double vals[5] = {bb.s.get(), rr.s.get(), hr.re.get(), hr.im.get(), hh.s.get()}, tot[5];
-  if (sys_sum<5>(vals, bk, q, st, partials, tot) && threadIdx.x == 0) {
+  double vals[5] = {bb.s.get(), rr.s.get(), hr.re.get(), hr.im.get(), hh.s.get()};
+  bstore<5>(vals, bk, partials);
+}
+
+// Per-system folds: one workgroup per system (grid = K) reduces the
+// partials of the producing kernel and applies the reference's scalar
+// update and tests for that system. Each system's state is written by its
+// own workgroup only.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_bfold_init(const BGeo *__restrict__ geo, BSys *sys,
+                                                       BInitArgs a,
+                                                       const double *__restrict__ partials) {
+  BSys *st = &sys[blockIdx.x];
+  if (st->status != ST_RUNNING) return;
+  double tot[5];
+  fold_sum<5>(geo[blockIdx.x], partials, tot);
+  if (threadIdx.x == 0) {
     st->bnorm2 = tot[0];
     st->rr = tot[1];
     st->rhrh = tot[4];
@@ -482,8 +523,18 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_alpha(const BBlock *__restrict
     hv.add(hs[r], y);
     vv.add(y);
   });
-  double vals[3] = {hv.re.get(), hv.im.get(), vv.s.get()}, tot[3];
-  if (sys_sum<3>(vals, bk, q, st, partials, tot) && threadIdx.x == 0) {
+  double vals[3] = {hv.re.get(), hv.im.get(), vv.s.get()};
+  bstore<3>(vals, bk, partials);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_bfold_alpha(const BGeo *__restrict__ geo, BSys *sys,
+                                                        const double *__restrict__ partials) {
+  BSys *st = &sys[blockIdx.x];
+  if (st->status != ST_RUNNING) return;
+  double tot[3];
+  fold_sum<3>(geo[blockIdx.x], partials, tot);
+  if (threadIdx.x == 0) {
     const zd rv = {tot[0], tot[1]};
     if (brk_small(sqrt(zabs2(rv)), sqrt(st->rhrh * tot[2]), st->scaled,
                   Tr<S>::C ? 1e-15 : 1e-30)) {
@@ -517,8 +568,17 @@ __global__ __launch_bounds__(kBlock) void k_bs(const BBlock *__restrict__ blocks
     sh[e] = PK == PK_JACOBI ? inv[e] * si : si;
     ss.add(si);
   }
-  double vals[1] = {ss.s.get()}, tot[1];
-  if (sys_sum<1>(vals, bk, q, st, partials, tot) && threadIdx.x == 0) {
+  double vals[1] = {ss.s.get()};
+  bstore<1>(vals, bk, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfold_s(const BGeo *__restrict__ geo, BSys *sys,
+                                                    const double *__restrict__ partials) {
+  BSys *st = &sys[blockIdx.x];
+  if (st->status != ST_RUNNING) return;
+  double tot[1];
+  fold_sum<1>(geo[blockIdx.x], partials, tot);
+  if (threadIdx.x == 0) {
     st->ss = tot[0];
     if (sqrt(tot[0]) < st->abs_tol) st->s_conv = 1;
   }
@@ -545,8 +605,18 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_omega(const BBlock *__restrict
     ts.add(y, sv[r]);
     tt.add(y);
   });
-  double vals[3] = {ts.re.get(), ts.im.get(), tt.s.get()}, tot[3];
-  if (sys_sum<3>(vals, bk, q, st, partials, tot) && threadIdx.x == 0) {
+  double vals[3] = {ts.re.get(), ts.im.get(), tt.s.get()};
+  bstore<3>(vals, bk, partials);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_bfold_omega(const BGeo *__restrict__ geo, BSys *sys,
+                                                        const double *__restrict__ partials) {
+  BSys *st = &sys[blockIdx.x];
+  if (st->status != ST_RUNNING || st->s_conv) return;
+  double tot[3];
+  fold_sum<3>(geo[blockIdx.x], partials, tot);
+  if (threadIdx.x == 0) {
     const bool brk = st->scaled ? tot[2] <= 1e-30 * st->ss
                                 : (Tr<S>::C ? tot[2] < 1e-15 : fabs(tot[2]) < 1e-30);
     if (brk) {
@@ -587,8 +657,19 @@ __global__ __launch_bounds__(kBlock) void k_bxr(const BBlock *__restrict__ block
       hr.add(rh[e], ri);
     }
   }
-  double vals[3] = {rr.s.get(), hr.re.get(), hr.im.get()}, tot[3];
-  if (sys_sum<3>(vals, bk, q, st, partials, tot) && threadIdx.x == 0) {
+  double vals[3] = {rr.s.get(), hr.re.get(), hr.im.get()};
+  bstore<3>(vals, bk, partials);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_bfold_xr(const BGeo *__restrict__ geo, BSys *sys,
+                                                     const double *__restrict__ partials) {
+  BSys *st = &sys[blockIdx.x];
+  if (st->status != ST_RUNNING) return;
+  const int sconv = st->s_conv;
+  double tot[3];
+  fold_sum<3>(geo[blockIdx.x], partials, tot);
+  if (threadIdx.x == 0) {
     st->iters += 1;
     if (sconv) {
       st->status = ST_CONVERGED;
@@ -977,22 +1058,28 @@ std::vector<BBlock> active_table(const ohm_batch *bt, const std::vector<char> &r
   return out;
 }
 
+// One iteration over the active block table: each reducing kernel is
+// followed by its per-system fold (grid = K systems).
 template <typename S, int PK>
 void enqueue_iter(ohm_batch *bt, BW<S> &w, int nblk) {
   hipStream_t st = bt->owner->stream;
-  const dim3 g(nblk), blk(kBlock);
+  const dim3 g(nblk), gk(bt->K), blk(kBlock);
   hipLaunchKernelGGL((k_bp<S, PK>), g, blk, 0, st, bt->d_active, bt->d_geo, bt->d_sys, w.r, w.p,
                      w.v, w.ph, w.inv);
   hipLaunchKernelGGL(k_bspmv_alpha<S>, g, blk, 0, st, bt->d_active, bt->d_geo, bt->d_sys, w.ph,
                      w.v, w.rh, bt->d_partials);
+  hipLaunchKernelGGL(k_bfold_alpha<S>, gk, blk, 0, st, bt->d_geo, bt->d_sys, bt->d_partials);
   hipLaunchKernelGGL((k_bs<S, PK>), g, blk, 0, st, bt->d_active, bt->d_geo, bt->d_sys, w.r, w.v,
                      w.s, w.sh, w.inv, bt->d_partials);
+  hipLaunchKernelGGL(k_bfold_s, gk, blk, 0, st, bt->d_geo, bt->d_sys, bt->d_partials);
   hipLaunchKernelGGL(k_bspmv_omega<S>, g, blk, 0, st, bt->d_active, bt->d_geo, bt->d_sys, w.sh,
                      w.t, w.s, bt->d_partials);
+  hipLaunchKernelGGL(k_bfold_omega<S>, gk, blk, 0, st, bt->d_geo, bt->d_sys, bt->d_partials);
   hipLaunchKernelGGL(k_bxr<S>, g, blk, 0, st, bt->d_active, bt->d_geo, bt->d_sys, w.x, w.r, w.rh,
                      w.ph, w.sh, w.s, w.t, bt->d_partials);
-  bt->dispatches += 5;
-  bt->owner->dispatches += 5;
+  hipLaunchKernelGGL(k_bfold_xr<S>, gk, blk, 0, st, bt->d_geo, bt->d_sys, bt->d_partials);
+  bt->dispatches += 9;
+  bt->owner->dispatches += 9;
 }
 
 struct BatchOpts {
@@ -1079,8 +1166,10 @@ ohm_status core_solve(ohm_batch *bt, BW<S> &w, const std::vector<char> &run, con
   hipLaunchKernelGGL(k_binit<S>, dim3(tab.size()), dim3(kBlock), 0, stream, bt->d_active,
                      bt->d_geo, bt->d_sys, w.b, w.ax, w.x, w.r, w.rh, w.p, w.v, ia,
                      bt->d_partials);
-  bt->dispatches++;
-  b->dispatches++;
+  hipLaunchKernelGGL(k_bfold_init<S>, dim3(K), dim3(kBlock), 0, stream, bt->d_geo, bt->d_sys, ia,
+                     bt->d_partials);
+  bt->dispatches += 2;
+  b->dispatches += 2;
   std::vector<char> live = run;
   int64_t done = 0;
   for (;;) {
@@ -1576,11 +1665,11 @@ ohm_status ohm_solve_complex(ohm_backend *b, size_t n, size_t nnz, const double 
   OHM_TRY(ohm_batch_create(b, 1, 1, &sys, &bt));
   ohm_batch_result r{};
   const int64_t d0 = b->dispatches, rb0 = b->readbacks;
-  hipEventRecord(b->ev0, b->stream);
+  (void)hipEventRecord(b->ev0, b->stream);
   ohm_status s = ohm_batch_solve(bt, rhs, x, params, precond, &r);
   std::string keep = s == OHM_OK ? "" : ohm_last_error();
-  hipEventRecord(b->ev1, b->stream);
-  hipEventSynchronize(b->ev1);
+  (void)hipEventRecord(b->ev1, b->stream);
+  (void)hipEventSynchronize(b->ev1);
   if (result) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);

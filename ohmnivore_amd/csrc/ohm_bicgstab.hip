@@ -63,8 +63,8 @@ struct Parts {
   double *ss;    // K3: s.s
   double *ts;    // K4: t.s, t.t
   double *rr;    // K5 (and k_init): r.r, r_hat.r
-  int n_rv, n_ss, n_ts, n_rr;  // partials each consumer sums (1 = ticket-folded)
-  bool t_rv, t_ss, t_ts, t_rr; // producer folds with the ticket
+  int n_rv, n_ss, n_ts, n_rr;  // partials each consumer sums (1 = folded)
+  bool f_rv, f_ss, f_ts, f_rr; // a k_fold launch reduces the producer's partials
 };
 
 // A workgroup-uniform read of the status word. Workgroup 0 of a consuming
@@ -161,12 +161,12 @@ __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
   double vals[4] = {bb.get(), rr.get(), rhr.get(), hh.get()}, tot[4];
   const bool last = grid_sum<4>(vals, pp.init, &st->counter, tot);
   // grid_sum leaves this workgroup's sums in thread 0's vals[]
-  if (!pp.t_rr && threadIdx.x == 0) {
+  if (!pp.f_rr && threadIdx.x == 0) {
     pp.rr[(size_t)blockIdx.x * 2] = vals[1];
     pp.rr[(size_t)blockIdx.x * 2 + 1] = vals[2];
   }
   if (last && threadIdx.x == 0) {
-    if (pp.t_rr) {
+    if (pp.f_rr) {  // already folded: the totals
       pp.rr[0] = tot[1];
       pp.rr[1] = tot[2];
     }
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_alpha(const uint32_t *__restric
     vv.add_prod(y, y);
   });
   double vals[2] = {acc.get(), vv.get()};
-  deposit<2>(vals, pp.rv, &st->counter, pp.t_rv);
+  deposit<2>(vals, pp.rv);
 }
 
 // K3 (Jacobi / none): alpha, s = r - alpha v, s_hat = inv * s; partial s.s.
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_s_update(int n, const T *__restrict_
     acc.add_prod(si, si);
   }
   double vals[1] = {acc.get()};
-  deposit<1>(vals, pp.ss, &st->counter, pp.t_ss);
+  deposit<1>(vals, pp.ss);
 }
 
 // K3 (ISAI): the s update fused into the first preconditioner SpMV
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_s_spmv(const uint32_t *__restrict__ 
     acc.add_prod(si, si);
   });
   double vals[1] = {acc.get()};
-  deposit<1>(vals, pp.ss, &st->counter, pp.t_ss);
+  deposit<1>(vals, pp.ss);
 }
 
 // K4: the ||s|| early-exit test, then t = A s_hat; partials t.s, t.t.
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_omega(const uint32_t *__restric
     tt.add_prod(y, y);
   });
   double vals[2] = {ts.get(), tt.get()};
-  deposit<2>(vals, pp.ts, &st->counter, pp.t_ts);
+  deposit<2>(vals, pp.ts);
 }
 
 // K5: omega = t.s / t.t with the t.t breakdown test; x += alpha p_hat +
@@ -383,7 +383,19 @@ __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
     rhr.add_prod(rh[i], ri);
   }
   double vals[2] = {rr.get(), rhr.get()};
-  deposit<2>(vals, pp.rr, &st->counter, pp.t_rr);
+  deposit<2>(vals, pp.rr);
+}
+
+// Reduces a producer's np partials (K each) to partials[0..K) for a
+// consumer whose workgroups would otherwise each re-read all of them.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_fold(double *partials, int np) {
+  double tot[K];
+  gather_sum<K>(partials, np, tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partials[k] = tot[k];
+  }
 }
 
 // Closing test of an enqueued chunk (one workgroup): publishes the stop
@@ -466,18 +478,17 @@ struct Ops<float> {
 static const DevCsr &isai_first(const PrecondDev &p) { return p.mm.rowptr ? p.mm : p.ml; }
 
 // Partial regions and consumer widths for one solve; sizes the backend's
-// partials buffer. A producer defers (plain per-workgroup partials) unless
-// its consumer would read too many of them: consumer workgroups x producer
-// workgroups beyond kDeferMaxReads, where the ticket's fixed tail is cheaper
-// than every consumer workgroup re-reading the partials.
+// partials buffer. A consumer re-sums its producer's partials unless that
+// means consumer workgroups x producer workgroups beyond kDeferMaxReads;
+// then a one-workgroup k_fold launch reduces them first.
 static ohm_status parts_for(ohm_backend *b, const DevCsr &a, const PrecondDev &p, int n,
                             Parts &pp) {
   const int64_t gv = grid_for(n, 4), ga = std::max(a.n_blocks, 1);
   const bool isai = p.kind == PK_ISAI;
   const int64_t gm = isai ? std::max(isai_first(p).n_blocks, 1) : gv;
   const int64_t g3 = gm;  // K3 runs on the first ISAI SpMV's grid, else gv
-  auto ticket = [](int64_t producer, int64_t consumer) {
-    return producer > kDeferMaxBlocks || producer * consumer > kDeferMaxReads;
+  auto fold = [](int64_t producer, int64_t consumer) {
+    return producer * consumer > kDeferMaxReads;
   };
   const size_t S = (size_t)std::max(std::max(gv, ga), g3);
   OHM_TRY(ensure_partials(b, std::max<size_t>(S * 11, (size_t)kMaxReduceBlocks * 4)));
@@ -487,15 +498,21 @@ static ohm_status parts_for(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   pp.ss = q + 6 * S;
   pp.ts = q + 7 * S;
   pp.rr = q + 9 * S;
-  pp.t_rr = ticket(gv, gm);  // K5 -> K1 (on gm workgroups: gv, or the ISAI SpMV's)
-  pp.t_rv = ticket(ga, gm);  // K2 -> K3
-  pp.t_ss = ticket(g3, ga);  // K3 -> K4
-  pp.t_ts = ticket(ga, gv);  // K4 -> K5
-  pp.n_rv = pp.t_rv ? 1 : (int)ga;
-  pp.n_ss = pp.t_ss ? 1 : (int)g3;
-  pp.n_ts = pp.t_ts ? 1 : (int)ga;
-  pp.n_rr = pp.t_rr ? 1 : (int)gv;
+  pp.f_rr = fold(gv, gm);  // K5 -> K1 (on gm workgroups: gv, or the ISAI SpMV's)
+  pp.f_rv = fold(ga, gm);  // K2 -> K3
+  pp.f_ss = fold(g3, ga);  // K3 -> K4
+  pp.f_ts = fold(ga, gv);  // K4 -> K5
+  pp.n_rv = pp.f_rv ? 1 : (int)ga;
+  pp.n_ss = pp.f_ss ? 1 : (int)g3;
+  pp.n_ts = pp.f_ts ? 1 : (int)ga;
+  pp.n_rr = pp.f_rr ? 1 : (int)gv;
   return OHM_OK;
+}
+
+template <int K>
+static void fold_launch(ohm_backend *b, double *partials, int np) {
+  hipLaunchKernelGGL(k_fold<K>, dim3(1), dim3(kBlock), 0, b->stream, partials, np);
+  b->dispatches++;
 }
 
 // One iteration: K1 .. K5 (ISAI: the p and s updates ride on the first
@@ -518,22 +535,28 @@ static void enqueue_iteration(ohm_backend *b, const DevCsr &a, const PrecondDev 
     if (two) Ops<T>::spmv(b, p.mu, v.tmp, v.ph, st, 1);
     hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks,
                        v.ph, v.v, v.rh, st, pp);
+    if (pp.f_rv) fold_launch<2>(b, pp.rv, a.n_blocks);
     T *y3 = two ? v.tmp : v.sh;
     hipLaunchKernelGGL(k_s_spmv<T>, dim3(m1.n_blocks), blk, 0, b->stream, m1.rowptr, m1.col,
                        Ops<T>::vals(m1), m1.blocks, v.r, v.v, v.s, y3, st, pp);
+    if (pp.f_ss) fold_launch<1>(b, pp.ss, m1.n_blocks);
     if (two) Ops<T>::spmv(b, p.mu, v.tmp, v.sh, st, 1);
   } else {
     hipLaunchKernelGGL((k_p_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.p, v.v, v.ph, inv,
                        st, pp);
     hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks,
                        v.ph, v.v, v.rh, st, pp);
+    if (pp.f_rv) fold_launch<2>(b, pp.rv, a.n_blocks);
     hipLaunchKernelGGL((k_s_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.v, v.s, v.sh, inv,
                        st, pp);
+    if (pp.f_ss) fold_launch<1>(b, pp.ss, (int)gv.x);
   }
   hipLaunchKernelGGL(k_spmv_omega<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks, v.sh,
                      v.t, v.s, st, pp);
+  if (pp.f_ts) fold_launch<2>(b, pp.ts, a.n_blocks);
   hipLaunchKernelGGL(k_xr_update<T>, gv, blk, 0, b->stream, n, v.x, v.r, v.rh, v.ph, v.sh, v.s,
                      v.t, st, pp);
+  if (pp.f_rr) fold_launch<2>(b, pp.rr, (int)gv.x);
   b->dispatches += 5;
 }
 
@@ -582,7 +605,8 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
   hipGraphExec_t exec = nullptr;
   for (auto &g : b->graphs)
     if (g.key == key) exec = g.exec;
-  const int64_t per_iter = 5 + (p.kind == PK_ISAI && !p.mm.rowptr ? 2 : 0);
+  const int64_t per_iter = 5 + (p.kind == PK_ISAI && !p.mm.rowptr ? 2 : 0) + pp.f_rv + pp.f_ss +
+                           pp.f_ts + pp.f_rr;
   if (!exec) {
     int64_t saved = b->dispatches;
     GraphEntry e;

@@ -40,8 +40,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kSpmvNnzPerBlock = 2048;   // LDS-staged products per row block
 constexpr int kSpmvRowsPerBlock = kBlock;
 constexpr int kMaxReduceBlocks = 2048;   // cap on grid size of reducing vector kernels
-constexpr int kDeferMaxBlocks = 2048;    // widest producer grid whose partials consumers re-sum
-constexpr int64_t kDeferMaxReads = int64_t(1) << 21;  // consumer x producer workgroups
+constexpr int64_t kDeferMaxReads = int64_t(1) << 21;  // consumer x producer workgroups re-summed
 
 // ----------------------------------------------------------- solve state --
 enum SolveStatus : int {

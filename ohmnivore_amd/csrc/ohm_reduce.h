@@ -1,6 +1,6 @@
 // ohm_reduce.h — wave64 / workgroup reductions, the deterministic "last
-// workgroup finishes the reduction" pattern, and deferred reductions whose
-// consumer re-sums the producer's partials. Partial sums are combined in a
+// workgroup finishes the reduction" pattern (one-off reductions), and
+// deferred reductions whose consumer re-sums the producer's partials. Partial sums are combined in a
 // fixed order, so results are bitwise reproducible run to run.
 #pragma once
 
@@ -107,28 +107,18 @@ __device__ __forceinline__ bool grid_sum(double (&v)[K], double *partials, unsig
   return true;
 }
 
-// Producer side of a deferred reduction. Normally each workgroup stores its
-// K block sums at partials[blockIdx.x * K ...] with plain stores and the
-// kernel boundary publishes them; the consuming kernel sums them itself
-// (gather_sum). Where every consumer workgroup would read too many partials
-// (wide producer and consumer grids) the ticket pattern (grid_sum) folds
-// them in the producer instead and the totals land in partials[0..K): the
-// consumer then gathers a single partial.
+// Producer side of a deferred reduction: each workgroup stores its K block
+// sums at partials[blockIdx.x * K ...] with plain stores and the kernel
+// boundary publishes them. The consuming kernel sums them itself
+// (gather_sum) or, where every consumer workgroup would read too many of
+// them, a one-workgroup fold kernel in between reduces them to
+// partials[0..K) and the consumer gathers that single partial.
 template <int K>
-__device__ __forceinline__ void deposit(double (&v)[K], double *partials, unsigned *counter,
-                                        bool ticket) {
-  if (ticket) {
-    double tot[K];
-    if (grid_sum<K>(v, partials, counter, tot) && threadIdx.x == 0) {
+__device__ __forceinline__ void deposit(double (&v)[K], double *partials) {
+  block_sum<K>(v);
+  if (threadIdx.x == 0) {
 #pragma unroll
-      for (int k = 0; k < K; k++) partials[k] = tot[k];
-    }
-  } else {
-    block_sum<K>(v);
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int k = 0; k < K; k++) partials[(size_t)blockIdx.x * K + k] = v[k];
-    }
+    for (int k = 0; k < K; k++) partials[(size_t)blockIdx.x * K + k] = v[k];
   }
 }
 
