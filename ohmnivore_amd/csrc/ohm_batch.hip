@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_bfold_init(const BGeo *__restrict__ 
     const zd rho = {tot[2], tot[3]};
     if (a.warm && sqrt(tot[1]) < st->abs_tol) {
       st->status = ST_CONVERGED;
-    } else if (brk_small(sqrt(zabs2(rho)), sqrt(tot[4] * tot[1]), st->scaled,
+    } else if (brk_small(sqrt(zabs2(rho)), sqrt(tot[4]) * sqrt(tot[1]), st->scaled,
                          Tr<S>::C ? 1e-15 : 1e-30)) {
       st->status = ST_BRK_RHO;
     } else {
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void k_bfold_alpha(const BGeo *__restrict__
   fold_sum<3>(geo[blockIdx.x], partials, tot);
   if (threadIdx.x == 0) {
     const zd rv = {tot[0], tot[1]};
-    if (brk_small(sqrt(zabs2(rv)), sqrt(st->rhrh * tot[2]), st->scaled,
+    if (brk_small(sqrt(zabs2(rv)), sqrt(st->rhrh) * sqrt(tot[2]), st->scaled,
                   Tr<S>::C ? 1e-15 : 1e-30)) {
       st->status = ST_BRK_RV;
     } else {
@@ -686,7 +686,7 @@ __global__ __launch_bounds__(kBlock) void k_bfold_xr(const BGeo *__restrict__ ge
       st->status = ST_BRK_OMEGA;
     } else if (st->iters >= st->max_iters) {
       st->status = ST_MAXITER;
-    } else if (brk_small(sqrt(zabs2(rho_new)), sqrt(st->rhrh * tot[0]), st->scaled,
+    } else if (brk_small(sqrt(zabs2(rho_new)), sqrt(st->rhrh) * sqrt(tot[0]), st->scaled,
                          Tr<S>::C ? 1e-15 : 1e-30)) {
       st->status = ST_BRK_RHO;
     } else {
@@ -789,7 +789,7 @@ __global__ __launch_bounds__(kBlock) void k_bpersist(const int *__restrict__ lis
                                               : a.tol * bnorm;
       if (a.warm && sqrt(rr) < abs_tol)
         status = ST_CONVERGED;
-      else if (brk_small(sqrt(zabs2(rho)), sqrt(rhrh * rr), scaled, th))
+      else if (brk_small(sqrt(zabs2(rho)), sqrt(rhrh) * sqrt(rr), scaled, th))
         status = ST_BRK_RHO;
     }
   } else {
@@ -824,7 +824,7 @@ __global__ __launch_bounds__(kBlock) void k_bpersist(const int *__restrict__ lis
     double d3[3] = {hv.re.get(), hv.im.get(), vv.s.get()};
     wg_reduce<3>(d3);
     const zd rv = {d3[0], d3[1]};
-    if (brk_small(sqrt(zabs2(rv)), sqrt(rhrh * d3[2]), scaled, th)) {
+    if (brk_small(sqrt(zabs2(rv)), sqrt(rhrh) * sqrt(d3[2]), scaled, th)) {
       status = ST_BRK_RV;
       break;
     }
@@ -886,7 +886,7 @@ __global__ __launch_bounds__(kBlock) void k_bpersist(const int *__restrict__ lis
       status = ST_BRK_OMEGA;
     } else if (iters >= max_iters) {
       status = ST_MAXITER;
-    } else if (brk_small(sqrt(zabs2(rho_new)), sqrt(rhrh * rr), scaled, th)) {
+    } else if (brk_small(sqrt(zabs2(rho_new)), sqrt(rhrh) * sqrt(rr), scaled, th)) {
       status = ST_BRK_RHO;
     } else {
       beta = zmul(zdiv(rho_new, rho), zdiv(alpha, omega));

@@ -53,7 +53,7 @@ __global__ void k_state_reset(SolveState *st) {
 // Breakdown tests: the reference's absolute thresholds, or the same 1e-30
 // applied to normalised quantities.
 __device__ __forceinline__ bool rho_breakdown(double rho, double hh, double rr, int scaled) {
-  return scaled ? fabs(rho) <= 1e-30 * sqrt(hh * rr) : fabs(rho) < 1e-30;
+  return scaled ? fabs(rho) <= 1e-30 * (sqrt(hh) * sqrt(rr)) : fabs(rho) < 1e-30;
 }
 
 // Partial-sum regions of one solve (doubles, `stride` workgroups each).
@@ -78,6 +78,33 @@ __device__ __forceinline__ bool block_running(const SolveState *st) {
   return s_run != 0;
 }
 
+// block_running and gather_sum in one pass: the status word (thread 0) and
+// the partials are requested together and published by one workgroup sum.
+template <int K>
+__device__ __forceinline__ bool gather_run(const SolveState *st,
+                                           const double *__restrict__ partials, int np,
+                                           double (&tot)[K]) {
+  __shared__ double bc[K];
+  __shared__ int s_run;
+  const int run0 = threadIdx.x == 0 ? (st->status == ST_RUNNING) : 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) tot[k] = 0.0;
+  for (int i = threadIdx.x; i < np; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] += partials[(size_t)i * K + k];
+  }
+  block_sum<K>(tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) bc[k] = tot[k];
+    s_run = run0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) tot[k] = bc[k];
+  return s_run != 0;
+}
+
 // End-of-iteration tests on ||r||^2, r_hat.r (src/solver/bicgstab.rs:158-185
 // in the reference's order), taken by the first kernel of the next iteration
 // and by the chunk's closing check. Returns the stop status or ST_RUNNING
@@ -98,9 +125,8 @@ __device__ __forceinline__ int decide_rho(const SolveState *st, double rr, doubl
 // p-update head: every workgroup forms beta from K5's partials; workgroup 0
 // publishes rho_new (or the stop status). False means: exit.
 __device__ __forceinline__ bool head_rho(SolveState *st, const Parts &pp, double &beta) {
-  if (!block_running(st)) return false;
   double tot[2];
-  gather_sum<2>(pp.rr, pp.n_rr, tot);
+  if (!gather_run<2>(st, pp.rr, pp.n_rr, tot)) return false;
   const int stat = decide_rho(st, tot[0], tot[1], beta);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rr = tot[0];
@@ -114,10 +140,9 @@ __device__ __forceinline__ bool head_rho(SolveState *st, const Parts &pp, double
 
 // s-update head: alpha = rho_new / r_hat.v with the r_hat.v breakdown test.
 __device__ __forceinline__ bool head_alpha(SolveState *st, const Parts &pp, double &alpha) {
-  if (!block_running(st)) return false;
   double tot[2];
-  gather_sum<2>(pp.rv, pp.n_rv, tot);
-  const bool brk = st->scaled ? fabs(tot[0]) <= 1e-30 * sqrt(st->rhrh * tot[1])
+  if (!gather_run<2>(st, pp.rv, pp.n_rv, tot)) return false;
+  const bool brk = st->scaled ? fabs(tot[0]) <= 1e-30 * (sqrt(st->rhrh) * sqrt(tot[1]))
                               : fabs(tot[0]) < 1e-30;
   const double rho = st->rho_new;
   alpha = rho / tot[0];
@@ -199,6 +224,13 @@ __device__ __forceinline__ T p_next(T r, T p, T v, T beta, T omega) {
   return fmac<T>(fmac<T>(-omega, v, p), beta, r);
 }
 
+// Vector kernels prefetch their first element before the head: the index
+// is clamped into the vector (n >= 1 for every launch) and the values are
+// used only when it is in range.
+__device__ __forceinline__ int first_index(int n) {
+  return min((int)(blockIdx.x * kBlock + threadIdx.x), n - 1);
+}
+
 // K1 (Jacobi / none): p update, p_hat = inv * p (or p).
 template <typename T, int PK>
 __global__ __launch_bounds__(kBlock) void k_p_update(int n, const T *__restrict__ r,
@@ -206,11 +238,19 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int n, const T *__restrict_
                                                      T *__restrict__ ph,
                                                      const T *__restrict__ inv, SolveState *st,
                                                      Parts pp) {
+  const int i0 = blockIdx.x * kBlock + threadIdx.x, ic = first_index(n);
+  const T r0 = r[ic], p0 = p[ic], v0 = v[ic], d0 = PK == PK_JACOBI ? inv[ic] : T(1);
   double beta_d;
   if (!head_rho(st, pp, beta_d)) return;
   const T beta = (T)beta_d, omega = (T)st->omega;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    T pi = p_next<T>(r[i], p[i], v[i], beta, omega);
+  if (i0 < n) {
+    const T pi = p_next<T>(r0, p0, v0, beta, omega);
+    p[i0] = pi;
+    if (PK == PK_JACOBI) ph[i0] = d0 * pi;
+    if (PK == PK_NONE) ph[i0] = pi;
+  }
+  for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock) {
+    const T pi = p_next<T>(r[i], p[i], v[i], beta, omega);
     p[i] = pi;
     if (PK == PK_JACOBI) ph[i] = inv[i] * pi;
     if (PK == PK_NONE) ph[i] = pi;
@@ -230,14 +270,28 @@ __global__ __launch_bounds__(kBlock) void k_p_spmv(const uint32_t *__restrict__ 
                                                    const T *__restrict__ v,
                                                    T *__restrict__ p_new, T *__restrict__ y,
                                                    SolveState *st, Parts pp) {
-  double beta_d;
-  if (!head_rho(st, pp, beta_d)) return;
-  const T beta = (T)beta_d, omega = (T)st->omega;
-  auto pf = [&](uint32_t j) { return p_next<T>(r[j], p_old[j], v[j], beta, omega); };
-  spmv_block_g<T>(rowptr, col, val, blocks, pf, [&](uint32_t i, T yi) {
-    y[i] = yi;
-    p_new[i] = pf(i);
-  });
+  T beta = T(0), omega = T(0);
+  spmv_block_h<T, 3>(
+      rowptr, col, val, blocks,
+      [&](uint32_t j) {
+        Opnd<T, 3> o;
+        o.v[0] = r[j];
+        o.v[1] = p_old[j];
+        o.v[2] = v[j];
+        return o;
+      },
+      [&] {
+        double beta_d;
+        if (!head_rho(st, pp, beta_d)) return false;
+        beta = (T)beta_d;
+        omega = (T)st->omega;
+        return true;
+      },
+      [&](const Opnd<T, 3> &o) { return p_next<T>(o.v[0], o.v[1], o.v[2], beta, omega); },
+      [&](uint32_t i, T yi) {
+        y[i] = yi;
+        p_new[i] = p_next<T>(r[i], p_old[i], v[i], beta, omega);
+      });
 }
 
 // K2: v = A p_hat; partials r_hat.v, v.v.
@@ -250,13 +304,21 @@ __global__ __launch_bounds__(kBlock) void k_spmv_alpha(const uint32_t *__restric
                                                        T *__restrict__ v,
                                                        const T *__restrict__ rh, SolveState *st,
                                                        Parts pp) {
-  if (!block_running(st)) return;
   Acc<T> acc, vv;
-  spmv_block<T>(rowptr, col, val, blocks, ph, [&](uint32_t r, T y) {
-    v[r] = y;
-    acc.add_prod(rh[r], y);
-    vv.add_prod(y, y);
-  });
+  const bool ran = spmv_block_h<T, 1>(
+      rowptr, col, val, blocks,
+      [&](uint32_t j) {
+        Opnd<T, 1> o;
+        o.v[0] = ph[j];
+        return o;
+      },
+      [&] { return block_running(st); }, [](const Opnd<T, 1> &o) { return o.v[0]; },
+      [&](uint32_t r, T y) {
+        v[r] = y;
+        acc.add_prod(rh[r], y);
+        vv.add_prod(y, y);
+      });
+  if (!ran) return;
   double vals[2] = {acc.get(), vv.get()};
   deposit<2>(vals, pp.rv);
 }
@@ -268,12 +330,21 @@ __global__ __launch_bounds__(kBlock) void k_s_update(int n, const T *__restrict_
                                                      T *__restrict__ sh,
                                                      const T *__restrict__ inv, SolveState *st,
                                                      Parts pp) {
+  const int i0 = blockIdx.x * kBlock + threadIdx.x, ic = first_index(n);
+  const T r0 = r[ic], v0 = v[ic], d0 = PK == PK_JACOBI ? inv[ic] : T(1);
   double alpha_d;
   if (!head_alpha(st, pp, alpha_d)) return;
   const T alpha = (T)alpha_d;
   Acc<T> acc;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    T si = fmac<T>(-alpha, v[i], r[i]);
+  if (i0 < n) {
+    const T si = fmac<T>(-alpha, v0, r0);
+    s[i0] = si;
+    if (PK == PK_JACOBI) sh[i0] = d0 * si;
+    if (PK == PK_NONE) sh[i0] = si;
+    acc.add_prod(si, si);
+  }
+  for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock) {
+    const T si = fmac<T>(-alpha, v[i], r[i]);
     s[i] = si;
     if (PK == PK_JACOBI) sh[i] = inv[i] * si;
     if (PK == PK_NONE) sh[i] = si;
@@ -293,17 +364,30 @@ __global__ __launch_bounds__(kBlock) void k_s_spmv(const uint32_t *__restrict__ 
                                                    const T *__restrict__ r,
                                                    const T *__restrict__ v, T *__restrict__ s,
                                                    T *__restrict__ y, SolveState *st, Parts pp) {
-  double alpha_d;
-  if (!head_alpha(st, pp, alpha_d)) return;
-  const T alpha = (T)alpha_d;
-  auto sf = [&](uint32_t j) { return fmac<T>(-alpha, v[j], r[j]); };
+  T alpha = T(0);
   Acc<T> acc;
-  spmv_block_g<T>(rowptr, col, val, blocks, sf, [&](uint32_t i, T yi) {
-    y[i] = yi;
-    T si = sf(i);
-    s[i] = si;
-    acc.add_prod(si, si);
-  });
+  const bool ran = spmv_block_h<T, 2>(
+      rowptr, col, val, blocks,
+      [&](uint32_t j) {
+        Opnd<T, 2> o;
+        o.v[0] = r[j];
+        o.v[1] = v[j];
+        return o;
+      },
+      [&] {
+        double alpha_d;
+        if (!head_alpha(st, pp, alpha_d)) return false;
+        alpha = (T)alpha_d;
+        return true;
+      },
+      [&](const Opnd<T, 2> &o) { return fmac<T>(-alpha, o.v[1], o.v[0]); },
+      [&](uint32_t i, T yi) {
+        y[i] = yi;
+        const T si = fmac<T>(-alpha, v[i], r[i]);
+        s[i] = si;
+        acc.add_prod(si, si);
+      });
+  if (!ran) return;
   double vals[1] = {acc.get()};
   deposit<1>(vals, pp.ss);
 }
@@ -318,21 +402,31 @@ __global__ __launch_bounds__(kBlock) void k_spmv_omega(const uint32_t *__restric
                                                        T *__restrict__ t,
                                                        const T *__restrict__ s, SolveState *st,
                                                        Parts pp) {
-  if (!block_running(st)) return;
-  double ss[1];
-  gather_sum<1>(pp.ss, pp.n_ss, ss);
-  const bool sconv = sqrt(ss[0]) < st->abs_tol;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->ss = ss[0];
-    if (sconv) st->s_conv = 1;
-  }
-  if (sconv) return;
   Acc<T> ts, tt;
-  spmv_block<T>(rowptr, col, val, blocks, sh, [&](uint32_t r, T y) {
-    t[r] = y;
-    ts.add_prod(y, s[r]);
-    tt.add_prod(y, y);
-  });
+  const bool ran = spmv_block_h<T, 1>(
+      rowptr, col, val, blocks,
+      [&](uint32_t j) {
+        Opnd<T, 1> o;
+        o.v[0] = sh[j];
+        return o;
+      },
+      [&] {
+        double ss[1];
+        if (!gather_run<1>(st, pp.ss, pp.n_ss, ss)) return false;
+        const bool sconv = sqrt(ss[0]) < st->abs_tol;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+          st->ss = ss[0];
+          if (sconv) st->s_conv = 1;
+        }
+        return !sconv;
+      },
+      [](const Opnd<T, 1> &o) { return o.v[0]; },
+      [&](uint32_t r, T y) {
+        t[r] = y;
+        ts.add_prod(y, s[r]);
+        tt.add_prod(y, y);
+      });
+  if (!ran) return;
   double vals[2] = {ts.get(), tt.get()};
   deposit<2>(vals, pp.ts);
 }
@@ -350,16 +444,18 @@ __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
                                                       const T *__restrict__ s,
                                                       const T *__restrict__ t, SolveState *st,
                                                       Parts pp) {
-  if (!block_running(st)) return;
+  const int i0 = blockIdx.x * kBlock + threadIdx.x, ic = first_index(n);
+  const T x0 = x[ic], ph0 = ph[ic], sh0 = sh[ic], s0 = s[ic], t0 = t[ic], rh0 = rh[ic];
+  double tot[2];
+  if (!gather_run<2>(st, pp.ts, pp.n_ts, tot)) return;
   const T alpha = (T)st->alpha;
+  const int stride = gridDim.x * kBlock;
   if (st->s_conv) {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-      x[i] = x[i] + alpha * ph[i];
+    if (i0 < n) x[i0] = x0 + alpha * ph0;
+    for (int i = i0 + stride; i < n; i += stride) x[i] = x[i] + alpha * ph[i];
     if (blockIdx.x == 0 && threadIdx.x == 0) st->iters += 1;
     return;
   }
-  double tot[2];
-  gather_sum<2>(pp.ts, pp.n_ts, tot);
   const bool brk = st->scaled ? tot[1] <= 1e-30 * st->ss : fabs(tot[1]) < 1e-30;
   const double omega_d = tot[0] / tot[1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -375,9 +471,16 @@ __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
   if (brk) return;
   const T omega = (T)omega_d;
   Acc<T> rr, rhr;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  if (i0 < n) {
+    x[i0] = (x0 + alpha * ph0) + omega * sh0;
+    const T ri = s0 - omega * t0;
+    r[i0] = ri;
+    rr.add_prod(ri, ri);
+    rhr.add_prod(rh0, ri);
+  }
+  for (int i = i0 + stride; i < n; i += stride) {
     x[i] = (x[i] + alpha * ph[i]) + omega * sh[i];
-    T ri = s[i] - omega * t[i];
+    const T ri = s[i] - omega * t[i];
     r[i] = ri;
     rr.add_prod(ri, ri);
     rhr.add_prod(rh[i], ri);
@@ -401,9 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_fold(double *partials, int np) {
 // Closing test of an enqueued chunk (one workgroup): publishes the stop
 // status of the chunk's last iteration so the host's poll sees it.
 __global__ __launch_bounds__(kBlock) void k_check(SolveState *st, Parts pp) {
-  if (!block_running(st)) return;
   double tot[2], beta;
-  gather_sum<2>(pp.rr, pp.n_rr, tot);
+  if (!gather_run<2>(st, pp.rr, pp.n_rr, tot)) return;
   const int stat = decide_rho(st, tot[0], tot[1], beta);
   if (threadIdx.x == 0 && stat != ST_RUNNING) st->status = stat;
 }
@@ -798,6 +900,8 @@ __global__ void k_correct(int n, const float *__restrict__ d, const double *__re
 static double *g_ir_r = nullptr;
 static float *g_ir_r32 = nullptr;
 static int g_ir_n = 0;
+static double *g_ir_x0 = nullptr;  // the caller's starting iterate (FP64 rerun)
+static int g_ir_x0_n = 0;
 
 ohm_status true_residual(ohm_backend *b, const DevCsr &a, const double *rhs, const double *x,
                          double &rel) {
@@ -838,7 +942,16 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
   OHM_TRY(csr_ensure_f32(b, a));
   OHM_TRY(precond_ensure_f32(b, p));
   OHM_TRY(ws_get(g_ws32, n));
-  if (!o.warm) OHM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, b->stream));
+  if (o.warm) {
+    if (g_ir_x0_n < n) {
+      if (g_ir_x0) OHM_HIP(hipFree(g_ir_x0));
+      OHM_HIP(hipMalloc(&g_ir_x0, sizeof(double) * n));
+      g_ir_x0_n = n;
+    }
+    OHM_HIP(hipMemcpyAsync(g_ir_x0, x, sizeof(double) * n, hipMemcpyDeviceToDevice, b->stream));
+  } else {
+    OHM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, b->stream));
+  }
   double rel = 0.0, prev = INFINITY, target = o.tol;
   int64_t total = 0;
   int last_status = ST_RUNNING, stalled = 0;
@@ -901,6 +1014,21 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
   out.polls += f64.polls;
   out.status = f64.status;
   out.fp64_fallback = true;
+  if (s != OHM_OK) {
+    // Continuing from the refined iterate failed (its residual is left in
+    // the modes BiCGSTAB reduces slowest, and a breakdown or divergence
+    // followed): solve the original system in FP64 from the caller's start,
+    // so FP32 refinement never fails where the FP64 solver alone succeeds.
+    if (o.warm)
+      OHM_HIP(hipMemcpyAsync(x, g_ir_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, b->stream));
+    else
+      OHM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, b->stream));
+    BicgOut re;
+    s = bicg_solve_f64(b, a, p, rhs, x, o, re);
+    out.iters += re.iters;
+    out.polls += re.polls;
+    out.status = re.status;
+  }
   (void)last_status;
   return s;
 }

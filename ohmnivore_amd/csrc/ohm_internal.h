@@ -79,7 +79,7 @@ struct DevCsr {
   uint32_t *rowptr = nullptr, *col = nullptr;
   double *val = nullptr;
   float *val32 = nullptr;          // f32 mirror (mixed precision)
-  uint32_t *blocks = nullptr;      // row-block boundaries for SpMV
+  uint32_t *blocks = nullptr;      // SpMV block table: (first row, first nonzero) pairs, n_blocks+1
   int n_blocks = 0;
   int32_t *diag_pos = nullptr;     // CSR position of (r, r) or -1
   // transpose pattern (built lazily for A^T x): CSR of A^T + map to A's slots

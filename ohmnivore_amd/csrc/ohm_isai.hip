@@ -24,6 +24,7 @@
 // product would be much denser than the two factors the apply stays
 // z = M_U (M_L r).
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <unordered_map>
 
@@ -446,7 +447,13 @@ static ohm_status isai_plan_build(ohm_backend *b, const DevCsr &a, int level, Pr
   OHM_HIP(hipMalloc(&pl->f_tmp, sizeof(double) * std::max<int64_t>(nf, 1)));
   OHM_TRY(build_side(n, level, true, frp, fcl, fdiag, pl->side[0], p.ml, b));
   OHM_TRY(build_side(n, level, false, frp, fcl, fdiag, pl->side[1], p.mu, b));
-  OHM_TRY(build_product(b, p));
+  // The apply is two SpMVs (M_L, then M_U), the first carrying BiCGSTAB's
+  // fused vector update. OHM_ISAI_PRODUCT=1 forms M = M_U M_L instead (one
+  // SpMV per apply): fewer launches, but on stencil patterns M has ~1.6x the
+  // nonzeros of the two factors and the fused update gathers three vectors
+  // per nonzero, so the two-SpMV form is faster there.
+  const char *env = getenv("OHM_ISAI_PRODUCT");
+  if (env && atoi(env) != 0) OHM_TRY(build_product(b, p));
   return OHM_OK;
 }
 

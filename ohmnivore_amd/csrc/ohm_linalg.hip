@@ -38,6 +38,17 @@ void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blo
   }
 }
 
+// Device block table: (first row, first nonzero) of every block boundary, so
+// a workgroup reads its row and nonzero ranges with one 16-byte scalar load.
+std::vector<uint32_t> spmv_block_pairs(const uint32_t *rowptr, const std::vector<uint32_t> &bounds) {
+  std::vector<uint32_t> t(2 * bounds.size());
+  for (size_t i = 0; i < bounds.size(); i++) {
+    t[2 * i] = bounds[i];
+    t[2 * i + 1] = rowptr[bounds[i]];
+  }
+  return t;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t *__restrict__ rowptr,
                                                  const uint32_t *__restrict__ col,
@@ -74,8 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_t(const uint32_t *__restrict__ 
                                                    const double *__restrict__ x,
                                                    double *__restrict__ y) {
   __shared__ double prod[kSpmvNnzPerBlock];
-  const uint32_t r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
-  const uint32_t k0 = rowptr[r0], k1 = rowptr[r1];
+  const uint32_t r0 = blocks[2 * blockIdx.x], k0 = blocks[2 * blockIdx.x + 1];
+  const uint32_t r1 = blocks[2 * blockIdx.x + 2], k1 = blocks[2 * blockIdx.x + 3];
   const int tid = threadIdx.x;
   if (k1 - k0 > (uint32_t)kSpmvNnzPerBlock) {
     double acc = 0.0;
@@ -231,9 +242,10 @@ ohm_status csr_upload(ohm_backend *b, DevCsr &m, const double *val, const uint32
   m.nnz = nnz;
   m.h_rowptr.assign(rowptr, rowptr + n + 1);
   m.h_col.assign(col, col + nnz);
-  std::vector<uint32_t> blocks;
-  build_spmv_blocks(rowptr, n, blocks);
-  m.n_blocks = (int)blocks.size() - 1;
+  std::vector<uint32_t> bounds;
+  build_spmv_blocks(rowptr, n, bounds);
+  m.n_blocks = (int)bounds.size() - 1;
+  const std::vector<uint32_t> blocks = spmv_block_pairs(rowptr, bounds);
   std::vector<int32_t> diag(n > 0 ? n : 1, -1);
   for (int r = 0; r < n; r++)
     for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; k++)
@@ -275,9 +287,10 @@ ohm_status csr_ensure_transpose(ohm_backend *b, DevCsr &m) {
       tcol[p] = (uint32_t)r;
       tsrc[p] = (uint32_t)k;
     }
-  std::vector<uint32_t> blocks;
-  build_spmv_blocks(trp.data(), n, blocks);
-  m.t_n_blocks = (int)blocks.size() - 1;
+  std::vector<uint32_t> bounds;
+  build_spmv_blocks(trp.data(), n, bounds);
+  m.t_n_blocks = (int)bounds.size() - 1;
+  const std::vector<uint32_t> blocks = spmv_block_pairs(trp.data(), bounds);
   OHM_HIP(hipMalloc(&m.t_rowptr, sizeof(uint32_t) * (n + 1)));
   OHM_HIP(hipMalloc(&m.t_col, sizeof(uint32_t) * tcol.size()));
   OHM_HIP(hipMalloc(&m.t_src, sizeof(uint32_t) * tsrc.size()));
@@ -332,14 +345,15 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
   const std::vector<uint64_t> key = {(uint64_t)m.n, (uint64_t)m.nnz, h};
   for (auto &e : b->tune_cache)
     if (e.first == key) {
-      std::vector<uint32_t> blocks;
-      build_spmv_blocks(m.h_rowptr.data(), m.n, blocks, e.second.first, e.second.second);
+      std::vector<uint32_t> bounds;
+      build_spmv_blocks(m.h_rowptr.data(), m.n, bounds, e.second.first, e.second.second);
+      const std::vector<uint32_t> blocks = spmv_block_pairs(m.h_rowptr.data(), bounds);
       uint32_t *nb = nullptr;
       OHM_HIP(hipMalloc(&nb, sizeof(uint32_t) * blocks.size()));
       OHM_HIP(hipMemcpy(nb, blocks.data(), sizeof(uint32_t) * blocks.size(), hipMemcpyHostToDevice));
       if (m.blocks) OHM_HIP(hipFree(m.blocks));
       m.blocks = nb;
-      m.n_blocks = (int)blocks.size() - 1;
+      m.n_blocks = (int)bounds.size() - 1;
       m.tuned_rows = e.second.first;
       m.tuned_nnz = e.second.second;
       if (rows_out) *rows_out = m.tuned_rows;
@@ -362,6 +376,7 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
   for (int rows : kRows)
     for (int nz : kNnz) {
       build_spmv_blocks(m.h_rowptr.data(), m.n, blocks, rows, nz);
+      blocks = spmv_block_pairs(m.h_rowptr.data(), blocks);
       if (blocks.size() > d_cap) {
         if (d_blk) (void)hipFree(d_blk);
         d_cap = blocks.size();
@@ -375,7 +390,7 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
       {
         DevCsr v = m;  // shallow view with the candidate table
         v.blocks = d_blk;
-        v.n_blocks = (int)blocks.size() - 1;
+        v.n_blocks = (int)blocks.size() / 2 - 1;
         v.h_rowptr.clear();
         v.h_col.clear();
         for (int w = 0; w < 3; w++) launch_spmv(b, v, x, y, nullptr, 0);
@@ -404,7 +419,7 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
     (void)hipStreamSynchronize(b->stream);
     if (m.blocks) (void)hipFree(m.blocks);
     m.blocks = nb;
-    m.n_blocks = (int)best_blocks.size() - 1;
+    m.n_blocks = (int)best_blocks.size() / 2 - 1;
     m.tuned_rows = best_r;
     m.tuned_nnz = best_z;
     graph_cache_clear(b);

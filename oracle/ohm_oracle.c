@@ -998,7 +998,7 @@ int oo_bicgstab(int64_t n, const uint32_t *rowptr, const uint32_t *col, const do
   double hh = dot(n, rh, rh);
   for (int64_t it = 0; it < max_iters; it++) {
     double rho_new = dot(n, rh, r);
-    if (scaled ? fabs(rho_new) <= 1e-30 * sqrt(hh * dot(n, r, r)) : fabs(rho_new) < 1e-30) {
+    if (scaled ? fabs(rho_new) <= 1e-30 * (sqrt(hh) * sqrt(dot(n, r, r))) : fabs(rho_new) < 1e-30) {
       rc = fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: rho ~ 0");
       goto done;
     }
@@ -1010,7 +1010,7 @@ int oo_bicgstab(int64_t n, const uint32_t *rowptr, const uint32_t *col, const do
     prec_apply(n, p, pp, ph, tmp);
     oo_spmv(n, rowptr, col, val, ph, v, 0);
     double rv = dot(n, rh, v);
-    if (scaled ? fabs(rv) <= 1e-30 * sqrt(hh * dot(n, v, v)) : fabs(rv) < 1e-30) {
+    if (scaled ? fabs(rv) <= 1e-30 * (sqrt(hh) * sqrt(dot(n, v, v))) : fabs(rv) < 1e-30) {
       rc = fail(OHM_ERR_SOLVE, "Solve error: BiCGSTAB breakdown: r_hat.v ~ 0");
       goto done;
     }

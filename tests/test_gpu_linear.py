@@ -280,3 +280,34 @@ def test_wide_grid_ticket_reductions(be, precond):
     ok, err = close(be.download_vec(xx), ref)
     assert ok, (err, s.iterations)
     assert s.iterations > 3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_isai_product_apply_matches_two_factor(be, precision, monkeypatch):
+    """OHM_ISAI_PRODUCT=1 applies M = M_U M_L formed on the device (one SpMV,
+    long pair lists summed by whole workgroups); the default applies M_L then
+    M_U. Both give the LU solution, and the product apply equals the
+    two-factor apply to rounding."""
+    mna = solver.compile(ir.amplifier_array(64, seed=3))  # VDD row: long product entries
+    a = mna.g
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(a.nrows)
+    ref = oracle.sparse_lu_solve(a, b)
+    outs = {}
+    for prod in ("0", "1"):
+        monkeypatch.setenv("OHM_ISAI_PRODUCT", prod)
+        m = be.upload_matrix(a.values, a.col_indices, a.row_pointers, a.nrows)
+        p = be.preconditioner(m, N.PREC_ISAI, 1)
+        br, bz = be.new_buffer(a.nrows), be.new_buffer(a.nrows)
+        be.upload_vec(b, br)
+        p.apply(br, bz)
+        bb, xx = be.new_buffer(a.nrows), be.new_buffer(a.nrows)
+        be.upload_vec(b, bb)
+        be.bicgstab(m, bb, xx, p, solver.bicgstab_params(tolerance=1e-13, max_iterations=20000,
+                                                         precision=precision))
+        x = be.download_vec(xx)
+        ok, err = close(x, ref)
+        assert ok, (prod, err)
+        outs[prod] = be.download_vec(bz)
+    scale = np.abs(outs["0"]).max()
+    assert np.allclose(outs["1"], outs["0"], rtol=1e-10, atol=1e-13 * scale)
