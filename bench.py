@@ -79,8 +79,13 @@ def headline(args, be, N, ir, solver):
     ckt = ir.diode_mesh(256, 256, seed=2)
     m = solver.compile(ckt)
     ctx = solver.NewtonContext(be, m.g, m.b_dc, m.diode_descriptors)
+    # bicgstab_floor: the sub-solves may stop at floor * ||b||. The assembled
+    # ||b|| (3.4e4) is set by the diodes at the 1 V corners, so the default
+    # 1e-15 stops Newton at ||F|| ~ 3e-11 and leaves ~5e-7 V on the mesh
+    # nodes; 1e-18 reaches ||F|| ~ 3e-14 and errors ~3e-10 V.
     newton_kw = dict(abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-6,
-                     bicgstab_max_iterations=20_000, forcing="ew", forcing_max=0.1)
+                     bicgstab_max_iterations=20_000, forcing="ew", forcing_max=0.1,
+                     bicgstab_floor=1e-18)
     forms = {}
     for prec in ("fp64", "fp32_ir", "fp64_fixed_tol"):
         kw = dict(newton_kw, precision=prec)
@@ -373,9 +378,9 @@ def main() -> int:
         args.only = "headline"
     h = headline(args, be, N, ir, solver)
     forms, errors = h["forms"], h["errors"]
-    best = max(forms, key=lambda p: forms[p]["solves_per_s"])
-    if errors and not errors[best]["within_1e-6_rel_1e-9_abs"]:
-        best = "fp64"
+    # the fastest form whose answer is within the parity bar (fp64 when none is)
+    ok = [p for p in forms if not errors or errors[p]["within_1e-6_rel_1e-9_abs"]]
+    best = max(ok, key=lambda p: forms[p]["solves_per_s"]) if ok else "fp64"
     workloads = {}
     speedups = None
     if args.only == "all":
@@ -405,7 +410,9 @@ def main() -> int:
             "form": best,
             "newton_abs_tol": h["newton_kw"]["abs_tol"],
             "bicgstab_tolerance_rel_to_newton_residual": h["newton_kw"]["bicgstab_tolerance"],
-            "forcing": "Eisenstat-Walker, eta in [1e-6, 0.1]",
+            "bicgstab_floor_rel_to_b": h["newton_kw"]["bicgstab_floor"],
+            "forcing": ("fixed" if best == "fp64_fixed_tol" else
+                        "Eisenstat-Walker, eta in [1e-6, 0.1]"),
         },
         "roofline": h["roofline"],
         "cpu_baseline": h["cpu"],
