@@ -327,12 +327,17 @@ typedef struct {
   int32_t forcing;                 /* OHM_FORCING_FIXED (default): every sub-solve to
                                       bicgstab_tolerance; OHM_FORCING_EW: Eisenstat-Walker
                                       choice 2, eta_k = 0.9 (||F_k|| / ||F_k-1||)^2 with the
-                                      0.9 eta_k-1^2 safeguard, clamped to
-                                      [bicgstab_tolerance, forcing_max] */
-  int32_t _pad;
-  double forcing_max;              /* OHM_FORCING_EW: largest eta (default 0.1) */
+                                      0.9 eta_k-1^2 safeguard; OHM_FORCING_EW1: choice 1,
+                                      eta_k = | ||F_k|| - ||F_k-1 + J_k-1 s_k-1|| | / ||F_k-1||
+                                      with the eta_k-1^((1+sqrt 5)/2) safeguard. Both clamped
+                                      to [bicgstab_tolerance, forcing_max] and evaluated on
+                                      the device from ||F_k|| as each sub-solve starts. */
+  int32_t forcing_safeguard;       /* 0 (default): the safeguard applies when it exceeds 0.1
+                                      (as published); 1: at every iteration, so eta falls at
+                                      most as fast as the safeguard allows */
+  double forcing_max;              /* OHM_FORCING_EW / EW1: largest eta (default 0.1) */
 } ohm_newton_params;
-enum { OHM_FORCING_FIXED = 0, OHM_FORCING_EW = 1 };
+enum { OHM_FORCING_FIXED = 0, OHM_FORCING_EW = 1, OHM_FORCING_EW1 = 2 };
 /* The sub-solve is warm-started from x_k, so its initial residual is the
  * Newton residual F(x_k) = b(x_k) - A(x_k) x_k and the stopping rule is
  * max(bicgstab_tolerance*||F||, bicgstab_floor*||b||) (an inexact Newton

@@ -63,7 +63,7 @@ class NewtonParams(ctypes.Structure):
                 ("bicgstab_max_iterations", ctypes.c_int64), ("bicgstab_tolerance", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("precond", ctypes.c_int32),
                 ("initial_guess", c_f64p), ("bicgstab_floor", ctypes.c_double),
-                ("forcing", ctypes.c_int32), ("_pad", ctypes.c_int32),
+                ("forcing", ctypes.c_int32), ("forcing_safeguard", ctypes.c_int32),
                 ("forcing_max", ctypes.c_double)]
 
 

@@ -46,6 +46,8 @@ __global__ void k_state_reset(SolveState *st) {
     z.counter = 0u;
     z.scaled = 1;
     z.r0 = 0.0;
+    z.res2 = 0.0;
+    z.eta = 0.0;
     *st = z;
   }
 }
@@ -130,6 +132,7 @@ __device__ __forceinline__ bool head_rho(SolveState *st, const Parts &pp, double
   const int stat = decide_rho(st, tot[0], tot[1], beta);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rr = tot[0];
+    if (!st->s_conv) st->res2 = tot[0];
     if (stat != ST_RUNNING)
       st->status = stat;
     else
@@ -158,6 +161,28 @@ __device__ __forceinline__ bool head_alpha(SolveState *st, const Parts &pp, doub
   return !brk;
 }
 
+// Inexact-Newton forcing term for a sub-solve starting at ||F_k|| = f
+// (Eisenstat & Walker, SIAM J. Sci. Comput. 17(1), 1996: choice 1 with the
+// eta^((1+sqrt 5)/2) safeguard, choice 2 with gamma 0.9, alpha 2), clamped to
+// [eta_min, eta_max]; eta_max on the first Newton iteration.
+__host__ __device__ inline double forcing_eta(const Forcing &fc, double f) {
+  double eta = fc.eta_max;
+  if (fc.f_prev > 0.0 && f == f && f < INFINITY) {
+    double safe;
+    if (fc.kind == OHM_FORCING_EW1) {
+      eta = fabs(f - fc.lin_prev) / fc.f_prev;
+      safe = pow(fc.eta_prev, 1.6180339887498949);
+    } else {
+      const double q = f / fc.f_prev;
+      eta = 0.9 * q * q;
+      safe = 0.9 * fc.eta_prev * fc.eta_prev;
+    }
+    if (safe > 0.1 || fc.rate_cap) eta = fmax(eta, safe);
+    eta = fmin(fc.eta_max, fmax(fc.eta_min, eta));
+  }
+  return eta;
+}
+
 // r = b - A x0 (warm) or b; r_hat = hashed or r; p = v = 0; x = 0 (cold).
 // Also leaves the r.r / r_hat.r partials where the first p update reads them.
 template <typename T>
@@ -167,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
                                                  T *__restrict__ p, T *__restrict__ v, int shadow,
                                                  int warm, double tol, double floor_rel,
                                                  int tol_mode, int max_iters, int scaled,
-                                                 SolveState *st, Parts pp) {
+                                                 Forcing fc, SolveState *st, Parts pp) {
   Acc<T> bb, rr, rhr, hh;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     T bi = b[i];
@@ -200,7 +225,12 @@ __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
     st->scaled = scaled;
     st->max_iters = max_iters;
     st->rr = tot[1];
+    st->res2 = tot[1];
     st->r0 = sqrt(tot[1]);
+    if (fc.kind) {
+      tol = forcing_eta(fc, sqrt(tot[1]));
+      st->eta = tol;
+    }
     double bnorm = sqrt(tot[0]);
     if (bnorm < 1e-30) {  // b == 0: x = 0 (src/solver/bicgstab.rs:93-97)
       st->status = ST_CONVERGED;
@@ -416,7 +446,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_omega(const uint32_t *__restric
         const bool sconv = sqrt(ss[0]) < st->abs_tol;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
           st->ss = ss[0];
-          if (sconv) st->s_conv = 1;
+          if (sconv) {
+            st->s_conv = 1;
+            st->res2 = ss[0];
+          }
         }
         return !sconv;
       },
@@ -507,7 +540,10 @@ __global__ __launch_bounds__(kBlock) void k_check(SolveState *st, Parts pp) {
   double tot[2], beta;
   if (!gather_run<2>(st, pp.rr, pp.n_rr, tot)) return;
   const int stat = decide_rho(st, tot[0], tot[1], beta);
-  if (threadIdx.x == 0 && stat != ST_RUNNING) st->status = stat;
+  if (threadIdx.x == 0) {
+    if (!st->s_conv) st->res2 = tot[0];
+    if (stat != ST_RUNNING) st->status = stat;
+  }
 }
 
 // ------------------------------------------------------------ workspace --
@@ -732,10 +768,13 @@ struct StatusWord {  // SolveState from `status` on
   int status, s_conv, iters, max_iters;
   unsigned counter;
   int scaled;
-  double r0;
+  double r0, res2, eta;
 };
 static_assert(offsetof(SolveState, r0) - offsetof(SolveState, status) ==
-                  offsetof(StatusWord, r0),
+                      offsetof(StatusWord, r0) &&
+                  offsetof(SolveState, eta) - offsetof(SolveState, status) ==
+                      offsetof(StatusWord, eta) &&
+                  sizeof(SolveState) - offsetof(SolveState, status) == sizeof(StatusWord),
               "StatusWord mirrors the tail of SolveState");
 
 // Runs BiCGSTAB on the workspace vectors; rhs in ws.r on entry? No: rhs is a
@@ -754,7 +793,7 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
                      v.r, v.rh, v.p, v.v, o.shadow == OHM_SHADOW_HASHED ? 1 : 0, o.warm ? 1 : 0,
                      o.tol, o.tol_floor, o.tol_mode,
                      (int)std::min<int64_t>(o.max_iters, 0x7fffffff), o.scaled_breakdown ? 1 : 0,
-                     st, pp);
+                     o.forcing, st, pp);
   b->dispatches += 2;
   StatusWord *h = reinterpret_cast<StatusWord *>(b->h_pinned);
   int64_t enqueued = 0;
@@ -791,6 +830,8 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   out.status = h->status;
   out.iters = h->iters;
   out.r0 = h->r0;
+  out.res = sqrt(h->res2);
+  out.eta = h->eta;
   OHM_HIP(hipGetLastError());
   return OHM_OK;
 }
@@ -959,8 +1000,16 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
   for (int64_t k = 0; k <= max_ref; k++) {
     OHM_TRY(true_residual(b, a, rhs, x, rel));
     double bn = sqrt(b->h_pinned[1]);
-    if (k == 0) out.r0 = rel * bn;
-    if (k == 0 && o.tol_mode == OHM_TOL_INITIAL) target = std::max(o.tol * rel, o.tol_floor);
+    if (k == 0) {
+      out.r0 = rel * bn;
+      double tol = o.tol;
+      if (o.forcing.kind) {
+        tol = forcing_eta(o.forcing, rel * bn);
+        out.eta = tol;
+      }
+      if (o.tol_mode == OHM_TOL_INITIAL || o.forcing.kind) target = std::max(tol * rel, o.tol_floor);
+    }
+    out.res = rel * bn;
     if (bn < 1e-30 || rel < target) {
       out.status = ST_CONVERGED;
       out.iters = total;
@@ -980,6 +1029,7 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
     BicgOpts io = o;
     io.warm = false;
     io.tol_mode = OHM_TOL_RHS;
+    io.forcing.kind = 0;
     io.tol = std::max(inner_tol, 0.5 * target / std::max(rel, 1e-300));
     io.tol = std::min(io.tol, 0.5);
     io.max_iters = std::max<int64_t>(1, std::min<int64_t>(kInnerMaxIters, o.max_iters - total));
@@ -1007,12 +1057,14 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
   fo.warm = true;
   fo.tol_mode = OHM_TOL_RHS;
   fo.tol = target;
+  fo.forcing.kind = 0;
   fo.max_iters = std::max<int64_t>(1, o.max_iters - total);
   BicgOut f64;
   ohm_status s = bicg_solve_f64(b, a, p, rhs, x, fo, f64);
   out.iters = total + f64.iters;
   out.polls += f64.polls;
   out.status = f64.status;
+  out.res = f64.res;
   out.fp64_fallback = true;
   if (s != OHM_OK) {
     // Continuing from the refined iterate failed (its residual is left in
@@ -1028,6 +1080,8 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
     out.iters += re.iters;
     out.polls += re.polls;
     out.status = re.status;
+    out.res = re.res;
+    if (o.forcing.kind) out.eta = re.eta;
   }
   (void)last_status;
   return s;

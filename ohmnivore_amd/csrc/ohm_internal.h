@@ -70,6 +70,8 @@ struct SolveState {
   unsigned counter;     // last-block-done ticket
   int scaled;           // OHM_BREAKDOWN_SCALED tests
   double r0;            // ||b - A x0|| of the solve (read with the status word)
+  double res2;          // squared residual norm at the last completed iteration
+  double eta;           // forcing term chosen on the device (inexact Newton)
 };
 
 // ------------------------------------------------------ device objects --

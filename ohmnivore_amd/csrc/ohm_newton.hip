@@ -172,7 +172,8 @@ __device__ __forceinline__ void scale_node(const double *xo, double *xn, uint32_
   if (t != OHM_SENTINEL) xn[t] = xo[t] + (xn[t] - xo[t]) * s;
 }
 
-__device__ void bjt_limit_one(const ohm_bjt_desc &q, const double *xo, double *xn) {
+__device__ bool bjt_limit_one(const ohm_bjt_desc &q, const double *xo, double *xn) {
+  bool lim = false;
   const double pol = q.polarity, is = q.is_val, nf = q.nf_vt, nr = q.nr_vt;
   const double vcf = nf * log(nf / (M_SQRT2 * is)), vcr = nr * log(nr / (M_SQRT2 * is));
   const uint32_t B = q.base_idx, E = q.emitter_idx, C = q.collector_idx;
@@ -182,6 +183,7 @@ __device__ void bjt_limit_one(const ohm_bjt_desc &q, const double *xo, double *x
     const double s = (vl - vo) / (vn - vo);
     if (!q.node_fixed[1]) scale_node(xo, xn, B, s);
     if (!q.node_fixed[2]) scale_node(xo, xn, E, s);
+    lim = true;
   }
   vo = pol * (xv(xo, B) - xv(xo, C));
   vn = pol * (xv(xn, B) - xv(xn, C));
@@ -190,10 +192,12 @@ __device__ void bjt_limit_one(const ohm_bjt_desc &q, const double *xo, double *x
     const double s = (vl - vo) / (vn - vo);
     if (!q.node_fixed[1]) scale_node(xo, xn, B, s);
     if (!q.node_fixed[0]) scale_node(xo, xn, C, s);
+    lim = true;
   }
+  return lim;
 }
 
-__device__ void diode_limit_one(const ohm_diode_desc &q, const double *xo, double *xn) {
+__device__ bool diode_limit_one(const ohm_diode_desc &q, const double *xo, double *xn) {
   const double nvt = q.n_vt, is = q.is_val;
   const double vcrit = nvt * log(nvt / (M_SQRT2 * is));
   const double vo = xv(xo, q.anode_idx) - xv(xo, q.cathode_idx);
@@ -203,31 +207,39 @@ __device__ void diode_limit_one(const ohm_diode_desc &q, const double *xo, doubl
     const double s = (vl - vo) / (vn - vo);
     scale_node(xo, xn, q.anode_idx, s);
     scale_node(xo, xn, q.cathode_idx, s);
+    return true;
   }
+  return false;
 }
 
 // serial = 1: one thread walks the devices in index order (used when two
 // devices of the pass share a node, where a parallel pass would race).
+// *limited is set to 1 when any step is cut (every writer stores the same
+// value, so the unordered plain stores are benign).
 __global__ void k_bjt_limit(int nq, const ohm_bjt_desc *__restrict__ d, const double *xo,
-                            double *xn, int serial) {
+                            double *xn, int serial, int *limited) {
+  bool lim = false;
   if (serial) {
     if (blockIdx.x == 0 && threadIdx.x == 0)
-      for (int k = 0; k < nq; k++) bjt_limit_one(d[k], xo, xn);
-    return;
+      for (int k = 0; k < nq; k++) lim |= bjt_limit_one(d[k], xo, xn);
+  } else {
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < nq; k += gridDim.x * kBlock)
+      lim |= bjt_limit_one(d[k], xo, xn);
   }
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < nq; k += gridDim.x * kBlock)
-    bjt_limit_one(d[k], xo, xn);
+  if (lim) *limited = 1;
 }
 
 __global__ void k_diode_limit(int nd, const ohm_diode_desc *__restrict__ d, const double *xo,
-                              double *xn, int serial) {
+                              double *xn, int serial, int *limited) {
+  bool lim = false;
   if (serial) {
     if (blockIdx.x == 0 && threadIdx.x == 0)
-      for (int k = 0; k < nd; k++) diode_limit_one(d[k], xo, xn);
-    return;
+      for (int k = 0; k < nd; k++) lim |= diode_limit_one(d[k], xo, xn);
+  } else {
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < nd; k += gridDim.x * kBlock)
+      lim |= diode_limit_one(d[k], xo, xn);
   }
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < nd; k += gridDim.x * kBlock)
-    diode_limit_one(d[k], xo, xn);
+  if (lim) *limited = 1;
 }
 
 __global__ void k_fill_u32(int n, unsigned v, unsigned *x) {
@@ -263,11 +275,17 @@ __global__ void k_mos_limit_reduce(int nm, const ohm_mosfet_desc *__restrict__ d
 
 // mosfet_voltage_limit_apply_ds (:1106-1130)
 __global__ void k_mos_limit_apply(int n, const unsigned *__restrict__ scale,
-                                  const double *__restrict__ xo, double *__restrict__ xn) {
+                                  const double *__restrict__ xo, double *__restrict__ xn,
+                                  int *limited) {
+  bool lim = false;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const unsigned s = scale[i];
-    if (s < kScaleOne) xn[i] = xo[i] + (xn[i] - xo[i]) * ((double)s / (double)kScaleOne);
+    if (s < kScaleOne) {
+      xn[i] = xo[i] + (xn[i] - xo[i]) * ((double)s / (double)kScaleOne);
+      lim = true;
+    }
   }
+  if (lim) *limited = 1;
 }
 
 // convergence_check_ds (:1257-1297): max |x - x_old| and a NaN/Inf flag.
@@ -275,6 +293,8 @@ struct NewtonWord {
   double max_diff;
   int nan;
   int done;
+  int limited;  // a limiter cut this iteration's step
+  int pad;
 };
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -493,18 +513,18 @@ void enqueue_limit(ohm_newton_ctx *c, const double *xo, double *xn) {
   const dim3 blk(kBlock);
   if (c->nq)
     hipLaunchKernelGGL(k_bjt_limit, dim3(grid_for(c->nq)), blk, 0, b->stream, c->nq, c->qd, xo, xn,
-                       c->serial_bjt ? 1 : 0);
+                       c->serial_bjt ? 1 : 0, &c->word->limited);
   if (c->nm) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(c->n)), blk, 0, b->stream, c->n, kScaleOne,
                        c->scale);
     hipLaunchKernelGGL(k_mos_limit_reduce, dim3(grid_for(c->nm)), blk, 0, b->stream, c->nm, c->md,
                        xo, xn, c->scale);
     hipLaunchKernelGGL(k_mos_limit_apply, dim3(grid_for(c->n)), blk, 0, b->stream, c->n, c->scale,
-                       xo, xn);
+                       xo, xn, &c->word->limited);
   }
   if (c->nd)
     hipLaunchKernelGGL(k_diode_limit, dim3(grid_for(c->nd)), blk, 0, b->stream, c->nd, c->dd, xo,
-                       xn, c->serial_diode ? 1 : 0);
+                       xn, c->serial_diode ? 1 : 0, &c->word->limited);
   b->dispatches += 5;
 }
 
@@ -554,7 +574,7 @@ void ohm_newton_default_params(ohm_newton_params *p) {
   p->initial_guess = nullptr;
   p->bicgstab_floor = 1e-15;
   p->forcing = OHM_FORCING_FIXED;
-  p->_pad = 0;
+  p->forcing_safeguard = 0;
   p->forcing_max = 0.1;
 }
 
@@ -779,18 +799,24 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
   ohm_status rc = OHM_OK;
   int64_t it = 0;
   bool converged = false;
-  // Eisenstat-Walker forcing (choice 2, gamma 0.9, alpha 2; Eisenstat &
-  // Walker, SIAM J. Sci. Comput. 17(1), 1996): loose sub-solves while Newton
-  // is far from the solution, tight ones as ||F|| falls.
-  const bool ew = prm.forcing == OHM_FORCING_EW;
-  const double eta_max = prm.forcing_max > 0 ? std::max(prm.forcing_max, prm.bicgstab_tolerance)
-                                             : 0.1;
-  double eta = ew ? eta_max : prm.bicgstab_tolerance, f_prev = -1.0;
+  // Eisenstat-Walker forcing (choice 1 or 2): loose sub-solves while Newton
+  // is far from the solution, tight ones as ||F|| falls. Each sub-solve
+  // computes its forcing term on the device from its own starting residual
+  // ||F(x_k)||; the host carries ||F_k-1||, the last linear residual and
+  // eta_k-1 forward.
+  Forcing fc;
+  if (prm.forcing == OHM_FORCING_EW || prm.forcing == OHM_FORCING_EW1) {
+    fc.kind = prm.forcing;
+    fc.eta_min = prm.bicgstab_tolerance;
+    fc.eta_max = prm.forcing_max > 0 ? std::max(prm.forcing_max, prm.bicgstab_tolerance) : 0.1;
+    fc.rate_cap = prm.forcing_safeguard != 0;
+  }
   for (it = 0; it < prm.max_iterations; it++) {
     hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n, 4)), dim3(kBlock), 0, st, n, c->x, c->x_old);
     enqueue_eval_assemble(c, c->x);
     OHM_TRY(precond_refresh(b, c->A, c->P));
-    o.tol = eta;
+    o.tol = prm.bicgstab_tolerance;
+    o.forcing = fc;
     BicgOut bo;
     ohm_status ls;
     if (prm.precision == OHM_PRECISION_FP32_IR) {
@@ -804,15 +830,10 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
     }
     total_bicg += bo.iters;
     if (iters_per_newton) iters_per_newton[it] = (int32_t)bo.iters;
-    if (ew) {
-      const double f = bo.r0;  // ||F(x_k)||
-      if (f_prev > 0.0 && std::isfinite(f)) {
-        double e = 0.9 * (f / f_prev) * (f / f_prev);
-        const double safe = 0.9 * eta * eta;
-        if (safe > 0.1) e = std::max(e, safe);
-        eta = std::min(eta_max, std::max(prm.bicgstab_tolerance, e));
-      }
-      f_prev = f;
+    if (fc.kind && std::isfinite(bo.r0)) {
+      fc.f_prev = bo.r0;  // ||F(x_k)||
+      fc.lin_prev = bo.res;
+      fc.eta_prev = bo.eta;
     }
     // the assembled-value NaN flag (src/solver/newton.rs:258-263)
     OHM_HIP(hipMemcpyAsync(&h->nan, c->asm_nan, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -826,6 +847,7 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
       rc = ls;
       break;
     }
+    OHM_HIP(hipMemsetAsync(&c->word->limited, 0, sizeof(int), st));
     enqueue_limit(c, c->x_old, c->x);
     hipLaunchKernelGGL(k_converge, dim3(grid_for(n, 4)), dim3(kBlock), 0, st, n, c->x_old, c->x,
                        b->d_partials, b->d_counter, c->asm_nan, c->word);
@@ -834,6 +856,10 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
     OHM_HIP(hipStreamSynchronize(st));
     b->readbacks++;
     max_diff = h->max_diff;
+    // A limited step is not the Newton step the forcing theory assumes: the
+    // residual change reflects the limiter, so the next sub-solve starts from
+    // eta_max again instead of tightening on it.
+    if (fc.kind && h->limited) fc.f_prev = -1.0;
     if (h->nan) {
       rc = fail(OHM_ERR_NEWTON_NUMERICAL, "Newton solver encountered numerical error at iteration %lld",
                 (long long)it);

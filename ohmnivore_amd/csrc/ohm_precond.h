@@ -34,6 +34,15 @@ void precond_apply_f32(ohm_backend *b, const PrecondDev &p, const float *in, flo
 bool matrix_has_zero_diag(ohm_backend *b, const DevCsr &a, const double *h_values);
 
 // ---- BiCGSTAB drivers (ohm_bicgstab.hip) --------------------------------
+// Forcing inputs: the previous sub-solve's ||F_k-1||, final linear residual
+// and forcing term (f_prev < 0: first Newton iteration, eta = eta_max).
+struct Forcing {
+  int kind = 0;  // 0: off, OHM_FORCING_EW, OHM_FORCING_EW1
+  double f_prev = -1.0, lin_prev = 0.0, eta_prev = 0.0;
+  double eta_min = 1e-6, eta_max = 0.1;
+  int rate_cap = 0;  // apply the safeguard at every iteration (eta falls at most quadratically)
+};
+
 struct BicgOpts {
   int64_t max_iters = 10000;
   double tol = 1e-5;
@@ -44,13 +53,21 @@ struct BicgOpts {
   bool scaled_breakdown = true;
   int tol_mode = OHM_TOL_RHS;
   double tol_floor = 1e-15;
+  // Inexact-Newton forcing evaluated on the device from ||b - A x0|| (the
+  // Newton residual ||F_k|| when x0 = x_k) as the solve starts:
+  // OHM_FORCING_EW (Eisenstat-Walker choice 2) or OHM_FORCING_EW1
+  // (choice 1); tol is then ignored and the forcing term becomes the
+  // relative tolerance (OHM_TOL_INITIAL).
+  Forcing forcing;
 };
 struct BicgOut {
   bool fp64_fallback = false;  // FP32_IR finished in FP64
   int status = ST_RUNNING;
   int64_t iters = 0;
   int64_t polls = 0;
-  double r0 = 0.0;  // ||b - A x0|| (the Newton residual when x0 = x_k)
+  double r0 = 0.0;   // ||b - A x0|| (the Newton residual when x0 = x_k)
+  double res = 0.0;  // residual norm at exit (recurrence; FP64 true residual under IR)
+  double eta = 0.0;  // forcing term the solve used (forcing on)
 };
 
 // FP64 solve on device vectors (rhs, x of length a.n).

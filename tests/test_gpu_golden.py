@@ -41,16 +41,18 @@ def test_ladder_10k_golden(be):
 
 @pytest.mark.parametrize("precision,forcing,floor", [("fp64", "fixed", 1e-15),
                                                      ("fp32_ir", "fixed", 1e-15),
+                                                     ("fp64", "ew1", 1e-18),
                                                      ("fp64", "ew", 1e-18)])
 def test_diode_mesh_256_golden(be, precision, forcing, floor):
     """The headline workload at full size against the LU fixture, in the
     fixed-tolerance forms and in the benchmark's form (Eisenstat-Walker
-    forcing, sub-solve floor 1e-18 ||b||)."""
+    forcing with the safeguard at every iteration, sub-solve floor
+    1e-18 ||b||)."""
     mna = solver.compile(ir.diode_mesh(256, 256, seed=2))
     x, st = solver.newton_solve(be, mna.g, mna.b_dc, mna.diode_descriptors, precision=precision,
                                 abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-6,
                                 bicgstab_max_iterations=20_000, forcing=forcing,
-                                bicgstab_floor=floor)
+                                forcing_safeguard="always", bicgstab_floor=floor)
     ok, err = close(x[::4], load("diode_mesh_256.npz")["x_every4"])
     assert ok, (err, st)
 
