@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # WRITE_SIZE, KiB per dispatch), each scaled by what the counter reads for a
 # copy kernel of known size (65,536 KiB each way): FETCH x 1.9994, WRITE x 1.0.
 # Summary: profiles/spmv_diode_mesh_256_counters.md.
-SPMV_TRAFFIC_BYTES = round((2911.07 * 1.9994 + 512.03 * 1.0) * 1024)
+SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
 
 
 def log(msg: str) -> None:
