@@ -137,6 +137,13 @@ def headline(args, be, N, ir, solver):
     big_gbs = big_b / (msb.value * 1e-3) / 1e9
     big32_gbs = big32_b / (msb32.value * 1e-3) / 1e9
     del abig, bbx, bby
+    # practical HBM ceilings: read-only streaming and copy over 1 GiB
+    stream_bytes = 1 << 30
+    ms_rd, ms_cp = ctypes.c_double(), ctypes.c_double()
+    N.check(N.lib().ohm_stream_timed(be.h, ctypes.c_int64(stream_bytes), 0, 20, ctypes.byref(ms_rd)))
+    N.check(N.lib().ohm_stream_timed(be.h, ctypes.c_int64(stream_bytes), 1, 20, ctypes.byref(ms_cp)))
+    read_gbs = stream_bytes / (ms_rd.value * 1e-3) / 1e9
+    copy_gbs = 2 * stream_bytes / (ms_cp.value * 1e-3) / 1e9
 
     cpu, errors = None, {}
     if not (args.no_cpu_baseline or args.profile):
@@ -147,6 +154,7 @@ def headline(args, be, N, ir, solver):
         per_iter = (time.perf_counter() - t0) / max(its, 1)
         cold_iters = 16  # oracle LU-Newton from x = 0 on this circuit (measured, same algorithm)
         cpu = {"value": 1.0 / (per_iter * cold_iters), "unit": "solves/s", "cores": 1,
+               "host": host_info(),
                "kind": "port",
                "sample": f"{its} LU-Newton iterations (RCM + banded LU, 1 thread) warm-started "
                          f"at the GPU answer, {per_iter:.2f} s/iteration; rate = 1/(s/iteration "
@@ -166,9 +174,28 @@ def headline(args, be, N, ir, solver):
         "spmv_large_working_set": {
             "copies": 64, "bytes_per_launch": big_b, "fp64_gbs": big_gbs,
             "fp64_frac": big_gbs / HBM_PEAK_GBS, "fp32_bytes_per_launch": big32_b,
-            "fp32_gbs": big32_gbs, "fp32_frac": big32_gbs / HBM_PEAK_GBS},
+            "fp32_gbs": big32_gbs, "fp32_frac": big32_gbs / HBM_PEAK_GBS,
+            "fp64_frac_of_measured_read_ceiling": big_gbs / read_gbs},
+        "hbm_measured": {
+            "buffer_bytes": stream_bytes, "stream_read_gbs": read_gbs,
+            "copy_gbs": copy_gbs, "read_frac_of_peak": read_gbs / HBM_PEAK_GBS},
     }
     return out
+
+
+def host_info() -> dict:
+    """The CPU the baseline ran on: logical CPUs visible and the model name."""
+    import os
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "threads_used": 1}
 
 
 # ------------------------------------------------------------ workloads --
@@ -424,6 +451,7 @@ def main() -> int:
         "forms": {p: {k: v for k, v in f.items() if k != "x"} for p, f in forms.items()},
         "error_vs_fp64_lu": errors or None,
         "spmv_large_working_set": h["spmv_large_working_set"],
+        "hbm_measured": h["hbm_measured"],
         "batched_vs_plain_speedup": speedups,
         "workloads": workloads or None,
     }

@@ -74,6 +74,12 @@ ohm_status ohm_spmv_transpose(ohm_backend *b, const ohm_matrix *a, const ohm_buf
  * launch in *avg_ms. */
 ohm_status ohm_spmv_timed(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x, ohm_buffer *y,
                           int reps, int precision, double *avg_ms);
+/* HBM stream probe (no reference counterpart): `reps` launches streaming a
+ * `bytes`-sized device buffer (mode 0: read only, 16-byte loads; mode 1:
+ * copy into a second buffer of the same size), average ms per launch. With a
+ * buffer beyond the 256 MB Infinity Cache this is the practical HBM ceiling
+ * the SpMV bandwidth is read against. */
+ohm_status ohm_stream_timed(ohm_backend *b, int64_t bytes, int mode, int reps, double *avg_ms);
 /* SpMV autotuner (no reference counterpart): times the matrix's SpMV under
  * each candidate row-block shape (rows per workgroup 32-256 x staged
  * products per workgroup 512-2048) and keeps the fastest; the chosen shape

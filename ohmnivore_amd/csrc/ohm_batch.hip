@@ -979,6 +979,13 @@ __global__ __launch_bounds__(kBlock) void k_bcorrect(const BBlock *__restrict__ 
 }
 
 // float copies of value arrays
+__global__ void k_form_a(int64_t nnz, const double *__restrict__ g, const double *__restrict__ c,
+                         double sigma, double *__restrict__ a) {
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * kBlock)
+    a[k] = g[k] + sigma * c[k];
+}
+
 __global__ void k_d2f_b(int64_t n, const double *__restrict__ x, float *__restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock)
@@ -998,6 +1005,13 @@ struct ohm_batch {
   int max_n = 0;
   std::vector<BGeo> h_geo;
   BGeo *d_geo = nullptr;
+  // The systems as created (G and C separate). For real systems with a C
+  // matrix the FP64 SpMV reads A_k = G_k + sigma_k C_k formed once per sigma
+  // (h_geo's g points there, c is null); FP32 mirrors, the transient RHS
+  // (which needs G x and C x) and re-forming read these originals.
+  std::vector<BGeo> h_src;
+  BGeo *d_src = nullptr;
+  std::vector<double *> formed;
   std::vector<BBlock> h_blocks;  // full table, ordered by system
   BBlock *d_full = nullptr, *d_active = nullptr;
   int *d_list = nullptr;         // active system ids (persistent kernel)
@@ -1036,6 +1050,32 @@ BW<S> ws_view(void *base, int64_t N) {
 int64_t ws_bytes(const ohm_batch *bt, bool f32) {
   const int64_t stride = (bt->N + 63) & ~int64_t(63);
   return 12 * std::max<int64_t>(stride, 64) * (int64_t)elem_bytes(bt->cplx, f32);
+}
+
+// FP64 values of real systems with a C matrix: A_k = G_k + sigma_k C_k,
+// formed on the device whenever sigma changes (h_geo then points at A_k with
+// no C); complex systems keep forming G + sigma C inside the SpMV.
+ohm_status form_values(ohm_batch *bt) {
+  for (int k = 0; k < bt->K; k++) {
+    const BGeo &o = bt->h_src[k];
+    BGeo &q = bt->h_geo[k];
+    const int64_t nnz = bt->h_nnz[k];
+    if (bt->cplx || !o.c || o.n == 0 || nnz == 0) {
+      q.g = o.g;
+      q.c = o.c;
+      continue;
+    }
+    if (!bt->formed[k]) {
+      OHM_HIP(hipMalloc(&bt->formed[k], sizeof(double) * nnz));
+      bt->allocs.push_back(bt->formed[k]);
+    }
+    hipLaunchKernelGGL(k_form_a, dim3(std::min<int64_t>((nnz + kBlock - 1) / kBlock, 4096)),
+                       dim3(kBlock), 0, bt->owner->stream, nnz, o.g, o.c, o.sig_re, bt->formed[k]);
+    q.g = bt->formed[k];
+    q.c = nullptr;
+  }
+  OHM_HIP(hipGetLastError());
+  return OHM_OK;
 }
 
 struct Poll {
@@ -1229,9 +1269,10 @@ ohm_status ensure_ws(ohm_batch *bt, bool f32) {
     std::map<const double *, const float *> done;
     for (int k = 0; k < bt->K; k++) {
       BGeo &q = bt->h_geo[k];
+      BGeo &o = bt->h_src[k];
       const int64_t nnz = bt->h_nnz[k];
       for (int which = 0; which < 2; which++) {
-        const double *src = which ? q.c : q.g;
+        const double *src = which ? o.c : o.g;
         if (!src) continue;
         const float *dst;
         auto it = done.find(src);
@@ -1248,12 +1289,14 @@ ohm_status ensure_ws(ohm_batch *bt, bool f32) {
           dst = f;
         }
         if (which)
-          q.c32 = dst;
+          q.c32 = o.c32 = dst;
         else
-          q.g32 = dst;
+          q.g32 = o.g32 = dst;
       }
     }
     OHM_HIP(hipMemcpyAsync(bt->d_geo, bt->h_geo.data(), sizeof(BGeo) * bt->K, hipMemcpyHostToDevice,
+                           bt->owner->stream));
+    OHM_HIP(hipMemcpyAsync(bt->d_src, bt->h_src.data(), sizeof(BGeo) * bt->K, hipMemcpyHostToDevice,
                            bt->owner->stream));
     bt->has_f32 = true;
   }
@@ -1502,7 +1545,10 @@ ohm_status ohm_batch_create(ohm_backend *b, int32_t is_complex, int64_t k,
   }
   bt->N = off;
   const size_t nb = std::max<size_t>(bt->h_blocks.size(), 1);
+  bt->h_src = bt->h_geo;
+  bt->formed.assign(k, nullptr);
   if (hipMalloc(&bt->d_geo, sizeof(BGeo) * k) != hipSuccess ||
+      hipMalloc(&bt->d_src, sizeof(BGeo) * k) != hipSuccess ||
       hipMalloc(&bt->d_full, sizeof(BBlock) * nb) != hipSuccess ||
       hipMalloc(&bt->d_active, sizeof(BBlock) * nb) != hipSuccess ||
       hipMalloc(&bt->d_list, sizeof(int) * k) != hipSuccess ||
@@ -1514,6 +1560,11 @@ ohm_status ohm_batch_create(ohm_backend *b, int32_t is_complex, int64_t k,
       hipHostMalloc((void **)&bt->h_sys, sizeof(BSys) * k, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&bt->h_res, sizeof(double) * 2 * k, hipHostMallocDefault) != hipSuccess)
     return bail(fail(OHM_ERR_SOLVE, "Solve error: out of device memory for the batch"));
+  OHM_HIP(hipMemcpy(bt->d_src, bt->h_src.data(), sizeof(BGeo) * k, hipMemcpyHostToDevice));
+  {
+    ohm_status fs = form_values(bt);
+    if (fs != OHM_OK) return bail(fs);
+  }
   OHM_HIP(hipMemcpy(bt->d_geo, bt->h_geo.data(), sizeof(BGeo) * k, hipMemcpyHostToDevice));
   if (!bt->h_blocks.empty())
     OHM_HIP(hipMemcpy(bt->d_full, bt->h_blocks.data(), sizeof(BBlock) * bt->h_blocks.size(),
@@ -1533,8 +1584,8 @@ void ohm_batch_free(ohm_batch *bt) {
     (void)hipSetDevice(bt->owner->device);
     (void)hipStreamSynchronize(bt->owner->stream);
     for (void *p : bt->allocs) (void)hipFree(p);
-    void *ds[] = {bt->d_geo, bt->d_full, bt->d_active, bt->d_list, bt->d_mask, bt->d_sys,
-                  bt->d_partials, bt->d_res, bt->d_scale, bt->ws64, bt->ws32};
+    void *ds[] = {bt->d_geo, bt->d_src, bt->d_full, bt->d_active, bt->d_list, bt->d_mask,
+                  bt->d_sys, bt->d_partials, bt->d_res, bt->d_scale, bt->ws64, bt->ws32};
     for (void *p : ds)
       if (p) (void)hipFree(p);
     if (bt->h_sys) (void)hipHostFree(bt->h_sys);
@@ -1549,10 +1600,13 @@ ohm_status ohm_batch_set_sigma(ohm_batch *bt, const double *sigma) {
   if (!bt || !sigma) return fail(OHM_ERR_INVALID, "null argument");
   if (!backend_alive(bt->owner)) return fail(OHM_ERR_INVALID, "backend destroyed");
   for (int k = 0; k < bt->K; k++) {
-    bt->h_geo[k].sig_re = sigma[2 * k];
-    bt->h_geo[k].sig_im = sigma[2 * k + 1];
+    bt->h_geo[k].sig_re = bt->h_src[k].sig_re = sigma[2 * k];
+    bt->h_geo[k].sig_im = bt->h_src[k].sig_im = sigma[2 * k + 1];
   }
+  OHM_TRY(form_values(bt));
   OHM_HIP(hipMemcpyAsync(bt->d_geo, bt->h_geo.data(), sizeof(BGeo) * bt->K, hipMemcpyHostToDevice,
+                         bt->owner->stream));
+  OHM_HIP(hipMemcpyAsync(bt->d_src, bt->h_src.data(), sizeof(BGeo) * bt->K, hipMemcpyHostToDevice,
                          bt->owner->stream));
   OHM_HIP(hipStreamSynchronize(bt->owner->stream));
   bt->jac64_valid = bt->jac32_valid = false;
@@ -1973,7 +2027,7 @@ ohm_status tran_rhs(ohm_transient *tr, double t_next, double t_prev, double sig_
   double *b = x + 10 * stride;
   TRhsArgs a{t_next, t_prev, sig_c, trap ? 1 : 0};
   hipLaunchKernelGGL(k_tran_rhs, dim3(bt->h_blocks.size()), dim3(kBlock), 0, bt->owner->stream,
-                     bt->d_full, bt->d_geo, x, b, tr->d_srp, tr->d_sent, tr->d_sslot, tr->d_src,
+                     bt->d_full, bt->d_src, x, b, tr->d_srp, tr->d_sent, tr->d_sslot, tr->d_src,
                      tr->d_pwl, a);
   bt->owner->dispatches++;
   return OHM_OK;
@@ -1985,7 +2039,7 @@ ohm_status tran_single_solve(ohm_transient *tr, double sig, int precond, const B
                              int precision, double inner_tol, int64_t max_ref, int64_t &iters) {
   ohm_backend *b = tr->owner;
   ohm_batch *bt = tr->bt;
-  const BGeo &q = bt->h_geo[0];
+  const BGeo &q = bt->h_src[0];
   const int64_t nnz = bt->h_nnz[0];
   if (!tr->A1.rowptr)
     OHM_TRY(csr_upload(b, tr->A1, nullptr, tr->h_col.data(), tr->h_rowptr.data(), (int)tr->n, nnz));

@@ -262,6 +262,15 @@ ohm_status ohm_matrix_autotune(ohm_backend *b, ohm_matrix *m, int32_t *rows_per_
   return OHM_OK;
 }
 
+ohm_status ohm_stream_timed(ohm_backend *b, int64_t bytes, int mode, int reps, double *avg_ms) {
+  if (!b || !avg_ms || reps <= 0 || bytes <= 0 || (mode != 0 && mode != 1))
+    return fail(OHM_ERR_INVALID, "bad argument");
+  double ms = 0.0;
+  OHM_TRY(stream_timed(b, bytes, mode, reps, ms));
+  *avg_ms = ms;
+  return OHM_OK;
+}
+
 ohm_status ohm_spmv_timed(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x, ohm_buffer *y,
                           int reps, int precision, double *avg_ms) {
   if (!b || !a || !avg_ms || reps <= 0) return fail(OHM_ERR_INVALID, "bad argument");

@@ -145,6 +145,7 @@ void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blo
                        int rows_cap = kSpmvRowsPerBlock, int nnz_cap = kSpmvNnzPerBlock);
 // Pick the fastest (rows_cap, nnz_cap) for m by timing each candidate.
 ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out);
+ohm_status stream_timed(ohm_backend *b, int64_t bytes, int mode, int reps, double &avg_ms);
 ohm_status csr_upload(ohm_backend *b, DevCsr &m, const double *val, const uint32_t *col,
                       const uint32_t *rowptr, int n, int64_t nnz);
 void csr_free(DevCsr &m);

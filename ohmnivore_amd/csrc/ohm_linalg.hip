@@ -445,6 +445,67 @@ void csr_free(DevCsr &m) {
   m = DevCsr();
 }
 
+// ---------------------------------------------------- HBM stream probes --
+// Streaming read (16-byte loads, four in flight per thread) and copy over a
+// buffer larger than the Infinity Cache: the practical HBM ceilings that the
+// SpMV's bandwidth is read against.
+__global__ __launch_bounds__(kBlock) void k_stream_read(int64_t n2, const double2 *__restrict__ x,
+                                                       double *__restrict__ sink) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n2; i += 4 * stride) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = i + u * stride;
+      v[u] = j < n2 ? x[j] : double2{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) s += v[u].x + v[u].y;
+  }
+  if (s == -1.0) sink[0] = s;  // data-dependent store keeps the loads
+}
+
+__global__ __launch_bounds__(kBlock) void k_stream_copy(int64_t n2, const double2 *__restrict__ x,
+                                                       double2 *__restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n2; i += stride) y[i] = x[i];
+}
+
+ohm_status stream_timed(ohm_backend *b, int64_t bytes, int mode, int reps, double &avg_ms) {
+  const int64_t n2 = std::max<int64_t>(bytes / 16, 1);
+  double2 *x = nullptr, *y = nullptr;
+  OHM_HIP(hipMalloc(&x, sizeof(double2) * n2));
+  if (hipMalloc(&y, mode == 1 ? sizeof(double2) * n2 : sizeof(double2)) != hipSuccess) {
+    (void)hipFree(x);
+    return fail(OHM_ERR_SOLVE, "Solve error: out of device memory for the stream probe");
+  }
+  const dim3 grid(32768), blk(kBlock);
+  auto one = [&]() {
+    if (mode == 1)
+      hipLaunchKernelGGL(k_stream_copy, grid, blk, 0, b->stream, n2, x, y);
+    else
+      hipLaunchKernelGGL(k_stream_read, grid, blk, 0, b->stream, n2, x, (double *)y);
+  };
+  ohm_status rc = OHM_OK;
+  if (hipMemsetAsync(x, 0, sizeof(double2) * n2, b->stream) != hipSuccess) rc = fail(OHM_ERR_SOLVE, "memset");
+  if (rc == OHM_OK) {
+    for (int w = 0; w < 2; w++) one();
+    (void)hipEventRecord(b->ev0, b->stream);
+    for (int k = 0; k < reps; k++) one();
+    (void)hipEventRecord(b->ev1, b->stream);
+    (void)hipEventSynchronize(b->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    avg_ms = (double)ms / reps;
+    if (hipGetLastError() != hipSuccess) rc = fail(OHM_ERR_SOLVE, "Solve error: stream probe failed");
+  }
+  (void)hipFree(x);
+  (void)hipFree(y);
+  b->dispatches += reps + 2;
+  return rc;
+}
+
 ohm_status ensure_partials(ohm_backend *b, size_t count) {
   if (count <= b->partials_cap) return OHM_OK;
   if (b->d_partials) OHM_HIP(hipFree(b->d_partials));
