@@ -201,17 +201,19 @@ static ohm_status upload(V **dst, const std::vector<V> &src) {
 }
 
 // Row-sorted pattern product pattern(P*P) (src/solver/preconditioner.rs:185-201).
-static void pattern_square(int n, const std::vector<uint32_t> &rp, const std::vector<uint32_t> &cl,
-                           std::vector<uint32_t> &orp, std::vector<uint32_t> &ocl) {
+// Sparsity pattern of the product A B (row-sorted column lists).
+static void pattern_mul(int n, const std::vector<uint32_t> &ra, const std::vector<uint32_t> &ca,
+                        const std::vector<uint32_t> &rb, const std::vector<uint32_t> &cb,
+                        std::vector<uint32_t> &orp, std::vector<uint32_t> &ocl) {
   orp.assign(n + 1, 0);
   ocl.clear();
   std::vector<int> mark(n, -1);
   for (int i = 0; i < n; i++) {
     size_t start = ocl.size();
-    for (uint32_t a = rp[i]; a < rp[i + 1]; a++) {
-      uint32_t k = cl[a];
-      for (uint32_t b = rp[k]; b < rp[k + 1]; b++) {
-        uint32_t j = cl[b];
+    for (uint32_t a = ra[i]; a < ra[i + 1]; a++) {
+      uint32_t k = ca[a];
+      for (uint32_t b = rb[k]; b < rb[k + 1]; b++) {
+        uint32_t j = cb[b];
         if (mark[j] != i) {
           mark[j] = i;
           ocl.push_back(j);
@@ -239,12 +241,12 @@ static ohm_status build_side(int n, int level, bool lower, const std::vector<uin
     }
     prp[r + 1] = (uint32_t)pcl.size();
   }
-  std::vector<uint32_t> mrp, mcl;  // pattern of M = pattern(P^(level+1))
-  if (level >= 1) {
-    pattern_square(n, prp, pcl, mrp, mcl);
-  } else {
-    mrp = prp;
-    mcl = pcl;
+  std::vector<uint32_t> mrp = prp, mcl = pcl;  // pattern of M = pattern(P^(level+1))
+  for (int l = 0; l < level; l++) {
+    std::vector<uint32_t> trp, tcl;
+    pattern_mul(n, mrp, mcl, prp, pcl, trp, tcl);
+    mrp.swap(trp);
+    mcl.swap(tcl);
   }
   // column view of M: for each column k, rows J ascending
   std::vector<uint32_t> ccp(n + 1, 0), crow(mcl.size());
