@@ -242,7 +242,9 @@ ohm_status ohm_ac_sweep(ohm_backend *b, int64_t n, const uint32_t *g_rowptr,
  * n_circuits * n_freqs entries (circuit-major) with the FP64 residual
  * ||b - A x|| / ||b|| of every system. device_ms (may be NULL): kernel time
  * of the direct part. OHM_AC_DIRECT_BUDGET_GB bounds its working memory
- * (default 64). */
+ * (default 64): that workspace stays allocated on the backend for later
+ * sweeps (re-allocating tens of GB per sweep costs seconds) and is released
+ * by ohm_backend_destroy. */
 enum { OHM_AC_AUTO = 0, OHM_AC_ITERATIVE = 1, OHM_AC_DIRECT = 2 };
 typedef struct {
   int64_t n;

@@ -23,9 +23,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ohm_internal.h"
@@ -374,6 +376,25 @@ std::vector<uint32_t> rcm(int64_t n, const std::vector<uint32_t> &rp,
   return order;
 }
 
+// f(0) .. f(n-1) on up to 16 host threads (independent items; f must not
+// throw).
+template <typename F>
+void parallel_for(int64_t n, F f) {
+  const int64_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int64_t T = std::min<int64_t>(n, hw);
+  if (T <= 1) {
+    for (int64_t i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  std::vector<std::thread> pool;
+  for (int64_t t = 0; t < T; t++)
+    pool.emplace_back([&] {
+      for (int64_t i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto &th : pool) th.join();
+}
+
 struct Prepared {
   int64_t n = 0;
   int band = -1;  // half bandwidth after RCM (max of kl, ku)
@@ -395,20 +416,31 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
   if (method < OHM_AC_AUTO || method > OHM_AC_DIRECT) return fail(OHM_ERR_INVALID, "unknown AC method %d", method);
   OHM_HIP(hipSetDevice(b->device));
   hipStream_t st = b->stream;
-  // symbolic preparation per circuit
+  // symbolic preparation per circuit (pattern union, RCM, bandwidth), on
+  // host threads; inputs are checked first so errors are deterministic
   std::vector<Prepared> prep(n_circuits);
   int band = 0;
   std::vector<char> direct(n_circuits, 0);
   for (int64_t c = 0; c < n_circuits; c++) {
     const ohm_ac_circuit &q = circ[c];
-    Prepared &p = prep[c];
-    p.n = q.n;
     if (q.n < 0 || q.n > 0x7fffffff) return fail(OHM_ERR_INVALID, "circuit %lld: bad size", (long long)c);
     if (q.n == 0) continue;
     if (!q.g_rowptr || !q.g_val || !q.b_ac || (q.c_val && !q.c_rowptr))
       return fail(OHM_ERR_INVALID, "circuit %lld: null arrays", (long long)c);
-    OHM_TRY(csr_union(q.n, q.g_rowptr, q.g_col, q.g_val, q.c_val ? q.c_rowptr : nullptr, q.c_col,
-                      q.c_val, p.rp, p.cl, p.g, p.c));
+  }
+  std::vector<ohm_status> pst(n_circuits, OHM_OK);
+  std::vector<std::string> pmsg(n_circuits);
+  parallel_for(n_circuits, [&](int64_t c) {
+    const ohm_ac_circuit &q = circ[c];
+    Prepared &p = prep[c];
+    p.n = q.n;
+    if (q.n == 0) return;
+    pst[c] = csr_union(q.n, q.g_rowptr, q.g_col, q.g_val, q.c_val ? q.c_rowptr : nullptr, q.c_col,
+                       q.c_val, p.rp, p.cl, p.g, p.c);
+    if (pst[c] != OHM_OK) {
+      pmsg[c] = ohm_last_error();  // this thread's message
+      return;
+    }
     p.perm = rcm(q.n, p.rp, p.cl);
     p.iperm.assign(q.n, 0);
     for (int64_t i = 0; i < q.n; i++) p.iperm[p.perm[i]] = (uint32_t)i;
@@ -417,6 +449,14 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
       for (uint32_t k = p.rp[r]; k < p.rp[r + 1]; k++)
         bw = std::max<int64_t>(bw, std::llabs((int64_t)p.iperm[r] - (int64_t)p.iperm[p.cl[k]]));
     p.band = (int)std::min<int64_t>(bw, 1 << 20);
+  });
+  for (int64_t c = 0; c < n_circuits; c++) {
+    if (pst[c] != OHM_OK) {
+      set_error(pmsg[c]);
+      return pst[c];
+    }
+    if (circ[c].n == 0) continue;
+    const Prepared &p = prep[c];
     direct[c] = method == OHM_AC_ITERATIVE ? 0 : (p.band <= 4);
     if (method == OHM_AC_DIRECT && !direct[c])
       return fail(OHM_ERR_INVALID,
@@ -465,35 +505,54 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
   };
   std::vector<const double *> d_gb(n_circuits), d_cb(n_circuits), d_b(n_circuits);
   std::vector<const uint32_t *> d_perm(n_circuits);
-  for (int64_t c = 0; c < n_circuits; c++) {
-    if (!direct[c] || circ[c].n == 0) continue;
-    const Prepared &p = prep[c];
-    const int64_t n = p.n;
-    std::vector<double> gb((size_t)n * W, 0.0), cb((size_t)n * W, 0.0), bp(2 * n);
-    for (int64_t r = 0; r < n; r++) {
-      const int64_t i = p.iperm[r];
-      for (uint32_t k = p.rp[r]; k < p.rp[r + 1]; k++) {
-        const int64_t jcol = p.iperm[p.cl[k]];
-        const int64_t d = jcol - i + B;
-        gb[i * W + d] += p.g[k];
-        cb[i * W + d] += p.c[k];
+  // band arrays in the RCM order, built on host threads 32 circuits at a
+  // time and uploaded from this thread
+  std::vector<int64_t> dlist;
+  for (int64_t c = 0; c < n_circuits; c++)
+    if (direct[c] && circ[c].n > 0) dlist.push_back(c);
+  struct BandHost {
+    std::vector<double> gb, cb, bp;
+  };
+  for (size_t c0 = 0; c0 < dlist.size(); c0 += 32) {
+    const size_t cn = std::min<size_t>(32, dlist.size() - c0);
+    std::vector<BandHost> hb(cn);
+    parallel_for((int64_t)cn, [&](int64_t t) {
+      const int64_t c = dlist[c0 + t];
+      const Prepared &p = prep[c];
+      const int64_t n = p.n;
+      BandHost &h = hb[t];
+      h.gb.assign((size_t)n * W, 0.0);
+      h.cb.assign((size_t)n * W, 0.0);
+      h.bp.assign(2 * n, 0.0);
+      for (int64_t r = 0; r < n; r++) {
+        const int64_t i = p.iperm[r];
+        for (uint32_t k = p.rp[r]; k < p.rp[r + 1]; k++) {
+          const int64_t d = (int64_t)p.iperm[p.cl[k]] - i + B;
+          h.gb[i * W + d] += p.g[k];
+          h.cb[i * W + d] += p.c[k];
+        }
+        h.bp[2 * i] = circ[c].b_ac[2 * r];
+        h.bp[2 * i + 1] = circ[c].b_ac[2 * r + 1];
       }
-      bp[2 * i] = circ[c].b_ac[2 * r];
-      bp[2 * i + 1] = circ[c].b_ac[2 * r + 1];
+    });
+    for (size_t t = 0; t < cn; t++) {
+      const int64_t c = dlist[c0 + t];
+      const int64_t n = prep[c].n;
+      const BandHost &h = hb[t];
+      void *pg, *pc, *pb, *pp;
+      OHM_TRY(dmal(&pg, sizeof(double) * h.gb.size()));
+      OHM_TRY(dmal(&pc, sizeof(double) * h.cb.size()));
+      OHM_TRY(dmal(&pb, sizeof(double) * h.bp.size()));
+      OHM_TRY(dmal(&pp, sizeof(uint32_t) * n));
+      OHM_HIP(hipMemcpy(pg, h.gb.data(), sizeof(double) * h.gb.size(), hipMemcpyHostToDevice));
+      OHM_HIP(hipMemcpy(pc, h.cb.data(), sizeof(double) * h.cb.size(), hipMemcpyHostToDevice));
+      OHM_HIP(hipMemcpy(pb, h.bp.data(), sizeof(double) * h.bp.size(), hipMemcpyHostToDevice));
+      OHM_HIP(hipMemcpy(pp, prep[c].perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+      d_gb[c] = (const double *)pg;
+      d_cb[c] = (const double *)pc;
+      d_b[c] = (const double *)pb;
+      d_perm[c] = (const uint32_t *)pp;
     }
-    void *pg, *pc, *pb, *pp;
-    OHM_TRY(dmal(&pg, sizeof(double) * gb.size()));
-    OHM_TRY(dmal(&pc, sizeof(double) * cb.size()));
-    OHM_TRY(dmal(&pb, sizeof(double) * bp.size()));
-    OHM_TRY(dmal(&pp, sizeof(uint32_t) * n));
-    OHM_HIP(hipMemcpy(pg, gb.data(), sizeof(double) * gb.size(), hipMemcpyHostToDevice));
-    OHM_HIP(hipMemcpy(pc, cb.data(), sizeof(double) * cb.size(), hipMemcpyHostToDevice));
-    OHM_HIP(hipMemcpy(pb, bp.data(), sizeof(double) * bp.size(), hipMemcpyHostToDevice));
-    OHM_HIP(hipMemcpy(pp, p.perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-    d_gb[c] = (const double *)pg;
-    d_cb[c] = (const double *)pc;
-    d_b[c] = (const double *)pb;
-    d_perm[c] = (const uint32_t *)pp;
   }
   // waves: up to 64 frequencies of one circuit; longest circuits first
   struct WaveItem {
@@ -526,6 +585,8 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
   OHM_HIP(hipMemsetAsync(d_res, 0, sizeof(double) * res_h.size(), st));
   float kernel_ms = 0.f;
   size_t pos = 0;
+  bool want_x = false;
+  for (int64_t c = 0; c < n_circuits; c++) want_x |= direct[c] && circ[c].x != nullptr;
   while (pos < items.size()) {
     size_t end = pos;
     double bytes = 0.0;
@@ -576,14 +637,27 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
       store += w.n;
       hw[k] = w;
     }
-    BandWave *d_w;
-    cz *dU, *dY, *dX;
-    double *d_part;
-    OHM_TRY(dmal((void **)&d_w, sizeof(BandWave) * n_w));
-    OHM_TRY(dmal((void **)&dU, sizeof(cz) * store * 64 * W));
-    OHM_TRY(dmal((void **)&dY, sizeof(cz) * store * 64));
-    OHM_TRY(dmal((void **)&dX, sizeof(cz) * std::max<int64_t>(out_total, 1)));
-    OHM_TRY(dmal((void **)&d_part, sizeof(double) * 2 * (size_t)n_w * 64 * kResChunks));
+    // chunk buffers carved from the backend's workspace (kept across sweeps:
+    // re-allocating tens of GB per chunk costs seconds)
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t bw_b = al(sizeof(BandWave) * n_w), u_b = al(sizeof(cz) * store * 64 * W),
+                 y_b = al(sizeof(cz) * store * 64),
+                 x_b = want_x ? al(sizeof(cz) * std::max<int64_t>(out_total, 1)) : 0,
+                 p_b = al(sizeof(double) * 2 * (size_t)n_w * 64 * kResChunks);
+    const size_t need = bw_b + u_b + y_b + x_b + p_b;
+    if (b->ac_ws_bytes < need) {
+      if (b->ac_ws) OHM_HIP(hipFree(b->ac_ws));
+      b->ac_ws = nullptr;
+      b->ac_ws_bytes = 0;
+      OHM_HIP(hipMalloc(&b->ac_ws, need));
+      b->ac_ws_bytes = need;
+    }
+    char *wsp = (char *)b->ac_ws;
+    BandWave *d_w = (BandWave *)wsp;
+    cz *dU = (cz *)(wsp + bw_b);
+    cz *dY = (cz *)(wsp + bw_b + u_b);
+    cz *dX = want_x ? (cz *)(wsp + bw_b + u_b + y_b) : nullptr;
+    double *d_part = (double *)(wsp + bw_b + u_b + y_b + x_b);
     OHM_HIP(hipMemcpyAsync(d_w, hw.data(), sizeof(BandWave) * n_w, hipMemcpyHostToDevice, st));
     OHM_HIP(hipEventRecord(b->ev0, st));
     const dim3 rg(kResChunks, n_w);
@@ -599,9 +673,12 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
     }
     hipLaunchKernelGGL(k_band_resid_sum, dim3((n_w * 64 + 255) / 256), dim3(256), 0, st, d_w, n_w,
                        d_part, d_res);
-    hipLaunchKernelGGL(k_band_unpermute, dim3(64, n_w), dim3(256), 0, st, d_w, dY, dX);
+    if (want_x) {
+      hipLaunchKernelGGL(k_band_unpermute, dim3(64, n_w), dim3(256), 0, st, d_w, dY, dX);
+      b->dispatches++;
+    }
     OHM_HIP(hipEventRecord(b->ev1, st));
-    b->dispatches += 4;
+    b->dispatches += 3;
     for (int64_t c = 0; c < n_circuits; c++) {
       if (f_hi[c] < 0 || !circ[c].x) continue;
       const int64_t n = circ[c].n;
@@ -613,10 +690,6 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
     (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
     kernel_ms += ms;
     b->readbacks++;
-    for (void *p : {(void *)d_w, (void *)dU, (void *)dY, (void *)dX, (void *)d_part}) {
-      (void)hipFree(p);
-      allocs.erase(std::find(allocs.begin(), allocs.end(), p));
-    }
     pos = end;
   }
   OHM_HIP(hipMemcpy(singular_h.data(), d_sing, sizeof(int) * singular_h.size(), hipMemcpyDeviceToHost));

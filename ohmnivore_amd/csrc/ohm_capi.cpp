@@ -102,6 +102,7 @@ void ohm_backend_destroy(ohm_backend *b) {
   if (b->d_counter) (void)hipFree(b->d_counter);
   if (b->d_scalars) (void)hipFree(b->d_scalars);
   if (b->h_pinned) (void)hipHostFree(b->h_pinned);
+  if (b->ac_ws) (void)hipFree(b->ac_ws);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);

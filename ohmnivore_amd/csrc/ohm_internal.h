@@ -118,6 +118,8 @@ struct ohm_backend {
   unsigned *d_counter = nullptr;     // ticket for one-shot reductions
   double *d_scalars = nullptr;       // device results of one-shot dots
   double *h_pinned = nullptr;        // pinned host staging for scalar readback
+  void *ac_ws = nullptr;             // banded AC solver's chunk workspace (grow-only)
+  size_t ac_ws_bytes = 0;
   int64_t dispatches = 0, readbacks = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
