@@ -101,6 +101,34 @@ class Pulse:
         return self.v1
 
 
+@dataclass
+class Sin:
+    """SIN(vo va freq [td [theta]]) — src/ir.rs:117-123."""
+    vo: float
+    va: float
+    freq: float
+    td: float = 0.0
+    theta: float = 0.0
+
+
+@dataclass
+class Exp:
+    """EXP(v1 v2 [td1 [tau1 [td2 [tau2]]]]) — src/ir.rs:127-134; absent times
+    are f64::MAX (never reached)."""
+    v1: float
+    v2: float
+    td1: float = 0.0
+    tau1: float = 1.7976931348623157e308
+    td2: float = 1.7976931348623157e308
+    tau2: float = 1.7976931348623157e308
+
+
+@dataclass
+class Pwl:
+    """PWL(t1 v1 t2 v2 ...) — src/ir.rs:124-126."""
+    pairs: list
+
+
 def is_ground(name: str) -> bool:
     """src/compiler.rs:178-180."""
     return name == "0" or name.lower() == "gnd"

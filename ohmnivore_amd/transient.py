@@ -52,6 +52,7 @@ def source_table(ckt: ir.Circuit, mna: ir.MnaSystem):
     names = ckt.node_names
     n_nodes = mna.n_nodes
     out = []
+    pwl = []  # (time, value) rows; a PWL entry names its run by offset and count
     branch = 0
     for k in range(comps.shape[0]):
         kind = int(comps["kind"][k])
@@ -63,9 +64,22 @@ def source_table(ckt: ir.Circuit, mna: ir.MnaSystem):
         e = np.zeros(1, dtype=ir.TRAN_SOURCE_DTYPE)[0]
         tr = ckt.tran.get(k)
         flags = int(comps["flags"][k])
-        if tr is not None:
+        if isinstance(tr, ir.Pulse):
             e["kind"] = ir.SRC_PULSE
             e["param"] = [tr.v1, tr.v2, tr.td, tr.tr, tr.tf, tr.pw, tr.per]
+        elif isinstance(tr, ir.Sin):
+            e["kind"] = ir.SRC_SIN
+            e["param"][:5] = [tr.vo, tr.va, tr.freq, tr.td, tr.theta]
+        elif isinstance(tr, ir.Exp):
+            e["kind"] = ir.SRC_EXP
+            e["param"][:6] = [tr.v1, tr.v2, tr.td1, tr.tau1, tr.td2, tr.tau2]
+        elif isinstance(tr, ir.Pwl):
+            e["kind"] = ir.SRC_PWL
+            e["pwl_offset"] = len(pwl)
+            e["pwl_count"] = len(tr.pairs)
+            pwl.extend((float(t), float(v)) for t, v in tr.pairs)
+        elif tr is not None:
+            raise TypeError(f"component {k}: unknown waveform {type(tr).__name__}")
         else:
             e["kind"] = ir.SRC_DC
             e["dc"] = float(comps["value"][k]) if flags & ir.HAS_DC else 0.0
@@ -86,7 +100,7 @@ def source_table(ckt: ir.Circuit, mna: ir.MnaSystem):
             e["n_idx"] = j
         out.append(e)
     table = np.array(out, dtype=ir.TRAN_SOURCE_DTYPE) if out else np.zeros(0, ir.TRAN_SOURCE_DTYPE)
-    return table, np.zeros((0, 2))
+    return table, np.array(pwl, dtype=np.float64).reshape(-1, 2)
 
 
 def c_on_g(mna: ir.MnaSystem) -> np.ndarray:
@@ -254,21 +268,36 @@ class TranAnalysis:
 
 def run(backend: HipBackend, ckt: ir.Circuit, mna: ir.MnaSystem, tstep: float, tstop: float,
         tstart: float = 0.0, uic: bool = False, method: str = "adaptive",
-        params: N.BicgstabParams | None = None, precond: int = N.PREC_JACOBI) -> TranAnalysis:
+        params: N.BicgstabParams | None = None, precond: int = N.PREC_JACOBI,
+        dc_kw: dict | None = None) -> TranAnalysis:
     """transient::run (src/analysis/transient.rs:25-199): initial state from
-    the DC operating point (linear circuits: G x = b_dc) or the UIC system,
-    both solved on the GPU, then GPU time stepping of every unknown."""
-    from .solver import GpuSolver
-    s = GpuSolver(backend, precond=N.PREC_AUTO,
-                  params=bicgstab_params(tolerance=1e-12, max_iterations=100000))
+    the DC operating point (dc::run, so nonlinear devices are solved by the
+    Newton cascade) or the UIC system, both on the GPU, then GPU time
+    stepping of every unknown. As in the reference, the time stepping itself
+    uses G and C only: diode, BJT and MOSFET currents enter through the
+    operating point. ``dc_kw``: dc.run's Newton settings for that point."""
     if uic:
+        from .solver import GpuSolver
+        s = GpuSolver(backend, precond=N.PREC_AUTO,
+                      params=bicgstab_params(tolerance=1e-12, max_iterations=100000))
         a, b = uic_system(ckt, mna)
         x0 = s.solve_real(a, b)
     else:
-        x0 = s.solve_real(mna.g, mna.b_dc)
+        from . import dc
+        x0 = dc.run(backend, mna, **(dc_kw or {})).x
     src, pwl = source_table(ckt, mna)
     tr = Transient(backend, mna, src, pwl)
-    res = tr.run(tstep, tstop, tstart, method=method, x0=x0, params=params, precond=precond)
+    # Every accepted step is recorded. Start from room for 4 records per
+    # tstep and rerun with more if the stepper filled it (steps never fall
+    # below h_min / 2 = tstep / 20000, which bounds the count).
+    cap = int(np.ceil(tstop / tstep * (1.0 if method in ("be", "fixed_be") else 4.0))) + 2
+    bound = int(np.ceil(tstop / tstep * 20000.0)) + 2
+    while True:
+        res = tr.run(tstep, tstop, tstart, method=method, x0=x0, params=params, precond=precond,
+                     max_records=cap)
+        if res.times.shape[0] < cap or cap >= bound:
+            break
+        cap = min(cap * 4, bound)
     w = res.waves[:, 0, :]
     nn = mna.n_nodes
     nodes = [(name, w[:, i].copy()) for i, name in enumerate(mna.node_names)]

@@ -1,0 +1,405 @@
+"""Netlist reader, compile-from-text and CSV writers on the CPU.
+
+Known answers are the ones the reference's parser tests hold
+(src/parser.rs:838-1911), its integration tests' netlists
+(tests/integration_test.rs, tests/diode_integration_test.rs,
+tests/transistor_integration_test.rs) and its regression netlists
+(tests/regression/circuits, copied below as fixtures: data, not code).
+Compilation is host code, so parse -> compile runs here too.
+"""
+import io
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from ohmnivore_amd import circuit as ir
+from ohmnivore_amd import netlist as nl
+from ohmnivore_amd import output
+from ohmnivore_amd import solver as S
+from ohmnivore_amd import transient as T
+from ohmnivore_amd._native import OhmnivoreError
+
+MAXF = 1.7976931348623157e308
+
+
+# --------------------------------------------------------------- values --
+@pytest.mark.parametrize("text,value", [
+    ("100", 100.0), ("4.7", 4.7), ("10k", 10e3), ("10K", 10e3), ("1MEG", 1e6),
+    ("2.2meg", 2.2e6), ("100m", 0.1), ("4.7u", 4.7e-6), ("100n", 100e-9), ("22p", 22e-12),
+    ("10f", 10e-15), ("1T", 1e12), ("2.5G", 2.5e9), ("1e3", 1000.0), ("-3.3", -3.3),
+    (".5", 0.5), ("5.", 5.0), ("+2k", 2e3), ("1.5e-3k", 1.5),
+])
+def test_eng_value_known_answers(text, value):
+    """test_eng_value_* (src/parser.rs:842-932)."""
+    v, rest = nl.eng_value(text)
+    assert rest == ""
+    assert abs(v - value) <= 1e-12 * abs(value)
+
+
+def test_eng_value_is_number_times_multiplier():
+    # the product is formed in binary64 exactly as num * mult: 10u is not 1e-5
+    assert nl.eng_value("10u")[0] == 10.0 * 1e-6 != 1e-5
+    assert nl.eng_value("10uF") == (10.0 * 1e-6, "F")      # unit letters are left over
+    assert nl.eng_value("1Mohm") == (1e-3, "ohm")           # M is milli, MEG is mega
+    assert nl.eng_value("3MEGohm") == (3e6, "ohm")
+    assert math.isinf(nl.eng_value("inf")[0]) and math.isnan(nl.eng_value("NaN")[0])
+    for bad in ("k", "", "e5", "1e", "1e+", "-", "."):
+        with pytest.raises(ValueError):
+            nl.parse_value(bad)
+
+
+# ------------------------------------------------------------- elements --
+def one(text):
+    c = nl.parse(text)
+    assert len(c.components) == 1
+    return c.components[0]
+
+
+def test_rlc_lines():
+    e = one("R1 1 0 10k")
+    assert (e.kind, e.name, e.nodes, e.value) == ("R", "R1", ("1", "0"), 10e3)
+    assert one("R2 in out 1MEG").value == 1e6
+    e = one("C1 2 0 100n")
+    assert (e.kind, e.nodes, e.value) == ("C", ("2", "0"), 100 * 1e-9)
+    e = one("L1 3 4 4.7u")
+    assert (e.kind, e.value) == ("L", 4.7e-6)
+    assert one("L1 1 2 1e-6").value == 1e-6
+    e = one("r1 1 0 10k")
+    assert e.name == "r1" and e.kind == "R"
+    e = one("R1 Vdd GND 4.7K")
+    assert e.nodes == ("Vdd", "GND") and e.value == 4.7e3
+    assert one("R1\t1\t0\t10k").value == 10e3
+    assert one("  R1  1  0  10k  ").value == 10e3
+    assert one("R1 n_1 0 10kOhm").value == 10e3       # trailing text after a value is ignored
+
+
+@pytest.mark.parametrize("line,dc,ac,tran", [
+    ("V1 1 0 DC 5", 5.0, None, None),
+    ("V1 1 0 12", 12.0, None, None),
+    ("V1 1 0 AC 1 0", None, (1.0, 0.0), None),
+    ("V1 1 0 DC 5 AC 1 90", 5.0, (1.0, 90.0), None),
+    ("V1 1 0 AC 1", None, (1.0, 0.0), None),
+    ("V1 1 0", None, None, None),
+    ("V1 1 0 DC -12", -12.0, None, None),
+    ("V1 1 0 DC", None, None, None),
+    ("V1 1 0 DC AC 2", None, (2.0, 0.0), None),
+    ("V1 1 0 foo", None, None, None),
+    ("V1 1 0 PULSE(0 5 0 1n 1n 10n 20n)", None, None,
+     ir.Pulse(0.0, 5.0, 0.0, 1e-9, 1e-9, 10 * 1e-9, 20 * 1e-9)),
+    ("V1 1 0 DC 5 PULSE(0 5 0 1n 1n 10n 20n)", 5.0, None,
+     ir.Pulse(0.0, 5.0, 0.0, 1e-9, 1e-9, 10 * 1e-9, 20 * 1e-9)),
+    ("V1 1 0 PULSE(0 5)", None, None, ir.Pulse(0.0, 5.0, 0.0, 0.0, 0.0, MAXF, MAXF)),
+    ("V1 1 0 SIN(0 1 1MEG)", None, None, ir.Sin(0.0, 1.0, 1e6, 0.0, 0.0)),
+    ("V1 1 0 sine(0.5 1 10k 1u 100)", None, None, ir.Sin(0.5, 1.0, 10e3, 1e-6, 100.0)),
+    ("V1 1 0 PWL(0 0 1u 5 2u 5 3u 0)", None, None,
+     ir.Pwl([(0.0, 0.0), (1e-6, 5.0), (2 * 1e-6, 5.0), (3 * 1e-6, 0.0)])),
+    ("V1 1 0 EXP(0 5 0 1u 5u 1u)", None, None, ir.Exp(0.0, 5.0, 0.0, 1e-6, 5 * 1e-6, 1e-6)),
+    ("V1 1 0 EXP(1 2)", None, None, ir.Exp(1.0, 2.0, 0.0, MAXF, MAXF, MAXF)),
+    ("V1 1 0 AC 1 PWL(0 1 1 2)", None, (1.0, 0.0), ir.Pwl([(0.0, 1.0), (1.0, 2.0)])),
+])
+def test_source_specs(line, dc, ac, tran):
+    """test_parse_vsource_* (src/parser.rs:993-1088, 1776-1905)."""
+    e = one(line)
+    assert (e.kind, e.name, e.nodes) == ("V", "V1", ("1", "0"))
+    assert e.dc == dc and e.ac == ac and e.tran == tran
+
+
+def test_current_sources():
+    e = one("I1 2 0 DC 1m")
+    assert e.kind == "I" and e.dc == 1e-3
+    e = one("I1 1 0 PULSE(0 1m 0 1n 1n 50n 100n)")
+    assert e.tran.v2 == 1e-3 and e.dc is None
+
+
+def test_devices():
+    e = one("D1 1 2 MYDIODE")
+    assert (e.kind, e.nodes, e.model) == ("D", ("1", "2"), "MYDIODE")
+    e = one("d2 anode cathode mymodel")
+    assert (e.name, e.nodes, e.model) == ("d2", ("anode", "cathode"), "mymodel")
+    e = one("Q1 3 2 1 MYBJT")
+    assert (e.kind, e.nodes, e.model) == ("Q", ("3", "2", "1"), "MYBJT")
+    assert one("q1 c b e mybjt").nodes == ("c", "b", "e")
+    e = one("M1 3 2 1 MYMOS")
+    assert (e.kind, e.nodes, e.model) == ("M", ("3", "2", "1"), "MYMOS")
+    e = one("M1 3 2 1 0 MYMOS")   # bulk node ignored
+    assert (e.nodes, e.model) == (("3", "2", "1"), "MYMOS")
+
+
+def test_models():
+    """test_parse_model_* (src/parser.rs:1413-1700)."""
+    m = nl.parse(".MODEL MYDIODE D(IS=1e-14 N=1.0)").models[0]
+    assert (m.name, m.is_, m.n) == ("MYDIODE", 1e-14, 1.0)
+    for text in (".MODEL D1N4148 D()", ".MODEL SIMPLE D", ".MODEL X D IS=5"):
+        m = nl.parse(text).models[0]
+        assert (m.is_, m.n) == (1e-14, 1.0)
+    m = nl.parse(".model mydiode d(is=2.52e-9 n=1.752)").models[0]
+    assert (m.name, m.is_, m.n) == ("mydiode", 2.52e-9, 1.752)
+    m = nl.parse(".MODEL DTEST D(N=1.5 IS=1e-12)").models[0]
+    assert (m.is_, m.n) == (1e-12, 1.5)
+    assert nl.parse(".MODEL DTEST D (IS=5e-10)").models[0].is_ == 5e-10
+    c = nl.parse(".MODEL 2N2222 NPN(IS=1e-14 BF=200 BR=2 NF=1.0 NR=1.0)")
+    m = c.bjt_models[0]
+    assert (m.name, m.is_npn, m.is_, m.bf, m.br, m.nf, m.nr) == ("2N2222", True, 1e-14, 200, 2, 1, 1)
+    m = nl.parse(".MODEL MPNP PNP(IS=1e-14 BF=100)").bjt_models[0]
+    assert (m.is_npn, m.bf, m.br, m.nf, m.nr) == (False, 100, 1, 1, 1)
+    m = nl.parse(".MODEL Q1 NPN()").bjt_models[0]
+    assert (m.is_, m.bf, m.br) == (1e-16, 100.0, 1.0)
+    m = nl.parse(".MODEL NMOS1 NMOS(VTO=0.7 KP=1.1e-4 LAMBDA=0.04)").mosfet_models[0]
+    assert (m.is_nmos, m.vto, m.kp, m.lambda_) == (True, 0.7, 1.1e-4, 0.04)
+    m = nl.parse(".MODEL PMOS1 PMOS(VTO=0.7 KP=4.5e-5 LAMBDA=0.05)").mosfet_models[0]
+    assert (m.is_nmos, m.kp) == (False, 4.5e-5)
+    m = nl.parse(".MODEL M1 NMOS()").mosfet_models[0]
+    assert (m.vto, m.kp, m.lambda_) == (1.0, 2e-5, 0.0)
+    c = nl.parse(".MODEL D1N4148 D(IS=2.52e-9 N=1.752)")
+    assert len(c.models) == 1 and not c.bjt_models and not c.mosfet_models
+
+
+# ------------------------------------------------------------- analyses --
+def test_analyses():
+    assert isinstance(nl.parse(".DC").analyses[0], nl.Dc)
+    assert isinstance(nl.parse(".OP").analyses[0], nl.Dc)
+    assert isinstance(nl.parse(".dc").analyses[0], nl.Dc)
+    assert nl.parse(".DC V1 0 5 0.1").analyses == []      # a swept .DC is not the operating point
+    assert nl.parse(".AC DEC 10 1 100k").analyses[0] == nl.Ac("dec", 10, 1.0, 100e3)
+    assert nl.parse(".AC LIN 100 1k 10k").analyses[0] == nl.Ac("lin", 100, 1e3, 10e3)
+    assert nl.parse(".AC OCT 20 100 10k").analyses[0] == nl.Ac("oct", 20, 100.0, 10e3)
+    assert nl.parse(".ac dec 10 1 100k").analyses[0].sweep == "dec"
+    assert nl.parse(".TRAN 1n 100n").analyses[0] == nl.Tran(1e-9, 100 * 1e-9, 0.0, False)
+    assert nl.parse(".TRAN 1n 100n 10n").analyses[0] == nl.Tran(1e-9, 100 * 1e-9, 10 * 1e-9, False)
+    assert nl.parse(".TRAN 1n 100n UIC").analyses[0].uic
+    t = nl.parse(".TRAN 1n 100n 10n UIC").analyses[0]
+    assert t.uic and t.tstart == 10 * 1e-9
+    t = nl.parse(".tran 1u 1m").analyses[0]
+    assert (t.tstep, t.tstop) == (1e-6, 1e-3)
+    assert nl.parse(".TRAN 1n 100n 0 10p UIC").analyses[0].uic is False  # tmax then UIC: ignored
+    assert nl.parse(".TITLE x\n.PARAM a=1\n.PRINT OP V(1)").analyses == []
+    assert len(nl.parse(".DC\n.AC DEC 10 1 1k").analyses) == 2
+
+
+def test_netlist_structure():
+    text = "* title\n\nV1 1 0 DC 10\n* note\nR1 1 2 1k\nR2 2 0 1k\n.DC\n.END\nR9 9 0 1\n"
+    c = nl.parse(text)
+    assert [e.name for e in c.components] == ["V1", "R1", "R2"] and len(c.analyses) == 1
+    assert len(nl.parse("R1 1 0 1\nR2 2 0 1").components) == 2        # no .END
+    assert len(nl.parse("R1 1 0 1\n.end\nR2 2 0 1").components) == 1
+    assert len(nl.parse("R1 1 0 1\r\nR2 1 0 2\r\n").components) == 2  # CRLF
+    c = nl.parse("")
+    assert not c.components and not c.analyses
+    assert not nl.parse("* just a comment\n* another one").components
+
+
+@pytest.mark.parametrize("text,message", [
+    ("X1 1 0 something", "Parse error: line 1: unknown element 'X': X1 1 0 something"),
+    ("R1 1 0 1k\n  x2 1 0", "Parse error: line 2: unknown element 'x':   x2 1 0"),
+    ("R1 1", "Parse error: line 1: failed to parse R component in: R1 1"),
+    ("C1 1 0 big", "Parse error: line 1: failed to parse C component in: C1 1 0 big"),
+    ("R1 n-1 0 1k", "Parse error: line 1: failed to parse R component in: R1 n-1 0 1k"),
+    (".AC DEC", "Parse error: line 1: expected integer in: .AC DEC"),
+    (".AC LOG 10 1 2", "Parse error: line 1: expected DEC, OCT, or LIN for AC sweep type in: "
+                       ".AC LOG 10 1 2"),
+    (".AC DEC 10 x 2", "Parse error: line 1: failed to parse AC f_start in: .AC DEC 10 x 2"),
+    (".TRAN x", "Parse error: line 1: failed to parse TRAN tstep in: .TRAN x"),
+    (".TRAN 1n", "Parse error: line 1: failed to parse TRAN tstop in: .TRAN 1n"),
+    ("V1 1 0 DC x", "Parse error: line 1: failed to parse DC value in: V1 1 0 DC x"),
+    ("V1 1 0 AC x", "Parse error: line 1: failed to parse AC magnitude in: V1 1 0 AC x"),
+    ("V1 1 0 AC 1 DC 5", "Parse error: line 1: failed to parse AC phase in: V1 1 0 AC 1 DC 5"),
+    ("V1 1", "Parse error: line 1: failed to parse source name/nodes in: V1 1"),
+    ("V1 1 0 PULSE 0 5", "Parse error: line 1: expected '(' after PULSE in: V1 1 0 PULSE 0 5"),
+    ("V1 1 0 SIN(0 1", "Parse error: line 1: expected ')' after SIN in: V1 1 0 SIN(0 1"),
+    ("V1 1 0 PULSE(0)", "Parse error: line 1: PULSE requires at least v1 and v2 in: "
+                        "V1 1 0 PULSE(0)"),
+    ("V1 1 0 SIN(0 1)", "Parse error: line 1: SIN requires at least vo, va, and freq in: "
+                        "V1 1 0 SIN(0 1)"),
+    ("V1 1 0 PWL(0 1 2)", "Parse error: line 1: PWL requires an even number of time-value "
+                          "pairs in: V1 1 0 PWL(0 1 2)"),
+    ("V1 1 0 EXP(1)", "Parse error: line 1: EXP requires at least v1 and v2 in: V1 1 0 EXP(1)"),
+    ("V1 1 0 PULSE(0 5 x)", "Parse error: line 1: failed to parse value in 'x' in: "
+                            "V1 1 0 PULSE(0 5 x)"),
+    ("D1 1 2", "Parse error: line 1: failed to parse diode element in: D1 1 2"),
+    ("Q1 1 2 3", "Parse error: line 1: failed to parse BJT element in: Q1 1 2 3"),
+    ("M1 1 2 3", "Parse error: line 1: expected model name (and optional bulk node) after "
+                 "MOSFET terminals in: M1 1 2 3"),
+    ("M1 1 2 3 4 5 MOD", "Parse error: line 1: expected model name (and optional bulk node) "
+                         "after MOSFET terminals in: M1 1 2 3 4 5 MOD"),
+    (".MODEL", "Parse error: line 1: expected model name after .MODEL in: .MODEL"),
+    (".MODEL X", "Parse error: line 1: expected model type after model name in: .MODEL X"),
+    (".MODEL X D(IS=1", "Parse error: line 1: missing closing ')' in .MODEL parameters in: "
+                        ".MODEL X D(IS=1"),
+    (".MODEL X R(A=1)", "Parse error: line 1: unsupported model type 'R', expected D, NPN, PNP, "
+                        "NMOS, or PMOS in: .MODEL X R(A=1)"),
+    (".MODEL X D(BV=5)", "Parse error: line 1: unknown diode model parameter: BV=5 in: "
+                         ".MODEL X D(BV=5)"),
+    (".MODEL X D(IS=abc)", "Parse error: line 1: failed to parse IS value: abc in: "
+                           ".MODEL X D(IS=abc)"),
+    (".MODEL X NPN(VAF=5)", "Parse error: line 1: unknown BJT model parameter: VAF=5 in: "
+                            ".MODEL X NPN(VAF=5)"),
+    (".MODEL X NMOS(LAMBDA=q)", "Parse error: line 1: failed to parse LAMBDA value: q in: "
+                                ".MODEL X NMOS(LAMBDA=q)"),
+    (".MODEL X PMOS(W=1u)", "Parse error: line 1: unknown MOSFET model parameter: W=1u in: "
+                            ".MODEL X PMOS(W=1u)"),
+    ("+ continuation", "Parse error: line 1: unknown element '+': + continuation"),
+])
+def test_error_messages(text, message):
+    with pytest.raises(OhmnivoreError) as ei:
+        nl.parse(text)
+    assert str(ei.value) == message and ei.value.kind == "Parse"
+
+
+# ------------------------------------------------------ text -> compile --
+VOLTAGE_DIVIDER = "V1 1 0 DC 10\nR1 1 2 1k\nR2 2 0 1k\n.DC\n.END\n"  # tests/fixtures
+
+
+def compile_text(text):
+    return S.compile(nl.parse(text).to_circuit())
+
+
+def test_divider_compiles_to_reference_system():
+    """tests/fixtures/voltage_divider.spice: nodes 1, 2 then V1's branch."""
+    m = compile_text(VOLTAGE_DIVIDER)
+    assert m.node_names == ["1", "2"] and m.branch_names == ["V1"] and m.size == 3
+    x = oracle.dense_solve(m.g, m.b_dc)
+    # the reference test's epsilons: the compiler's GMIN shunts move I(V1) by 1e-11
+    assert abs(x[0] - 10.0) < 1e-6 and abs(x[1] - 5.0) < 1e-6 and abs(x[2] + 0.005) < 1e-9
+
+
+def test_diode_descriptor_from_text():
+    """test_diode_parse_compile_produces_descriptors (tests/diode_integration_test.rs:9-33)."""
+    m = compile_text("V1 1 0 DC 5\nR1 1 2 1k\n.MODEL D1N4148 D(IS=2.52e-9 N=1.752)\n"
+                     "D1 2 0 D1N4148\n.DC\n.END\n")
+    d = m.diode_descriptors[0]
+    assert len(m.diode_descriptors) == 1
+    assert d["anode_idx"] == 1 and d["cathode_idx"] == ir.SENTINEL
+    assert d["is_val"] == np.float32(2.52e-9)
+    assert abs(d["n_vt"] - np.float32(1.752 * 0.02585)) < 1e-5
+
+
+def test_model_may_follow_its_device():
+    a = compile_text("V1 1 0 DC 5\nR1 1 2 1k\nD1 2 0 LATE\n.MODEL LATE D(IS=1e-12 N=2)\n")
+    b = compile_text("V1 1 0 DC 5\nR1 1 2 1k\n.MODEL LATE D(IS=1e-12 N=2)\nD1 2 0 LATE\n")
+    assert a.diode_descriptors.tobytes() == b.diode_descriptors.tobytes()
+    assert a.diode_descriptors[0]["is_val"] == np.float32(1e-12)
+
+
+@pytest.mark.parametrize("kind,line", [("Diode", "D1 2 0 NOPE"), ("BJT", "Q1 2 1 0 NOPE"),
+                                       ("MOSFET", "M1 2 1 0 NOPE")])
+def test_undefined_model(kind, line):
+    """test_*_undefined_model_error (src/compiler.rs:1449-1900)."""
+    with pytest.raises(OhmnivoreError) as ei:
+        compile_text(f"V1 1 0 DC 5\nR1 1 2 1k\n{line}\n")
+    assert str(ei.value) == f"Compile error: {kind} {line.split()[0]} references undefined " \
+                            f"model 'NOPE'"
+
+
+def test_transistor_descriptors_from_text():
+    """test_bjt/mosfet_parse_compile_produces_descriptors
+    (tests/transistor_integration_test.rs:129-201)."""
+    m = compile_text("V1 vcc 0 DC 5\nV2 vb 0 DC 1.0\nR1 vcc vc 1k\nR2 vb base 100k\n"
+                     ".MODEL Q2N2222 NPN(IS=1e-14 BF=200 BR=2 NF=1.0 NR=1.0)\n"
+                     "Q1 vc base 0 Q2N2222\n.DC\n.END\n")
+    d = m.bjt_descriptors[0]
+    assert d["polarity"] == 1.0 and d["is_val"] == np.float32(1e-14)
+    assert (d["bf"], d["br"]) == (200.0, 2.0) and d["emitter_idx"] == ir.SENTINEL
+    assert abs(d["nf_vt"] - np.float32(0.02585)) < 1e-5
+    m = compile_text("V1 vdd 0 DC 5\nV2 gate 0 DC 2.0\nR1 vdd drain 1k\n"
+                     ".MODEL NMOD NMOS(VTO=0.7 KP=1.1e-4 LAMBDA=0.04)\nM1 drain gate 0 0 NMOD\n")
+    d = m.mosfet_descriptors[0]
+    assert d["polarity"] == 1.0 and d["source_idx"] == ir.SENTINEL
+    assert d["vto"] == np.float32(0.7) and d["kp"] == np.float32(1.1e-4)
+    assert d["lambda"] == np.float32(0.04)
+
+
+def test_source_table_waveforms_from_text():
+    """SIN / EXP / PWL sources reach the transient source table with the
+    right values: the oracle's evaluation at points the waveform fixes."""
+    text = ("V1 1 0 SIN(0.5 2 1k 1m 0)\nV2 2 0 EXP(1 3 1u 2u 5u 4u)\n"
+            "I1 0 3 PWL(0 0 1m 2 3m 2 4m -1)\nR1 1 2 1k\nR2 2 3 1k\nR3 3 0 1k\n")
+    c = nl.parse(text).to_circuit()
+    m = S.compile(c)
+    src, pwl = T.source_table(c, m)
+    assert [int(k) for k in src["kind"]] == [ir.SRC_SIN, ir.SRC_EXP, ir.SRC_PWL]
+    val = lambda i, t: oracle.source_value(src[i], t, pwl)  # noqa: E731
+    # SIN: vo before td; vo + va a quarter period after td; vo half a period after
+    assert val(0, 0.5e-3) == 0.5
+    assert abs(val(0, 1e-3 + 0.25e-3) - 2.5) < 1e-12
+    assert abs(val(0, 1e-3 + 0.5e-3) - 0.5) < 1e-12
+    # EXP: v1 before td1; one tau1 into the rise; long after td2 it returns to v1
+    assert val(1, 0.5e-6) == 1.0
+    assert abs(val(1, 1e-6 + 2e-6) - (1 + 2 * (1 - math.exp(-1)))) < 1e-12
+    assert abs(val(1, 1.0) - 1.0) < 1e-9
+    # PWL: breakpoints, a midpoint, clamping at both ends
+    for t, v in ((0.0, 0.0), (1e-3, 2.0), (2e-3, 2.0), (3.5e-3, 0.5), (4e-3, -1.0), (9.0, -1.0),
+                 (-1.0, 0.0)):
+        assert abs(val(2, t) - v) < 1e-12
+    assert pwl.shape == (4, 2) and src[2]["pwl_count"] == 4
+
+
+# ---------------------------------------------------------------- CSV --
+@pytest.mark.parametrize("x,text", [
+    (5.0, "5"), (-0.005, "-0.005"), (0.0, "0"), (-0.0, "-0"), (1e-9, "0.000000001"),
+    (1e20, "100000000000000000000"), (0.1 + 0.2, "0.30000000000000004"), (2 / 3, "0.6666666666666666"),
+    (1.2345678901234568e17, "123456789012345680"), (float("nan"), "NaN"), (float("inf"), "inf"),
+    (float("-inf"), "-inf"), (10.0 * 1e-6, "0.000009999999999999999"),
+])
+def test_f64_display(x, text):
+    """f64 Display: shortest round-trip digits, positional."""
+    assert output.fmt(x) == text
+    if math.isfinite(x):
+        assert float(text) == x
+
+
+class _R:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def test_dc_csv():
+    r = _R(node_voltages=[("1", 10.0), ("2", 5.0)], branch_currents=[("V1", -0.005)])
+    f = io.StringIO()
+    output.write_dc_csv(r, f)
+    assert f.getvalue() == "Variable,Value\nV(1),10\nV(2),5\nI(V1),-0.005\n"
+
+
+def test_ac_csv():
+    """write_ac_csv: header, one row per frequency, magnitude and degrees."""
+    z = np.array([1 + 0j, 1j, -1 - 1j])
+    r = _R(frequencies=np.array([1.0, 10.0, 100.0]), node_voltages=[("2", z)],
+           branch_currents=[("V1", np.array([-1 + 0j, -1j, 1 + 1j]))])
+    f = io.StringIO()
+    output.write_ac_csv(r, f)
+    lines = f.getvalue().splitlines()
+    assert lines[0] == "Frequency,V(2)_mag,V(2)_phase_deg,I(V1)_mag,I(V1)_phase_deg"
+    assert lines[1] == "1,1,0,1,180"            # atan2(+0, -1) = +pi
+    assert lines[2] == "10,1,90,1,-90"
+    row = [float(v) for v in lines[3].split(",")]
+    assert row[0] == 100 and row[1] == math.sqrt(2) and row[2] == -135.0 and row[4] == 45.0
+
+
+def test_tran_csv():
+    r = _R(times=np.array([0.0, 1e-5]), node_voltages=[("1", np.array([0.0, 5.0])),
+                                                       ("2", np.array([0.0, 0.05]))],
+           branch_currents=[("V1", np.array([0.0, -0.00495]))])
+    f = io.StringIO()
+    output.write_tran_csv(r, f)
+    assert f.getvalue() == "time,V(1),V(2),I(V1)\n0,0,0,0\n0.00001,5,0.05,-0.00495\n"
+
+
+def test_frequency_grids():
+    """test_ac_linear_sweep / test_ac_octave_sweep (tests/integration_test.rs:182-219)."""
+    f = S.generate_frequencies("lin", 100, 1.0, 1000.0)
+    assert f.shape[0] == 100 and f[0] == 1.0 and abs(f[-1] - 1000.0) < 1e-9
+    f = S.generate_frequencies("oct", 5, 100.0, 800.0)
+    assert f.shape[0] == math.ceil(5 * math.log2(8.0)) and f[0] == 100.0
+    f = S.generate_frequencies("dec", 10, 1.0, 1e6)
+    assert f.shape[0] == 60 and f[0] == 1.0      # ceil(10 * 6) points: f_stop itself is not one
+
+
+def test_cli_refuses_cpu(tmp_path, capsys):
+    from ohmnivore_amd import cli
+    p = tmp_path / "d.spice"
+    p.write_text(VOLTAGE_DIVIDER)
+    assert cli.main([str(p), "--cpu"]) == 2
+    assert "no CPU solver" in capsys.readouterr().err
+    assert cli.main([str(tmp_path / "missing.spice")]) == 1
+    assert capsys.readouterr().err.startswith("Error reading ")
