@@ -9,6 +9,7 @@ Compilation is host code, so parse -> compile runs here too.
 """
 import io
 import math
+import os
 
 import numpy as np
 import pytest
@@ -403,3 +404,24 @@ def test_cli_refuses_cpu(tmp_path, capsys):
     assert "no CPU solver" in capsys.readouterr().err
     assert cli.main([str(tmp_path / "missing.spice")]) == 1
     assert capsys.readouterr().err.startswith("Error reading ")
+
+
+# ------------------------------------------- reference netlist fixtures --
+NETLISTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "netlists")
+FIXTURES = sorted(f for f in os.listdir(NETLISTS) if f.endswith(".spice"))
+ANALYSES = {"cmos_inverter_chain_tran.spice": nl.Tran, "rc_lowpass.spice": nl.Ac,
+            "example_rc_lowpass.spice": nl.Ac, "tran_pulse_rc_filter.spice": nl.Tran,
+            "tran_rc_charging.spice": nl.Tran, "tran_rl_step.spice": nl.Tran,
+            "example_pulse_filter.spice": nl.Tran, "example_rc_charging.spice": nl.Tran}
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_netlists_compile_bit_identically(name):
+    """The reference's regression circuits, examples and test fixture
+    (tests/regression/circuits, examples/, tests/fixtures): parser output
+    compiled by the product compiler and by the oracle's, bit for bit."""
+    from test_capi_cpu import assert_same_mna
+    c = nl.parse_file(os.path.join(NETLISTS, name))
+    assert len(c.analyses) == 1 and isinstance(c.analyses[0], ANALYSES.get(name, nl.Dc))
+    ckt = c.to_circuit()
+    assert_same_mna(S.compile(ckt), oracle.compile(ckt))
