@@ -397,6 +397,55 @@ ohm_status ohm_newton_get_x(ohm_newton_ctx *ctx, double *x_out);
 ohm_status ohm_newton_get_system(ohm_newton_ctx *ctx, double *values_out, double *b_out);
 void ohm_newton_free(ohm_newton_ctx *ctx);
 
+/* ---- dc::run (src/analysis/dc.rs:25-52): the DC operating point of a
+ * compiled system. Linear systems: one ohm_solve_real (AUTO preconditioner,
+ * relative tolerance 1e-12). Nonlinear systems: the continuation cascade of
+ * run_nonlinear (:60-293) on one device-resident Newton context whose
+ * values and right-hand side are replaced per step (ohm_newton_update):
+ *   direct Newton (params->max_iterations, 200 BiCGSTAB iterations);
+ *   source stepping when MOSFETs are present (:480-640; sources scaled by
+ *     alpha from 0.1, the step doubled on success and halved on failure);
+ *   adaptive GMIN stepping (:152-260; extra node-diagonal conductance from
+ *     0.1 down by 1000x, subdivided by 10x on failure) and a final true-GMIN
+ *     solve; pseudo-transient continuation (:340-470; alpha I on node rows
+ *     anchored at b + alpha x_prev) when that final solve fails.
+ * Where the reference retries a failed step on the CPU with a sparse LU,
+ * the step is retried on the GPU in FP64 with a BiCGSTAB budget of
+ * max(10 x budget, 20000) (and 15 Newton iterations where the reference's
+ * retry uses 15). params: the caller's NewtonParams (abs_tol, sub-solve
+ * tolerance and forcing, precision, preconditioner apply to every step; the
+ * cascade sets the iteration caps and initial guesses as the reference does);
+ * NULL = defaults. x_out: mna->size doubles. stats and trace may be NULL;
+ * trace receives the first trace_cap Newton runs, stats->runs counts all. */
+enum {
+  OHM_DC_LINEAR = 0,
+  OHM_DC_DIRECT = 1,
+  OHM_DC_SOURCE_STEPPING = 2,
+  OHM_DC_GMIN_STEPPING = 3,
+  OHM_DC_GMIN_FINAL = 4,
+  OHM_DC_PSEUDO_TRANSIENT = 5,
+  OHM_DC_PSEUDO_TRANSIENT_FINAL = 6
+};
+typedef struct {
+  int32_t stage;       /* OHM_DC_* */
+  int32_t converged;   /* 1 if this run converged */
+  double level;        /* source scale, extra GMIN or alpha of the step (0 otherwise) */
+  int64_t iterations;  /* Newton iterations of the run */
+  int32_t retry;       /* 1 for the FP64 larger-budget retry of a failed run */
+  int32_t _pad;
+} ohm_dc_step;
+typedef struct {
+  int32_t stage;               /* OHM_DC_* of the run that produced x */
+  int32_t _pad;
+  int64_t runs;                /* Newton runs (continuation steps and retries) */
+  int64_t newton_iterations;   /* summed over every run (Stats::newton_iterations) */
+  int64_t bicgstab_iterations; /* summed over every run */
+  double device_ms;
+} ohm_dc_stats;
+ohm_status ohm_dc_run(ohm_backend *b, const ohm_mna *mna, const ohm_newton_params *params,
+                      double *x_out, ohm_dc_stats *stats, ohm_dc_step *trace,
+                      int64_t trace_cap);
+
 /* ---- NonlinearBackend kernels (src/solver/nonlinear.rs:31-150) on a
  * Newton context's device buffers: evaluate + stamp into A, b at x. */
 ohm_status ohm_newton_assemble_at(ohm_newton_ctx *ctx, const double *x, double *values_out,

@@ -73,6 +73,22 @@ class NewtonResult(ctypes.Structure):
                 ("fp64_fallbacks", ctypes.c_int64)]
 
 
+class DcStep(ctypes.Structure):
+    _fields_ = [("stage", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("level", ctypes.c_double), ("iterations", ctypes.c_int64),
+                ("retry", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+
+
+class DcStats(ctypes.Structure):
+    _fields_ = [("stage", ctypes.c_int32), ("_pad", ctypes.c_int32), ("runs", ctypes.c_int64),
+                ("newton_iterations", ctypes.c_int64), ("bicgstab_iterations", ctypes.c_int64),
+                ("device_ms", ctypes.c_double)]
+
+
+DC_STAGES = {0: "linear", 1: "direct", 2: "source", 3: "gmin", 4: "final", 5: "ptc",
+             6: "ptc-final"}
+
+
 class BatchSystem(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("rowptr", c_u32p), ("col", c_u32p),
                 ("g_val", c_f64p), ("c_val", c_f64p), ("sigma_re", ctypes.c_double),

@@ -111,7 +111,7 @@ def simulate(backend: S.HipBackend, text: str, out: TextIO | None = None,
             r = dc.run(backend, mna, **newton)
             if stats:
                 stats.add_phase("DC analysis", time.perf_counter() - t)
-                stats.newton_iterations += sum(p[3] for p in r.path)
+                stats.newton_iterations += r.newton_iterations
             if out is not None:
                 output.write_dc_csv(r, out)
         elif isinstance(a, netlist.Ac):

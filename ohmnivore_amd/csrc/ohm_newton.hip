@@ -778,8 +778,12 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
   // the assembled matrix has a zero diagonal, else Jacobi. Device stamps only
   // add to node diagonals, so the base matrix decides.
   int kind = prm.precond;
-  // the assembled values start from base
+  // the assembled values and right-hand side start from base: the stamp
+  // gathers rewrite only the positions devices touch, so every other entry
+  // (voltage-source rows of b among them) must come from the current base,
+  // which ohm_newton_update may have replaced since the last run
   OHM_HIP(hipMemcpyAsync(c->A.val, c->base_val, sizeof(double) * c->nnz, hipMemcpyDeviceToDevice, st));
+  OHM_HIP(hipMemcpyAsync(c->rhs, c->base_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
   if (kind == OHM_PREC_AUTO) {
     if (c->zero_diag < 0) c->zero_diag = matrix_has_zero_diag(b, c->A, nullptr) ? 1 : 0;
     kind = c->zero_diag ? OHM_PREC_ISAI : OHM_PREC_JACOBI;

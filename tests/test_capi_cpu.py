@@ -139,3 +139,19 @@ def test_generate_frequencies_reference_cases():
     assert np.allclose(f, 10.0 ** (np.arange(60) / 10.0), rtol=1e-15)
     assert solver.generate_frequencies("lin", 1, 5.0, 9.0).tolist() == [5.0]
     assert solver.generate_frequencies("dec", 10, 10.0, 1.0).shape == (0,)  # ceil < 0 -> 0 points
+
+
+def test_dc_run_rejects_bad_arguments():
+    """ohm_dc_run validates its arguments before touching a device."""
+    import ctypes
+    from ohmnivore_amd import dc
+    mna = solver.compile(ir.resistor_ladder(4, seed=1))
+    view = dc._MnaView(mna)
+    x = np.zeros(mna.size)
+    L = N.lib()
+    assert L.ohm_dc_run(None, ctypes.byref(view.struct), None,
+                        x.ctypes.data_as(N.c_f64p), None, None, ctypes.c_int64(0)) == 11
+    assert L.ohm_dc_run(ctypes.c_void_p(1), ctypes.byref(view.struct), None, None, None, None,
+                        ctypes.c_int64(0)) == 11
+    assert L.ohm_dc_run(ctypes.c_void_p(1), ctypes.byref(view.struct), None,
+                        x.ctypes.data_as(N.c_f64p), None, None, ctypes.c_int64(4)) == 11

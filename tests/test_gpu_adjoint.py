@@ -170,3 +170,22 @@ def test_newton_update_matches_fresh_context(be):
         prob.ctx.update(mosfets=bad)
     with pytest.raises(ValueError):
         prob.ctx.update(values=np.zeros(3))
+
+
+def test_newton_update_of_the_rhs_reaches_source_rows(be):
+    """A new base right-hand side replaces every entry, the voltage-source
+    rows that no device stamps included: a context re-run with b scaled by
+    0.3 lands where a fresh context built with that b lands."""
+    mna = solver.compile(ir.amplifier_array(32, seed=11))
+    kw = dict(abs_tol=1e-13, bicgstab_tolerance=1e-14)
+    descs = (mna.diode_descriptors, mna.bjt_descriptors, mna.mosfet_descriptors)
+    ctx = solver.NewtonContext(be, mna.g, mna.b_dc, *descs)
+    ctx.run(**kw)
+    ctx.update(base_b=0.3 * mna.b_dc)
+    ctx.run(**kw)
+    fresh = solver.NewtonContext(be, mna.g, 0.3 * mna.b_dc, *descs)
+    fresh.run(**kw)
+    x_upd, x_new = ctx.x(), fresh.x()
+    assert np.allclose(x_upd, x_new, rtol=1e-9, atol=1e-12), np.abs(x_upd - x_new).max()
+    vdd = mna.node_names.index("vdd")
+    assert abs(x_upd[vdd] - 0.3 * 1.8) < 1e-12

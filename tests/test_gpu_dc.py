@@ -67,3 +67,60 @@ def test_source_stepping_path(be):
     assert labels[0] == "direct" and "source" in labels
     ok, err = close(x, oracle_op(mna))
     assert ok, err
+
+
+def cmos_chain(stages):
+    """The reference's cmos_inverter_chain_dc netlist family: one PMOS/NMOS
+    pair per stage, VIN = VDD = 5."""
+    ckt = ir.Circuit()
+    ckt.add_mosfet_model("NMOD", 0.7, 1.1e-4, 0.04, True)
+    ckt.add_mosfet_model("PMOD", -0.7, 5.5e-5, 0.04, False)
+    ckt.vsource("VDD", "vdd", "0", dc=5.0)
+    ckt.vsource("VIN", "in", "0", dc=5.0)
+    prev = "in"
+    for k in range(1, stages + 1):
+        ckt.mosfet(f"MP{k}", f"out_{k}", prev, "vdd", "PMOD")
+        ckt.mosfet(f"MN{k}", f"out_{k}", prev, "0", "NMOD")
+        prev = f"out_{k}"
+    return ckt
+
+
+def oracle_source_stepping(mna):
+    """The oracle's LU Newton along the cascade's source-stepping schedule
+    (steps converged to 1e-10, the last one polished to 1e-12)."""
+    def newton(scale, x0, tol, iters):
+        x, _, _ = oracle.newton_solve(mna.g, scale * mna.b_dc, mna.diode_descriptors,
+                                      mna.bjt_descriptors, mna.mosfet_descriptors,
+                                      abs_tol=tol, max_iterations=iters, initial_guess=x0)
+        return x
+    x, last, target = None, 0.0, 0.1
+    while True:
+        try:
+            x = newton(target, x, 1e-10, 30)
+        except oracle.OracleError:
+            target = last + (target - last) / 2.0
+            assert target - last > 1e-6
+            continue
+        if target == 1.0:
+            try:
+                return newton(1.0, x, 1e-12, 50)
+            except oracle.OracleError:
+                return x
+        target, last = min(target + 2.0 * (target - last), 1.0), target
+
+
+@pytest.mark.parametrize("stages", [10, 200])
+def test_cmos_chain_cascade(be, stages):
+    """A CMOS inverter chain needs the cascade (direct Newton from zero does
+    not converge); the native cascade's stages, counters and answer."""
+    mna = solver.compile(cmos_chain(stages))
+    res = dc.run(be, mna, **TIGHT)
+    stages_run = [p[0] for p in res.path]
+    assert stages_run[0] == "direct" and not res.path[0][2]
+    assert res.stage in ("source", "final", "ptc-final") and res.path[-1][2]
+    assert res.newton_iterations == sum(p[3] for p in res.path)
+    ok, err = close(res.x, oracle_source_stepping(mna))
+    assert ok, err
+    v = dict(res.node_voltages)
+    for k in range(1, stages + 1):
+        assert (v[f"out_{k}"] < 0.5) if k % 2 else (v[f"out_{k}"] > 4.5)
