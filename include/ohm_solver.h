@@ -446,6 +446,34 @@ ohm_status ohm_dc_run(ohm_backend *b, const ohm_mna *mna, const ohm_newton_param
                       double *x_out, ohm_dc_stats *stats, ohm_dc_step *trace,
                       int64_t trace_cap);
 
+/* ---- Nonlinear transient (no counterpart in the reference, whose transient
+ * steps G and C only, src/analysis/transient.rs:84-178, leaving diode, BJT
+ * and MOSFET currents out of the time loop). Every timestep is a Newton
+ * solve on the device (ohm_newton_* on one context):
+ *   BE:   (G + C/h) x + i(x) = b(t+h) + (C/h) x_n
+ *   TRAP: (G + 2C/h) x + i(x) = b(t+h) + (2C/h) x_n + d_n,
+ *         d_n = b(t) - G x_n - i(x_n)  (= C dx/dt at t_n)
+ * which for a circuit without devices is the reference's companion form
+ * (:285-352). Step control is the reference's: FIXED_BE steps at tstep;
+ * ADAPTIVE takes a BE first step, then TRAP with a BE companion solve for
+ * the LTE estimate (:84-178, same h_min, h_max and tolerance); a Newton
+ * failure rejects the step like an LTE failure (h halved, next step BE).
+ * Sources: the ohm_tran_source table of ohm_transient_create, evaluated on
+ * the host per step (b(t) starts from zero, evaluate_b_at_time :265-280).
+ * mna: the compiled system (G, C with C's pattern inside G's, descriptors).
+ * params->method and ->tstep/tstop/tstart apply; the linear-solver fields
+ * of params are unused (newton->bicgstab_* govern the sub-solves).
+ * x0: n doubles or NULL (zero). Records as ohm_transient_run with one
+ * instance: times[r], waves[r][probe]. stats->linear_solves counts Newton
+ * iterations. */
+ohm_status ohm_transient_nonlinear(ohm_backend *b, const ohm_mna *mna, int64_t n_sources,
+                                   const ohm_tran_source *sources, int64_t n_pwl,
+                                   const double *pwl_pairs, const ohm_tran_params *params,
+                                   const ohm_newton_params *newton, const double *x0,
+                                   int64_t n_probes, const uint32_t *probes, int64_t max_records,
+                                   double *times, double *waves, double *x_final,
+                                   ohm_tran_stats *stats);
+
 /* ---- NonlinearBackend kernels (src/solver/nonlinear.rs:31-150) on a
  * Newton context's device buffers: evaluate + stamp into A, b at x. */
 ohm_status ohm_newton_assemble_at(ohm_newton_ctx *ctx, const double *x, double *values_out,

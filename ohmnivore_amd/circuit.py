@@ -375,6 +375,45 @@ class MnaC(ctypes.Structure):
     ]
 
 
+class MnaView:
+    """An ``ohm_mna`` pointing into an MnaSystem's arrays (kept alive here):
+    what the entry points that take a compiled system read (G, C, b_dc and
+    the device descriptors)."""
+
+    def __init__(self, mna: "MnaSystem"):
+        g, c = mna.g, mna.c
+        arrs = [np.ascontiguousarray(g.row_pointers, np.uint32),
+                np.ascontiguousarray(g.col_indices, np.uint32),
+                np.ascontiguousarray(g.values, np.float64),
+                np.ascontiguousarray(c.row_pointers, np.uint32),
+                np.ascontiguousarray(c.col_indices, np.uint32),
+                np.ascontiguousarray(c.values, np.float64),
+                np.ascontiguousarray(mna.b_dc, np.float64)]
+        descs = []
+        for arr, dt in ((mna.diode_descriptors, DIODE_DESC_DTYPE),
+                        (mna.bjt_descriptors, BJT_DESC_DTYPE),
+                        (mna.mosfet_descriptors, MOSFET_DESC_DTYPE)):
+            descs.append(np.ascontiguousarray(arr if arr is not None else np.zeros(0, dt),
+                                              dtype=dt))
+        self.keep = arrs + descs
+        u32p, f64p = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)
+        m = MnaC()
+        m.size, m.n_nodes = mna.size, mna.n_nodes
+        m.n_branches = mna.size - mna.n_nodes
+        m.g_nnz, m.c_nnz = arrs[2].shape[0], arrs[5].shape[0]
+        m.g_rowptr, m.g_col = arrs[0].ctypes.data_as(u32p), arrs[1].ctypes.data_as(u32p)
+        m.g_val = arrs[2].ctypes.data_as(f64p)
+        m.c_rowptr, m.c_col = arrs[3].ctypes.data_as(u32p), arrs[4].ctypes.data_as(u32p)
+        m.c_val = arrs[5].ctypes.data_as(f64p)
+        m.b_dc = arrs[6].ctypes.data_as(f64p)
+        m.n_diodes, m.n_bjts, m.n_mosfets = (len(d) for d in descs)
+        m.diodes, m.bjts, m.mosfets = (d.ctypes.data if len(d) else None for d in descs)
+        self.struct = m
+
+    def ref(self):
+        return ctypes.byref(self.struct)
+
+
 @dataclass
 class CsrMatrix:
     """CsrMatrix<T> (src/sparse.rs:9-21) with u32 indices."""

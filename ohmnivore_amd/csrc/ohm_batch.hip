@@ -37,6 +37,7 @@
 
 #include "ohm_precond.h"
 #include "ohm_reduce.h"
+#include "ohm_sources.h"
 
 namespace ohm {
 
@@ -1860,62 +1861,6 @@ int64_t ohm_generate_frequencies(int32_t sweep, int64_t n_points, double f_start
 // per step, so a timestep never leaves the GPU: rhs -> warm-started solve
 // (x_n is already in the iterate) -> probe gather.
 namespace ohm {
-
-struct TSrc {
-  int kind;       // OHM_SRC_*
-  int n_idx;
-  uint32_t idx[2];
-  double sign[2];
-  double dc;
-  double p[7];
-  int pwl_off, pwl_n;
-};
-
-__device__ __forceinline__ double src_value(const TSrc &s, const double *__restrict__ pwl, double t) {
-  switch (s.kind) {
-    case OHM_SRC_PULSE: {  // eval_pulse, transient_source.rs:39-71
-      const double v1 = s.p[0], v2 = s.p[1], td = s.p[2], tr = s.p[3], tf = s.p[4], pw = s.p[5],
-                   per = s.p[6];
-      if (t < td) return v1;
-      const double t_rel = (per > 0.0 && per < 1.7976931348623157e308) ? fmod(t - td, per) : t - td;
-      if (t_rel < tr) return tr > 0.0 ? v1 + (v2 - v1) * t_rel / tr : v2;
-      if (t_rel < tr + pw) return v2;
-      if (t_rel < tr + pw + tf) return tf > 0.0 ? v2 + (v1 - v2) * (t_rel - tr - pw) / tf : v1;
-      return v1;
-    }
-    case OHM_SRC_SIN: {  // eval_sin, :73-83
-      const double vo = s.p[0], va = s.p[1], freq = s.p[2], td = s.p[3], theta = s.p[4];
-      if (t < td) return vo;
-      const double dt = t - td;
-      const double env = theta != 0.0 ? exp(-dt * theta) : 1.0;
-      return vo + va * sin(2.0 * M_PI * freq * dt) * env;
-    }
-    case OHM_SRC_EXP: {  // eval_exp, :106-116
-      const double v1 = s.p[0], v2 = s.p[1], td1 = s.p[2], tau1 = s.p[3], td2 = s.p[4],
-                   tau2 = s.p[5];
-      if (t < td1) return v1;
-      const double rise = (v2 - v1) * (1.0 - exp(-(t - td1) / tau1));
-      if (t < td2) return v1 + rise;
-      const double fall = (v1 - v2) * (1.0 - exp(-(t - td2) / tau2));
-      return v1 + rise + fall;
-    }
-    case OHM_SRC_PWL: {  // eval_pwl, :85-104
-      const double *q = pwl + 2 * s.pwl_off;
-      const int m = s.pwl_n;
-      if (m == 0) return 0.0;
-      if (t <= q[0]) return q[1];
-      if (t >= q[2 * (m - 1)]) return q[2 * (m - 1) + 1];
-      for (int i = 1; i < m; i++)
-        if (t <= q[2 * i]) {
-          const double t0 = q[2 * i - 2], v0 = q[2 * i - 1], t1 = q[2 * i], v1 = q[2 * i + 1];
-          return v0 + (v1 - v0) * ((t - t0) / (t1 - t0));
-        }
-      return q[2 * (m - 1) + 1];
-    }
-    default:
-      return s.dc;
-  }
-}
 
 struct TRhsArgs {
   double t_next, t_prev;

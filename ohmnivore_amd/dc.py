@@ -38,40 +38,11 @@ class DcResult:
     device_ms: float = 0.0
 
 
-class _MnaView:
-    """An ``ohm_mna`` pointing into an MnaSystem's arrays (kept alive here)."""
-
-    def __init__(self, mna: ir.MnaSystem):
-        g = mna.g
-        self.keep = [np.ascontiguousarray(g.row_pointers, np.uint32),
-                     np.ascontiguousarray(g.col_indices, np.uint32),
-                     np.ascontiguousarray(g.values, np.float64),
-                     np.ascontiguousarray(mna.b_dc, np.float64)]
-        descs = []
-        for arr, dt in ((mna.diode_descriptors, ir.DIODE_DESC_DTYPE),
-                        (mna.bjt_descriptors, ir.BJT_DESC_DTYPE),
-                        (mna.mosfet_descriptors, ir.MOSFET_DESC_DTYPE)):
-            a = np.ascontiguousarray(arr if arr is not None else np.zeros(0, dt), dtype=dt)
-            descs.append(a)
-        self.keep += descs
-        m = ir.MnaC()
-        m.size, m.n_nodes = mna.size, mna.n_nodes
-        m.n_branches = mna.size - mna.n_nodes
-        m.g_nnz = self.keep[2].shape[0]
-        m.g_rowptr = self.keep[0].ctypes.data_as(N.c_u32p)
-        m.g_col = self.keep[1].ctypes.data_as(N.c_u32p)
-        m.g_val = self.keep[2].ctypes.data_as(N.c_f64p)
-        m.b_dc = self.keep[3].ctypes.data_as(N.c_f64p)
-        m.n_diodes, m.n_bjts, m.n_mosfets = (len(d) for d in descs)
-        m.diodes, m.bjts, m.mosfets = (d.ctypes.data if len(d) else None for d in descs)
-        self.struct = m
-
-
 def run(backend: HipBackend, mna: ir.MnaSystem, **newton_kw) -> DcResult:
     """dc::run (src/analysis/dc.rs:25-52). ``newton_kw``: solver.newton_params
     keywords applied to every Newton run (max_iterations caps the direct
     attempt; the cascade sets the other steps' caps as the reference does)."""
-    view = _MnaView(mna)
+    view = ir.MnaView(mna)
     p, _keep = newton_params(**newton_kw)
     x = np.zeros(max(mna.size, 1))
     st = N.DcStats()

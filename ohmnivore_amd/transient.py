@@ -266,16 +266,71 @@ class TranAnalysis:
     result: TranResult
 
 
+def has_devices(mna: ir.MnaSystem) -> bool:
+    return any(d is not None and len(d) for d in
+               (mna.diode_descriptors, mna.bjt_descriptors, mna.mosfet_descriptors))
+
+
+def run_nonlinear(backend: HipBackend, ckt: ir.Circuit, mna: ir.MnaSystem, tstep: float,
+                  tstop: float, tstart: float = 0.0, x0=None, method: str = "adaptive",
+                  newton_kw: dict | None = None, probes=None,
+                  max_records: int | None = None) -> TranResult:
+    """ohm_transient_nonlinear: a Newton solve on the GPU per timestep, with
+    the diode / BJT / MOSFET currents in the time loop (BE, or the
+    reference's adaptive TRAP/BE step control). ``newton_kw``: newton_params
+    keywords for the per-step solves (default abs_tol 1e-9, sub-solves to
+    1e-10). ``x0`` None = zero state."""
+    from .solver import newton_params
+    kw = dict(abs_tol=1e-9, bicgstab_tolerance=1e-10)
+    kw.update(newton_kw or {})
+    npar, _keep = newton_params(**kw)
+    src, pwl = source_table(ckt, mna)
+    src = np.ascontiguousarray(src, dtype=ir.TRAN_SOURCE_DTYPE)
+    pw = np.ascontiguousarray(pwl if pwl is not None else np.zeros((0, 2)), dtype=np.float64)
+    view = ir.MnaView(mna)
+    n = mna.size
+    pr = np.ascontiguousarray(np.arange(n) if probes is None else probes, dtype=np.uint32)
+    p = TranParams()
+    lib().ohm_tran_default_params(ctypes.byref(p))
+    p.tstep, p.tstop, p.tstart = tstep, tstop, tstart
+    p.method = METHODS[method]
+    x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    cap = max_records or int(np.ceil(tstop / tstep * (1.0 if method in ("be", "fixed_be") else 4.0))) + 2
+    bound = int(np.ceil(tstop / tstep * 20000.0)) + 2
+    while True:
+        times = np.zeros(cap)
+        waves = np.zeros((cap, pr.shape[0]))
+        xf = np.zeros(n)
+        st = TranStats()
+        check(lib().ohm_transient_nonlinear(
+            backend.h, view.ref(), ctypes.c_int64(src.shape[0]),
+            src.ctypes.data_as(ctypes.POINTER(TranSource)), ctypes.c_int64(pw.shape[0]),
+            pw.ctypes.data_as(N.c_f64p), ctypes.byref(p), ctypes.byref(npar),
+            x0a.ctypes.data_as(N.c_f64p) if x0a is not None else None,
+            ctypes.c_int64(pr.shape[0]), pr.ctypes.data_as(N.c_u32p), ctypes.c_int64(cap),
+            times.ctypes.data_as(N.c_f64p), waves.ctypes.data_as(N.c_f64p),
+            xf.ctypes.data_as(N.c_f64p), ctypes.byref(st)))
+        if max_records or st.records < cap or cap >= bound:
+            break
+        cap = min(cap * 4, bound)
+    r = st.records
+    return TranResult(times[:r], waves[:r, None, :], xf[None, :], st.steps_accepted,
+                      st.steps_rejected, st.linear_solves, st.bicgstab_iterations, st.device_ms)
+
+
 def run(backend: HipBackend, ckt: ir.Circuit, mna: ir.MnaSystem, tstep: float, tstop: float,
         tstart: float = 0.0, uic: bool = False, method: str = "adaptive",
         params: N.BicgstabParams | None = None, precond: int = N.PREC_JACOBI,
-        dc_kw: dict | None = None) -> TranAnalysis:
+        dc_kw: dict | None = None, devices: bool = True) -> TranAnalysis:
     """transient::run (src/analysis/transient.rs:25-199): initial state from
     the DC operating point (dc::run, so nonlinear devices are solved by the
     Newton cascade) or the UIC system, both on the GPU, then GPU time
-    stepping of every unknown. As in the reference, the time stepping itself
-    uses G and C only: diode, BJT and MOSFET currents enter through the
-    operating point. ``dc_kw``: dc.run's Newton settings for that point."""
+    stepping of every unknown. ``dc_kw``: dc.run's Newton settings for that
+    point (and for the per-step Newton solves of a circuit with devices).
+    The reference's time stepping uses G and C only, so diode, BJT and
+    MOSFET currents enter only through the operating point; here a circuit
+    with devices is stepped with them (run_nonlinear) unless ``devices`` is
+    False, which reproduces the reference's linear stepping."""
     if uic:
         from .solver import GpuSolver
         s = GpuSolver(backend, precond=N.PREC_AUTO,
@@ -285,6 +340,14 @@ def run(backend: HipBackend, ckt: ir.Circuit, mna: ir.MnaSystem, tstep: float, t
     else:
         from . import dc
         x0 = dc.run(backend, mna, **(dc_kw or {})).x
+    if devices and has_devices(mna):
+        res = run_nonlinear(backend, ckt, mna, tstep, tstop, tstart, x0=x0, method=method,
+                            newton_kw=dc_kw)
+        w = res.waves[:, 0, :]
+        nn = mna.n_nodes
+        nodes = [(name, w[:, i].copy()) for i, name in enumerate(mna.node_names)]
+        branches = [(name, w[:, nn + i].copy()) for i, name in enumerate(mna.branch_names)]
+        return TranAnalysis(res.times, nodes, branches, res)
     src, pwl = source_table(ckt, mna)
     tr = Transient(backend, mna, src, pwl)
     # Every accepted step is recorded. Start from room for 4 records per

@@ -380,3 +380,103 @@ def source_value(src_entry, t: float, pwl=None) -> float:
     e = np.ascontiguousarray(np.array([src_entry], dtype=_ir.TRAN_SOURCE_DTYPE))
     pw = np.ascontiguousarray(pwl if pwl is not None else np.zeros((0, 2)), dtype=np.float64)
     return lib().oo_source_value(e.ctypes.data_as(ctypes.c_void_p), _p(pw, f64p), ctypes.c_double(t))
+
+
+def transient_nonlinear(mna: _ir.MnaSystem, sources, pwl=None, method="adaptive", tstep=1e-9,
+                        tstop=1e-7, tstart=0.0, x0=None, abs_tol=1e-12, max_iterations=100):
+    """Transient with the device currents in the time loop (no reference
+    counterpart: the reference steps G and C only). Each step is the
+    oracle's LU Newton (newton_solve) on
+        BE:   (G + C/h) x + i(x) = b(t+h) + (C/h) x_n
+        TRAP: (G + 2C/h) x + i(x) = b(t+h) + (2C/h) x_n + b(t) - G x_n - i(x_n)
+    with the step control of transient::run (src/analysis/transient.rs:84-178):
+    BE first step, TRAP with a BE companion for the LTE estimate, h in
+    [tstep/1e4, tstep], LTE tolerance 1e-3; a Newton failure rejects the step
+    and halves h. Without devices this is transient() exactly.
+    Returns (times, waves[records, n], x_final, accepted, rejected)."""
+    n = mna.size
+    g, c = mna.g, mna.c
+    descs = (mna.diode_descriptors, mna.bjt_descriptors, mna.mosfet_descriptors)
+    grp = g.row_pointers.astype(np.int64)
+    cg = np.zeros(g.nnz)
+    for r in range(n):
+        cols = g.col_indices[grp[r]:grp[r + 1]]
+        for q in range(int(c.row_pointers[r]), int(c.row_pointers[r + 1])):
+            pos = int(np.searchsorted(cols, c.col_indices[q]))
+            if pos >= cols.shape[0] or cols[pos] != c.col_indices[q]:
+                raise OracleError(11, "C entry outside G's pattern")
+            cg[grp[r] + pos] += c.values[q]
+    src = np.atleast_1d(np.asarray(sources, dtype=_ir.TRAN_SOURCE_DTYPE))
+
+    def b_at(t):
+        b = np.zeros(n)
+        for e in src:
+            v = source_value(e, t, pwl)
+            for j in range(int(e["n_idx"])):
+                b[int(e["idx"][j])] += float(e["sign"][j]) * v
+        return b
+
+    def csr(vals):
+        return _ir.CsrMatrix(n, n, vals, g.col_indices, g.row_pointers)
+
+    def newton(sig, rhs, guess):
+        x, _, _ = newton_solve(csr(g.values + sig * cg), rhs, *descs, abs_tol=abs_tol,
+                               max_iterations=max_iterations, initial_guess=guess)
+        return x
+
+    def be_step(x, t_next, h):
+        return newton(1.0 / h, b_at(t_next) + spmv(csr(cg), x) / h, x)
+
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    times, waves = [], []
+    if tstart <= 0.0:
+        times.append(0.0)
+        waves.append(x.copy())
+    h_min, h_max, lte_tol = tstep / 10000.0, tstep, 1e-3
+    t, h = 0.0, min(tstep, h_max)
+    acc = rej = 0
+    use_be, fails = True, 0
+    while t < tstop:
+        if t + h > tstop:
+            h = tstop - t
+        if h < h_min * 0.5:
+            break
+        t_next = t + h
+        lte, ok = 0.0, True
+        try:
+            if method == "be" or use_be:
+                x_new = be_step(x, t_next, h)
+            else:
+                vals, bb = newton_assemble(g, np.zeros(n), x, *descs)
+                d = b_at(t) + bb - spmv(csr(vals), x)
+                x_tr = newton(2.0 / h, b_at(t_next) + 2.0 * spmv(csr(cg), x) / h + d, x)
+                x_be = be_step(x, t_next, h)
+                lte = (2.0 / 3.0) * float(np.max(np.abs(x_tr - x_be), initial=0.0))
+                x_new = x_tr
+        except OracleError:
+            if method == "be":
+                raise
+            ok = False
+        if method != "be" and (not ok or lte > lte_tol):
+            h_new = 0.5 * h if not ok else h * min(2.0, max(0.5, 0.9 * np.sqrt(lte_tol / lte)))
+            h = max(h_new, h_min)
+            fails += 1
+            if fails >= 10 and h <= h_min * 1.01:
+                raise OracleError(8, "Timestep error: consecutive failures at minimum timestep")
+            rej += 1
+            use_be = True
+            continue
+        if method != "be":
+            if use_be:
+                use_be = False
+            else:
+                grow = 2.0 if lte == 0.0 else min(2.0, max(0.5, 0.9 * np.sqrt(lte_tol / lte)))
+                h = min(h_max, max(h_min, h * grow))  # sqrt(tol / 0) = inf -> 2 in C
+        fails = 0
+        acc += 1
+        x = x_new
+        t = t_next
+        if t >= tstart:
+            times.append(t)
+            waves.append(x.copy())
+    return np.array(times), np.array(waves).reshape(len(times), n), x, acc, rej

@@ -144,9 +144,8 @@ def test_generate_frequencies_reference_cases():
 def test_dc_run_rejects_bad_arguments():
     """ohm_dc_run validates its arguments before touching a device."""
     import ctypes
-    from ohmnivore_amd import dc
     mna = solver.compile(ir.resistor_ladder(4, seed=1))
-    view = dc._MnaView(mna)
+    view = ir.MnaView(mna)
     x = np.zeros(mna.size)
     L = N.lib()
     assert L.ohm_dc_run(None, ctypes.byref(view.struct), None,

@@ -374,15 +374,28 @@ def test_waveform_sources(be, src):
 
 
 def test_nonlinear_transient_starts_at_operating_point(be):
-    """Without UIC the transient starts from dc::run's Newton answer; the
-    stepping itself uses G and C (src/analysis/transient.rs:43-56)."""
+    """Without UIC the transient starts from dc::run's Newton answer
+    (src/analysis/transient.rs:43-56); the diode stays in the time loop
+    (ohm_transient_nonlinear), checked against the oracle's per-step LU
+    Newton up to the first point where the LTE-driven step sequences part."""
     text = ("V1 1 0 DC 5\nR1 1 2 1k\n.MODEL DMOD D(IS=1e-14 N=1.0)\nD1 2 0 DMOD\nC1 2 0 1u\n"
             ".TRAN 10u 100u\n.END\n")
     res, _ = run(be, text)
     a, r = res[0]
-    _, m = system_of(text)
-    assert close(r.result.waves[0, 0, :], oracle_op(m))[0]
-    check_against_oracle(text, r, a)
+    c, m = system_of(text)
+    x0 = oracle_op(m)
+    assert close(r.result.waves[0, 0, :], x0)[0]
+    src, pwl = T.source_table(c, m)
+    times, waves, _, _, _ = oracle.transient_nonlinear(m, src, pwl, method="adaptive",
+                                                       tstep=a.tstep, tstop=a.tstop, x0=x0)
+    n = min(times.shape[0], r.times.shape[0])
+    same = np.isclose(r.times[:n], times[:n], rtol=1e-8, atol=0)
+    k = n if same.all() else int(np.argmin(same))
+    assert k >= n // 2
+    ok, err = close(r.result.waves[:k, 0, :], waves[:k])
+    assert ok, err
+    # at the operating point nothing moves
+    assert np.abs(r.result.waves[:, 0, :] - x0).max() < 1e-6
 
 
 def test_cli_end_to_end(tmp_path):
@@ -404,8 +417,9 @@ def test_cli_end_to_end(tmp_path):
 
 
 NETLISTS = os.path.join(ROOT, "tests", "golden", "netlists")
-# The transient of the CMOS chain needs device currents inside the time
-# stepping, which the reference's transient leaves out (G and C only).
+# The CMOS chain's transient keeps its MOSFETs in the time loop, which the
+# linear transient oracle used here cannot check: tests/test_gpu_trannl.py
+# covers it.
 FIXTURES = sorted(f for f in os.listdir(NETLISTS)
                   if f.endswith(".spice") and f != "cmos_inverter_chain_tran.spice")
 

@@ -151,3 +151,56 @@ def test_oracle_backward_euler_recurrence():
     for k in range(1, times.shape[0]):
         v = (v + a * 5.0) / (1 + a)
         assert abs(waves[k, i2] - v) < 1e-9 * 5
+
+
+# ------------------------------------------- transient with device currents --
+def _rlc_pulse():
+    ckt = ir.Circuit()
+    ckt.vsource("V1", "1", "0", tran=ir.Pulse(0.0, 5.0, 0.0, 1e-6, 1e-6, 5e-6, 20e-6))
+    ckt.resistor("R1", "1", "2", 1e3)
+    ckt.capacitor("C1", "2", "0", 1e-9)
+    ckt.inductor("L1", "2", "3", 1e-3)
+    ckt.resistor("R2", "3", "0", 50.0)
+    return ckt
+
+
+def test_nonlinear_oracle_without_devices_is_the_linear_oracle():
+    """With no devices the per-step Newton is one LU solve of the reference's
+    companion system: fixed-step BE agrees sample for sample; the adaptive
+    run takes the same accepted / rejected steps, its step sizes drifting
+    only by the LTE estimate's rounding."""
+    ckt = _rlc_pulse()
+    mna = solver.compile(ckt)
+    src, pwl = T.source_table(ckt, mna)
+    a = oracle.transient(mna, src, pwl, method="be", tstep=1e-6, tstop=40e-6)
+    b = oracle.transient_nonlinear(mna, src, pwl, method="be", tstep=1e-6, tstop=40e-6)
+    assert np.array_equal(a[0], b[0]) and np.allclose(a[1], b[1], rtol=1e-12, atol=1e-14)
+    a = oracle.transient(mna, src, pwl, method="adaptive", tstep=1e-6, tstop=40e-6,
+                         max_records=5000)
+    b = oracle.transient_nonlinear(mna, src, pwl, method="adaptive", tstep=1e-6, tstop=40e-6)
+    assert (a[3], a[4]) == (b[3], b[4]) and a[0].shape == b[0].shape
+    assert np.allclose(a[0], b[0], rtol=1e-6, atol=0) and np.abs(a[1] - b[1]).max() < 1e-4
+
+
+def test_nonlinear_oracle_diode_discharge_closed_form():
+    """C dv/dt = -Is e^(v/nVt) for a capacitor discharging through a diode
+    from 0.65 V: v(t) = -nVt ln(e^(-v0/nVt) + Is t/(C nVt)) while v >> nVt,
+    against backward Euler at the reference's 2 % transient tolerance."""
+    ckt = ir.Circuit()
+    ckt.add_diode_model("D", 1e-14, 1.0)
+    ckt.capacitor("C1", "1", "0", 1e-6)
+    ckt.diode("D1", "1", "0", "D")
+    mna = solver.compile(ckt)
+    d = mna.diode_descriptors[0]
+    i_s, nvt = float(d["is_val"]), float(d["n_vt"])
+    src, pwl = T.source_table(ckt, mna)
+    times, waves, _, acc, _ = oracle.transient_nonlinear(mna, src, pwl, method="be", tstep=2e-6,
+                                                         tstop=4e-3, x0=[0.65])
+    expect = -nvt * np.log(np.exp(-0.65 / nvt) + i_s * times / (1e-6 * nvt))
+    assert acc == 2000 and expect[-1] > 0.45
+    assert np.all(np.abs(waves[:, 0] - expect) < 0.02 * 0.65)
+    # TRAP (adaptive) is second order: tighter than BE on the same span
+    times2, waves2, _, _, _ = oracle.transient_nonlinear(mna, src, pwl, method="adaptive",
+                                                         tstep=2e-6, tstop=4e-3, x0=[0.65])
+    e2 = -nvt * np.log(np.exp(-0.65 / nvt) + i_s * times2 / (1e-6 * nvt))
+    assert np.abs(waves2[:, 0] - e2).max() < np.abs(waves[:, 0] - expect).max()
