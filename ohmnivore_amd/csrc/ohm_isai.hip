@@ -156,6 +156,61 @@ __global__ void k_isai_sweep(int64_t m, const uint32_t *__restrict__ ptr,
   }
 }
 
+// Small patterns: every sweep of one side in a single 1024-thread
+// workgroup, a barrier between sweeps, so a pattern whose dependency depth
+// asks for hundreds of sweeps (a long inverter chain) costs one launch, not
+// hundreds. Same arithmetic per sweep as the kernels above; the result ends
+// in the buffer the sweep-per-launch loop would leave it in.
+constexpr int kSweepWg = 1024;
+constexpr int64_t kSweepWgMax = 1 << 16;  // entries up to which one workgroup sweeps
+
+__global__ __launch_bounds__(kSweepWg) void k_ilu_sweeps_wg(
+    int64_t nf, const int32_t *__restrict__ from_a, const int32_t *__restrict__ div,
+    const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ pairs,
+    const double *__restrict__ aval, double *f0, double *f1, int sweeps) {
+  double *fin = f0, *fout = f1;
+  for (int s = 0; s < sweeps; s++) {
+    for (int64_t e = threadIdx.x; e < nf; e += kSweepWg) {
+      int32_t src = from_a[e];
+      double v = src >= 0 ? aval[src] : 0.0;
+      for (uint32_t q = ptr[e]; q < ptr[e + 1]; q++) v -= fin[pairs[2 * q]] * fin[pairs[2 * q + 1]];
+      int32_t d = div[e];
+      if (d >= 0) {
+        double piv = fin[d];
+        v = fabs(piv) < 1e-30 ? 0.0 : v / piv;
+      }
+      fout[e] = v;
+    }
+    __syncthreads();
+    double *t = fin;
+    fin = fout;
+    fout = t;
+  }
+}
+
+__global__ __launch_bounds__(kSweepWg) void k_isai_sweeps_wg(
+    int64_t m, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ dep,
+    const int32_t *__restrict__ diag, const uint8_t *__restrict__ is_k,
+    const double *__restrict__ f, double *m0, double *m1, int sweeps) {
+  double *min = m0, *mout = m1;
+  for (int s = 0; s < sweeps; s++) {
+    for (int64_t e = threadIdx.x; e < m; e += kSweepWg) {
+      double v = is_k[e] ? 1.0 : 0.0;
+      for (uint32_t q = ptr[e]; q < ptr[e + 1]; q++) v -= f[dep[2 * q]] * min[dep[2 * q + 1]];
+      int32_t d = diag[e];
+      if (d >= 0) {
+        double piv = f[d];
+        v = fabs(piv) < 1e-30 ? 0.0 : v / piv;
+      }
+      mout[e] = v;
+    }
+    __syncthreads();
+    double *t = min;
+    min = mout;
+    mout = t;
+  }
+}
+
 __global__ void k_scatter(int64_t m, const uint32_t *__restrict__ pos, const double *__restrict__ src,
                           double *__restrict__ dst, float *__restrict__ dst32) {
   for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < m;
@@ -469,11 +524,19 @@ static ohm_status isai_numeric(ohm_backend *b, const DevCsr &a, PrecondDev &p) {
     b->dispatches++;
     pl->f_init = true;
   }
-  for (int s = 0; s < p.ilu_sweeps; s++) {
-    hipLaunchKernelGGL(k_ilu_sweep, gf, blk, 0, b->stream, nf, pl->f_from_a, pl->f_div,
-                       pl->cp_ptr, pl->cp_pairs, a.val, pl->f_val, pl->f_tmp);
-    std::swap(pl->f_val, pl->f_tmp);
+  if (nf <= kSweepWgMax && p.ilu_sweeps > 1) {
+    hipLaunchKernelGGL(k_ilu_sweeps_wg, dim3(1), dim3(kSweepWg), 0, b->stream, nf, pl->f_from_a,
+                       pl->f_div, pl->cp_ptr, pl->cp_pairs, a.val, pl->f_val, pl->f_tmp,
+                       p.ilu_sweeps);
+    if (p.ilu_sweeps & 1) std::swap(pl->f_val, pl->f_tmp);
     b->dispatches++;
+  } else {
+    for (int s = 0; s < p.ilu_sweeps; s++) {
+      hipLaunchKernelGGL(k_ilu_sweep, gf, blk, 0, b->stream, nf, pl->f_from_a, pl->f_div,
+                         pl->cp_ptr, pl->cp_pairs, a.val, pl->f_val, pl->f_tmp);
+      std::swap(pl->f_val, pl->f_tmp);
+      b->dispatches++;
+    }
   }
   for (int sdx = 0; sdx < 2; sdx++) {
     IsaiPlan::Side &sd = pl->side[sdx];
@@ -482,11 +545,18 @@ static ohm_status isai_numeric(ohm_backend *b, const DevCsr &a, PrecondDev &p) {
     const dim3 gm(grid_for(sd.m, 4));
     double *src = sd.m0, *dst = sd.m1;
     OHM_HIP(hipMemsetAsync(src, 0, sizeof(double) * sd.m, b->stream));
-    for (int s = 0; s < sd.sweeps; s++) {
-      hipLaunchKernelGGL(k_isai_sweep, gm, blk, 0, b->stream, sd.m, sd.dep_ptr, sd.dep, sd.diag,
-                         sd.is_k, pl->f_val, src, dst);
-      std::swap(src, dst);
+    if (sd.m <= kSweepWgMax && sd.sweeps > 1) {
+      hipLaunchKernelGGL(k_isai_sweeps_wg, dim3(1), dim3(kSweepWg), 0, b->stream, sd.m, sd.dep_ptr,
+                         sd.dep, sd.diag, sd.is_k, pl->f_val, src, dst, sd.sweeps);
+      if (sd.sweeps & 1) std::swap(src, dst);
       b->dispatches++;
+    } else {
+      for (int s = 0; s < sd.sweeps; s++) {
+        hipLaunchKernelGGL(k_isai_sweep, gm, blk, 0, b->stream, sd.m, sd.dep_ptr, sd.dep, sd.diag,
+                           sd.is_k, pl->f_val, src, dst);
+        std::swap(src, dst);
+        b->dispatches++;
+      }
     }
     hipLaunchKernelGGL(k_scatter, gm, blk, 0, b->stream, sd.m, sd.out_pos, src, M.val,
                        p.has_f32 ? M.val32 : nullptr);
