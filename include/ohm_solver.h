@@ -344,8 +344,14 @@ typedef struct {
                                       (as published); 1: at every iteration, so eta falls at
                                       most as fast as the safeguard allows */
   double forcing_max;              /* OHM_FORCING_EW / EW1: largest eta (default 0.1) */
+  int32_t linear_solver;           /* OHM_LINEAR_BICGSTAB (default): the preconditioned
+                                      BiCGSTAB sub-solve above; OHM_LINEAR_DENSE_LU: every
+                                      Newton step by dense LU with partial pivoting on the GPU
+                                      (systems of at most 4096 unknowns; the reference's
+                                      NewtonLinearMode::CpuSparseLu, src/solver/newton.rs:22-30) */
 } ohm_newton_params;
 enum { OHM_FORCING_FIXED = 0, OHM_FORCING_EW = 1, OHM_FORCING_EW1 = 2 };
+enum { OHM_LINEAR_BICGSTAB = 0, OHM_LINEAR_DENSE_LU = 1 };
 /* The sub-solve is warm-started from x_k, so its initial residual is the
  * Newton residual F(x_k) = b(x_k) - A(x_k) x_k and the stopping rule is
  * max(bicgstab_tolerance*||F||, bicgstab_floor*||b||) (an inexact Newton
@@ -410,7 +416,8 @@ void ohm_newton_free(ohm_newton_ctx *ctx);
  *     solve; pseudo-transient continuation (:340-470; alpha I on node rows
  *     anchored at b + alpha x_prev) when that final solve fails.
  * Where the reference retries a failed step on the CPU with a sparse LU,
- * the step is retried on the GPU in FP64 with a BiCGSTAB budget of
+ * the step is retried on the GPU with OHM_LINEAR_DENSE_LU (systems of at
+ * most 4096 unknowns) or else in FP64 with a BiCGSTAB budget of
  * max(10 x budget, 20000) (and 15 Newton iterations where the reference's
  * retry uses 15). params: the caller's NewtonParams (abs_tol, sub-solve
  * tolerance and forcing, precision, preconditioner apply to every step; the

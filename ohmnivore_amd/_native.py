@@ -64,7 +64,7 @@ class NewtonParams(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("precond", ctypes.c_int32),
                 ("initial_guess", c_f64p), ("bicgstab_floor", ctypes.c_double),
                 ("forcing", ctypes.c_int32), ("forcing_safeguard", ctypes.c_int32),
-                ("forcing_max", ctypes.c_double)]
+                ("forcing_max", ctypes.c_double), ("linear_solver", ctypes.c_int32)]
 
 
 class NewtonResult(ctypes.Structure):

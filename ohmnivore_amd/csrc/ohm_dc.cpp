@@ -81,8 +81,14 @@ struct Cascade {
     p.initial_guess = guess;
     s = run_once(stage, level, p, 0);
     if (s != OHM_OK && recoverable(s) && retry_max_it > 0) {
-      p.precision = OHM_PRECISION_FP64;
-      p.bicgstab_max_iterations = std::max<int64_t>(10 * budget, 20000);
+      // the reference's CPU sparse-LU retry: a direct solve per Newton step
+      // where the system fits the dense LU, else a larger FP64 budget
+      if (m->size <= ohm::kDenseMaxRows) {
+        p.linear_solver = OHM_LINEAR_DENSE_LU;
+      } else {
+        p.precision = OHM_PRECISION_FP64;
+        p.bicgstab_max_iterations = std::max<int64_t>(10 * budget, 20000);
+      }
       p.max_iterations = retry_max_it;
       s = run_once(stage, level, p, 1);
     }

@@ -187,6 +187,10 @@ void launch_dot(ohm_backend *b, int n, const double *x, const double *y, int slo
 
 // Jacobi inverse diagonal (with the Schur-complement rule for zero diagonals)
 void launch_jacobi_setup(ohm_backend *b, const DevCsr &a, double *inv, float *inv32);
+// Dense LU with partial pivoting on the GPU (ohm_dense.hip): x = A^-1 rhs
+// for systems of at most kDenseMaxRows unknowns; x and rhs are device arrays.
+constexpr int kDenseMaxRows = 4096;
+ohm_status dense_lu_solve(ohm_backend *b, const DevCsr &a, const double *rhs, double *x);
 void launch_mul(ohm_backend *b, int n, const double *d, const double *x, double *y);
 
 // ---------------------------------------------------------- BiCGSTAB core --

@@ -576,6 +576,7 @@ void ohm_newton_default_params(ohm_newton_params *p) {
   p->forcing = OHM_FORCING_FIXED;
   p->forcing_safeguard = 0;
   p->forcing_max = 0.1;
+  p->linear_solver = OHM_LINEAR_BICGSTAB;
 }
 
 void ohm_newton_free(ohm_newton_ctx *c) {
@@ -770,6 +771,11 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
   ohm_backend *b = c->b;
   hipStream_t st = b->stream;
   const int n = c->n;
+  if (prm.linear_solver != OHM_LINEAR_BICGSTAB && prm.linear_solver != OHM_LINEAR_DENSE_LU)
+    return fail(OHM_ERR_INVALID, "unknown linear_solver %d", prm.linear_solver);
+  if (prm.linear_solver == OHM_LINEAR_DENSE_LU && n > kDenseMaxRows)
+    return fail(OHM_ERR_INVALID, "dense LU is limited to %d unknowns (system has %d)",
+                kDenseMaxRows, n);
   if (prm.initial_guess)
     OHM_HIP(hipMemcpyAsync(c->x, prm.initial_guess, sizeof(double) * n, hipMemcpyHostToDevice, st));
   else
@@ -823,7 +829,11 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
     o.forcing = fc;
     BicgOut bo;
     ohm_status ls;
-    if (prm.precision == OHM_PRECISION_FP32_IR) {
+    if (prm.linear_solver == OHM_LINEAR_DENSE_LU) {
+      // direct step: no forcing bookkeeping (r0 is not formed)
+      bo.r0 = NAN;
+      ls = dense_lu_solve(b, c->A, c->rhs, c->x);
+    } else if (prm.precision == OHM_PRECISION_FP32_IR) {
       int64_t refs = 0;
       // FP32 refinement hands over to FP64 when kappa * 2^-24 >~ 1 (diodes
       // still off, mesh grounded only through GMIN)

@@ -522,12 +522,14 @@ class NewtonStats:
 def newton_params(max_iterations=50, abs_tol=1e-4, bicgstab_max_iterations=10_000,
                   bicgstab_tolerance=1e-5, precision="fp64", precond=N.PREC_AUTO,
                   initial_guess=None, bicgstab_floor=1e-15, forcing="fixed", forcing_max=0.1,
-                  forcing_safeguard="published"):
+                  forcing_safeguard="published", linear_solver="bicgstab"):
     """NewtonParams (src/solver/newton.rs:34-55). bicgstab_tolerance is
     relative to the Newton residual ||F(x_k)|| (inexact Newton); forcing
     "ew" (Eisenstat-Walker choice 2) or "ew1" (choice 1) adapts it per
     iteration down to that value; forcing_safeguard "always" applies the
-    forcing safeguard at every iteration instead of only above 0.1."""
+    forcing safeguard at every iteration instead of only above 0.1.
+    linear_solver "dense" solves every Newton step by GPU dense LU (systems
+    of at most 4096 unknowns)."""
     p = N.NewtonParams()
     lib().ohm_newton_default_params(ctypes.byref(p))
     p.max_iterations = max_iterations
@@ -540,6 +542,7 @@ def newton_params(max_iterations=50, abs_tol=1e-4, bicgstab_max_iterations=10_00
     p.forcing = {"fixed": 0, "ew": 1, "ew2": 1, "ew1": 2}[forcing]
     p.forcing_safeguard = {"published": 0, "always": 1}[forcing_safeguard]
     p.forcing_max = forcing_max
+    p.linear_solver = {"bicgstab": 0, "dense": 1, "dense_lu": 1}[linear_solver]
     keep = None
     if initial_guess is not None:
         keep = np.ascontiguousarray(initial_guess, dtype=np.float64)

@@ -8,6 +8,7 @@ import oracle
 from ohmnivore_amd import circuit as ir
 from ohmnivore_amd import dc
 from ohmnivore_amd import solver
+from ohmnivore_amd import _native as N
 
 pytestmark = pytest.mark.gpu
 TIGHT = dict(abs_tol=1e-12, bicgstab_tolerance=1e-10)
@@ -124,3 +125,43 @@ def test_cmos_chain_cascade(be, stages):
     v = dict(res.node_voltages)
     for k in range(1, stages + 1):
         assert (v[f"out_{k}"] < 0.5) if k % 2 else (v[f"out_{k}"] > 4.5)
+
+
+def test_dense_lu_newton_matches_oracle(be):
+    """OHM_LINEAR_DENSE_LU: every Newton step by the GPU dense LU (the
+    reference's CpuSparseLu mode) lands on the oracle's operating point."""
+    for ckt in (cmos_chain(10), ir.amplifier_array(16, seed=5)):
+        mna = solver.compile(ckt)
+        ref = oracle_source_stepping(mna)
+        ctx = solver.NewtonContext(be, mna.g, mna.b_dc, mna.diode_descriptors,
+                                   mna.bjt_descriptors, mna.mosfet_descriptors)
+        # from the oracle's answer perturbed: the direct steps must pull it back
+        st = ctx.run(abs_tol=1e-12, linear_solver="dense", max_iterations=50,
+                     initial_guess=ref * (1 + 1e-3))
+        ok, err = close(ctx.x(), ref)
+        assert ok, err
+        assert st.bicgstab_iterations == 0
+
+
+def test_dense_lu_reports_singular_system(be):
+    """Two ideal sources in parallel make the MNA matrix singular."""
+    ckt = ir.Circuit()
+    ckt.add_diode_model("D", 1e-14, 1.0)
+    ckt.vsource("V1", "1", "0", dc=1.0)
+    ckt.vsource("V2", "1", "0", dc=2.0)
+    ckt.diode("D1", "1", "0", "D")
+    mna = solver.compile(ckt)
+    ctx = solver.NewtonContext(be, mna.g, mna.b_dc, mna.diode_descriptors)
+    with pytest.raises(N.OhmnivoreError, match="singular"):
+        ctx.run(linear_solver="dense")
+
+
+def test_long_cmos_chain_cascade(be):
+    """1,000 inverter stages: BiCGSTAB sub-solves break down on the chain's
+    nearly floating nodes, so the cascade's retries solve each Newton step
+    by dense LU (the reference's CPU sparse-LU retry)."""
+    mna = solver.compile(cmos_chain(1000))
+    res = dc.run(be, mna, **TIGHT)
+    assert any(p[4] for p in res.path)  # a retry ran
+    ok, err = close(res.x, oracle_source_stepping(mna))
+    assert ok, err
