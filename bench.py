@@ -21,6 +21,9 @@ The other workloads are measured once each (they are not the headline):
   amplifier_adjoint    4,096 stages (8,194 unknowns): operating point and
                        the adjoint gradient of sum (V_out - 0.9)^2 with
                        respect to 12,288 conductances and 2,048 kp
+  inverter_chain_netlists  the reference's own benchmark netlists (CMOS
+                       inverter chains of 10-200 stages, .DC and .TRAN 1n 100n)
+                       from netlist text to CSV, wall time per netlist
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--only headline]
 """
@@ -375,6 +378,48 @@ def roofline_only(be, N, ir, solver) -> int:
     return 0
 
 
+def chain_netlist(stages, analysis):
+    """The reference's own benchmark circuit family (bench/generate_chain.py,
+    run by bench/run_benchmark.sh for 10-200 stages): a CMOS inverter chain
+    with VDD = 5, VIN = 5 V DC (.DC) or PULSE(0 5 10n 1n 1n 50n 100n)
+    (.TRAN 1n 100n)."""
+    lines = [f"* CMOS Inverter Chain - {stages} stages - {analysis.upper()}",
+             ".MODEL NMOD NMOS(VTO=0.7 KP=1.1e-4 LAMBDA=0.04)",
+             ".MODEL PMOD PMOS(VTO=-0.7 KP=5.5e-5 LAMBDA=0.04)", "VDD vdd 0 DC 5",
+             "VIN in 0 DC 5" if analysis == "dc" else "VIN in 0 PULSE(0 5 10n 1n 1n 50n 100n)"]
+    for i in range(1, stages + 1):
+        gate = "in" if i == 1 else f"out_{i - 1}"
+        lines += [f"MP{i} out_{i} {gate} vdd vdd PMOD", f"MN{i} out_{i} {gate} 0 0 NMOD"]
+    lines += [".DC" if analysis == "dc" else ".TRAN 1n 100n", ".END", ""]
+    return "\n".join(lines)
+
+
+def inverter_chains(be):
+    """Netlist text -> parse -> compile -> GPU analysis -> CSV, wall time per
+    netlist, on the reference's benchmark chains (DC through the continuation
+    cascade, TRAN with the MOSFETs in the time loop)."""
+    import io
+    from ohmnivore_amd import cli
+    cli.simulate(be, chain_netlist(5, "dc"), io.StringIO())  # first-use setup
+    out = {}
+    for stages in (10, 50, 100, 200):
+        r = {}
+        for analysis in ("dc", "tran"):
+            t0 = time.perf_counter()
+            res = cli.simulate(be, chain_netlist(stages, analysis), io.StringIO())
+            ms = 1e3 * (time.perf_counter() - t0)
+            a = res[0][1]
+            if analysis == "dc":
+                r["dc"] = {"ms": ms, "stage": a.stage, "newton_iterations": a.newton_iterations}
+            else:
+                r["tran"] = {"ms": ms, "steps": a.result.steps_accepted,
+                             "newton_iterations": a.result.linear_solves}
+        out[f"stages_{stages}"] = r
+        log(f"inverter chain {stages}: dc {r['dc']['ms']:.0f} ms ({r['dc']['stage']}), "
+            f"tran {r['tran']['ms']:.0f} ms ({r['tran']['steps']} steps)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,6 +465,7 @@ def main() -> int:
                                            args.mc_steps)
         workloads["rlc_ac_256x101"] = rlc_ac(be, N, ir, solver, args.ac_circuits)
         workloads["amplifier_adjoint_4096"] = adjoint(be, N, ir, solver)
+        workloads["inverter_chain_netlists"] = inverter_chains(be)
         speedups = {"rc_grid_512_monte_carlo": workloads["rc_grid_512"]["monte_carlo"]["speedup"],
                     "rlc_ac_256x101": workloads["rlc_ac_256x101"]["speedup"]}
     line = {

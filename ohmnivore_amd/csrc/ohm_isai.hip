@@ -162,7 +162,11 @@ __global__ void k_isai_sweep(int64_t m, const uint32_t *__restrict__ ptr,
 // hundreds. Same arithmetic per sweep as the kernels above; the result ends
 // in the buffer the sweep-per-launch loop would leave it in.
 constexpr int kSweepWg = 1024;
-constexpr int64_t kSweepWgMax = 1 << 16;  // entries up to which one workgroup sweeps
+// Entries up to which one workgroup sweeps: about 8 per thread, where a
+// sweep inside the workgroup costs less than a launch; beyond that the
+// sweep-per-launch grid is faster (an 8,194-unknown amplifier array's
+// factors took 16 ms per operating point in one workgroup against 9 ms).
+constexpr int64_t kSweepWgMax = 8192;
 
 __global__ __launch_bounds__(kSweepWg) void k_ilu_sweeps_wg(
     int64_t nf, const int32_t *__restrict__ from_a, const int32_t *__restrict__ div,
