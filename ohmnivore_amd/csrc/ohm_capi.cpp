@@ -103,6 +103,7 @@ void ohm_backend_destroy(ohm_backend *b) {
   if (b->d_scalars) (void)hipFree(b->d_scalars);
   if (b->h_pinned) (void)hipHostFree(b->h_pinned);
   if (b->ac_ws) (void)hipFree(b->ac_ws);
+  if (b->dense_ws) (void)hipFree(b->dense_ws);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -5,8 +5,9 @@
 // src/solver/newton.rs:22-30,286-289).
 //
 // Layout: the CSR matrix is scattered into a row-major n x n FP64 array in
-// HBM (134 MB at the 4096-row limit). Elimination step k is two launches:
-// a one-workgroup pivot search + row swap (matrix and right-hand side), then
+// HBM (134 MB at the 4096-row limit, kept on the backend for later solves).
+// Elimination step k is two launches: a one-workgroup pivot search + row
+// swap (matrix and right-hand side), then
 // a trailing update over rows k+1..n-1 and columns k+1..n-1 (plus the
 // right-hand side as column n) with the multiplier a_ik / a_kk recomputed
 // per element, so column k is only read during the update and no multiplier
@@ -147,21 +148,19 @@ ohm_status dense_lu_solve(ohm_backend *b, const DevCsr &a, const double *rhs, do
     return fail(OHM_ERR_INVALID, "dense LU is limited to %d unknowns (system has %d)",
                 kDenseMaxRows, n);
   hipStream_t st = b->stream;
-  double *d = nullptr, *y = nullptr;
-  int *flag = nullptr;
-  struct Free {
-    double *&d, *&y;
-    int *&f;
-    ~Free() {
-      if (d) (void)hipFree(d);
-      if (y) (void)hipFree(y);
-      if (f) (void)hipFree(f);
-    }
-  } guard{d, y, flag};
+  // workspace kept on the backend between solves (a Newton run in this mode
+  // solves once per iteration): n*n matrix, n right-hand side, one flag
   const int64_t nn = (int64_t)n * n;
-  OHM_HIP(hipMalloc(&d, sizeof(double) * nn));
-  OHM_HIP(hipMalloc(&y, sizeof(double) * n));
-  OHM_HIP(hipMalloc(&flag, sizeof(int)));
+  const size_t need = sizeof(double) * (size_t)(nn + n) + 256;
+  if (b->dense_ws_bytes < need) {
+    if (b->dense_ws) OHM_HIP(hipFree(b->dense_ws));
+    b->dense_ws = nullptr;
+    b->dense_ws_bytes = 0;
+    OHM_HIP(hipMalloc(&b->dense_ws, need));
+    b->dense_ws_bytes = need;
+  }
+  double *d = reinterpret_cast<double *>(b->dense_ws), *y = d + nn;
+  int *flag = reinterpret_cast<int *>(y + n);
   OHM_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
   hipLaunchKernelGGL(k_dense_zero, dim3((unsigned)std::min<int64_t>((nn + kBlock - 1) / kBlock, 8192)),
                      dim3(kBlock), 0, st, nn, d);

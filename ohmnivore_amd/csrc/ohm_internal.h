@@ -120,6 +120,8 @@ struct ohm_backend {
   double *h_pinned = nullptr;        // pinned host staging for scalar readback
   void *ac_ws = nullptr;             // banded AC solver's chunk workspace (grow-only)
   size_t ac_ws_bytes = 0;
+  void *dense_ws = nullptr;          // dense LU: n*n matrix + rhs + flag (grow-only)
+  size_t dense_ws_bytes = 0;
   int64_t dispatches = 0, readbacks = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
