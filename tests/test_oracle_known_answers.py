@@ -526,3 +526,34 @@ def test_integration_rc_lowpass_cutoff():
 def test_pulse_waveform(args, t, expect):
     assert abs(oracle.eval_pulse(*args, t) - expect) < 1e-12
     assert abs(ir.Pulse(*args)(t) - expect) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# tests/transistor_integration_test.rs:238-657, tests/diode_integration_test.rs:
+# 34-86: the reference's device circuits, solved by the oracle's LU Newton,
+# land inside the bounds the reference's tests assert
+# --------------------------------------------------------------------------
+def test_device_circuits_meet_reference_bounds():
+    from ohmnivore_amd import netlist as nl
+    from tests.test_gpu_device_netlists import CASES, oracle_point
+    for name, (text, bounds) in sorted(CASES.items()):
+        m = oracle.compile(nl.parse(text).to_circuit())
+        x = oracle_point(m)
+        v = {n: float(x[i]) for i, n in enumerate(m.node_names)}
+        assert bounds(v), (name, v)
+
+
+def test_pnp_is_the_negated_npn():
+    """src/compiler.rs:482-522: PNP = NPN with the polarity flag, so the
+    mirrored circuit's solution is the exact negation."""
+    from ohmnivore_amd import netlist as nl
+    from tests.test_gpu_device_netlists import nl as dc_netlist
+    xs = []
+    for pol, s in (("NPN", 1), ("PNP", -1)):
+        text = dc_netlist(f"V1 vcc 0 DC {5 * s}", f"V2 base 0 DC {0.7 * s}", "R1 vcc vc 1k",
+                          f".MODEL Q {pol}(IS=1e-14 BF=100)", "Q1 vc base 0 Q")
+        m = oracle.compile(nl.parse(text).to_circuit())
+        xs.append(oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, m.bjt_descriptors,
+                                      m.mosfet_descriptors, abs_tol=1e-13, max_iterations=200)[0])
+    assert np.array_equal(xs[0], -xs[1])
+    assert 0.0 < xs[0][2] < 0.5  # V(vc): saturated NPN
