@@ -674,7 +674,30 @@ def vp(a: np.ndarray):
 
 def newton_solve(backend: HipBackend, g: ir.CsrMatrix, base_b, diodes=None, bjts=None,
                  mosfets=None, **params):
-    """newton_solve (src/solver/newton.rs:65-78). Returns (x, NewtonStats)."""
-    ctx = NewtonContext(backend, g, base_b, diodes, bjts, mosfets)
-    stats = ctx.run(**params)
-    return ctx.x(), stats
+    """The reference's free function newton_solve (src/solver/newton.rs:65-78)
+    through its one-shot C entry point ohm_newton_solve: base matrix (values
+    and CSR pattern), RHS and device descriptors in; (x, NewtonStats) out.
+    Errors carry the reference's texts (NewtonNotConverged,
+    NewtonNumericalError)."""
+    v = np.ascontiguousarray(g.values, dtype=np.float64)
+    c = np.ascontiguousarray(g.col_indices, dtype=np.uint32)
+    r = np.ascontiguousarray(g.row_pointers, dtype=np.uint32)
+    b = np.ascontiguousarray(base_b, dtype=np.float64)
+    n = g.nrows
+    if b.shape[0] != n:
+        raise ValueError(f"base_b: expected {n} entries, got {b.shape[0]}")
+    d, nd = _descs(diodes, ir.DIODE_DESC_DTYPE)
+    q, nq = _descs(bjts, ir.BJT_DESC_DTYPE)
+    m, nm = _descs(mosfets, ir.MOSFET_DESC_DTYPE)
+    p, keep = newton_params(**params)
+    res = N.NewtonResult()
+    per = np.zeros(max(p.max_iterations, 1), dtype=np.int32)
+    x = np.zeros(n)
+    check(lib().ohm_newton_solve(
+        backend.h, ptr(v, N.c_f64p), ptr(c, N.c_u32p), ptr(r, N.c_u32p), ptr(b, N.c_f64p),
+        vp(d), ctypes.c_size_t(nd), vp(q), ctypes.c_size_t(nq), vp(m), ctypes.c_size_t(nm),
+        ctypes.c_size_t(n), ctypes.c_size_t(v.shape[0]), ctypes.byref(p), ptr(x, N.c_f64p),
+        ctypes.byref(res), ptr(per, N.c_i32p)))
+    del keep
+    return x, NewtonStats(res.iterations, res.bicgstab_iterations, per[: res.iterations].tolist(),
+                          res.max_diff, res.device_ms, res.fp64_fallbacks)

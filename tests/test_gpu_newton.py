@@ -109,3 +109,82 @@ def test_newton_full_size_diode_mesh_residual(be):
     r = A.spmv(x) - b
     assert np.abs(r).max() <= 1e-9 * np.abs(b).max()
     assert st.newton_iterations < 50
+
+
+def _mosfet_cell(nmos: bool):
+    """One MOSFET between nodes d, g, s, each tied to ground by 1 kOhm so
+    that any (v_d, v_g, v_s) is a valid assembly point."""
+    c = ir.Circuit()
+    c.add_mosfet_model("M", vto=0.7 if nmos else -0.7, kp=1.1e-4 if nmos else 5.5e-5, lam=0.04,
+                       nmos=nmos)
+    for node in ("d", "g", "s"):
+        c.resistor(f"R{node}", node, "0", 1e3)
+    c.mosfet("M1", "d", "g", "s", "M")
+    return c
+
+
+# (v_gs, v_ds) points of the reference's smooth-MOSFET tests
+# (src/solver/gpu_shaders.rs:1495-1656): deep saturation, deep cutoff, at
+# threshold, the continuity sweep across threshold, linear region, the
+# finite-difference points; the PMOS case at its physical (-2, -4).
+_MOS_POINTS = ([(3.0, 5.0), (0.0, 3.0), (0.7, 3.0), (3.0, 0.5)]
+               + [(0.6 + 0.01 * i, 3.0) for i in range(21)]
+               + [(v + d, 3.0) for v in (0.5, 0.7, 0.9, 1.5, 2.5) for d in (-1e-4, 0.0, 1e-4)])
+
+
+@pytest.mark.parametrize("nmos", [True, False])
+def test_mosfet_stamps_at_reference_points(be, nmos):
+    """GPU evaluation + stamping of one MOSFET at each point equals the
+    oracle's assembly to 1e-13 (the oracle's model is pinned to the same
+    points in tests/test_oracle_solvers.py)."""
+    m = solver.compile(_mosfet_cell(nmos))
+    ctx = ctx_for(be, m)
+    idx = {n: i for i, n in enumerate(m.node_names)}
+    pts = _MOS_POINTS if nmos else [(-2.0, -4.0)] + [(-g, -d) for g, d in _MOS_POINTS]
+    for vgs, vds in pts:
+        x = np.zeros(m.size)
+        x[idx["s"]] = 0.3
+        x[idx["g"]] = 0.3 + vgs
+        x[idx["d"]] = 0.3 + vds
+        v, b = ctx.assemble_at(x)
+        rv, rb = oracle.newton_assemble(m.g, m.b_dc, x, m.diode_descriptors, m.bjt_descriptors,
+                                        m.mosfet_descriptors)
+        assert np.allclose(v, rv, rtol=1e-13, atol=1e-300), (vgs, vds)
+        assert np.allclose(b, rb, rtol=1e-13, atol=1e-18), (vgs, vds)
+
+
+def test_one_shot_newton_solve_matches_context_and_oracle(be):
+    """ohm_newton_solve (the reference's newton_solve, src/solver/newton.rs:
+    65-78, as one C call): the same iterate sequence as a device-resident
+    context run with the same parameters, and the oracle's LU Newton answer."""
+    # diodes (a mesh) and BJTs + MOSFETs (the amplifier array)
+    for m in (solver.compile(ir.diode_mesh(40, 36, seed=11)),
+              solver.compile(ir.amplifier_array(64, seed=5))):
+        kw = dict(abs_tol=1e-10, bicgstab_tolerance=1e-8, max_iterations=200)
+        x, st = solver.newton_solve(be, m.g, m.b_dc, m.diode_descriptors, m.bjt_descriptors,
+                                    m.mosfet_descriptors, **kw)
+        ctx = ctx_for(be, m)
+        st2 = ctx.run(**kw)
+        assert x.tobytes() == ctx.x().tobytes()
+        assert st.bicgstab_iters_per_newton == st2.bicgstab_iters_per_newton
+        assert st.newton_iterations == len(st.bicgstab_iters_per_newton) > 0
+        ref, _ = newton_ref(m, abs_tol=1e-10, max_iterations=200)
+        assert_close(x, ref)
+
+
+def test_one_shot_newton_solve_errors(be):
+    """The one-shot call reports the reference's NewtonNotConverged text,
+    and a linear system (no devices) is one solve."""
+    m = solver.compile(ir.diode_mesh(12, 12, seed=1, diode_every=2))
+    with pytest.raises(N.OhmnivoreError, match="did not converge after 2 iterations") as e:
+        solver.newton_solve(be, m.g, m.b_dc, m.diode_descriptors, max_iterations=2,
+                            abs_tol=1e-15)
+    assert e.value.kind == "NewtonNotConverged"
+    lin = ir.Circuit()
+    lin.vsource("V1", "1", "0", dc=10.0)
+    lin.resistor("R1", "1", "2", 1e3)
+    lin.resistor("R2", "2", "0", 1e3)
+    ml = solver.compile(lin)
+    x, st = solver.newton_solve(be, ml.g, ml.b_dc, abs_tol=1e-12, bicgstab_tolerance=1e-12)
+    assert_close(x, oracle.dense_solve(ml.g, ml.b_dc))
+    assert abs(x[1] - 5.0) < 1e-6
