@@ -89,9 +89,11 @@ def headline(args, be, N, ir, solver):
     # forcing: Eisenstat-Walker choice 1 with its safeguard at every
     # iteration, so eta cannot collapse on the large residual drops of the
     # diode turn-on phase (a sub-solve to 1e-6 there is wasted on the
-    # limited steps that follow).
+    # limited steps that follow). eta_max 0.9 is Eisenstat and Walker's
+    # published ceiling; against 0.1 it took fewer BiCGSTAB iterations in
+    # both forms on this mesh and on a 192x160 mesh of another seed.
     newton_kw = dict(abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-6,
-                     bicgstab_max_iterations=20_000, forcing="ew1", forcing_max=0.1,
+                     bicgstab_max_iterations=20_000, forcing="ew1", forcing_max=0.9,
                      forcing_safeguard="always", bicgstab_floor=1e-18)
     forms = {}
     for prec in ("fp64", "fp32_ir", "fp64_fixed_tol"):
@@ -490,7 +492,8 @@ def main() -> int:
             "bicgstab_floor_rel_to_b": h["newton_kw"]["bicgstab_floor"],
             "forcing": ("fixed" if best == "fp64_fixed_tol" else
                         "Eisenstat-Walker choice 1, safeguard at every iteration, "
-                        "eta in [1e-6, 0.1]"),
+                        f"eta in [{h['newton_kw']['bicgstab_tolerance']:g}, "
+                        f"{h['newton_kw']['forcing_max']:g}]"),
         },
         "roofline": h["roofline"],
         "cpu_baseline": h["cpu"],
