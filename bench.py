@@ -440,7 +440,7 @@ def main() -> int:
                          "counter calibration, 20 copies of a 64 MiB vector")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+        print(f"bench.py measures one GPU: the backend has no multi-GPU path (asked for "
               f"--gpus {args.gpus}).", file=sys.stderr)
         return 2
 

@@ -144,11 +144,11 @@ def main(argv=None) -> int:
                                  description="GPU circuit simulation (MI355X)")
     ap.add_argument("netlist", help="SPICE netlist file to simulate")
     ap.add_argument("--cpu", action="store_true",
-                    help="not available: every solve of this build runs on the GPU")
+                    help="not available: every solve runs on the GPU")
     ap.add_argument("--stats", action="store_true", help="print performance stats to stderr")
     args = ap.parse_args(argv)
     if args.cpu:
-        print("--cpu: this build has no CPU solver; every analysis runs on the GPU",
+        print("--cpu: there is no CPU solver; every analysis runs on the GPU",
               file=sys.stderr)
         return 2
     try:

@@ -2,8 +2,8 @@
 
 ctypes wrapper over ``oracle/libohm_oracle.so`` (built from ohm_oracle.c by
 oracle/Makefile). Only tests/, ``__graft_entry__.smoke()`` and bench.py's
-``cpu_baseline`` leg may import this package, and only as the checker or the
-timed CPU baseline. Each function restates the reference algorithm cited in
+``cpu_baseline`` leg may import this package, and only as the reference answer
+or the timed CPU baseline. Each function restates the reference algorithm cited in
 ohm_oracle.c / ohm_oracle.h.
 """
 from __future__ import annotations

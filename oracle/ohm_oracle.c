@@ -2,7 +2,7 @@
  * ohm_oracle.c — CPU oracle (plain C) for the ohmnivore solve path.
  *
  * TEST INFRASTRUCTURE ONLY: a restatement of the reference's algorithms used
- * as the checker in tests/ and as the timed CPU baseline in bench.py. The
+ * as the reference answer in tests/ and as the timed CPU baseline in bench.py. The
  * product (ohmnivore_amd/) never links or calls it.
  *
  * All citations are to files of the reference repository

@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference's
  * algorithms (each function cites the reference file:line it follows). Only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
- * it, and only as the checker or the timed CPU baseline — never as part of
+ * it, and only as the reference answer or the timed CPU baseline — never as part of
  * the product path.
  *
  * Parity status of each function is recorded in oracle/README.md.

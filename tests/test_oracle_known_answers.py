@@ -2,7 +2,7 @@
 
 Each test names the reference test it restates. These run on the CPU only
 and check the oracle (oracle/ohm_oracle.c), which the GPU parity tests then
-use as the checker.
+compare against.
 """
 import cmath
 import math
