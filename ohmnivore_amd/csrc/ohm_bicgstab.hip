@@ -288,39 +288,38 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int n, const T *__restrict_
 }
 
 // K1 (ISAI): the p update fused into the first preconditioner SpMV:
-// y = M1 p_new with p_new formed on the fly from p_old, so p is double
-// buffered (p_old is read at neighbour rows while p_new is written).
+// y = M1 p with p_j = r_j + beta q_j formed on the fly, where
+// q = p_prev - omega v_prev was written by the previous K5 (zeroed by k_init,
+// so the first p is r). The two roundings are p_next's, split across the
+// two kernels; p is written for the rows of this block.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_p_spmv(const uint32_t *__restrict__ rowptr,
                                                    const uint32_t *__restrict__ col,
                                                    const T *__restrict__ val,
                                                    const uint32_t *__restrict__ blocks,
                                                    const T *__restrict__ r,
-                                                   const T *__restrict__ p_old,
-                                                   const T *__restrict__ v,
-                                                   T *__restrict__ p_new, T *__restrict__ y,
+                                                   const T *__restrict__ q,
+                                                   T *__restrict__ p, T *__restrict__ y,
                                                    SolveState *st, Parts pp) {
-  T beta = T(0), omega = T(0);
-  spmv_block_h<T, 3>(
+  T beta = T(0);
+  spmv_block_h<T, 2>(
       rowptr, col, val, blocks,
       [&](uint32_t j) {
-        Opnd<T, 3> o;
+        Opnd<T, 2> o;
         o.v[0] = r[j];
-        o.v[1] = p_old[j];
-        o.v[2] = v[j];
+        o.v[1] = q[j];
         return o;
       },
       [&] {
         double beta_d;
         if (!head_rho(st, pp, beta_d)) return false;
         beta = (T)beta_d;
-        omega = (T)st->omega;
         return true;
       },
-      [&](const Opnd<T, 3> &o) { return p_next<T>(o.v[0], o.v[1], o.v[2], beta, omega); },
+      [&](const Opnd<T, 2> &o) { return fmac<T>(o.v[1], beta, o.v[0]); },
       [&](uint32_t i, T yi) {
         y[i] = yi;
-        p_new[i] = p_next<T>(r[i], p_old[i], v[i], beta, omega);
+        p[i] = fmac<T>(q[i], beta, r[i]);
       });
 }
 
@@ -475,7 +474,10 @@ __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
                                                       const T *__restrict__ ph,
                                                       const T *__restrict__ sh,
                                                       const T *__restrict__ s,
-                                                      const T *__restrict__ t, SolveState *st,
+                                                      const T *__restrict__ t,
+                                                      const T *__restrict__ p,
+                                                      const T *__restrict__ v,
+                                                      T *__restrict__ q, SolveState *st,
                                                       Parts pp) {
   const int i0 = blockIdx.x * kBlock + threadIdx.x, ic = first_index(n);
   const T x0 = x[ic], ph0 = ph[ic], sh0 = sh[ic], s0 = s[ic], t0 = t[ic], rh0 = rh[ic];
@@ -510,6 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
     r[i0] = ri;
     rr.add_prod(ri, ri);
     rhr.add_prod(rh0, ri);
+    if (q) q[i0] = fmac<T>(-omega, v[i0], p[i0]);
   }
   for (int i = i0 + stride; i < n; i += stride) {
     x[i] = (x[i] + alpha * ph[i]) + omega * sh[i];
@@ -517,6 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
     r[i] = ri;
     rr.add_prod(ri, ri);
     rhr.add_prod(rh[i], ri);
+    if (q) q[i] = fmac<T>(-omega, v[i], p[i]);
   }
   double vals[2] = {rr.get(), rhr.get()};
   deposit<2>(vals, pp.rr);
@@ -655,21 +659,20 @@ static void fold_launch(ohm_backend *b, double *partials, int np) {
 
 // One iteration: K1 .. K5 (ISAI: the p and s updates ride on the first
 // preconditioner SpMV; a two-factor apply adds its second SpMV after each).
-// `parity` selects which p buffer is current for the fused ISAI p update.
 template <typename T, int PK>
 static void enqueue_iteration(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
-                              BicgVecs<T> &v, SolveState *st, int n, const Parts &pp,
-                              int parity) {
+                              BicgVecs<T> &v, SolveState *st, int n, const Parts &pp) {
   const dim3 gv(grid_for(n, 4)), blk(kBlock), ga(a.n_blocks);
   const T *inv = Ops<T>::inv(p);
   const T *av = Ops<T>::vals(a);
   if (PK == PK_ISAI) {
     const DevCsr &m1 = isai_first(p);
     const bool two = !p.mm.rowptr;
-    T *p_old = parity ? v.p2 : v.p, *p_new = parity ? v.p : v.p2;
+    // q (= p - omega v, written by K5) lives in v.p, which k_init zeroes;
+    // p itself in v.p2
     T *y1 = two ? v.tmp : v.ph;
     hipLaunchKernelGGL(k_p_spmv<T>, dim3(m1.n_blocks), blk, 0, b->stream, m1.rowptr, m1.col,
-                       Ops<T>::vals(m1), m1.blocks, v.r, p_old, v.v, p_new, y1, st, pp);
+                       Ops<T>::vals(m1), m1.blocks, v.r, v.p, v.p2, y1, st, pp);
     if (two) Ops<T>::spmv(b, p.mu, v.tmp, v.ph, st, 1);
     hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks,
                        v.ph, v.v, v.rh, st, pp);
@@ -693,20 +696,20 @@ static void enqueue_iteration(ohm_backend *b, const DevCsr &a, const PrecondDev 
                      v.t, v.s, st, pp);
   if (pp.f_ts) fold_launch<2>(b, pp.ts, a.n_blocks);
   hipLaunchKernelGGL(k_xr_update<T>, gv, blk, 0, b->stream, n, v.x, v.r, v.rh, v.ph, v.sh, v.s,
-                     v.t, st, pp);
+                     v.t, v.p2, v.v, PK == PK_ISAI ? v.p : nullptr, st, pp);
   if (pp.f_rr) fold_launch<2>(b, pp.rr, (int)gv.x);
   b->dispatches += 5;
 }
 
 template <typename T>
 static void enqueue_any(ohm_backend *b, const DevCsr &a, const PrecondDev &p, BicgVecs<T> &v,
-                        SolveState *st, int n, const Parts &pp, int parity) {
+                        SolveState *st, int n, const Parts &pp) {
   if (p.kind == PK_JACOBI)
-    enqueue_iteration<T, PK_JACOBI>(b, a, p, v, st, n, pp, parity);
+    enqueue_iteration<T, PK_JACOBI>(b, a, p, v, st, n, pp);
   else if (p.kind == PK_ISAI)
-    enqueue_iteration<T, PK_ISAI>(b, a, p, v, st, n, pp, parity);
+    enqueue_iteration<T, PK_ISAI>(b, a, p, v, st, n, pp);
   else
-    enqueue_iteration<T, PK_NONE>(b, a, p, v, st, n, pp, parity);
+    enqueue_iteration<T, PK_NONE>(b, a, p, v, st, n, pp);
 }
 
 static void enqueue_check(ohm_backend *b, SolveState *st, const Parts &pp) {
@@ -715,7 +718,6 @@ static void enqueue_check(ohm_backend *b, SolveState *st, const Parts &pp) {
 }
 
 constexpr int kGraphIters = 8;
-static_assert(kGraphIters % 2 == 0, "each replay must start on p buffer 0");
 
 void graph_cache_clear(ohm_backend *b) {
   for (auto &g : b->graphs) {
@@ -750,7 +752,7 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
     GraphEntry e;
     e.key = key;
     OHM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-    for (int k = 0; k < kGraphIters; k++) enqueue_any<T>(b, a, p, v, st, n, pp, k & 1);
+    for (int k = 0; k < kGraphIters; k++) enqueue_any<T>(b, a, p, v, st, n, pp);
     enqueue_check(b, st, pp);
     OHM_HIP(hipStreamEndCapture(b->stream, &e.graph));
     OHM_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
@@ -816,7 +818,7 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
       chunk = reps * kGraphIters;
     } else {
       for (int k = 0; k < chunk; k++)
-        enqueue_any<T>(b, a, p, v, st, n, pp, (int)((enqueued + k) & 1));
+        enqueue_any<T>(b, a, p, v, st, n, pp);
       enqueue_check(b, st, pp);
     }
     enqueued += chunk;
