@@ -799,13 +799,22 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   b->dispatches += 2;
   StatusWord *h = reinterpret_cast<StatusWord *>(b->h_pinned);
   int64_t enqueued = 0;
-  for (;;) {
-    OHM_HIP(hipMemcpyAsync(h, &st->status, sizeof(StatusWord), hipMemcpyDeviceToHost, b->stream));
-    OHM_HIP(hipStreamSynchronize(b->stream));
-    b->readbacks++;
-    out.polls++;
-    if (h->status != ST_RUNNING) break;
-    int64_t done = h->iters;
+  // The first chunk goes out behind k_init without a poll: the rare solve
+  // that k_init ends (b = 0, rho breakdown; under OHM_TOL_INITIAL a warm
+  // start only once ||r0|| is below the floor) turns every kernel of the
+  // chunk into a no-op at its head. A warm start against tol * ||b|| polls
+  // first, since k_init ends it whenever x already meets that tolerance.
+  for (bool poll = o.warm && o.tol_mode != OHM_TOL_INITIAL;; poll = true) {
+    int64_t done = 0;
+    if (poll) {
+      OHM_HIP(hipMemcpyAsync(h, &st->status, sizeof(StatusWord), hipMemcpyDeviceToHost,
+                             b->stream));
+      OHM_HIP(hipStreamSynchronize(b->stream));
+      b->readbacks++;
+      out.polls++;
+      if (h->status != ST_RUNNING) break;
+      done = h->iters;
+    }
     int chunk = o.check_interval > 0
                     ? o.check_interval
                     : (int)std::min<int64_t>(32, std::max<int64_t>(4, done / 2));
