@@ -717,7 +717,11 @@ static void enqueue_check(ohm_backend *b, SolveState *st, const Parts &pp) {
   b->dispatches++;
 }
 
-constexpr int kGraphIters = 8;
+// Captured chunk lengths: short chunks while a solve is young (most Newton
+// sub-solves end within a few iterations, and a replay runs to its end even
+// when the solve stops inside it), long ones once the polled chunk reaches
+// kGraphIters.
+constexpr int kGraphItersShort = 4, kGraphIters = 8;
 
 void graph_cache_clear(ohm_backend *b) {
   for (auto &g : b->graphs) {
@@ -727,12 +731,12 @@ void graph_cache_clear(ohm_backend *b) {
   b->graphs.clear();
 }
 
-// Replays `reps` launches of a captured kGraphIters-iteration chunk. The key
+// Replays `reps` launches of a captured `iters`-iteration chunk. The key
 // holds every pointer and launch size baked into the captured kernels.
 template <typename T>
 static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
                                BicgVecs<T> &v, SolveState *st, int n, const Parts &pp,
-                               int reps) {
+                               int iters, int reps) {
   std::vector<uintptr_t> key = {
       sizeof(T), (uintptr_t)p.kind, (uintptr_t)n, (uintptr_t)a.rowptr, (uintptr_t)a.col,
       (uintptr_t)Ops<T>::vals(a), (uintptr_t)a.blocks, (uintptr_t)a.n_blocks,
@@ -741,7 +745,8 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
       (uintptr_t)Ops<T>::vals(p.mu), (uintptr_t)p.mu.blocks, (uintptr_t)p.mu.n_blocks,
       (uintptr_t)p.mm.rowptr, (uintptr_t)Ops<T>::vals(p.mm), (uintptr_t)p.mm.blocks,
       (uintptr_t)p.mm.n_blocks,
-      (uintptr_t)v.x, (uintptr_t)v.tmp, (uintptr_t)st, (uintptr_t)b->d_partials, kGraphIters};
+      (uintptr_t)v.x, (uintptr_t)v.tmp, (uintptr_t)st, (uintptr_t)b->d_partials,
+      (uintptr_t)iters};
   hipGraphExec_t exec = nullptr;
   for (auto &g : b->graphs)
     if (g.key == key) exec = g.exec;
@@ -752,7 +757,7 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
     GraphEntry e;
     e.key = key;
     OHM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-    for (int k = 0; k < kGraphIters; k++) enqueue_any<T>(b, a, p, v, st, n, pp);
+    for (int k = 0; k < iters; k++) enqueue_any<T>(b, a, p, v, st, n, pp);
     enqueue_check(b, st, pp);
     OHM_HIP(hipStreamEndCapture(b->stream, &e.graph));
     OHM_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
@@ -762,7 +767,7 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
     exec = e.exec;
   }
   for (int r = 0; r < reps; r++) OHM_HIP(hipGraphLaunch(exec, b->stream));
-  b->dispatches += (per_iter * kGraphIters + 1) * reps;
+  b->dispatches += (per_iter * iters + 1) * reps;
   return OHM_OK;
 }
 
@@ -820,11 +825,12 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
                     : (int)std::min<int64_t>(32, std::max<int64_t>(4, done / 2));
     chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, o.max_iters - enqueued));
     if (b->use_graphs) {
-      // replay a captured 8-iteration chunk; surplus iterations are no-ops
-      // once the state leaves RUNNING
-      int reps = (chunk + kGraphIters - 1) / kGraphIters;
-      OHM_TRY(replay_chunk<T>(b, a, p, v, st, n, pp, reps));
-      chunk = reps * kGraphIters;
+      // replay captured chunks; surplus iterations are no-ops once the
+      // state leaves RUNNING
+      const int gi = chunk < kGraphIters ? kGraphItersShort : kGraphIters;
+      int reps = (chunk + gi - 1) / gi;
+      OHM_TRY(replay_chunk<T>(b, a, p, v, st, n, pp, gi, reps));
+      chunk = reps * gi;
     } else {
       for (int k = 0; k < chunk; k++)
         enqueue_any<T>(b, a, p, v, st, n, pp);
