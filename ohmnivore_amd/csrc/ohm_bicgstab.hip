@@ -717,10 +717,11 @@ static void enqueue_check(ohm_backend *b, SolveState *st, const Parts &pp) {
   b->dispatches++;
 }
 
-// Captured chunk lengths: short chunks while a solve is young (most Newton
-// sub-solves end within a few iterations, and a replay runs to its end even
-// when the solve stops inside it), long ones once the polled chunk reaches
-// kGraphIters.
+// Captured chunk lengths: an inexact-Newton sub-solve (tolerance relative to
+// its own starting residual) opens with a short chunk, since most of them end
+// within a few iterations and a replay runs to its end even when the solve
+// stops inside it; every later chunk, and every other solve, replays long
+// chunks.
 constexpr int kGraphItersShort = 4, kGraphIters = 8;
 
 void graph_cache_clear(ohm_backend *b) {
@@ -827,7 +828,9 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
     if (b->use_graphs) {
       // replay captured chunks; surplus iterations are no-ops once the
       // state leaves RUNNING
-      const int gi = chunk < kGraphIters ? kGraphItersShort : kGraphIters;
+      const bool newton_sub = o.tol_mode == OHM_TOL_INITIAL || o.forcing.kind != 0;
+      const int gi =
+          newton_sub && enqueued == 0 && chunk < kGraphIters ? kGraphItersShort : kGraphIters;
       int reps = (chunk + gi - 1) / gi;
       OHM_TRY(replay_chunk<T>(b, a, p, v, st, n, pp, gi, reps));
       chunk = reps * gi;
