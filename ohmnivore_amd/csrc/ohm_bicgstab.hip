@@ -823,7 +823,7 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
     }
     int chunk = o.check_interval > 0
                     ? o.check_interval
-                    : (int)std::min<int64_t>(32, std::max<int64_t>(4, done / 2));
+                    : (int)std::min<int64_t>(16, std::max<int64_t>(4, done / 2));
     chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, o.max_iters - enqueued));
     if (b->use_graphs) {
       // replay captured chunks; surplus iterations are no-ops once the
