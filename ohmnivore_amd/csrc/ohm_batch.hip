@@ -1238,7 +1238,7 @@ ohm_status core_solve(ohm_batch *bt, BW<S> &w, const std::vector<char> &run, con
     }
     int chunk = o.check_interval > 0
                     ? o.check_interval
-                    : (int)std::min<int64_t>(32, std::max<int64_t>(4, min_iters / 2));
+                    : (int)std::min<int64_t>(16, std::max<int64_t>(4, min_iters / 2));
     for (int c = 0; c < chunk; c++) {
       if (pk == PK_JACOBI)
         enqueue_iter<S, PK_JACOBI>(bt, w, (int)tab.size());
