@@ -849,16 +849,16 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
       fc.lin_prev = bo.res;
       fc.eta_prev = bo.eta;
     }
-    // the assembled-value NaN flag (src/solver/newton.rs:258-263)
-    OHM_HIP(hipMemcpyAsync(&h->nan, c->asm_nan, sizeof(int), hipMemcpyDeviceToHost, st));
-    OHM_HIP(hipStreamSynchronize(st));
-    if (h->nan) {
-      rc = fail(OHM_ERR_NEWTON_NUMERICAL, "Newton solver encountered numerical error at iteration %lld",
-                (long long)it);
-      break;
-    }
+    // the assembled-value NaN flag (src/solver/newton.rs:258-263): k_converge
+    // folds it into the status word read below; a failed sub-solve reads it
+    // here, since a NaN in the assembly is then the error to report
     if (ls != OHM_OK) {
-      rc = ls;
+      OHM_HIP(hipMemcpyAsync(&h->nan, c->asm_nan, sizeof(int), hipMemcpyDeviceToHost, st));
+      OHM_HIP(hipStreamSynchronize(st));
+      rc = h->nan ? fail(OHM_ERR_NEWTON_NUMERICAL,
+                         "Newton solver encountered numerical error at iteration %lld",
+                         (long long)it)
+                  : ls;
       break;
     }
     OHM_HIP(hipMemsetAsync(&c->word->limited, 0, sizeof(int), st));
