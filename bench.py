@@ -428,7 +428,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["headline", "all"], default="all")
+    ap.add_argument("--only", choices=["headline", "all", "ac"], default="all")
     ap.add_argument("--tran-steps", type=int, default=1000)
     ap.add_argument("--mc-instances", type=int, default=8)
     ap.add_argument("--mc-steps", type=int, default=50)
@@ -449,6 +449,9 @@ def main() -> int:
     from ohmnivore_amd import solver
 
     be = solver.HipBackend()
+    if args.only == "ac":
+        print(json.dumps(rlc_ac(be, N, ir, solver, args.ac_circuits)))
+        return 0
     if args.roofline_only:
         return roofline_only(be, N, ir, solver)
     if args.profile:

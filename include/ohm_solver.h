@@ -261,6 +261,42 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits,
                                  const ohm_bicgstab_params *params, int32_t precond,
                                  ohm_batch_result *results, double *device_ms);
 
+/* ---- The direct AC sweep split into its symbolic and numeric halves (the
+ * reference re-forms and re-solves every (circuit, frequency) system from
+ * scratch, src/analysis/ac.rs:56-70; a sparse direct solver's analyse /
+ * factor split, as faer's sp_lu has, src/solver/sparse_direct.rs:39-47).
+ * ohm_ac_plan_create: per circuit the union pattern of G and C, a reverse
+ * Cuthill-McKee ordering and the band rows of G and C in that order, built
+ * on host threads and uploaded once (every circuit's half bandwidth after
+ * RCM must be at most 4; the circuits' x fields are ignored).
+ * ohm_ac_plan_solve: every (circuit, frequency) system by banded LU with
+ * partial pivoting on the device (at half bandwidth 1 each system's rows
+ * are cut into parts eliminated in parallel, joined through a reduced
+ * system of the parts' boundary unknowns); x (may be NULL, else n_circuits pointers,
+ * each NULL or n_freqs * 2n doubles) receives solutions; results
+ * (n_circuits * n_freqs, circuit-major) the FP64 residual of every system.
+ * When the sweep fits the device in one chunk its solutions stay resident:
+ * ohm_ac_plan_get_x copies frequencies [f_first, f_first + f_count) of one
+ * circuit (f_count * 2n doubles) until the next sweep on the backend. */
+typedef struct ohm_ac_plan ohm_ac_plan;
+ohm_status ohm_ac_plan_create(ohm_backend *b, int64_t n_circuits, const ohm_ac_circuit *circuits,
+                              ohm_ac_plan **out);
+ohm_status ohm_ac_plan_solve(ohm_ac_plan *plan, int64_t n_freqs, const double *freqs,
+                             double *const *x, ohm_batch_result *results, double *device_ms);
+ohm_status ohm_ac_plan_get_x(ohm_ac_plan *plan, int64_t circuit, int64_t f_first,
+                             int64_t f_count, double *x);
+/* Figures of the last solve (no counterpart in the reference; for
+ * benchmarks and tests): phase_ms (6 doubles, HIP events on the backend's
+ * stream) = part relations, reduced boundary systems, part solves (the
+ * whole one-lane LU when unpartitioned), residuals, one-lane redo of
+ * systems whose partitioned residual exceeded 1e-12, unpermute; the rows
+ * per part (0: one lane per whole system; picked by the autotuner on the
+ * plan's first solve unless OHM_AC_PART_ROWS is set), the systems redone,
+ * and the chunks the sweep needed. Any pointer may be NULL. */
+ohm_status ohm_ac_plan_stats(const ohm_ac_plan *plan, double *phase_ms, int64_t *rows_per_part,
+                             int64_t *fallback_systems, int32_t *chunks);
+void ohm_ac_plan_free(ohm_ac_plan *plan);
+
 /* generate_frequencies (src/analysis/ac.rs:80-109). Writes up to `cap`
  * points and returns how many the sweep has. */
 enum { OHM_SWEEP_DEC = 0, OHM_SWEEP_OCT = 1, OHM_SWEEP_LIN = 2 };
@@ -368,6 +404,10 @@ typedef struct {
 } ohm_newton_result;
 
 void ohm_newton_default_params(ohm_newton_params *p);
+/* bicgstab_iters_per_newton (ohm_newton_solve, ohm_newton_run): NULL, or an
+ * array of at least params->max_iterations int32 entries; entry k receives
+ * the BiCGSTAB iterations of Newton iteration k (Stats, newton.rs:430-435),
+ * and entries past result->iterations are left untouched. */
 ohm_status ohm_newton_solve(ohm_backend *b, const double *base_values,
                             const uint32_t *col_indices, const uint32_t *row_pointers,
                             const double *base_b, const ohm_diode_desc *diodes, size_t n_diodes,

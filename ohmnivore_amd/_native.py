@@ -132,7 +132,8 @@ def lib() -> ctypes.CDLL:
         for name in ("ohm_backend_destroy", "ohm_buffer_free", "ohm_matrix_free",
                      "ohm_precond_free", "ohm_newton_free", "ohm_mna_free",
                      "ohm_bicgstab_default_params", "ohm_newton_default_params",
-                     "ohm_batch_free", "ohm_transient_free", "ohm_tran_default_params"):
+                     "ohm_batch_free", "ohm_transient_free", "ohm_tran_default_params",
+                     "ohm_ac_plan_free"):
             getattr(L, name).restype = None
         _lib = L
     return _lib

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -57,10 +58,12 @@ struct BandWave {
   const double *b;         // permuted right-hand side, (re, im) pairs
   const uint32_t *perm;    // permuted row i holds original unknown perm[i]
   const double *omega;     // omega of lane 0 (consecutive lanes follow)
-  int64_t store_off;       // complex offset of this wave's U / Y region (per lane-row)
+  int64_t store_off;       // lane-row offset of this wave's U / Y region (U: W entries per lane-row)
   int64_t out_off;         // complex offset of lane 0's x in the output (lanes n apart)
   int nf;                  // active lanes
   int sys;                 // result slot of lane 0
+  int pbase;               // partitioned solve: index of the wave's first part
+  int nparts;              //   and its number of parts
 };
 
 constexpr int kBandTile = 64;
@@ -98,6 +101,7 @@ __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ wav
   const int64_t n = q.n;
   cz *Ub = U + q.store_off * W;
   cz *Yb = Y + q.store_off;
+  const int S = q.nf;  // lane stride of the wave's rows
   cz win[B + 1][W], y[B + 1];
 #pragma unroll
   for (int r = 0; r <= B; r++) {
@@ -164,8 +168,8 @@ __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ wav
       }
       if (live) {
 #pragma unroll
-        for (int c = 0; c < W; c++) Ub[(j * W + c) * 64 + lane] = win[0][c];
-        Yb[j * 64 + lane] = y[0];
+        for (int c = 0; c < W; c++) Ub[(j * W + c) * S + lane] = win[0][c];
+        Yb[j * S + lane] = y[0];
       }
 #pragma unroll
       for (int r = 0; r < B; r++) {
@@ -200,8 +204,8 @@ __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ wav
       const int64_t j = hi - g;
       if (j >= 0) {
 #pragma unroll
-        for (int c = 0; c < W; c++) u[g][c] = Ub[(j * W + c) * 64 + lane];
-        yy[g] = Yb[j * 64 + lane];
+        for (int c = 0; c < W; c++) u[g][c] = Ub[(j * W + c) * S + lane];
+        yy[g] = Yb[j * S + lane];
       }
     }
 #pragma unroll
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ wav
 #pragma unroll
         for (int c = 1; c < W; c++) acc = acc - u[g][c] * xw[c - 1];
         const cz xj = acc * cinv(u[g][0]);
-        Yb[j * 64 + lane] = xj;
+        Yb[j * S + lane] = xj;
 #pragma unroll
         for (int k = W - 2; k > 0; k--) xw[k] = xw[k - 1];
         xw[0] = xj;
@@ -232,13 +236,15 @@ __global__ __launch_bounds__(256) void k_band_resid(const BandWave *__restrict__
                                                     double *__restrict__ partial) {
   constexpr int W = 2 * B + 1;
   const int lane = threadIdx.x & 63, slot = threadIdx.x >> 6;
-  const BandWave q = waves[blockIdx.y];
+  const int wave = blockIdx.x / kResChunks, chunk = blockIdx.x % kResChunks;
+  const BandWave q = waves[wave];
   const bool live = lane < q.nf;
   const double om = live ? q.omega[lane] : 0.0;
   const cz *Yb = Y + q.store_off;
+  const int S = q.nf;
   double rr = 0.0, bb = 0.0;
   if (live)
-    for (int64_t i = (int64_t)blockIdx.x * 4 + slot; i < q.n; i += (int64_t)gridDim.x * 4) {
+    for (int64_t i = (int64_t)chunk * 4 + slot; i < q.n; i += (int64_t)kResChunks * 4) {
       cz acc = {q.b[2 * i], q.b[2 * i + 1]};
       bb += cabs2(acc);
 #pragma unroll
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(256) void k_band_resid(const BandWave *__restrict__
         const int64_t col = i - B + d;
         if (col >= 0 && col < q.n) {
           const cz a = {q.gb[i * W + d], om * q.cb[i * W + d]};
-          acc = acc - a * Yb[col * 64 + lane];
+          acc = acc - a * Yb[col * S + lane];
         }
       }
       rr += cabs2(acc);
@@ -261,7 +267,7 @@ __global__ __launch_bounds__(256) void k_band_resid(const BandWave *__restrict__
       a += s_rr[k][lane];
       c += s_bb[k][lane];
     }
-    const size_t o = ((size_t)blockIdx.y * 64 + lane) * kResChunks + blockIdx.x;
+    const size_t o = ((size_t)wave * 64 + lane) * kResChunks + chunk;
     partial[2 * o] = a;
     partial[2 * o + 1] = c;
   }
@@ -284,16 +290,400 @@ __global__ void k_band_resid_sum(const BandWave *__restrict__ waves, int n_waves
 }
 
 // x back to the original unknown order: out[out_off + lane n + perm[i]] = x_i
+// grid: 64 workgroups of 256 threads per wave
 __global__ void k_band_unpermute(const BandWave *__restrict__ waves, const cz *__restrict__ Y,
                                  cz *__restrict__ out) {
   const int lane = threadIdx.x & 63;
-  const BandWave q = waves[blockIdx.y];
+  const int wave = blockIdx.x / 64, part = blockIdx.x % 64;
+  const BandWave q = waves[wave];
   if (lane >= q.nf) return;
   const cz *Yb = Y + q.store_off;
   const int rows_per = blockDim.x / 64;
-  for (int64_t i = blockIdx.x * rows_per + (threadIdx.x >> 6); i < q.n;
-       i += (int64_t)gridDim.x * rows_per)
-    out[q.out_off + (int64_t)lane * q.n + q.perm[i]] = Yb[i * 64 + lane];
+  for (int64_t i = part * rows_per + (threadIdx.x >> 6); i < q.n; i += (int64_t)64 * rows_per)
+    out[q.out_off + (int64_t)lane * q.n + q.perm[i]] = Yb[i * q.nf + lane];
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned solve of systems of half bandwidth 1 (the RLC ladders after
+// RCM). One lane per system leaves the AC workload with about 400 waves for
+// 25,856 systems, each walking up to 190k dependent rows; here the rows of
+// every system are cut into parts of R rows, eliminated independently:
+//   phase 1 (k_part_rel): every part eliminated top-down and, in a second
+//     wave, bottom-up (LU with partial pivoting of the row-reversed part),
+//     its couplings to the neighbouring parts' boundary unknowns carried as
+//     two extra right-hand sides. Nothing is stored but, per part and
+//     direction, the relation
+//         x_end = alpha - k_prev x_prev - k_next x_next
+//     for its last (top-down) or first (bottom-up) unknown, where x_prev is
+//     the previous part's last unknown and x_next the next part's first.
+//   phase 2 (k_part_reduce): per system, the 2P boundary unknowns from those
+//     relations: a banded system of half bandwidth 2, LU with partial
+//     pivoting, one lane per system.
+//   phase 3 (k_part_solve): every part eliminated again top-down with the
+//     now known boundary values moved to its right-hand side, its U rows
+//     stored and back-substituted, x written where the one-lane solve
+//     writes it.
+// Phase 1 reads only the band rows shared by the wave's lanes, so the bytes
+// stored and read per (system, row) are those of the one-lane solve.
+// Systems whose phase-3 residual is not small (a part block close to
+// singular) are solved again by k_band_lu.
+constexpr int kPartW = 3;  // band width at half bandwidth 1
+
+struct PartTile {
+  double g[kBandTile][kPartW], c[kBandTile][kPartW], b[kBandTile][2];
+};
+
+// direction-order rows k0 .. k0+63 of part [s, s+m) (rev: row k is s+m-1-k,
+// band entries mirrored) into this lane's registers
+__device__ __forceinline__ void part_row_load(const BandWave &q, int64_t s, int64_t m, bool rev,
+                                              int64_t k, double (&rg)[kPartW],
+                                              double (&rc)[kPartW], double (&rb)[2]) {
+  if (k < m) {
+    const int64_t i = rev ? s + m - 1 - k : s + k;
+#pragma unroll
+    for (int d = 0; d < kPartW; d++) {
+      const int dd = rev ? kPartW - 1 - d : d;
+      rg[d] = q.gb[i * kPartW + dd];
+      rc[d] = q.cb[i * kPartW + dd];
+    }
+    rb[0] = q.b[2 * i];
+    rb[1] = q.b[2 * i + 1];
+  } else {
+#pragma unroll
+    for (int d = 0; d < kPartW; d++) rg[d] = rc[d] = 0.0;
+    rb[0] = rb[1] = 0.0;
+  }
+}
+
+// Eliminates the part [s, s+m) in direction order for one lane. NR = 3:
+// right-hand sides f, the start coupling (direction row 0's entry left of
+// the part) and the end coupling (direction row m-1's entry right of it);
+// returns the last pivot row's y and diagonal. NR = 1: xs / xe are the
+// known unknowns beyond the start / end, moved to f; with STORE (top-down
+// only) the U rows and y go to Ub / Yb at their global rows.
+template <int NR, bool STORE>
+__device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int64_t m, bool rev,
+                                               double om, bool live, cz xs, cz xe, cz *Ub, cz *Yb,
+                                               int S, PartTile &t, cz (&ly)[NR], cz &lu,
+                                               int &bad) {
+  constexpr int W = kPartW;
+  const int lane = threadIdx.x;
+  cz win[2][W], y[2][NR];
+  double rg[W], rc[W], rb[2];
+  // direction rows 0 and 1
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    part_row_load(q, s, m, rev, r, rg, rc, rb);
+#pragma unroll
+    for (int c = 0; c < W; c++) win[r][c] = cz{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NR; k++) y[r][k] = cz{0.0, 0.0};
+    if (r < m) {
+      y[r][0] = cz{rb[0], rb[1]};
+#pragma unroll
+      for (int d = 0; d < W; d++) {
+        const cz a = {rg[d], om * rc[d]};
+        const int64_t col = r - 1 + d;  // direction column
+        if (col < 0) {
+          if (NR == 3) y[r][1] = a;
+          else y[r][0] = y[r][0] - a * xs;
+        } else if (col >= m) {
+          if (NR == 3) y[r][NR - 1] = a;
+          else y[r][0] = y[r][0] - a * xe;
+        } else {
+          win[r][col] = a;
+        }
+      }
+    }
+  }
+  part_row_load(q, s, m, rev, 2 + lane, rg, rc, rb);
+  for (int64_t t0 = 0; t0 < m; t0 += kBandTile) {
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < W; d++) {
+      t.g[lane][d] = rg[d];
+      t.c[lane][d] = rc[d];
+    }
+    t.b[lane][0] = rb[0];
+    t.b[lane][1] = rb[1];
+    __syncthreads();
+    part_row_load(q, s, m, rev, 2 + t0 + kBandTile + lane, rg, rc, rb);  // next tile in flight
+    const int64_t t1 = t0 + kBandTile < m ? t0 + kBandTile : m;
+    for (int64_t j = t0; j < t1; j++) {
+      const double a0 = cabs2(win[0][0]), a1 = cabs2(win[1][0]);
+      const bool sw = j + 1 < m && a1 > a0;
+      if (!((sw ? a1 : a0) > 0.0)) bad = 1;
+      if (sw) {
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+          const cz u = win[0][c];
+          win[0][c] = win[1][c];
+          win[1][c] = u;
+        }
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+          const cz u = y[0][k];
+          y[0][k] = y[1][k];
+          y[1][k] = u;
+        }
+      }
+      const cz l = win[1][0] * cinv(win[0][0]);
+#pragma unroll
+      for (int c = 1; c < W; c++) win[1][c] = win[1][c] - l * win[0][c];
+#pragma unroll
+      for (int k = 0; k < NR; k++) y[1][k] = y[1][k] - l * y[0][k];
+      if (STORE && live) {
+        const int64_t i = s + j;
+#pragma unroll
+        for (int c = 0; c < W; c++) Ub[(i * W + c) * S + lane] = win[0][c];
+        Yb[i * S + lane] = y[0][0];
+      }
+      if (j == m - 1) {
+        lu = win[0][0];
+#pragma unroll
+        for (int k = 0; k < NR; k++) ly[k] = y[0][k];
+      }
+      // shift; direction row j+2 enters (tile slot j - t0)
+#pragma unroll
+      for (int c = 0; c + 1 < W; c++) win[0][c] = win[1][c + 1];
+      win[0][W - 1] = cz{0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < NR; k++) {
+        y[0][k] = y[1][k];
+        y[1][k] = cz{0.0, 0.0};
+      }
+      const int64_t kr = j + 2;
+      const int sl = (int)(j - t0);
+#pragma unroll
+      for (int c = 0; c < W; c++) win[1][c] = cz{0.0, 0.0};
+      if (kr < m) {
+        y[1][0] = cz{t.b[sl][0], t.b[sl][1]};
+#pragma unroll
+        for (int d = 0; d < W; d++) {
+          const cz a = {t.g[sl][d], om * t.c[sl][d]};
+          if (j + 1 + d >= m) {  // right of the part: only row m-1, d = 2
+            if (NR == 3) y[1][NR - 1] = a;
+            else y[1][0] = y[1][0] - a * xe;
+          } else {
+            win[1][d] = a;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Phase 1: blockIdx.x = 2 ip + direction; relation of (part ip, direction)
+// to rel[((ip * 2 + dir) * 3 + k) * 64 + lane], k = alpha, k_prev, k_next.
+__global__ __launch_bounds__(64) void k_part_rel(const BandWave *__restrict__ waves,
+                                                 const int *__restrict__ ip_item, int64_t R,
+                                                 cz *__restrict__ rel, int *__restrict__ singular) {
+  __shared__ PartTile t;
+  const int lane = threadIdx.x;
+  const int ip = blockIdx.x >> 1, dir = blockIdx.x & 1;
+  const BandWave q = waves[ip_item[ip]];
+  const int p = ip - q.pbase;
+  const int64_t s = (int64_t)p * R, m = min(R, q.n - s);
+  const bool live = lane < q.nf;
+  const double om = live ? q.omega[lane] : 0.0;
+  cz ly[3], lu = {1.0, 0.0};
+  int bad = 0;
+  part_eliminate<3, false>(q, s, m, dir == 1, om, live, cz{0.0, 0.0}, cz{0.0, 0.0}, nullptr,
+                           nullptr, q.nf, t, ly, lu, bad);
+  if (!live) return;
+  if (bad || !(cabs2(lu) > 0.0)) {
+    singular[q.sys + lane] = 2;
+    lu = cz{1.0, 0.0};
+  }
+  const cz iu = cinv(lu);
+  // top-down: start = prev, end = next; bottom-up: start = next, end = prev
+  const cz alpha = ly[0] * iu, ks = ly[1] * iu, ke = ly[2] * iu;
+  cz *o = rel + (size_t)(ip * 2 + dir) * 3 * 64 + lane;
+  o[0] = alpha;
+  o[64] = dir == 0 ? ks : ke;
+  o[128] = dir == 0 ? ke : ks;
+}
+
+// Phase 2: one lane per system. Boundary unknowns F_p = x(first row of part
+// p) at index 2p and L_p = x(last row) at 2p+1; their equations
+//   F_p + k_prev L_{p-1} + k_next F_{p+1} = alpha   (bottom-up relation)
+//   L_p + k_prev L_{p-1} + k_next F_{p+1} = alpha   (top-down relation)
+// form a band of half bandwidth 2; solved by LU with partial pivoting (U rows
+// of 5 entries and y to red, x to bnd[(2 pbase + r) * 64 + lane]).
+__global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__ waves,
+                                                    const cz *__restrict__ rel,
+                                                    cz *__restrict__ red, cz *__restrict__ bnd,
+                                                    int *__restrict__ singular) {
+  constexpr int KB = 2, W = 2 * KB + 1;
+  const int lane = threadIdx.x;
+  const BandWave q = waves[blockIdx.x];
+  if (lane >= q.nf) return;
+  const int P = q.nparts, n = 2 * P;
+  const size_t rb = (size_t)2 * q.pbase;
+  // band row r of the reduced system: entries for columns r-2 .. r+2
+  auto row = [&](int r, cz (&a)[W], cz &f) {
+#pragma unroll
+    for (int d = 0; d < W; d++) a[d] = cz{0.0, 0.0};
+    f = cz{0.0, 0.0};
+    if (r >= n) return;
+    const int p = r >> 1, dir = (r & 1) ? 0 : 1;
+    const cz *o = rel + (size_t)((q.pbase + p) * 2 + dir) * 3 * 64 + lane;
+    f = o[0];
+    const cz kp = p > 0 ? o[64] : cz{0.0, 0.0}, kn = p + 1 < P ? o[128] : cz{0.0, 0.0};
+    if (dir == 1) {  // F_p: L_{p-1} at d 1, F_p at d 2, F_{p+1} at d 4
+      a[1] = kp;
+      a[2] = cz{1.0, 0.0};
+      a[4] = kn;
+    } else {         // L_p: L_{p-1} at d 0, L_p at d 2, F_{p+1} at d 3
+      a[0] = kp;
+      a[2] = cz{1.0, 0.0};
+      a[3] = kn;
+    }
+  };
+  cz win[KB + 1][W], y[KB + 1];
+#pragma unroll
+  for (int r = 0; r <= KB; r++) {
+    cz a[W], f;
+    row(r, a, f);
+    y[r] = f;
+#pragma unroll
+    for (int c = 0; c < W; c++) win[r][c] = cz{0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < W; d++) {
+      const int col = r - KB + d;
+      if (col >= 0 && col < W) win[r][col] = a[d];
+    }
+  }
+  int bad = 0;
+  for (int j = 0; j < n; j++) {
+    int pv = 0;
+    double best = cabs2(win[0][0]);
+#pragma unroll
+    for (int r = 1; r <= KB; r++) {
+      const double v = cabs2(win[r][0]);
+      if (j + r < n && v > best) {
+        best = v;
+        pv = r;
+      }
+    }
+    if (!(best > 0.0)) bad = 1;
+#pragma unroll
+    for (int r = 1; r <= KB; r++)
+      if (pv == r) {
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+          const cz u = win[0][c];
+          win[0][c] = win[r][c];
+          win[r][c] = u;
+        }
+        const cz u = y[0];
+        y[0] = y[r];
+        y[r] = u;
+      }
+    const cz pinv = best > 0.0 ? cinv(win[0][0]) : cz{0.0, 0.0};
+#pragma unroll
+    for (int r = 1; r <= KB; r++) {
+      const cz l = win[r][0] * pinv;
+#pragma unroll
+      for (int c = 1; c < W; c++) win[r][c] = win[r][c] - l * win[0][c];
+      y[r] = y[r] - l * y[0];
+    }
+    cz *u = red + ((rb + j) * (W + 1)) * 64 + lane;
+#pragma unroll
+    for (int c = 0; c < W; c++) u[c * 64] = win[0][c];
+    u[W * 64] = y[0];
+#pragma unroll
+    for (int r = 0; r < KB; r++) {
+#pragma unroll
+      for (int c = 0; c + 1 < W; c++) win[r][c] = win[r + 1][c + 1];
+      win[r][W - 1] = cz{0.0, 0.0};
+      y[r] = y[r + 1];
+    }
+    cz a[W], f;
+    row(j + 1 + KB, a, f);
+#pragma unroll
+    for (int d = 0; d < W; d++) win[KB][d] = a[d];
+    y[KB] = f;
+  }
+  cz xw[W - 1];
+#pragma unroll
+  for (int k = 0; k < W - 1; k++) xw[k] = cz{0.0, 0.0};
+  for (int j = n - 1; j >= 0; j--) {
+    const cz *u = red + ((rb + j) * (W + 1)) * 64 + lane;
+    cz acc = u[W * 64];
+#pragma unroll
+    for (int c = 1; c < W; c++) acc = acc - u[c * 64] * xw[c - 1];
+    const cz d = u[0];
+    const cz xj = cabs2(d) > 0.0 ? acc * cinv(d) : cz{0.0, 0.0};
+    bnd[(rb + j) * 64 + lane] = xj;
+#pragma unroll
+    for (int k = W - 2; k > 0; k--) xw[k] = xw[k - 1];
+    xw[0] = xj;
+  }
+  if (bad) singular[q.sys + lane] = 2;
+}
+
+// Phase 3: blockIdx.x = ip; the part's top-down elimination with its
+// boundary values known, U rows and y stored, then back substitution.
+__global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ waves,
+                                                   const int *__restrict__ ip_item, int64_t R,
+                                                   const cz *__restrict__ bnd, cz *__restrict__ U,
+                                                   cz *__restrict__ Y,
+                                                   int *__restrict__ singular) {
+  constexpr int W = kPartW;
+  __shared__ PartTile t;
+  const int lane = threadIdx.x;
+  const int ip = blockIdx.x;
+  const BandWave q = waves[ip_item[ip]];
+  const int p = ip - q.pbase;
+  const int64_t s = (int64_t)p * R, m = min(R, q.n - s);
+  const bool live = lane < q.nf;
+  const double om = live ? q.omega[lane] : 0.0;
+  const size_t rb = (size_t)2 * q.pbase;
+  cz xs = {0.0, 0.0}, xe = {0.0, 0.0};
+  if (live) {
+    if (p > 0) xs = bnd[(rb + 2 * (p - 1) + 1) * 64 + lane];
+    if (p + 1 < q.nparts) xe = bnd[(rb + 2 * (p + 1)) * 64 + lane];
+  }
+  const int S = q.nf;
+  cz *Ub = U + q.store_off * W;
+  cz *Yb = Y + q.store_off;
+  cz ly[1], lu;
+  int bad = 0;
+  part_eliminate<1, true>(q, s, m, false, om, live, xs, xe, Ub, Yb, S, t, ly, lu, bad);
+  if (!live) return;
+  // back substitution over the part's rows (columns past it carry no entries)
+  constexpr int G = 8;
+  cz xw[W - 1];
+#pragma unroll
+  for (int k = 0; k < W - 1; k++) xw[k] = cz{0.0, 0.0};
+  for (int64_t hi = s + m - 1; hi >= s; hi -= G) {
+    cz u[G][W], yy[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const int64_t j = hi - g;
+      if (j >= s) {
+#pragma unroll
+        for (int c = 0; c < W; c++) u[g][c] = Ub[(j * W + c) * S + lane];
+        yy[g] = Yb[j * S + lane];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const int64_t j = hi - g;
+      if (j >= s) {
+        cz acc = yy[g];
+#pragma unroll
+        for (int c = 1; c < W; c++) acc = acc - u[g][c] * xw[c - 1];
+        const cz xj = acc * cinv(u[g][0]);
+        Yb[j * S + lane] = xj;
+#pragma unroll
+        for (int k = W - 2; k > 0; k--) xw[k] = xw[k - 1];
+        xw[0] = xj;
+      }
+    }
+  }
+  if (bad) singular[q.sys + lane] = 2;
 }
 
 }  // namespace ohm
@@ -402,9 +792,641 @@ struct Prepared {
   std::vector<double> g, c;  // union-pattern values
 };
 
+// Symbolic preparation of one circuit: pattern union of G and C, RCM,
+// half bandwidth. Runs on a host thread; the message of a failure is kept.
+ohm_status prepare_circuit(const ohm_ac_circuit &q, Prepared &p, std::string &msg) {
+  p.n = q.n;
+  if (q.n == 0) return OHM_OK;
+  ohm_status s = csr_union(q.n, q.g_rowptr, q.g_col, q.g_val, q.c_val ? q.c_rowptr : nullptr,
+                           q.c_col, q.c_val, p.rp, p.cl, p.g, p.c);
+  if (s != OHM_OK) {
+    msg = ohm_last_error();
+    return s;
+  }
+  p.perm = rcm(q.n, p.rp, p.cl);
+  p.iperm.assign(q.n, 0);
+  for (int64_t i = 0; i < q.n; i++) p.iperm[p.perm[i]] = (uint32_t)i;
+  int64_t bw = 0;
+  for (int64_t r = 0; r < q.n; r++)
+    for (uint32_t k = p.rp[r]; k < p.rp[r + 1]; k++)
+      bw = std::max<int64_t>(bw, std::llabs((int64_t)p.iperm[r] - (int64_t)p.iperm[p.cl[k]]));
+  p.band = (int)std::min<int64_t>(bw, 1 << 20);
+  return OHM_OK;
+}
+
+ohm_status check_circuits(int64_t n_circuits, const ohm_ac_circuit *circ) {
+  for (int64_t c = 0; c < n_circuits; c++) {
+    const ohm_ac_circuit &q = circ[c];
+    if (q.n < 0 || q.n > 0x7fffffff) return fail(OHM_ERR_INVALID, "circuit %lld: bad size", (long long)c);
+    if (q.n == 0) continue;
+    if (!q.g_rowptr || !q.g_col || !q.g_val || !q.b_ac || (q.c_val && (!q.c_rowptr || !q.c_col)))
+      return fail(OHM_ERR_INVALID, "circuit %lld: null arrays", (long long)c);
+  }
+  return OHM_OK;
+}
+
+constexpr int kMaxDirectBand = 4;
+
+}  // namespace
+
+// The symbolic half of a banded AC sweep, resident on the device: per
+// circuit the band rows of G and C in RCM order (n (2B+1) doubles each), the
+// permuted right-hand side and the permutation. The last solve's wave table
+// and per-system results stay with it; its solutions stay in the backend's
+// AC workspace while ws_gen matches.
+struct ohm_ac_plan {
+  ohm_backend *b = nullptr;
+  int64_t nc = 0;
+  int B = 1;
+  std::vector<int64_t> n;
+  void *d_band = nullptr;
+  std::vector<const double *> gb, cb, bp;
+  std::vector<const uint32_t *> perm;
+  // last solve
+  int64_t nf = 0;
+  double *d_omega = nullptr;
+  int *d_sing = nullptr;
+  double *d_res = nullptr;
+  size_t sys_cap = 0, omega_cap = 0;
+  bool resident = false;
+  uint64_t ws_gen = 0;
+  std::vector<BandWave> waves;  // the resident chunk's wave table
+  size_t y_off = 0;             // byte offset of its solutions in the workspace
+  int64_t tuned_rows = -1;      // autotuned rows per part (0: one lane per system)
+  int64_t part_rows = 0;        // rows per part of the last solve
+  int chunks = 0;
+  int64_t fallback_systems = 0; // systems of the last solve redone in one lane
+  // device ms of the last solve: relations, reduced systems, part solves
+  // (or one-lane LU), residuals, one-lane redo, unpermute
+  double phase_ms[6] = {0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+void plan_release(ohm_ac_plan *p) {
+  if (!p) return;
+  if (backend_alive(p->b)) {
+    (void)hipSetDevice(p->b->device);
+    if (p->d_band) (void)hipFree(p->d_band);
+    if (p->d_omega) (void)hipFree(p->d_omega);
+    if (p->d_sing) (void)hipFree(p->d_sing);
+    if (p->d_res) (void)hipFree(p->d_res);
+    for (hipEvent_t e : p->ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  delete p;
+}
+
+ohm_status plan_build(ohm_backend *b, int64_t n_circuits, const ohm_ac_circuit *circ,
+                      std::vector<Prepared> &prep, ohm_ac_plan **out) {
+  std::unique_ptr<ohm_ac_plan, void (*)(ohm_ac_plan *)> p(new ohm_ac_plan, plan_release);
+  p->b = b;
+  p->nc = n_circuits;
+  p->n.resize(n_circuits);
+  int band = 0;
+  for (int64_t c = 0; c < n_circuits; c++) {
+    p->n[c] = circ[c].n;
+    if (circ[c].n == 0) continue;
+    if (prep[c].band > kMaxDirectBand)
+      return fail(OHM_ERR_INVALID,
+                  "circuit %lld: half bandwidth %d after RCM exceeds the direct solver's %d",
+                  (long long)c, prep[c].band, kMaxDirectBand);
+    band = std::max(band, prep[c].band);
+  }
+  const int B = band <= 1 ? 1 : band <= 2 ? 2 : 4;
+  const int W = 2 * B + 1;
+  p->B = B;
+  // one device allocation for every circuit's band data
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  std::vector<size_t> off(n_circuits + 1, 0);
+  for (int64_t c = 0; c < n_circuits; c++) {
+    const size_t n = (size_t)circ[c].n;
+    off[c + 1] = off[c] + 2 * al(sizeof(double) * n * W) + al(sizeof(double) * 2 * n) +
+                 al(sizeof(uint32_t) * n);
+  }
+  OHM_HIP(hipMalloc(&p->d_band, std::max<size_t>(off[n_circuits], 256)));
+  p->gb.assign(n_circuits, nullptr);
+  p->cb.assign(n_circuits, nullptr);
+  p->bp.assign(n_circuits, nullptr);
+  p->perm.assign(n_circuits, nullptr);
+  // band arrays in the RCM order, built on host threads 32 circuits at a
+  // time and uploaded from this thread
+  struct BandHost {
+    std::vector<double> gb, cb, bp;
+  };
+  hipStream_t st = b->stream;
+  for (int64_t c0 = 0; c0 < n_circuits; c0 += 32) {
+    const int64_t cn = std::min<int64_t>(32, n_circuits - c0);
+    std::vector<BandHost> hb(cn);
+    parallel_for(cn, [&](int64_t t) {
+      const int64_t c = c0 + t;
+      const Prepared &pr = prep[c];
+      const int64_t n = pr.n;
+      BandHost &h = hb[t];
+      h.gb.assign((size_t)n * W, 0.0);
+      h.cb.assign((size_t)n * W, 0.0);
+      h.bp.assign(2 * n, 0.0);
+      for (int64_t r = 0; r < n; r++) {
+        const int64_t i = pr.iperm[r];
+        for (uint32_t k = pr.rp[r]; k < pr.rp[r + 1]; k++) {
+          const int64_t d = (int64_t)pr.iperm[pr.cl[k]] - i + B;
+          h.gb[i * W + d] += pr.g[k];
+          h.cb[i * W + d] += pr.c[k];
+        }
+        h.bp[2 * i] = circ[c].b_ac[2 * r];
+        h.bp[2 * i + 1] = circ[c].b_ac[2 * r + 1];
+      }
+    });
+    for (int64_t t = 0; t < cn; t++) {
+      const int64_t c = c0 + t;
+      const size_t n = (size_t)circ[c].n;
+      if (n == 0) continue;
+      char *base = (char *)p->d_band + off[c];
+      const size_t gsz = al(sizeof(double) * n * W);
+      double *dg = (double *)base, *dc = (double *)(base + gsz),
+             *db = (double *)(base + 2 * gsz);
+      uint32_t *dp = (uint32_t *)(base + 2 * gsz + al(sizeof(double) * 2 * n));
+      OHM_HIP(hipMemcpyAsync(dg, hb[t].gb.data(), sizeof(double) * n * W, hipMemcpyHostToDevice, st));
+      OHM_HIP(hipMemcpyAsync(dc, hb[t].cb.data(), sizeof(double) * n * W, hipMemcpyHostToDevice, st));
+      OHM_HIP(hipMemcpyAsync(db, hb[t].bp.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, st));
+      OHM_HIP(hipMemcpyAsync(dp, prep[c].perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, st));
+      p->gb[c] = dg;
+      p->cb[c] = dc;
+      p->bp[c] = db;
+      p->perm[c] = dp;
+    }
+    // the host band arrays of this group are freed on return
+    OHM_HIP(hipStreamSynchronize(st));
+  }
+  *out = p.release();
+  return OHM_OK;
+}
+
+
+// Device-memory budget of one chunk of waves: OHM_AC_DIRECT_BUDGET_GB, or
+// 90 % of what is free (counting the workspace the backend already holds).
+double direct_budget(ohm_backend *b) {
+  if (const char *env = getenv("OHM_AC_DIRECT_BUDGET_GB")) return atof(env) * 1e9;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 64e9;
+  return 0.9 * (double)(fr + b->ac_ws_bytes);
+}
+
+// residual ||b - A x|| / ||b|| above which a partitioned system is solved
+// again in one lane
+constexpr double kPartResidMax = 1e-12;
+// rows per part when neither the environment nor the autotuner sets it
+constexpr int64_t kDefaultPartRows = 2048;
+
+ohm_status ensure_events(ohm_ac_plan *p) {
+  for (auto &e : p->ev)
+    if (!e) OHM_HIP(hipEventCreate(&e));
+  return OHM_OK;
+}
+
+// One numeric sweep with rows-per-part R (0: one lane per whole system).
+ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, int64_t R) {
+  ohm_backend *b = p->b;
+  hipStream_t st = b->stream;
+  const int64_t nc = p->nc;
+  const int B = p->B, W = 2 * B + 1;
+  const bool part = B == 1 && R > 0;
+  p->resident = false;
+  p->waves.clear();
+  p->fallback_systems = 0;
+  for (double &t : p->phase_ms) t = 0.0;
+  const size_t n_sys = (size_t)std::max<int64_t>(nc * n_freqs, 1);
+  OHM_HIP(hipMemsetAsync(p->d_sing, 0, sizeof(int) * n_sys, st));
+  OHM_HIP(hipMemsetAsync(p->d_res, 0, sizeof(double) * 2 * n_sys, st));
+  // waves: up to 64 frequencies of one circuit; longest circuits first
+  struct WaveItem {
+    int64_t c, f0;
+    int nf;
+  };
+  std::vector<WaveItem> items;
+  for (int64_t c = 0; c < nc; c++)
+    if (p->n[c] > 0)
+      for (int64_t f0 = 0; f0 < n_freqs; f0 += 64)
+        items.push_back({c, f0, (int)std::min<int64_t>(64, n_freqs - f0)});
+  std::stable_sort(items.begin(), items.end(), [&](const WaveItem &a, const WaveItem &b2) {
+    return p->n[a.c] > p->n[b2.c];
+  });
+  const double budget = direct_budget(b);
+  bool want_x = false;
+  if (x)
+    for (int64_t c = 0; c < nc; c++) want_x |= x[c] != nullptr;
+  size_t pos = 0;
+  int chunks = 0;
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  auto parts_of = [&](int64_t n) { return part ? (int)((n + R - 1) / R) : 0; };
+  while (pos < items.size()) {
+    size_t end = pos;
+    double bytes = 0.0;
+    while (end < items.size()) {
+      const double n = (double)p->n[items[end].c];
+      const double nfw = items[end].nf;
+      const double np = parts_of(p->n[items[end].c]);
+      const double wave_b = n * nfw * 16.0 * (W + 1) + (want_x ? n * nfw * 16.0 : 0.0) +
+                            64.0 * kResChunks * 16.0 + sizeof(BandWave) +
+                            np * (4.0 + 64.0 * 16.0 * (6.0 + 12.0 + 2.0));
+      if (end > pos && bytes + wave_b > budget) break;
+      bytes += wave_b;
+      end++;
+    }
+    const int n_w = (int)(end - pos);
+    // output in (circuit, frequency) order so each circuit's block is contiguous
+    std::vector<size_t> by_cf(n_w);
+    std::iota(by_cf.begin(), by_cf.end(), 0);
+    std::stable_sort(by_cf.begin(), by_cf.end(), [&](size_t a, size_t b2) {
+      const WaveItem &u = items[pos + a], &v = items[pos + b2];
+      return u.c != v.c ? u.c < v.c : u.f0 < v.f0;
+    });
+    std::vector<int64_t> out_off(n_w);
+    std::vector<int64_t> f_lo(nc, INT64_MAX), f_hi(nc, -1), x_at(nc, 0);
+    int64_t out_total = 0;
+    for (size_t k : by_cf) {
+      const WaveItem &it = items[pos + k];
+      out_off[k] = out_total;
+      if (it.f0 < f_lo[it.c]) {
+        f_lo[it.c] = it.f0;
+        x_at[it.c] = out_total;
+      }
+      f_hi[it.c] = std::max<int64_t>(f_hi[it.c], it.f0 + it.nf - 1);
+      out_total += p->n[it.c] * it.nf;
+    }
+    std::vector<BandWave> hw(n_w);
+    std::vector<int> ip_item;
+    int64_t store = 0;  // lane-rows
+    for (int k = 0; k < n_w; k++) {
+      const WaveItem &it = items[pos + k];
+      BandWave w{};
+      w.n = p->n[it.c];
+      w.gb = p->gb[it.c];
+      w.cb = p->cb[it.c];
+      w.b = p->bp[it.c];
+      w.perm = p->perm[it.c];
+      w.omega = p->d_omega + it.f0;
+      w.store_off = store;
+      w.out_off = out_off[k];
+      w.nf = it.nf;
+      w.sys = (int)(it.c * n_freqs + it.f0);
+      w.pbase = (int)ip_item.size();
+      w.nparts = parts_of(w.n);
+      for (int q = 0; q < w.nparts; q++) ip_item.push_back(k);
+      store += w.n * it.nf;
+      hw[k] = w;
+    }
+    const size_t n_ip = ip_item.size();
+    // chunk buffers carved from the backend's workspace (kept across sweeps:
+    // re-allocating tens of GB per chunk costs seconds)
+    const size_t bw_b = al(sizeof(BandWave) * n_w), u_b = al(sizeof(cz) * store * W),
+                 y_b = al(sizeof(cz) * store),
+                 x_b = want_x ? al(sizeof(cz) * std::max<int64_t>(out_total, 1)) : 0,
+                 p_b = al(sizeof(double) * 2 * (size_t)n_w * 64 * kResChunks),
+                 ip_b = al(sizeof(int) * std::max<size_t>(n_ip, 1)),
+                 rel_b = al(sizeof(cz) * n_ip * 6 * 64),
+                 red_b = al(sizeof(cz) * n_ip * 12 * 64),
+                 bnd_b = al(sizeof(cz) * n_ip * 2 * 64);
+    const size_t need = bw_b + u_b + y_b + x_b + p_b + (part ? ip_b + rel_b + red_b + bnd_b : 0);
+    if (b->ac_ws_bytes < need) {
+      if (b->ac_ws) OHM_HIP(hipFree(b->ac_ws));
+      b->ac_ws = nullptr;
+      b->ac_ws_bytes = 0;
+      OHM_HIP(hipMalloc(&b->ac_ws, need));
+      b->ac_ws_bytes = need;
+    }
+    b->ac_ws_gen++;
+    char *wsp = (char *)b->ac_ws;
+    BandWave *d_w = (BandWave *)wsp;
+    cz *dU = (cz *)(wsp + bw_b);
+    cz *dY = (cz *)(wsp + bw_b + u_b);
+    cz *dX = want_x ? (cz *)(wsp + bw_b + u_b + y_b) : nullptr;
+    double *d_part = (double *)(wsp + bw_b + u_b + y_b + x_b);
+    char *pw = wsp + bw_b + u_b + y_b + x_b + p_b;
+    int *d_ip = (int *)pw;
+    cz *d_rel = (cz *)(pw + ip_b), *d_red = (cz *)(pw + ip_b + rel_b),
+       *d_bnd = (cz *)(pw + ip_b + rel_b + red_b);
+    OHM_HIP(hipMemcpyAsync(d_w, hw.data(), sizeof(BandWave) * n_w, hipMemcpyHostToDevice, st));
+    if (part)
+      OHM_HIP(hipMemcpyAsync(d_ip, ip_item.data(), sizeof(int) * n_ip, hipMemcpyHostToDevice, st));
+    hipEvent_t *ev = p->ev;
+    OHM_HIP(hipEventRecord(ev[0], st));
+    if (part) {
+      hipLaunchKernelGGL(k_part_rel, dim3((unsigned)(2 * n_ip)), dim3(64), 0, st, d_w, d_ip, R,
+                         d_rel, p->d_sing);
+      OHM_HIP(hipGetLastError());
+      OHM_HIP(hipEventRecord(ev[1], st));
+      hipLaunchKernelGGL(k_part_reduce, dim3(n_w), dim3(64), 0, st, d_w, d_rel, d_red, d_bnd,
+                         p->d_sing);
+      OHM_HIP(hipGetLastError());
+      OHM_HIP(hipEventRecord(ev[2], st));
+      hipLaunchKernelGGL(k_part_solve, dim3((unsigned)n_ip), dim3(64), 0, st, d_w, d_ip, R, d_bnd,
+                         dU, dY, p->d_sing);
+      OHM_HIP(hipGetLastError());
+      b->dispatches += 3;
+    } else {
+      OHM_HIP(hipEventRecord(ev[1], st));
+      OHM_HIP(hipEventRecord(ev[2], st));
+      if (B == 1)
+        hipLaunchKernelGGL(k_band_lu<1>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, p->d_sing);
+      else if (B == 2)
+        hipLaunchKernelGGL(k_band_lu<2>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, p->d_sing);
+      else
+        hipLaunchKernelGGL(k_band_lu<4>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, p->d_sing);
+      OHM_HIP(hipGetLastError());
+      b->dispatches++;
+    }
+    OHM_HIP(hipEventRecord(ev[3], st));
+    auto resid = [&](int nw, const BandWave *dw) -> ohm_status {
+      if (B == 1)
+        hipLaunchKernelGGL(k_band_resid<1>, dim3(nw * kResChunks), dim3(256), 0, st, dw, dY, d_part);
+      else if (B == 2)
+        hipLaunchKernelGGL(k_band_resid<2>, dim3(nw * kResChunks), dim3(256), 0, st, dw, dY, d_part);
+      else
+        hipLaunchKernelGGL(k_band_resid<4>, dim3(nw * kResChunks), dim3(256), 0, st, dw, dY, d_part);
+      OHM_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_band_resid_sum, dim3((nw * 64 + 255) / 256), dim3(256), 0, st, dw, nw,
+                         d_part, p->d_res);
+      OHM_HIP(hipGetLastError());
+      b->dispatches += 2;
+      return OHM_OK;
+    };
+    OHM_TRY(resid(n_w, d_w));
+    OHM_HIP(hipEventRecord(ev[4], st));
+    OHM_HIP(hipEventSynchronize(ev[4]));
+    float ms[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 4; k++) OHM_HIP(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+    p->phase_ms[0] += ms[0];
+    p->phase_ms[1] += ms[1];
+    p->phase_ms[2] += ms[2];
+    p->phase_ms[3] += ms[3];
+    if (part) {
+      // systems whose partitioned residual is not small: their waves again, one lane each
+      std::vector<int> sing(n_sys);
+      std::vector<double> res(2 * n_sys);
+      OHM_HIP(hipMemcpy(sing.data(), p->d_sing, sizeof(int) * n_sys, hipMemcpyDeviceToHost));
+      OHM_HIP(hipMemcpy(res.data(), p->d_res, sizeof(double) * 2 * n_sys, hipMemcpyDeviceToHost));
+      std::vector<BandWave> redo;
+      for (int k = 0; k < n_w; k++) {
+        const BandWave &w = hw[k];
+        bool again = false;
+        for (int l = 0; l < w.nf; l++) {
+          const size_t q = (size_t)w.sys + l;
+          const double rr = res[2 * q], bb = res[2 * q + 1];
+          const double r = bb > 0 ? sqrt(rr / bb) : 0.0;
+          if (sing[q] != 0 || !(r <= kPartResidMax)) again = true;
+        }
+        if (again) {
+          redo.push_back(w);
+          for (int l = 0; l < w.nf; l++) sing[(size_t)w.sys + l] = 0;
+          p->fallback_systems += w.nf;
+        }
+      }
+      if (!redo.empty()) {
+        OHM_HIP(hipMemcpyAsync(p->d_sing, sing.data(), sizeof(int) * n_sys, hipMemcpyHostToDevice, st));
+        // wave table of the redone items after the chunk's (the chunk's table stays for get_x)
+        BandWave *d_redo = nullptr;
+        OHM_HIP(hipMalloc((void **)&d_redo, sizeof(BandWave) * redo.size()));
+        std::unique_ptr<BandWave, void (*)(BandWave *)> g(d_redo, [](BandWave *q) { (void)hipFree(q); });
+        OHM_HIP(hipMemcpyAsync(d_redo, redo.data(), sizeof(BandWave) * redo.size(),
+                               hipMemcpyHostToDevice, st));
+        OHM_HIP(hipEventRecord(ev[0], st));
+        hipLaunchKernelGGL(k_band_lu<1>, dim3((unsigned)redo.size()), dim3(64), 0, st, d_redo, dU,
+                           dY, p->d_sing);
+        OHM_HIP(hipGetLastError());
+        b->dispatches++;
+        OHM_TRY(resid((int)redo.size(), d_redo));
+        OHM_HIP(hipEventRecord(ev[1], st));
+        OHM_HIP(hipEventSynchronize(ev[1]));
+        float fm = 0.f;
+        OHM_HIP(hipEventElapsedTime(&fm, ev[0], ev[1]));
+        p->phase_ms[4] += fm;
+      }
+    }
+    if (want_x) {
+      OHM_HIP(hipEventRecord(ev[0], st));
+      hipLaunchKernelGGL(k_band_unpermute, dim3(n_w * 64), dim3(256), 0, st, d_w, dY, dX);
+      OHM_HIP(hipGetLastError());
+      OHM_HIP(hipEventRecord(ev[1], st));
+      b->dispatches++;
+      for (int64_t c = 0; c < nc; c++) {
+        if (f_hi[c] < 0 || !x[c]) continue;
+        const int64_t n = p->n[c];
+        OHM_HIP(hipMemcpyAsync(x[c] + f_lo[c] * 2 * n, dX + x_at[c],
+                               sizeof(cz) * n * (f_hi[c] - f_lo[c] + 1), hipMemcpyDeviceToHost, st));
+      }
+      OHM_HIP(hipStreamSynchronize(st));
+      float um = 0.f;
+      OHM_HIP(hipEventElapsedTime(&um, ev[0], ev[1]));
+      p->phase_ms[5] += um;
+    }
+    b->readbacks++;
+    chunks++;
+    if (end == items.size() && chunks == 1) {
+      p->waves = std::move(hw);
+      p->y_off = bw_b + u_b;
+    }
+    pos = end;
+  }
+  // solutions of a one-chunk sweep stay in the workspace for ohm_ac_plan_get_x
+  p->resident = chunks == 1;
+  p->ws_gen = b->ac_ws_gen;
+  p->chunks = chunks;
+  p->part_rows = part ? R : 0;
+  return OHM_OK;
+}
+
+// rows per part: OHM_AC_PART_ROWS, else the autotuned value of this plan
+// (the fastest of the candidates, each timed over one whole sweep on the
+// plan's first solve), else 2048
+ohm_status plan_solve(ohm_ac_plan *p, int64_t n_freqs, const double *freqs, double *const *x,
+                      ohm_batch_result *results, double *device_ms) {
+  ohm_backend *b = p->b;
+  OHM_HIP(hipSetDevice(b->device));
+  hipStream_t st = b->stream;
+  const int64_t nc = p->nc;
+  OHM_TRY(ensure_events(p));
+  p->nf = n_freqs;
+  const size_t n_sys = (size_t)std::max<int64_t>(nc * n_freqs, 1);
+  if (p->sys_cap < n_sys) {
+    if (p->d_sing) OHM_HIP(hipFree(p->d_sing));
+    if (p->d_res) OHM_HIP(hipFree(p->d_res));
+    p->d_sing = nullptr;
+    p->d_res = nullptr;
+    p->sys_cap = 0;
+    OHM_HIP(hipMalloc((void **)&p->d_sing, sizeof(int) * n_sys));
+    OHM_HIP(hipMalloc((void **)&p->d_res, sizeof(double) * 2 * n_sys));
+    p->sys_cap = n_sys;
+  }
+  const size_t n_om = (size_t)std::max<int64_t>(n_freqs, 1);
+  if (p->omega_cap < n_om) {
+    if (p->d_omega) OHM_HIP(hipFree(p->d_omega));
+    p->d_omega = nullptr;
+    p->omega_cap = 0;
+    OHM_HIP(hipMalloc((void **)&p->d_omega, sizeof(double) * n_om));
+    p->omega_cap = n_om;
+  }
+  std::vector<double> omega(n_om, 0.0);
+  for (int64_t f = 0; f < n_freqs; f++) omega[f] = 2.0 * M_PI * freqs[f];  // ac.rs:57
+  OHM_HIP(hipMemcpyAsync(p->d_omega, omega.data(), sizeof(double) * n_om, hipMemcpyHostToDevice, st));
+  int64_t R = kDefaultPartRows;
+  if (const char *env = getenv("OHM_AC_PART_ROWS")) {
+    R = atoll(env);
+  } else if (p->tuned_rows >= 0) {
+    R = p->tuned_rows;
+  } else if (b->autotune && p->B == 1) {
+    const int64_t cand[] = {0, 1024, 2048, 4096, 8192};
+    double best = 1e300;
+    for (int64_t c : cand) {
+      OHM_TRY(plan_solve_once(p, n_freqs, nullptr, c));
+      double t = 0.0;
+      for (double v : p->phase_ms) t += v;
+      if (t < best) {
+        best = t;
+        R = c;
+      }
+    }
+    p->tuned_rows = R;
+  }
+  OHM_TRY(plan_solve_once(p, n_freqs, x, R));
+  std::vector<int> sing(n_sys, 0);
+  std::vector<double> res(2 * n_sys, 0.0);
+  OHM_HIP(hipMemcpy(sing.data(), p->d_sing, sizeof(int) * n_sys, hipMemcpyDeviceToHost));
+  OHM_HIP(hipMemcpy(res.data(), p->d_res, sizeof(double) * 2 * n_sys, hipMemcpyDeviceToHost));
+  ohm_status first = OHM_OK;
+  std::string first_msg;
+  for (int64_t c = 0; c < nc; c++) {
+    for (int64_t f = 0; f < n_freqs; f++) {
+      const int64_t k = c * n_freqs + f;
+      ohm_batch_result &r = results[k];
+      r.iterations = 0;
+      r.refinements = 0;
+      if (p->n[c] == 0) {
+        r.residual_norm = 0.0;
+        r.status = OHM_OK;
+        r.reason = 1;
+        continue;
+      }
+      const double rr = res[2 * k], bb = res[2 * k + 1];
+      r.residual_norm = bb > 0 ? sqrt(rr / bb) : 0.0;
+      const bool bad = sing[k] != 0 || !std::isfinite(r.residual_norm);
+      r.status = bad ? OHM_ERR_SOLVE : OHM_OK;
+      r.reason = bad ? 0 : 1;
+      if (bad && first == OHM_OK) {
+        first = fail(OHM_ERR_SOLVE,
+                     "Solve error: banded LU: singular matrix (circuit %lld, frequency %lld)",
+                     (long long)c, (long long)f);
+        first_msg = ohm_last_error();
+      }
+    }
+  }
+  double total = 0.0;
+  for (int k = 0; k < 5; k++) total += p->phase_ms[k];
+  if (device_ms) *device_ms = total;
+  if (first != OHM_OK) return fail(first, "%s", first_msg.c_str());
+  return OHM_OK;
+}
+
+// x of lanes [l0, l1) of one wave back to the original unknown order:
+// out[(lane - l0) n + perm[i]] = x_i
+__global__ void k_band_gather_x(BandWave q, int l0, int l1, const cz *__restrict__ Y,
+                                cz *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t span = l1 - l0;
+  if (t >= q.n * span) return;
+  const int64_t i = t / span;
+  const int lane = l0 + (int)(t % span);
+  out[(int64_t)(lane - l0) * q.n + q.perm[i]] = Y[q.store_off + i * q.nf + lane];
+}
+
 }  // namespace
 
 extern "C" {
+
+ohm_status ohm_ac_plan_create(ohm_backend *b, int64_t n_circuits, const ohm_ac_circuit *circ,
+                              ohm_ac_plan **out) {
+  if (!b || (n_circuits > 0 && !circ) || !out) return fail(OHM_ERR_INVALID, "null argument");
+  if (n_circuits < 0) return fail(OHM_ERR_INVALID, "negative size");
+  *out = nullptr;
+  OHM_TRY(check_circuits(n_circuits, circ));
+  OHM_HIP(hipSetDevice(b->device));
+  std::vector<Prepared> prep(n_circuits);
+  std::vector<ohm_status> pst(n_circuits, OHM_OK);
+  std::vector<std::string> pmsg(n_circuits);
+  parallel_for(n_circuits, [&](int64_t c) { pst[c] = prepare_circuit(circ[c], prep[c], pmsg[c]); });
+  for (int64_t c = 0; c < n_circuits; c++)
+    if (pst[c] != OHM_OK) {
+      set_error(pmsg[c]);
+      return pst[c];
+    }
+  return plan_build(b, n_circuits, circ, prep, out);
+}
+
+ohm_status ohm_ac_plan_solve(ohm_ac_plan *p, int64_t n_freqs, const double *freqs,
+                             double *const *x, ohm_batch_result *results, double *device_ms) {
+  if (!p || (n_freqs > 0 && !freqs) || !results) return fail(OHM_ERR_INVALID, "null argument");
+  if (n_freqs < 0) return fail(OHM_ERR_INVALID, "negative size");
+  if (!backend_alive(p->b)) return fail(OHM_ERR_INVALID, "the plan's backend was destroyed");
+  return plan_solve(p, n_freqs, freqs, x, results, device_ms);
+}
+
+ohm_status ohm_ac_plan_get_x(ohm_ac_plan *p, int64_t circuit, int64_t f_first, int64_t f_count,
+                             double *x) {
+  if (!p || (f_count > 0 && !x)) return fail(OHM_ERR_INVALID, "null argument");
+  if (!backend_alive(p->b)) return fail(OHM_ERR_INVALID, "the plan's backend was destroyed");
+  if (circuit < 0 || circuit >= p->nc) return fail(OHM_ERR_INVALID, "circuit %lld out of range", (long long)circuit);
+  if (f_first < 0 || f_count < 0 || f_first + f_count > p->nf)
+    return fail(OHM_ERR_INVALID, "frequencies [%lld, %lld) out of range (the last solve had %lld)",
+                (long long)f_first, (long long)(f_first + f_count), (long long)p->nf);
+  ohm_backend *b = p->b;
+  if (!p->resident || p->ws_gen != b->ac_ws_gen)
+    return fail(OHM_ERR_INVALID,
+                "the last solve's solutions are not resident (it ran in several chunks, or "
+                "another sweep has used the workspace since): pass x to ohm_ac_plan_solve");
+  const int64_t n = p->n[circuit];
+  if (n == 0 || f_count == 0) return OHM_OK;
+  OHM_HIP(hipSetDevice(b->device));
+  hipStream_t st = b->stream;
+  cz *d_out = nullptr;
+  OHM_HIP(hipMalloc((void **)&d_out, sizeof(cz) * n * f_count));
+  const cz *dY = (const cz *)((const char *)b->ac_ws + p->y_off);
+  ohm_status s = OHM_OK;
+  for (const BandWave &w : p->waves) {
+    const int64_t c = w.sys / p->nf, f0 = w.sys % p->nf;
+    if (c != circuit) continue;
+    const int64_t lo = std::max<int64_t>(f0, f_first), hi = std::min<int64_t>(f0 + w.nf, f_first + f_count);
+    if (lo >= hi) continue;
+    const int64_t cnt = n * (hi - lo);
+    hipLaunchKernelGGL(k_band_gather_x, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, w,
+                       (int)(lo - f0), (int)(hi - f0), dY, d_out + (lo - f_first) * n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      s = hip_fail(e, "k_band_gather_x", __FILE__, __LINE__);
+      break;
+    }
+  }
+  if (s == OHM_OK) {
+    hipError_t e = hipMemcpyAsync(x, d_out, sizeof(cz) * n * f_count, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) s = hip_fail(e, "get_x copy", __FILE__, __LINE__);
+  }
+  (void)hipFree(d_out);
+  return s;
+}
+
+ohm_status ohm_ac_plan_stats(const ohm_ac_plan *p, double *phase_ms, int64_t *rows_per_part,
+                             int64_t *fallback_systems, int32_t *chunks) {
+  if (!p) return fail(OHM_ERR_INVALID, "null plan");
+  if (phase_ms)
+    for (int k = 0; k < 6; k++) phase_ms[k] = p->phase_ms[k];
+  if (rows_per_part) *rows_per_part = p->part_rows;
+  if (fallback_systems) *fallback_systems = p->fallback_systems;
+  if (chunks) *chunks = p->chunks;
+  return OHM_OK;
+}
+
+void ohm_ac_plan_free(ohm_ac_plan *p) { plan_release(p); }
 
 ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_ac_circuit *circ,
                                  int64_t n_freqs, const double *freqs, int32_t method,
@@ -414,55 +1436,28 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
     return fail(OHM_ERR_INVALID, "null argument");
   if (n_circuits < 0 || n_freqs < 0) return fail(OHM_ERR_INVALID, "negative size");
   if (method < OHM_AC_AUTO || method > OHM_AC_DIRECT) return fail(OHM_ERR_INVALID, "unknown AC method %d", method);
+  OHM_TRY(check_circuits(n_circuits, circ));
   OHM_HIP(hipSetDevice(b->device));
-  hipStream_t st = b->stream;
-  // symbolic preparation per circuit (pattern union, RCM, bandwidth), on
-  // host threads; inputs are checked first so errors are deterministic
+  if (device_ms) *device_ms = 0.0;
+  // symbolic preparation per circuit on host threads; inputs are checked
+  // first so errors are deterministic
   std::vector<Prepared> prep(n_circuits);
-  int band = 0;
-  std::vector<char> direct(n_circuits, 0);
-  for (int64_t c = 0; c < n_circuits; c++) {
-    const ohm_ac_circuit &q = circ[c];
-    if (q.n < 0 || q.n > 0x7fffffff) return fail(OHM_ERR_INVALID, "circuit %lld: bad size", (long long)c);
-    if (q.n == 0) continue;
-    if (!q.g_rowptr || !q.g_val || !q.b_ac || (q.c_val && !q.c_rowptr))
-      return fail(OHM_ERR_INVALID, "circuit %lld: null arrays", (long long)c);
-  }
   std::vector<ohm_status> pst(n_circuits, OHM_OK);
   std::vector<std::string> pmsg(n_circuits);
-  parallel_for(n_circuits, [&](int64_t c) {
-    const ohm_ac_circuit &q = circ[c];
-    Prepared &p = prep[c];
-    p.n = q.n;
-    if (q.n == 0) return;
-    pst[c] = csr_union(q.n, q.g_rowptr, q.g_col, q.g_val, q.c_val ? q.c_rowptr : nullptr, q.c_col,
-                       q.c_val, p.rp, p.cl, p.g, p.c);
-    if (pst[c] != OHM_OK) {
-      pmsg[c] = ohm_last_error();  // this thread's message
-      return;
-    }
-    p.perm = rcm(q.n, p.rp, p.cl);
-    p.iperm.assign(q.n, 0);
-    for (int64_t i = 0; i < q.n; i++) p.iperm[p.perm[i]] = (uint32_t)i;
-    int64_t bw = 0;
-    for (int64_t r = 0; r < q.n; r++)
-      for (uint32_t k = p.rp[r]; k < p.rp[r + 1]; k++)
-        bw = std::max<int64_t>(bw, std::llabs((int64_t)p.iperm[r] - (int64_t)p.iperm[p.cl[k]]));
-    p.band = (int)std::min<int64_t>(bw, 1 << 20);
-  });
+  if (method != OHM_AC_ITERATIVE)
+    parallel_for(n_circuits, [&](int64_t c) { pst[c] = prepare_circuit(circ[c], prep[c], pmsg[c]); });
+  std::vector<char> direct(n_circuits, 0);
   for (int64_t c = 0; c < n_circuits; c++) {
     if (pst[c] != OHM_OK) {
       set_error(pmsg[c]);
       return pst[c];
     }
     if (circ[c].n == 0) continue;
-    const Prepared &p = prep[c];
-    direct[c] = method == OHM_AC_ITERATIVE ? 0 : (p.band <= 4);
+    direct[c] = method == OHM_AC_ITERATIVE ? 0 : (prep[c].band <= kMaxDirectBand);
     if (method == OHM_AC_DIRECT && !direct[c])
       return fail(OHM_ERR_INVALID,
-                  "circuit %lld: half bandwidth %d after RCM exceeds the direct solver's 4",
-                  (long long)c, p.band);
-    if (direct[c]) band = std::max(band, p.band);
+                  "circuit %lld: half bandwidth %d after RCM exceeds the direct solver's %d",
+                  (long long)c, prep[c].band, kMaxDirectBand);
   }
   ohm_status first = OHM_OK;
   std::string first_msg;
@@ -488,230 +1483,36 @@ ohm_status ohm_ac_sweep_circuits(ohm_backend *b, int64_t n_circuits, const ohm_a
     note(s);
     if (s != OHM_OK && s != OHM_ERR_SOLVE) return s;
   }
-  const int B = band <= 1 ? 1 : band <= 2 ? 2 : 4;
-  const int W = 2 * B + 1;
-  // device band data per direct circuit
-  std::vector<void *> allocs;
-  struct Free {
-    std::vector<void *> &v;
-    ~Free() {
-      for (void *p : v) (void)hipFree(p);
-    }
-  } guard{allocs};
-  auto dmal = [&](void **p, size_t bytes) -> ohm_status {
-    OHM_HIP(hipMalloc(p, std::max<size_t>(bytes, 16)));
-    allocs.push_back(*p);
-    return OHM_OK;
-  };
-  std::vector<const double *> d_gb(n_circuits), d_cb(n_circuits), d_b(n_circuits);
-  std::vector<const uint32_t *> d_perm(n_circuits);
-  // band arrays in the RCM order, built on host threads 32 circuits at a
-  // time and uploaded from this thread
-  std::vector<int64_t> dlist;
+  // direct circuits: one plan over them, solved once
+  std::vector<int64_t> dl;
   for (int64_t c = 0; c < n_circuits; c++)
-    if (direct[c] && circ[c].n > 0) dlist.push_back(c);
-  struct BandHost {
-    std::vector<double> gb, cb, bp;
-  };
-  for (size_t c0 = 0; c0 < dlist.size(); c0 += 32) {
-    const size_t cn = std::min<size_t>(32, dlist.size() - c0);
-    std::vector<BandHost> hb(cn);
-    parallel_for((int64_t)cn, [&](int64_t t) {
-      const int64_t c = dlist[c0 + t];
-      const Prepared &p = prep[c];
-      const int64_t n = p.n;
-      BandHost &h = hb[t];
-      h.gb.assign((size_t)n * W, 0.0);
-      h.cb.assign((size_t)n * W, 0.0);
-      h.bp.assign(2 * n, 0.0);
-      for (int64_t r = 0; r < n; r++) {
-        const int64_t i = p.iperm[r];
-        for (uint32_t k = p.rp[r]; k < p.rp[r + 1]; k++) {
-          const int64_t d = (int64_t)p.iperm[p.cl[k]] - i + B;
-          h.gb[i * W + d] += p.g[k];
-          h.cb[i * W + d] += p.c[k];
-        }
-        h.bp[2 * i] = circ[c].b_ac[2 * r];
-        h.bp[2 * i + 1] = circ[c].b_ac[2 * r + 1];
-      }
-    });
-    for (size_t t = 0; t < cn; t++) {
-      const int64_t c = dlist[c0 + t];
-      const int64_t n = prep[c].n;
-      const BandHost &h = hb[t];
-      void *pg, *pc, *pb, *pp;
-      OHM_TRY(dmal(&pg, sizeof(double) * h.gb.size()));
-      OHM_TRY(dmal(&pc, sizeof(double) * h.cb.size()));
-      OHM_TRY(dmal(&pb, sizeof(double) * h.bp.size()));
-      OHM_TRY(dmal(&pp, sizeof(uint32_t) * n));
-      OHM_HIP(hipMemcpy(pg, h.gb.data(), sizeof(double) * h.gb.size(), hipMemcpyHostToDevice));
-      OHM_HIP(hipMemcpy(pc, h.cb.data(), sizeof(double) * h.cb.size(), hipMemcpyHostToDevice));
-      OHM_HIP(hipMemcpy(pb, h.bp.data(), sizeof(double) * h.bp.size(), hipMemcpyHostToDevice));
-      OHM_HIP(hipMemcpy(pp, prep[c].perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-      d_gb[c] = (const double *)pg;
-      d_cb[c] = (const double *)pc;
-      d_b[c] = (const double *)pb;
-      d_perm[c] = (const uint32_t *)pp;
+    if (direct[c]) dl.push_back(c);
+  if (!dl.empty()) {
+    std::vector<ohm_ac_circuit> dc(dl.size());
+    std::vector<Prepared> dp(dl.size());
+    std::vector<double *> dx(dl.size());
+    for (size_t k = 0; k < dl.size(); k++) {
+      dc[k] = circ[dl[k]];
+      dp[k] = std::move(prep[dl[k]]);
+      dx[k] = circ[dl[k]].x;
     }
+    ohm_ac_plan *p = nullptr;
+    OHM_TRY(plan_build(b, (int64_t)dl.size(), dc.data(), dp, &p));
+    p->tuned_rows = kDefaultPartRows;  // a one-off sweep is not worth tuning
+    std::vector<ohm_batch_result> dr(dl.size() * std::max<int64_t>(n_freqs, 1));
+    double ms = 0.0;
+    ohm_status s = plan_solve(p, n_freqs, freqs, dx.data(), dr.data(), &ms);
+    std::string msg = s != OHM_OK ? ohm_last_error() : std::string();
+    plan_release(p);
+    if (s != OHM_OK && s != OHM_ERR_SOLVE) return fail(s, "%s", msg.c_str());
+    if (s != OHM_OK) {
+      set_error(msg);
+      note(s);
+    }
+    for (size_t k = 0; k < dl.size(); k++)
+      for (int64_t f = 0; f < n_freqs; f++) results[dl[k] * n_freqs + f] = dr[k * n_freqs + f];
+    if (device_ms) *device_ms = ms;
   }
-  // waves: up to 64 frequencies of one circuit; longest circuits first
-  struct WaveItem {
-    int64_t c, f0;
-    int nf;
-  };
-  std::vector<WaveItem> items;
-  for (int64_t c = 0; c < n_circuits; c++)
-    if (direct[c] && circ[c].n > 0)
-      for (int64_t f0 = 0; f0 < n_freqs; f0 += 64)
-        items.push_back({c, f0, (int)std::min<int64_t>(64, n_freqs - f0)});
-  std::stable_sort(items.begin(), items.end(), [&](const WaveItem &a, const WaveItem &b2) {
-    return circ[a.c].n > circ[b2.c].n;
-  });
-  std::vector<double> omega(std::max<int64_t>(n_freqs, 1));
-  for (int64_t f = 0; f < n_freqs; f++) omega[f] = 2.0 * M_PI * freqs[f];  // ac.rs:57
-  double *d_omega = nullptr;
-  OHM_TRY(dmal((void **)&d_omega, sizeof(double) * omega.size()));
-  OHM_HIP(hipMemcpy(d_omega, omega.data(), sizeof(double) * omega.size(), hipMemcpyHostToDevice));
-  // chunks bounded by a device-memory budget for U, Y and the output
-  const char *env = getenv("OHM_AC_DIRECT_BUDGET_GB");
-  const double budget = (env ? atof(env) : 64.0) * 1e9;
-  std::vector<int> singular_h(std::max<int64_t>(n_circuits * n_freqs, 1), 0);
-  std::vector<double> res_h(2 * std::max<int64_t>(n_circuits * n_freqs, 1), 0.0);
-  int *d_sing = nullptr;
-  double *d_res = nullptr;
-  OHM_TRY(dmal((void **)&d_sing, sizeof(int) * singular_h.size()));
-  OHM_TRY(dmal((void **)&d_res, sizeof(double) * res_h.size()));
-  OHM_HIP(hipMemsetAsync(d_sing, 0, sizeof(int) * singular_h.size(), st));
-  OHM_HIP(hipMemsetAsync(d_res, 0, sizeof(double) * res_h.size(), st));
-  float kernel_ms = 0.f;
-  size_t pos = 0;
-  bool want_x = false;
-  for (int64_t c = 0; c < n_circuits; c++) want_x |= direct[c] && circ[c].x != nullptr;
-  while (pos < items.size()) {
-    size_t end = pos;
-    double bytes = 0.0;
-    while (end < items.size()) {
-      const double n = (double)circ[items[end].c].n;
-      const double wave_b = n * 64.0 * 16.0 * (W + 1) + n * items[end].nf * 16.0 +
-                            64.0 * kResChunks * 16.0;
-      if (end > pos && bytes + wave_b > budget) break;
-      bytes += wave_b;
-      end++;
-    }
-    const int n_w = (int)(end - pos);
-    // output in (circuit, frequency) order so each circuit's block is contiguous
-    std::vector<size_t> by_cf(n_w);
-    std::iota(by_cf.begin(), by_cf.end(), 0);
-    std::stable_sort(by_cf.begin(), by_cf.end(), [&](size_t a, size_t b2) {
-      const WaveItem &x = items[pos + a], &y = items[pos + b2];
-      return x.c != y.c ? x.c < y.c : x.f0 < y.f0;
-    });
-    std::vector<int64_t> out_off(n_w);
-    std::vector<int64_t> f_lo(n_circuits, INT64_MAX), f_hi(n_circuits, -1), x_at(n_circuits, 0);
-    int64_t out_total = 0;
-    for (size_t k : by_cf) {
-      const WaveItem &it = items[pos + k];
-      out_off[k] = out_total;
-      if (it.f0 < f_lo[it.c]) {
-        f_lo[it.c] = it.f0;
-        x_at[it.c] = out_total;
-      }
-      f_hi[it.c] = std::max<int64_t>(f_hi[it.c], it.f0 + it.nf - 1);
-      out_total += circ[it.c].n * it.nf;
-    }
-    std::vector<BandWave> hw(n_w);
-    int64_t store = 0;
-    for (int k = 0; k < n_w; k++) {
-      const WaveItem &it = items[pos + k];
-      BandWave w{};
-      w.n = circ[it.c].n;
-      w.gb = d_gb[it.c];
-      w.cb = d_cb[it.c];
-      w.b = d_b[it.c];
-      w.perm = d_perm[it.c];
-      w.omega = d_omega + it.f0;
-      w.store_off = store * 64;
-      w.out_off = out_off[k];
-      w.nf = it.nf;
-      w.sys = (int)(it.c * n_freqs + it.f0);
-      store += w.n;
-      hw[k] = w;
-    }
-    // chunk buffers carved from the backend's workspace (kept across sweeps:
-    // re-allocating tens of GB per chunk costs seconds)
-    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-    const size_t bw_b = al(sizeof(BandWave) * n_w), u_b = al(sizeof(cz) * store * 64 * W),
-                 y_b = al(sizeof(cz) * store * 64),
-                 x_b = want_x ? al(sizeof(cz) * std::max<int64_t>(out_total, 1)) : 0,
-                 p_b = al(sizeof(double) * 2 * (size_t)n_w * 64 * kResChunks);
-    const size_t need = bw_b + u_b + y_b + x_b + p_b;
-    if (b->ac_ws_bytes < need) {
-      if (b->ac_ws) OHM_HIP(hipFree(b->ac_ws));
-      b->ac_ws = nullptr;
-      b->ac_ws_bytes = 0;
-      OHM_HIP(hipMalloc(&b->ac_ws, need));
-      b->ac_ws_bytes = need;
-    }
-    char *wsp = (char *)b->ac_ws;
-    BandWave *d_w = (BandWave *)wsp;
-    cz *dU = (cz *)(wsp + bw_b);
-    cz *dY = (cz *)(wsp + bw_b + u_b);
-    cz *dX = want_x ? (cz *)(wsp + bw_b + u_b + y_b) : nullptr;
-    double *d_part = (double *)(wsp + bw_b + u_b + y_b + x_b);
-    OHM_HIP(hipMemcpyAsync(d_w, hw.data(), sizeof(BandWave) * n_w, hipMemcpyHostToDevice, st));
-    OHM_HIP(hipEventRecord(b->ev0, st));
-    const dim3 rg(kResChunks, n_w);
-    if (B == 1) {
-      hipLaunchKernelGGL(k_band_lu<1>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, d_sing);
-      hipLaunchKernelGGL(k_band_resid<1>, rg, dim3(256), 0, st, d_w, dY, d_part);
-    } else if (B == 2) {
-      hipLaunchKernelGGL(k_band_lu<2>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, d_sing);
-      hipLaunchKernelGGL(k_band_resid<2>, rg, dim3(256), 0, st, d_w, dY, d_part);
-    } else {
-      hipLaunchKernelGGL(k_band_lu<4>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, d_sing);
-      hipLaunchKernelGGL(k_band_resid<4>, rg, dim3(256), 0, st, d_w, dY, d_part);
-    }
-    hipLaunchKernelGGL(k_band_resid_sum, dim3((n_w * 64 + 255) / 256), dim3(256), 0, st, d_w, n_w,
-                       d_part, d_res);
-    if (want_x) {
-      hipLaunchKernelGGL(k_band_unpermute, dim3(64, n_w), dim3(256), 0, st, d_w, dY, dX);
-      b->dispatches++;
-    }
-    OHM_HIP(hipEventRecord(b->ev1, st));
-    b->dispatches += 3;
-    for (int64_t c = 0; c < n_circuits; c++) {
-      if (f_hi[c] < 0 || !circ[c].x) continue;
-      const int64_t n = circ[c].n;
-      OHM_HIP(hipMemcpyAsync(circ[c].x + f_lo[c] * 2 * n, dX + x_at[c],
-                             sizeof(cz) * n * (f_hi[c] - f_lo[c] + 1), hipMemcpyDeviceToHost, st));
-    }
-    OHM_HIP(hipStreamSynchronize(st));
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
-    kernel_ms += ms;
-    b->readbacks++;
-    pos = end;
-  }
-  OHM_HIP(hipMemcpy(singular_h.data(), d_sing, sizeof(int) * singular_h.size(), hipMemcpyDeviceToHost));
-  OHM_HIP(hipMemcpy(res_h.data(), d_res, sizeof(double) * res_h.size(), hipMemcpyDeviceToHost));
-  for (int64_t c = 0; c < n_circuits; c++) {
-    if (!direct[c]) continue;
-    for (int64_t f = 0; f < n_freqs; f++) {
-      const int64_t k = c * n_freqs + f;
-      ohm_batch_result &r = results[k];
-      r.iterations = 0;
-      r.refinements = 0;
-      const double rr = res_h[2 * k], bb = res_h[2 * k + 1];
-      r.residual_norm = bb > 0 ? sqrt(rr / bb) : 0.0;
-      const bool sing = singular_h[k] != 0 || !std::isfinite(r.residual_norm);
-      r.status = sing ? OHM_ERR_SOLVE : OHM_OK;
-      r.reason = sing ? 0 : 1;
-      if (sing)
-        note(fail(OHM_ERR_SOLVE, "Solve error: banded LU: singular matrix (circuit %lld, frequency %lld)",
-                  (long long)c, (long long)f));
-    }
-  }
-  if (device_ms) *device_ms = kernel_ms;
   if (first != OHM_OK) return fail(first, "%s", first_msg.c_str());
   return OHM_OK;
 }

@@ -120,6 +120,7 @@ struct ohm_backend {
   double *h_pinned = nullptr;        // pinned host staging for scalar readback
   void *ac_ws = nullptr;             // banded AC solver's chunk workspace (grow-only)
   size_t ac_ws_bytes = 0;
+  uint64_t ac_ws_gen = 0;            // bumped by every chunk written into ac_ws
   void *dense_ws = nullptr;          // dense LU: n*n matrix + rhs + flag (grow-only)
   size_t dense_ws_bytes = 0;
   int64_t dispatches = 0, readbacks = 0;

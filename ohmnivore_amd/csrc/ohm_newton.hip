@@ -547,13 +547,20 @@ bool same_topology(const ohm_mosfet_desc &a, const ohm_mosfet_desc &b) {
          !std::memcmp(a.b_idx, b.b_idx, sizeof a.b_idx);
 }
 template <typename D>
-ohm_status replace_desc(std::vector<D> &host, D *dev, const D *in, const char *what,
-                        hipStream_t st) {
+ohm_status check_desc(const std::vector<D> &host, const D *in, const char *what) {
   if (!in) return OHM_OK;
   for (size_t k = 0; k < host.size(); k++)
     if (!same_topology(host[k], in[k]))
       return fail(OHM_ERR_INVALID,
                   "%s descriptor %zu changes nodes or stamp slots: create a new context", what, k);
+  return OHM_OK;
+}
+
+// only after check_desc accepted every family, so a rejected update leaves
+// the context as it was
+template <typename D>
+ohm_status replace_desc(std::vector<D> &host, D *dev, const D *in, hipStream_t st) {
+  if (!in) return OHM_OK;
   host.assign(in, in + host.size());
   if (!host.empty())
     OHM_HIP(hipMemcpyAsync(dev, host.data(), sizeof(D) * host.size(), hipMemcpyHostToDevice, st));
@@ -736,9 +743,12 @@ ohm_status ohm_newton_update(ohm_newton_ctx *c, const double *base_values, const
         return fail(OHM_ERR_NEWTON_NUMERICAL,
                     "Newton solver encountered numerical error at iteration 0");
   }
-  OHM_TRY(replace_desc(c->h_dd, c->dd, diodes, "diode", st));
-  OHM_TRY(replace_desc(c->h_qd, c->qd, bjts, "BJT", st));
-  OHM_TRY(replace_desc(c->h_md, c->md, mosfets, "MOSFET", st));
+  OHM_TRY(check_desc(c->h_dd, diodes, "diode"));
+  OHM_TRY(check_desc(c->h_qd, bjts, "BJT"));
+  OHM_TRY(check_desc(c->h_md, mosfets, "MOSFET"));
+  OHM_TRY(replace_desc(c->h_dd, c->dd, diodes, st));
+  OHM_TRY(replace_desc(c->h_qd, c->qd, bjts, st));
+  OHM_TRY(replace_desc(c->h_md, c->md, mosfets, st));
   if (base_values && c->nnz) {
     OHM_HIP(hipMemcpyAsync(c->base_val, base_values, sizeof(double) * c->nnz,
                            hipMemcpyHostToDevice, st));

@@ -469,13 +469,9 @@ class AcSweepResult:
     device_ms: float
 
 
-def ac_sweep_circuits(backend: HipBackend, mnas, freqs, method: str = "auto",
-                      params: N.BicgstabParams | None = None, precond: int = N.PREC_JACOBI,
-                      keep_x: bool = True, raise_on_failure: bool = True) -> AcSweepResult:
-    """ac::run over many circuits at once (ohm_ac_sweep_circuits): banded
-    direct solves for chain-like circuits, batched BiCGSTAB for the rest."""
-    f = np.ascontiguousarray(freqs, dtype=np.float64)
-    nf = f.shape[0]
+def _ac_circuits(mnas, nf: int, keep_x: bool):
+    """ohm_ac_circuit array over compiled circuits (+ the arrays it points
+    into, and per-circuit x outputs when keep_x)."""
     k = len(mnas)
     cs = (N.AcCircuit * max(k, 1))()
     keep, xs = [], []
@@ -494,17 +490,94 @@ def ac_sweep_circuits(backend: HipBackend, mnas, freqs, method: str = "auto",
                             ptr(arrs[2], N.c_f64p), ptr(arrs[3], N.c_u32p), ptr(arrs[4], N.c_u32p),
                             ptr(arrs[5], N.c_f64p), arrs[6].ctypes.data_as(N.c_f64p),
                             x.ctypes.data_as(N.c_f64p) if x is not None else None)
+    return cs, keep, xs
+
+
+def _stats(res, k: int, nf: int):
+    return [[BatchStats(r.iterations, r.refinements, r.residual_norm, r.status, r.reason)
+             for r in res[i * nf:(i + 1) * nf]] for i in range(k)]
+
+
+def ac_sweep_circuits(backend: HipBackend, mnas, freqs, method: str = "auto",
+                      params: N.BicgstabParams | None = None, precond: int = N.PREC_JACOBI,
+                      keep_x: bool = True, raise_on_failure: bool = True) -> AcSweepResult:
+    """ac::run over many circuits at once (ohm_ac_sweep_circuits): banded
+    direct solves for chain-like circuits, batched BiCGSTAB for the rest."""
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    nf = f.shape[0]
+    k = len(mnas)
+    cs, keep, xs = _ac_circuits(mnas, nf, keep_x)
     res = (N.BatchResult * max(k * nf, 1))()
     ms = ctypes.c_double(0.0)
     prm = params if params is not None else bicgstab_params()
     rc = lib().ohm_ac_sweep_circuits(backend.h, ctypes.c_int64(k), cs, ctypes.c_int64(nf),
                                      ptr(f, N.c_f64p), AC_METHODS[method], ctypes.byref(prm),
                                      int(precond), res, ctypes.byref(ms))
-    stats = [[BatchStats(r.iterations, r.refinements, r.residual_norm, r.status, r.reason)
-              for r in res[i * nf:(i + 1) * nf]] for i in range(k)]
+    del keep
     if rc != 0 and raise_on_failure:
         check(rc)
-    return AcSweepResult(xs, stats, ms.value)
+    return AcSweepResult(xs, _stats(res, k, nf), ms.value)
+
+
+class AcPlan:
+    """The direct AC sweep split into its symbolic half (ohm_ac_plan_create:
+    pattern union, RCM and band rows per circuit, uploaded once) and its
+    numeric half (solve: every (circuit, frequency) system by banded LU on
+    the device). Solutions of a sweep that fits the device in one chunk stay
+    resident for x()."""
+
+    def __init__(self, backend: HipBackend, mnas):
+        self.backend = backend
+        self.sizes = [m.size for m in mnas]
+        cs, keep, _ = _ac_circuits(mnas, 0, False)
+        self.h = ctypes.c_void_p()
+        check(lib().ohm_ac_plan_create(backend.h, ctypes.c_int64(len(mnas)), cs,
+                                       ctypes.byref(self.h)))
+        del keep
+        self.n_freqs = 0
+
+    def solve(self, freqs, keep_x: bool = False, raise_on_failure: bool = True) -> AcSweepResult:
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        nf = f.shape[0]
+        k = len(self.sizes)
+        xs = [np.zeros((nf, n), dtype=np.complex128) for n in self.sizes] if keep_x else None
+        xp = None
+        if keep_x:
+            xp = (N.c_f64p * max(k, 1))(*[x.ctypes.data_as(N.c_f64p) for x in xs])
+        res = (N.BatchResult * max(k * nf, 1))()
+        ms = ctypes.c_double(0.0)
+        rc = lib().ohm_ac_plan_solve(self.h, ctypes.c_int64(nf), ptr(f, N.c_f64p), xp, res,
+                                     ctypes.byref(ms))
+        self.n_freqs = nf
+        if rc != 0 and raise_on_failure:
+            check(rc)
+        return AcSweepResult(xs if keep_x else [None] * k, _stats(res, k, nf), ms.value)
+
+    def stats(self) -> dict:
+        """ohm_ac_plan_stats of the last solve: device ms per phase, rows per
+        part (0: one lane per whole system), systems redone in one lane,
+        chunks."""
+        ms = (ctypes.c_double * 6)()
+        rows, redo, chunks = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        check(lib().ohm_ac_plan_stats(self.h, ms, ctypes.byref(rows), ctypes.byref(redo),
+                                      ctypes.byref(chunks)))
+        names = ("relations", "reduced", "part_solve", "residual", "redo", "unpermute")
+        return {"phase_ms": dict(zip(names, list(ms))), "rows_per_part": rows.value,
+                "redone_systems": redo.value, "chunks": chunks.value}
+
+    def x(self, circuit: int, f_first: int = 0, f_count: int | None = None) -> np.ndarray:
+        """Resident solutions of one circuit: [f_count, n] complex."""
+        if f_count is None:
+            f_count = self.n_freqs - f_first
+        out = np.zeros((f_count, self.sizes[circuit]), dtype=np.complex128)
+        check(lib().ohm_ac_plan_get_x(self.h, ctypes.c_int64(circuit), ctypes.c_int64(f_first),
+                                      ctypes.c_int64(f_count), out.ctypes.data_as(N.c_f64p)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and not sys.is_finalizing():
+            lib().ohm_ac_plan_free(self.h)
+            self.h = None
 
 
 # ----------------------------------------------------------------- Newton --

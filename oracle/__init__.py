@@ -480,3 +480,24 @@ def transient_nonlinear(mna: _ir.MnaSystem, sources, pwl=None, method="adaptive"
             times.append(t)
             waves.append(x.copy())
     return np.array(times), np.array(waves).reshape(len(times), n), x, acc, rej
+
+
+def ac_sweep(mna: _ir.MnaSystem, freqs) -> np.ndarray:
+    """ac::run over one circuit (src/analysis/ac.rs:56-70) with the sparse LU
+    as the linear solve: x[f] = (G + j 2 pi f C)^-1 b_ac for every f
+    (oo_ac_sweep; the ordering is computed once per circuit). Releases the
+    GIL while it runs, so circuits can be swept on host threads."""
+    n = mna.size
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    g, c = mna.g, mna.c
+    grp, gcl, gv = _csr_args(g)
+    crp, ccl, cv = _csr_args(c) if c is not None else (None, None, None)
+    bb = np.ascontiguousarray(mna.b_ac, dtype=np.complex128)
+    x = np.zeros((f.shape[0], n), dtype=np.complex128)
+    _check(lib().oo_ac_sweep(ctypes.c_int64(n), _p(grp, u32p), _p(gcl, u32p), _p(gv, f64p),
+                             _p(crp, u32p) if c is not None else None,
+                             _p(ccl, u32p) if c is not None else None,
+                             _p(cv, f64p) if c is not None else None,
+                             _p(bb.view(np.float64), f64p), ctypes.c_int64(f.shape[0]),
+                             _p(f, f64p), _p(x.view(np.float64), f64p)))
+    return x

@@ -773,15 +773,14 @@ static void rcm_order(int64_t n, const uint32_t *rowptr, const uint32_t *col, in
   free(tmp);
 }
 
-int oo_sparse_lu_solve(int64_t n, const uint32_t *rowptr, const uint32_t *col, const double *val,
-                       const double *b, double *x, int cplx_vals) {
-  if (n == 0) return OHM_OK;
+/* Numeric banded LU with partial pivoting under a given symmetric ordering
+ * (perm: new position -> old row), then the two triangular solves. */
+static int band_lu_solve(int64_t n, const uint32_t *rowptr, const uint32_t *col, const double *val,
+                         const double *b, double *x, int cplx_vals, const int64_t *perm) {
   int w = cplx_vals ? 2 : 1;
   for (int64_t k = 0; k < (int64_t)rowptr[n] * w; k++)
     if (!isfinite(val[k])) return fail(OHM_ERR_SOLVE, "Solve error: sparse LU input contains NaN/Inf");
-  int64_t *perm = (int64_t *)xmalloc(sizeof(int64_t) * n);
   int64_t *iperm = (int64_t *)xmalloc(sizeof(int64_t) * n);
-  rcm_order(n, rowptr, col, perm);
   for (int64_t i = 0; i < n; i++) iperm[perm[i]] = i;
   int64_t kl = 0, ku = 0;
   for (int64_t r = 0; r < n; r++)
@@ -926,8 +925,69 @@ int oo_sparse_lu_solve(int64_t n, const uint32_t *rowptr, const uint32_t *col, c
 #undef AB
   free(ab);
   free(ipiv);
-  free(perm);
   free(iperm);
+  return rc;
+}
+
+int oo_sparse_lu_solve(int64_t n, const uint32_t *rowptr, const uint32_t *col, const double *val,
+                       const double *b, double *x, int cplx_vals) {
+  if (n == 0) return OHM_OK;
+  int64_t *perm = (int64_t *)xmalloc(sizeof(int64_t) * n);
+  rcm_order(n, rowptr, col, perm);
+  int rc = band_lu_solve(n, rowptr, col, val, b, x, cplx_vals, perm);
+  free(perm);
+  return rc;
+}
+
+/* ac::run over one circuit (src/analysis/ac.rs:56-70) with the sparse LU as
+ * the linear solve: for every frequency f, A = G + j 2 pi f C
+ * (form_complex_matrix, src/sparse.rs:149-178: G's and C's entries summed
+ * position by position on the union of the two patterns, rows in ascending
+ * column order) and x_f = A^-1 b_ac. The union pattern does not depend on f,
+ * so its ordering is computed once. x: n_freqs * n complex, frequency-major. */
+int oo_ac_sweep(int64_t n, const uint32_t *g_rp, const uint32_t *g_col, const double *g_val,
+                const uint32_t *c_rp, const uint32_t *c_col, const double *c_val,
+                const double *b_ac, int64_t n_freqs, const double *freqs, double *x) {
+  if (n == 0 || n_freqs == 0) return OHM_OK;
+  /* union pattern: merge the two sorted rows */
+  int64_t cap = (int64_t)g_rp[n] + (c_val ? (int64_t)c_rp[n] : 0);
+  uint32_t *rp = (uint32_t *)xmalloc(sizeof(uint32_t) * (n + 1));
+  uint32_t *cl = (uint32_t *)xcalloc(cap > 0 ? cap : 1, sizeof(uint32_t));
+  double *gv = (double *)xcalloc(cap > 0 ? cap : 1, sizeof(double));
+  double *cv = (double *)xcalloc(cap > 0 ? cap : 1, sizeof(double));
+  int64_t k = 0;
+  rp[0] = 0;
+  for (int64_t r = 0; r < n; r++) {
+    uint32_t a = g_rp[r], ae = g_rp[r + 1];
+    uint32_t c = c_val ? c_rp[r] : 0, ce = c_val ? c_rp[r + 1] : 0;
+    while (a < ae || c < ce) {
+      uint32_t ca = a < ae ? g_col[a] : 0xffffffffu, cc = c < ce ? c_col[c] : 0xffffffffu;
+      uint32_t j = ca < cc ? ca : cc;
+      cl[k] = j; /* a repeated column (never from the compiler) is summed by the LU's scatter */
+      gv[k] = ca == j ? g_val[a++] : 0.0;
+      cv[k] = cc == j ? c_val[c++] : 0.0;
+      k++;
+    }
+    rp[r + 1] = (uint32_t)k;
+  }
+  int64_t *perm = (int64_t *)xmalloc(sizeof(int64_t) * n);
+  rcm_order(n, rp, cl, perm);
+  double *av = (double *)xmalloc(sizeof(double) * 2 * (k > 0 ? k : 1));
+  int rc = OHM_OK;
+  for (int64_t f = 0; f < n_freqs && rc == OHM_OK; f++) {
+    const double om = 2.0 * 3.14159265358979323846 * freqs[f]; /* ac.rs:57 */
+    for (int64_t q = 0; q < k; q++) {
+      av[2 * q] = gv[q];
+      av[2 * q + 1] = om * cv[q];
+    }
+    rc = band_lu_solve(n, rp, cl, av, b_ac, x + 2 * n * f, 1, perm);
+  }
+  free(av);
+  free(perm);
+  free(rp);
+  free(cl);
+  free(gv);
+  free(cv);
   return rc;
 }
 

@@ -48,6 +48,15 @@ int oo_dense_solve(int64_t nrows, int64_t ncols, int64_t nb, const uint32_t *row
 int oo_sparse_lu_solve(int64_t n, const uint32_t *rowptr, const uint32_t *col, const double *val,
                        const double *b, double *x, int cplx);
 
+/* ac::run over one circuit (src/analysis/ac.rs:56-70) with oo_sparse_lu_solve's
+ * factorisation as the linear solve; A_f = G + j 2 pi f C on the union
+ * pattern (form_complex_matrix, src/sparse.rs:149-178), ordering computed
+ * once. b_ac: 2n doubles; x: n_freqs * 2n doubles, frequency-major. c_val may
+ * be NULL (no capacitive part). */
+int oo_ac_sweep(int64_t n, const uint32_t *g_rp, const uint32_t *g_col, const double *g_val,
+                const uint32_t *c_rp, const uint32_t *c_col, const double *c_val,
+                const double *b_ac, int64_t n_freqs, const double *freqs, double *x);
+
 /* Preconditioner kinds for the oracle BiCGSTAB. */
 enum { OO_PREC_NONE = 0, OO_PREC_JACOBI = 1, OO_PREC_ISAI = 2 };
 

@@ -13,6 +13,8 @@ Fixtures (numpy .npz, FP64):
   diode_mesh_256.npz    every 4th unknown of the 256x256 diode-mesh operating point
   rc_grid_512.npz       64 probe voltages over the first 50 backward-Euler
                         steps (h = 1 ps) of the 512x512 RC grid from its DC point
+                        (waves), and at every 50th step through step 1,000
+                        (waves_every)
   amplifier_4096.npz    the 4,096 stage outputs of the amplifier array and the
                         adjoint gradient for 64 seeded parameters
 """
@@ -57,7 +59,7 @@ def diode_mesh():
                         newton_iterations=its)
 
 
-def rc_grid(steps=50, h=1e-12):
+def rc_grid(steps=1000, h=1e-12, dense_steps=50, every=50):
     ckt = ir.rc_grid(512, 512, 1024, seed=3)
     mna = solver.compile(ckt)
     g = csr(mna.g).tocsc()
@@ -68,19 +70,24 @@ def rc_grid(steps=50, h=1e-12):
     lu = sla.splu((csr(mna.g) + cm / h).tocsc())
     probes = np.arange(0, mna.size, 4099)
     waves = [x[probes]]
+    sparse_waves = [x[probes]]
     t = 0.0
-    for _ in range(steps):
-        t_next = t + h
+    for k in range(1, steps + 1):
+        t_next = t + h  # t accumulates as in transient.rs:84-178
         b = np.zeros(mna.size)
         for e in src:  # evaluate_b_at_time (src/analysis/transient.rs:265-280)
             v = oracle.source_value(e, t_next)
             for j in range(int(e["n_idx"])):
                 b[int(e["idx"][j])] += float(e["sign"][j]) * v
         x = lu.solve(b + (1.0 / h) * (cm @ x))
-        waves.append(x[probes])
+        if k <= dense_steps:
+            waves.append(x[probes])
+        if k % every == 0:
+            sparse_waves.append(x[probes])
         t = t_next
     np.savez_compressed(os.path.join(OUT, "rc_grid_512.npz"), probes=probes,
-                        waves=np.array(waves), h=h)
+                        waves=np.array(waves), h=h, waves_every=np.array(sparse_waves),
+                        every=every)
 
 
 def amplifier():

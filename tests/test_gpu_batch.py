@@ -314,3 +314,48 @@ def test_ac_direct_full_size_ladders(be):
                              m.g.col_indices, m.g.row_pointers)
             ok, err = close(x[i], oracle.sparse_lu_solve(a, m.b_ac))
             assert ok, (m.size, freqs[i], err)
+
+
+@pytest.mark.parametrize("rows", ["0", "1", "2", "7", "64", "1000"])
+def test_ac_plan_partitioned_vs_lu(be, rows, monkeypatch):
+    """The direct sweep's partitioned solve (each system's rows cut into
+    parts of `rows` rows, joined through the parts' boundary unknowns; 0 =
+    one lane per whole system): every system's residual below 1e-12 and
+    sampled points against the oracle's LU, from resident solutions
+    (ohm_ac_plan_get_x) equal to the copied-out ones."""
+    monkeypatch.setenv("OHM_AC_PART_ROWS", rows)
+    ckts, freqs = ir.rlc_sweep(n_circuits=5, lo=20, hi=3000, seed=4)
+    mnas = [solver.compile(c) for c in ckts]
+    plan = solver.AcPlan(be, mnas)
+    res = plan.solve(freqs, keep_x=True)
+    st = plan.stats()
+    assert st["rows_per_part"] == int(rows) and st["chunks"] == 1
+    for k, (m, x, sts) in enumerate(zip(mnas, res.x, res.stats)):
+        assert all(s.converged and s.residual_norm < 1e-12 for s in sts)
+        cg = c_on_g_matrix(m)
+        for i in (0, 13, 50, 100):
+            a = ir.CsrMatrix(m.size, m.size, m.g.values + 2j * np.pi * freqs[i] * cg,
+                             m.g.col_indices, m.g.row_pointers)
+            ok, err = close(x[i], oracle.sparse_lu_solve(a, m.b_ac))
+            assert ok, (m.size, freqs[i], err)
+    plan.solve(freqs)  # solutions left resident
+    for k in (0, 3):
+        assert plan.x(k).tobytes() == res.x[k].tobytes()
+        assert plan.x(k, 60, 5).tobytes() == res.x[k][60:65].tobytes()
+
+
+def test_ac_plan_errors(be):
+    ckts, freqs = ir.rlc_sweep(n_circuits=2, lo=10, hi=40, seed=4)
+    plan = solver.AcPlan(be, [solver.compile(c) for c in ckts])
+    with pytest.raises(N.OhmnivoreError, match="not resident"):
+        plan.x(0)  # nothing solved yet
+    plan.solve(freqs[:7])
+    with pytest.raises(N.OhmnivoreError, match="out of range"):
+        plan.x(2)
+    with pytest.raises(N.OhmnivoreError, match="out of range"):
+        plan.x(0, 5, 3)
+    other = solver.AcPlan(be, [solver.compile(ckts[0])])
+    other.solve(freqs[:3])  # another sweep on the backend takes the workspace
+    with pytest.raises(N.OhmnivoreError, match="not resident"):
+        plan.x(0)
+    assert other.x(0).shape == (3, other.sizes[0])

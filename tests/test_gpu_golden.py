@@ -59,6 +59,10 @@ def test_diode_mesh_256_golden(be, precision, forcing, floor):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
 def test_rc_grid_512_golden(be, precision):
+    """The transient workload at full length (1,000 backward-Euler steps of
+    1 ps, warm-started ISAI(1)-BiCGSTAB at the benchmark's 1e-9 relative
+    residual) against the SuperLU fixture: every step of the first 50 and
+    every 50th step through step 1,000."""
     g = load("rc_grid_512.npz")
     ckt = ir.rc_grid(512, 512, 1024, seed=3)
     mna = solver.compile(ckt)
@@ -66,13 +70,19 @@ def test_rc_grid_512_golden(be, precision):
                           params=solver.bicgstab_params(tolerance=1e-13, max_iterations=100_000))
     x0 = dc.solve_real(mna.g, mna.b_dc)
     src, _ = T.source_table(ckt, mna)
-    steps = g["waves"].shape[0] - 1
+    every = int(g["every"])
+    steps = (g["waves_every"].shape[0] - 1) * every
+    assert steps == 1000
     res = T.Transient(be, mna, src).run(
         float(g["h"]), steps * float(g["h"]), method="be", x0=x0, probes=g["probes"],
-        params=solver.bicgstab_params(tolerance=1e-12, max_iterations=20_000, precision=precision),
+        params=solver.bicgstab_params(tolerance=1e-9, max_iterations=20_000, precision=precision),
         precond=N.PREC_ISAI)
     assert res.steps_accepted == steps
-    ok, err = close(res.waves[:, 0, :], g["waves"])
+    w = res.waves[:, 0, :]
+    dense = g["waves"].shape[0]
+    ok, err = close(w[:dense], g["waves"])
+    assert ok, err
+    ok, err = close(w[::every], g["waves_every"])
     assert ok, err
 
 

@@ -191,26 +191,50 @@ def test_zero_rhs_empty_and_dimension_errors(be):
         gs.solve_real(m.g, np.zeros(m.size + 1))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
-def test_graph_replay_matches_eager(be, precision):
+@pytest.mark.parametrize("precision,tol_mode", [("fp64", "rhs"), ("fp32_ir", "rhs"),
+                                                ("fp64", "initial")])
+def test_graph_replay_matches_eager(be, precision, tol_mode):
     """Graph-captured iteration chunks, replayed many times with eager solves
-    and host syncs in between, give bitwise the eager result every time."""
+    and host syncs in between, give bitwise the eager result every time.
+    tol_mode 'initial' with a warm start takes the 4-iteration opening
+    chunk (the inexact-Newton sub-solve form) before the 8-iteration ones."""
     m = solver.compile(ir.diode_mesh(40, 40, seed=9))
     a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
     p = be.preconditioner(a, N.PREC_ISAI, 1)
     b, x = be.new_buffer(m.size), be.new_buffer(m.size)
     be.upload_vec(m.b_dc, b)
-    prm = solver.bicgstab_params(tolerance=1e-11, precision=precision)
+    warm = tol_mode == "initial"
+    prm = solver.bicgstab_params(tolerance=1e-11, precision=precision, tol_mode=tol_mode,
+                                 warm_start=warm)
+    x0 = np.linspace(0.0, 0.5, m.size) if warm else np.zeros(m.size)
     results = []
     for mode in (False, True, False, True, True):
         be.set_graphs(mode)
-        be.upload_vec(np.zeros(m.size), x)
+        be.upload_vec(x0, x)
         s = be.bicgstab(a, b, x, p, prm)
         be.sync()
         results.append((be.download_vec(x).tobytes(), s.iterations))
     be.set_graphs(True)
     assert all(r == results[0] for r in results)
     assert results[0][1] > 8  # several replays per solve
+
+
+def test_graph_replay_matches_eager_in_newton(be):
+    """A Newton run with Eisenstat-Walker forcing (warm-started sub-solves
+    with the opening 4-iteration chunk, no poll before it) gives bitwise the
+    same iterate and per-Newton BiCGSTAB counts with and without graphs."""
+    m = solver.compile(ir.diode_mesh(48, 40, seed=5))
+    ctx = solver.NewtonContext(be, m.g, m.b_dc, m.diode_descriptors)
+    kw = dict(abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-6, forcing="ew1",
+              forcing_max=0.9, forcing_safeguard="always", bicgstab_floor=1e-18)
+    out = []
+    for mode in (False, True, True, False):
+        be.set_graphs(mode)
+        st = ctx.run(**kw)
+        out.append((ctx.x().tobytes(), st.bicgstab_iters_per_newton))
+    be.set_graphs(True)
+    assert all(o == out[0] for o in out)
+    assert len(out[0][1]) > 3 and sum(out[0][1]) > 8
 
 
 def test_warm_start_and_determinism(be):

@@ -188,3 +188,42 @@ def test_one_shot_newton_solve_errors(be):
     x, st = solver.newton_solve(be, ml.g, ml.b_dc, abs_tol=1e-12, bicgstab_tolerance=1e-12)
     assert_close(x, oracle.dense_solve(ml.g, ml.b_dc))
     assert abs(x[1] - 5.0) < 1e-6
+
+
+@pytest.mark.parametrize("linear_solver", ["bicgstab", "dense"])
+def test_non_finite_device_parameter_is_numerical_error(be, linear_solver):
+    """A diode whose saturation current is infinite makes the assembled
+    system non-finite at the first evaluation: NewtonNumericalError at
+    iteration 0 (newton.rs:459-463), for the BiCGSTAB sub-solve (the failed
+    sub-solve reads the flag) and for dense LU (the convergence word
+    carries it)."""
+    m = solver.compile(ir.diode_mesh(8, 8, seed=3, diode_every=4))
+    d = np.array(m.diode_descriptors, dtype=ir.DIODE_DESC_DTYPE, copy=True)
+    d["is_val"][1] = np.inf
+    with pytest.raises(N.OhmnivoreError, match="numerical error at iteration 0") as e:
+        solver.newton_solve(be, m.g, m.b_dc, d, abs_tol=1e-9, linear_solver=linear_solver)
+    assert e.value.kind == "NewtonNumericalError"
+
+
+def test_rejected_update_leaves_context_unchanged(be):
+    """ohm_newton_update checks every device family before replacing any:
+    an update whose BJT descriptors change a node is refused, and the
+    diode parameters passed beside it are not applied."""
+    ckt = ir.amplifier_array(16, seed=5)
+    ckt.add_diode_model("DX", is_=1e-14, n=1.0)
+    ckt.resistor("Rx", "vdd", "dx", 1e4)
+    ckt.diode("Dx", "dx", "0", "DX")
+    m = solver.compile(ckt)
+    ctx = ctx_for(be, m)
+    kw = dict(abs_tol=1e-10, bicgstab_tolerance=1e-8, max_iterations=100)
+    ctx.run(**kw)
+    x0 = ctx.x()
+    q = np.array(m.bjt_descriptors, dtype=ir.BJT_DESC_DTYPE, copy=True)
+    q["collector_idx"][0] = q["emitter_idx"][0]
+    d = np.array(m.diode_descriptors, dtype=ir.DIODE_DESC_DTYPE, copy=True)
+    assert len(d) == 1
+    d["is_val"][:] *= 10
+    with pytest.raises(N.OhmnivoreError, match="changes nodes or stamp slots"):
+        ctx.update(diodes=d, bjts=q)
+    ctx.run(**kw)
+    assert ctx.x().tobytes() == x0.tobytes()

@@ -310,3 +310,35 @@ def test_mosfet_limit_encoded_scale():
     d["node_fixed_packed"] = 2
     xn = oracle.newton_limit(2, [0.0, 0.0], [0.2, 2.0], mosfets=d)
     assert xn[1] == 2.0
+
+
+def test_ac_sweep_matches_per_frequency_lu_and_closed_form():
+    """oo_ac_sweep (ac::run with the sparse LU, ordering computed once per
+    circuit) equals oo_sparse_lu_solve on G + j 2 pi f C formed per
+    frequency, agrees with dense GE, and reproduces an RC low-pass's
+    closed form |H| = 1 / sqrt(1 + (2 pi f R C)^2) (tests/integration_test.rs:
+    132-180)."""
+    from ohmnivore_amd import solver
+    ckts, freqs = ir.rlc_sweep(n_circuits=3, lo=30, hi=200, seed=4)
+    for ckt in ckts:
+        m = solver.compile(ckt)
+        x = oracle.ac_sweep(m, freqs[::7])
+        cg = np.zeros(m.g.nnz)
+        for r in range(m.size):
+            for q in range(m.c.row_pointers[r], m.c.row_pointers[r + 1]):
+                cg[m.g.value_index(r, int(m.c.col_indices[q]))] += m.c.values[q]
+        for k, f in enumerate(freqs[::7]):
+            a = ir.CsrMatrix(m.size, m.size, m.g.values + 2j * np.pi * f * cg,
+                             m.g.col_indices, m.g.row_pointers)
+            assert np.array_equal(x[k], oracle.sparse_lu_solve(a, m.b_ac))
+            xd = oracle.dense_solve(a, m.b_ac)
+            assert np.allclose(x[k], xd, rtol=1e-9, atol=1e-12 * np.abs(xd).max())
+    rc = ir.Circuit()
+    rc.vsource("V1", "in", "0", ac=(1.0, 0.0))
+    rc.resistor("R1", "in", "out", 1e3)
+    rc.capacitor("C1", "out", "0", 1e-6)
+    m = solver.compile(rc)
+    f = np.array([1.0, 159.15494309189535, 1e4])
+    x = oracle.ac_sweep(m, f)
+    h = np.abs(x[:, m.node_names.index("out")])
+    assert np.allclose(h, 1.0 / np.sqrt(1.0 + (2 * np.pi * f * 1e3 * 1e-6) ** 2), rtol=1e-12)
