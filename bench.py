@@ -1,23 +1,27 @@
 """Benchmark of the MI355X solve backend on one GPU.
 
-Headline ("step" = one solve; the value of the JSON line): the nonlinear DC
-operating point of the 256x256 diode mesh — 65,540 unknowns, 326,664 nnz,
-four 1 V corner sources, 8,192 diodes (IS 1e-14, N 1) from every 8th node —
-solved from x = 0 by GPU Newton-Raphson with ISAI(1)-preconditioned
-BiCGSTAB, in plain FP64 and with the FP32 inner solver inside FP64
-iterative refinement. Inputs are uploaded once; the timed region is K full
-solves. Its error against the oracle's FP64 LU-Newton, the SpMV kernel's
-HBM roofline position (HIP events, live) and the CPU baseline (oracle on
-this host) ride along.
+Headline ("step" = one sweep; the value of the JSON line): the AC sweep of
+256 seeded RLC ladders of uneven size (1k-64k sections, 24.9 M unknowns in
+all) over 101 log-spaced frequencies from 1 Hz to 1 GHz — 25,856 complex
+systems, every one solved by banded LU with partial pivoting after a reverse
+Cuthill-McKee ordering, each system's rows cut into parts eliminated in
+parallel and joined through the parts' boundary unknowns. The symbolic half
+(pattern union, RCM, band rows) is built once and resident in HBM when the
+timed region starts; a step is the numeric sweep over all 25,856 systems.
+Every system of the last timed sweep is then compared with the oracle's
+FP64 sparse LU (ac::run with a direct solve) on the host's cores, and that
+run is the CPU baseline. The dominant kernel's HBM roofline position comes
+from HIP events on the backend's stream.
 
 The other workloads are measured once each (they are not the headline):
+  diode_mesh_256       nonlinear DC, 65,540 unknowns: GPU Newton with
+                       ISAI(1)-BiCGSTAB in FP64 and FP32+FP64 refinement,
+                       against the committed FP64 LU fixture
   resistor_ladder_10k  linear DC, 10,002 unknowns, Jacobi-BiCGSTAB to 1e-10
   rc_grid_512          transient, 262,148 unknowns, 1,000 backward-Euler
-                       steps of 1 ps, warm-started; plus K Monte Carlo
-                       instances batched versus the same instances one by one
-  rlc_ac_256x101       256 uneven RLC ladders x 101 frequencies = 25,856
-                       complex systems, batched banded direct solve versus
-                       one system per call
+                       steps of 1 ps, warm-started, against the committed
+                       SuperLU fixture; plus K Monte Carlo instances batched
+                       versus the same instances one by one
   amplifier_adjoint    4,096 stages (8,194 unknowns): operating point and
                        the adjoint gradient of sum (V_out - 0.9)^2 with
                        respect to 12,288 conductances and 2,048 kp
@@ -25,7 +29,7 @@ The other workloads are measured once each (they are not the headline):
                        inverter chains of 10-200 stages, .DC and .TRAN 1n 100n)
                        from netlist text to CSV, wall time per netlist
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only headline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only headline|ac|all]
 """
 from __future__ import annotations
 
@@ -45,6 +49,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # copy kernel of known size (65,536 KiB each way): FETCH x 1.9994, WRITE x 1.0.
 # Summary: profiles/spmv_diode_mesh_256_counters.md.
 SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
+# HBM traffic of one launch of the AC sweep's dominant kernel, from the same
+# two counter passes over `bench.py --roofline-only` (None until measured).
+AC_TRAFFIC_BYTES = None
 
 
 def log(msg: str) -> None:
@@ -78,117 +85,226 @@ def within(x, ref, rel=1e-6, abs_=1e-9):
 
 
 # ------------------------------------------------------------- headline --
-def headline(args, be, N, ir, solver):
+def ac_bytes(sizes, n_freqs: int):
+    """Algorithmic bytes of one sweep's kernels. Waves hold up to 64
+    frequencies of one circuit; the band rows of G and C and the permuted
+    b (3 + 3 + 2 doubles per row at half bandwidth 1) are read once per wave
+    and direction; per (system, row) the part solve writes the U row with
+    fill (3 complex) and the eliminated right-hand side, reads both back and
+    writes x (144 bytes)."""
+    waves = -(-n_freqs // 64)
+    rows = float(sum(sizes))
+    band = 64.0
+    return {"part_solve": rows * (144.0 * n_freqs + band * waves),
+            "relations": rows * band * 2 * waves,
+            "residual": rows * (16.0 * n_freqs + band * waves)}
+
+
+def check_all_systems(plan, mnas, freqs, threads: int):
+    """Every (circuit, frequency) solution of the plan's last sweep against
+    the oracle's FP64 sparse LU (oracle.ac_sweep: the union pattern's RCM
+    once per circuit, one banded LU per frequency), circuits spread over
+    `threads` host threads. Returns the parity figures and the oracle's
+    wall time."""
+    import concurrent.futures as cf
+
+    import oracle
+
+    def job(c):
+        t = time.perf_counter()
+        ref = oracle.ac_sweep(mnas[c], freqs)
+        return ref, time.perf_counter() - t
+
+    n_sys = len(mnas) * len(freqs)
+    bad, worst_abs, worst_rel, busy = 0, 0.0, 0.0, 0.0
+    order = iter(sorted(range(len(mnas)), key=lambda c: -mnas[c].size))
+    pending = {}
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        def submit():
+            c = next(order, None)
+            if c is not None:
+                pending[ex.submit(job, c)] = c
+
+        for _ in range(2 * threads):
+            submit()
+        while pending:
+            done, _ = cf.wait(list(pending), return_when=cf.FIRST_COMPLETED)
+            for fut in done:
+                c = pending.pop(fut)
+                ref, sec = fut.result()
+                busy += sec
+                x = plan.x(c)
+                err = np.abs(x - ref)
+                bound = np.maximum(1e-6 * np.abs(ref), 1e-9)
+                bad += int(np.count_nonzero((err > bound).any(axis=1)))
+                worst_abs = max(worst_abs, float(err.max(initial=0.0)))
+                worst_rel = max(worst_rel, float((err / np.maximum(np.abs(ref), 1e-300)).max()))
+                del x, ref, err, bound
+                submit()
+    wall = time.perf_counter() - t0
+    return {"systems_checked": n_sys, "systems_outside_1e-6_rel_1e-9_abs": bad,
+            "max_abs_err": worst_abs, "max_rel_err": worst_rel}, wall, busy
+
+
+def ac_headline(args, be, N, ir, solver):
+    t0 = time.perf_counter()
+    ckts, freqs = ir.rlc_sweep(n_circuits=args.ac_circuits, seed=4)
+    mnas = [solver.compile(c) for c in ckts]
+    del ckts
+    t_build = time.perf_counter() - t0
+    nf = int(freqs.shape[0])
+    n_sys = len(mnas) * nf
+    sizes = [m.size for m in mnas]
+    log(f"AC workload: {len(mnas)} circuits, {sum(sizes):,} unknowns, {n_sys:,} systems "
+        f"(built in {t_build:.1f} s)")
+    t0 = time.perf_counter()
+    plan = solver.AcPlan(be, mnas)
+    t_plan = time.perf_counter() - t0
+    first = None
+    for w in range(max(args.warmup, 1)):  # the first sweep autotunes the rows per part
+        t0 = time.perf_counter()
+        plan.solve(freqs)
+        if w == 0:
+            first = time.perf_counter() - t0
+    be.sync()
+    t0 = time.perf_counter()
+    res = None
+    for _ in range(args.steps):
+        res = plan.solve(freqs)
+    be.sync()
+    dt = time.perf_counter() - t0
+    st = plan.stats()
+    worst = max(s.residual_norm for sts in res.stats for s in sts)
+    failed = sum(1 for sts in res.stats for s in sts if not s.converged)
+    ms = st["phase_ms"]
+    by = ac_bytes(sizes, nf)
+    kernel = {"part_solve": "k_part_solve" if st["rows_per_part"] else "k_band_lu<1>",
+              "relations": "k_part_rel", "residual": "k_band_resid<1> + k_band_resid_sum"}
+    dom = max(("part_solve", "relations", "residual"), key=lambda k: ms[k])
+    achieved = by[dom] / (ms[dom] * 1e-3) / 1e9
+    log(f"AC sweep: {1e3 * dt / args.steps:.1f} ms/sweep ({n_sys * args.steps / dt:,.0f} "
+        f"systems/s), device {sum(ms.values()):.1f} ms {ms}, rows/part {st['rows_per_part']}, "
+        f"redone {st['redone_systems']}, worst residual {worst:.1e}")
+    parity, cpu = None, None
+    if not args.no_cpu_baseline:
+        import os
+        threads = max(1, min(16, os.cpu_count() or 1))
+        parity, wall, busy = check_all_systems(plan, mnas, freqs, threads)
+        cpu = {"value": n_sys / wall, "unit": "solves/s", "cores": threads, "kind": "port",
+               "host": host_info(threads),
+               "sample": f"all {n_sys:,} systems of the sweep: oracle.ac_sweep (ac::run with the "
+                         f"oracle's FP64 sparse LU: RCM once per circuit, banded LU with partial "
+                         f"pivoting per frequency) on {threads} host threads, {wall:.1f} s wall, "
+                         f"{busy:.1f} thread-s ({1e3 * busy / n_sys:.2f} ms per system on one "
+                         f"thread)"}
+        log(f"AC parity: {parity}; CPU oracle {n_sys / wall:,.0f} systems/s on {threads} threads")
+    out = {
+        "value": n_sys * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+        "systems": n_sys, "circuits": len(mnas), "frequencies": nf,
+        "unknowns_per_sweep": int(sum(sizes)) * nf,
+        "rows_per_part": st["rows_per_part"], "redone_systems": st["redone_systems"],
+        "chunks": st["chunks"], "device_ms_per_sweep": ms,
+        "worst_residual": worst, "failed_systems": failed,
+        "build_s": t_build, "symbolic_plan_s": t_plan, "first_sweep_s_with_autotune": first,
+        "end_to_end_s": t_plan + dt / args.steps,
+        "roofline": {"bound": "hbm", "kernel": kernel[dom], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": AC_TRAFFIC_BYTES,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                     "copy-calibrated)" if AC_TRAFFIC_BYTES else None,
+                     "bytes_per_launch": by[dom], "avg_launch_ms": ms[dom],
+                     "bytes_per_launch_all": by},
+        "parity": parity, "cpu_baseline": cpu,
+    }
+    return out, mnas, freqs
+
+
+def mesh_newton(args, be, N, ir, solver):
+    """Nonlinear DC on the 256x256 diode mesh, FP64 and FP32-IR, against the
+    committed FP64 LU-Newton fixture (every 4th unknown)."""
+    import os
     ckt = ir.diode_mesh(256, 256, seed=2)
     m = solver.compile(ckt)
     ctx = solver.NewtonContext(be, m.g, m.b_dc, m.diode_descriptors)
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                "diode_mesh_256.npz"))
+    ref = gold["x_every4"]
     # bicgstab_floor: the sub-solves may stop at floor * ||b||. The assembled
     # ||b|| (3.4e4) is set by the diodes at the 1 V corners, so the default
     # 1e-15 stops Newton at ||F|| ~ 3e-11 and leaves ~5e-7 V on the mesh
     # nodes; 1e-18 reaches ||F|| ~ 3e-14 and errors ~3e-10 V.
     # forcing: Eisenstat-Walker choice 1 with its safeguard at every
     # iteration, so eta cannot collapse on the large residual drops of the
-    # diode turn-on phase (a sub-solve to 1e-6 there is wasted on the
-    # limited steps that follow). eta_max 0.9 is Eisenstat and Walker's
-    # published ceiling; against 0.1 it took fewer BiCGSTAB iterations in
-    # both forms on this mesh and on a 192x160 mesh of another seed.
+    # diode turn-on phase. eta_max 0.9 is Eisenstat and Walker's published
+    # ceiling.
     newton_kw = dict(abs_tol=1e-10, max_iterations=100, bicgstab_tolerance=1e-6,
                      bicgstab_max_iterations=20_000, forcing="ew1", forcing_max=0.9,
                      forcing_safeguard="always", bicgstab_floor=1e-18)
     forms = {}
+    steps = max(1, min(args.steps, 5))
     for prec in ("fp64", "fp32_ir", "fp64_fixed_tol"):
         kw = dict(newton_kw, precision=prec)
         if prec == "fp64_fixed_tol":  # every sub-solve to 1e-6 (no forcing)
             kw.update(precision="fp64", forcing="fixed")
-        for _ in range(args.warmup):
-            ctx.run(**kw)
+        ctx.run(**kw)
         be.sync()
         t0 = time.perf_counter()
         stats = None
-        for _ in range(args.steps):
+        for _ in range(steps):
             stats = ctx.run(**kw)
         be.sync()
         dt = time.perf_counter() - t0
+        e, ok = within(ctx.x()[::4], ref)
         forms[prec] = {
-            "solves_per_s": args.steps / dt,
-            "ms_per_solve": 1e3 * dt / args.steps,
+            "solves_per_s": steps / dt, "ms_per_solve": 1e3 * dt / steps,
             "newton_iterations": stats.newton_iterations,
             "bicgstab_iterations": stats.bicgstab_iterations,
-            "bicgstab_iters_per_newton": stats.bicgstab_iters_per_newton,
             "fp64_fallbacks": stats.fp64_fallbacks,
-            "x": ctx.x(),
-        }
+            "max_abs_err_vs_lu_fixture": e, "within_1e-6_rel_1e-9_abs": ok,
+            "us_per_bicgstab_iteration": 1e3 * dt / steps / max(stats.bicgstab_iterations, 1)}
         log(f"diode mesh {prec}: {forms[prec]['ms_per_solve']:.2f} ms/solve, "
-            f"{stats.newton_iterations} Newton, {stats.bicgstab_iterations} BiCGSTAB its")
-
-    # SpMV roofline (dominant kernel), HIP events on the backend stream
+            f"{stats.newton_iterations} Newton, {stats.bicgstab_iterations} BiCGSTAB its, "
+            f"err {e:.1e}")
+    it_b = isai_iteration_bytes(m.g)
+    us = forms["fp64"]["us_per_bicgstab_iteration"]
+    out = {"unknowns": m.size, "nnz": m.g.nnz, "forms": forms,
+           "iteration_roofline": {
+               "bytes_per_iteration": it_b, "us_per_iteration_incl_newton": us,
+               "achieved_gbs": it_b / (us * 1e-6) / 1e9,
+               "frac": it_b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS}}
+    # stand-alone SpMV of the system matrix, HIP events on the backend stream
     a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
     bx, by = be.new_buffer(m.size), be.new_buffer(m.size)
     be.upload_vec(np.ones(m.size), bx)
-    ms = ctypes.c_double()
-    N.check(N.lib().ohm_spmv_timed(be.h, a.h, bx.h, by.h, 200, 0, ctypes.byref(ms)))
-    spmv_b = spmv_bytes(m.size, m.g.nnz, 8)
-    spmv_gbs = spmv_b / (ms.value * 1e-3) / 1e9
-    val, col, rp, nbig = replicated(m.g, 64)
-    abig = be.upload_matrix(val, col, rp, nbig)
-    del val, col, rp
-    bbx, bby = be.new_buffer(nbig), be.new_buffer(nbig)
-    be.upload_vec(np.ones(nbig), bbx)
-    msb, msb32 = ctypes.c_double(), ctypes.c_double()
-    N.check(N.lib().ohm_spmv_timed(be.h, abig.h, bbx.h, bby.h, 50, 0, ctypes.byref(msb)))
-    N.check(N.lib().ohm_spmv_timed(be.h, abig.h, bbx.h, bby.h, 50, 1, ctypes.byref(msb32)))
-    big_b = spmv_bytes(nbig, m.g.nnz * 64, 8)
-    big32_b = spmv_bytes(nbig, m.g.nnz * 64, 4)
-    big_gbs = big_b / (msb.value * 1e-3) / 1e9
-    big32_gbs = big32_b / (msb32.value * 1e-3) / 1e9
-    del abig, bbx, bby
-    # practical HBM ceilings: read-only streaming and copy over 1 GiB
-    stream_bytes = 1 << 30
-    ms_rd, ms_cp = ctypes.c_double(), ctypes.c_double()
-    N.check(N.lib().ohm_stream_timed(be.h, ctypes.c_int64(stream_bytes), 0, 20, ctypes.byref(ms_rd)))
-    N.check(N.lib().ohm_stream_timed(be.h, ctypes.c_int64(stream_bytes), 1, 20, ctypes.byref(ms_cp)))
-    read_gbs = stream_bytes / (ms_rd.value * 1e-3) / 1e9
-    copy_gbs = 2 * stream_bytes / (ms_cp.value * 1e-3) / 1e9
-
-    cpu, errors = None, {}
-    if not (args.no_cpu_baseline or args.profile):
-        import oracle
-        t0 = time.perf_counter()
-        x_ref, its, _ = oracle.newton_solve(m.g, m.b_dc, m.diode_descriptors, abs_tol=1e-12,
-                                            max_iterations=20, initial_guess=forms["fp64"]["x"])
-        per_iter = (time.perf_counter() - t0) / max(its, 1)
-        cold_iters = 16  # oracle LU-Newton from x = 0 on this circuit (measured, same algorithm)
-        cpu = {"value": 1.0 / (per_iter * cold_iters), "unit": "solves/s", "cores": 1,
-               "host": host_info(),
-               "kind": "port",
-               "sample": f"{its} LU-Newton iterations (RCM + banded LU, 1 thread) warm-started "
-                         f"at the GPU answer, {per_iter:.2f} s/iteration; rate = 1/(s/iteration "
-                         f"x {cold_iters} cold-start iterations)"}
-        for prec in forms:
-            e, ok = within(forms[prec]["x"], x_ref)
-            errors[prec] = {"max_abs_err": e, "within_1e-6_rel_1e-9_abs": ok}
-    out = {
-        "forms": forms, "errors": errors, "cpu": cpu, "newton_kw": newton_kw,
-        "roofline": {
-            "bound": "hbm", "kernel": "k_spmv<double> on the 326,664-nnz mesh matrix",
-            "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": SPMV_TRAFFIC_BYTES,
-            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "bytes_per_launch": spmv_b,
-            "avg_launch_us": ms.value * 1e3},
-        "spmv_large_working_set": {
-            "copies": 64, "bytes_per_launch": big_b, "fp64_gbs": big_gbs,
-            "fp64_frac": big_gbs / HBM_PEAK_GBS, "fp32_bytes_per_launch": big32_b,
-            "fp32_gbs": big32_gbs, "fp32_frac": big32_gbs / HBM_PEAK_GBS,
-            "fp64_frac_of_measured_read_ceiling": big_gbs / read_gbs},
-        "hbm_measured": {
-            "buffer_bytes": stream_bytes, "stream_read_gbs": read_gbs,
-            "copy_gbs": copy_gbs, "read_frac_of_peak": read_gbs / HBM_PEAK_GBS},
-    }
+    msv = ctypes.c_double()
+    N.check(N.lib().ohm_spmv_timed(be.h, a.h, bx.h, by.h, 200, 0, ctypes.byref(msv)))
+    b = spmv_bytes(m.size, m.g.nnz, 8)
+    out["spmv"] = {"bytes_per_launch": b, "avg_launch_us": msv.value * 1e3,
+                   "achieved_gbs": b / (msv.value * 1e-3) / 1e9,
+                   "frac": b / (msv.value * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "traffic_bytes_rocprof": SPMV_TRAFFIC_BYTES}
     return out
 
 
-def host_info() -> dict:
+def isai_iteration_bytes(g) -> float:
+    """Algorithmic bytes of one ISAI(1)-BiCGSTAB iteration in FP64: two SpMVs
+    with A, two applies of M_U M_L (M_L on pattern(L^2), M_U on
+    pattern(U^2) of A's lower / upper parts with the diagonal), and the
+    vector traffic of the fused schedule (22 n doubles)."""
+    import scipy.sparse as sp
+    n = g.nrows
+    a = sp.csr_matrix((np.ones(g.nnz), g.col_indices, g.row_pointers), shape=(n, n))
+    a = a + sp.identity(n, format="csr")
+    lo = sp.tril(a, format="csr")
+    up = sp.triu(a, format="csr")
+    nl = (lo @ lo).nnz
+    nu = (up @ up).nnz
+    return float(2 * spmv_bytes(n, g.nnz, 8) + 2 * (spmv_bytes(n, nl, 8) + spmv_bytes(n, nu, 8))
+                 + 22 * n * 8)
+
+
+def host_info(threads: int = 1) -> dict:
     """The CPU the baseline ran on: logical CPUs visible and the model name."""
     import os
     model = "unknown"
@@ -200,7 +316,7 @@ def host_info() -> dict:
                     break
     except OSError:
         pass
-    return {"nproc": os.cpu_count(), "cpu_model": model, "threads_used": 1}
+    return {"nproc": os.cpu_count(), "cpu_model": model, "threads_used": threads}
 
 
 # ------------------------------------------------------------ workloads --
@@ -242,22 +358,39 @@ def rc_grid(be, N, ir, solver, steps, mc_k, mc_steps):
                           params=solver.bicgstab_params(tolerance=1e-12, max_iterations=100_000))
     x0 = dc.solve_real(mna.g, mna.b_dc)
     src, _ = T.source_table(ckt, mna)
-    probes = np.arange(0, mna.size, 4099)
+    import os
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                "rc_grid_512.npz"))
+    probes = gold["probes"]
+    every = int(gold["every"])
     out = {"unknowns": mna.size, "nnz": mna.g.nnz, "steps": steps, "h": 1e-12}
+    it_b = isai_iteration_bytes(mna.g)
     for name, pc, prec in (("fp64_isai", N.PREC_ISAI, "fp64"), ("fp64_jacobi", N.PREC_JACOBI, "fp64"),
                            ("fp32_ir_isai", N.PREC_ISAI, "fp32_ir")):
-        n_steps = steps if prec == "fp64" else max(steps // 10, 1)
         tr = T.Transient(be, mna, src)
         prm = solver.bicgstab_params(tolerance=1e-9, max_iterations=20_000, precision=prec)
         t0 = time.perf_counter()
-        r = tr.run(1e-12, n_steps * 1e-12, method="be", x0=x0, probes=probes, params=prm,
+        r = tr.run(1e-12, steps * 1e-12, method="be", x0=x0, probes=probes, params=prm,
                    precond=pc)
         dt = time.perf_counter() - t0
         out[name] = {"steps": r.steps_accepted, "ms_per_step": 1e3 * dt / r.steps_accepted,
                      "steps_per_s": r.steps_accepted / dt,
                      "bicgstab_iters_per_step": r.bicgstab_iterations / r.steps_accepted}
+        w = r.waves[:, 0, :]
+        if w.shape[0] > every * (gold["waves_every"].shape[0] - 1):
+            e, ok = within(w[::every][: gold["waves_every"].shape[0]], gold["waves_every"])
+            e2, ok2 = within(w[: gold["waves"].shape[0]], gold["waves"])
+            out[name].update({"max_abs_err_vs_superlu_fixture": max(e, e2),
+                              "within_1e-6_rel_1e-9_abs": ok and ok2})
+        if pc == N.PREC_ISAI and prec == "fp64":
+            us = 1e6 * dt / max(r.bicgstab_iterations, 1)
+            out[name]["iteration_roofline"] = {
+                "bytes_per_iteration": it_b, "us_per_iteration_incl_step": us,
+                "achieved_gbs": it_b / (us * 1e-6) / 1e9,
+                "frac": it_b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS}
         log(f"rc grid {name}: {out[name]['ms_per_step']:.2f} ms/step, "
-            f"{out[name]['bicgstab_iters_per_step']:.1f} its/step")
+            f"{out[name]['bicgstab_iters_per_step']:.1f} its/step, "
+            f"parity {out[name].get('within_1e-6_rel_1e-9_abs')}")
     # Monte Carlo: K perturbed instances batched vs one at a time (both FP64, Jacobi)
     cg = T.c_on_g(mna)
     rng = np.random.default_rng(7)
@@ -288,29 +421,28 @@ def rc_grid(be, N, ir, solver, steps, mc_k, mc_steps):
     return out
 
 
-def rlc_ac(be, N, ir, solver, n_circuits):
-    t0 = time.perf_counter()
-    ckts, freqs = ir.rlc_sweep(n_circuits=n_circuits, seed=4)
-    mnas = [solver.compile(c) for c in ckts]
-    del ckts
-    t_build = time.perf_counter() - t0
-    n_sys = n_circuits * freqs.shape[0]
-    t0 = time.perf_counter()
-    res = solver.ac_sweep_circuits(be, mnas, freqs, method="direct", keep_x=False)
-    t_batched = time.perf_counter() - t0
-    worst = max(s.residual_norm for sts in res.stats for s in sts)
-    # plain: one system per call, same direct method (sample of 8 systems)
-    picks = [(i * (n_circuits // 8), (i * 13) % freqs.shape[0]) for i in range(8)]
-    t0 = time.perf_counter()
-    for c, f in picks:
-        solver.ac_sweep_circuits(be, [mnas[c]], freqs[f:f + 1], method="direct", keep_x=False)
-    t_plain_per = (time.perf_counter() - t0) / len(picks)
-    # plain single-system Jacobi-BiCGSTAB (the reference's AC method) on a sample
+def ac_plain(be, N, ir, solver, mnas, freqs):
+    """One system per call on the same device path (a plan per circuit,
+    its symbolic half cached, one frequency per solve; device time only),
+    on 8 sampled (circuit, frequency) pairs; and the reference's own AC
+    method, plain Jacobi-BiCGSTAB on one system (src/solver/gpu.rs:210-735),
+    on 2 of them."""
     from ohmnivore_amd import transient as T
+    nc, nf = len(mnas), int(freqs.shape[0])
+    picks = [(i * (nc // 8), (i * 13) % nf) for i in range(8)]
+    dev_ms, wall = [], []
+    for c, f in picks:
+        p = solver.AcPlan(be, [mnas[c]])
+        p.solve(freqs[f:f + 1])  # first solve: autotunes its rows per part
+        t0 = time.perf_counter()
+        r = p.solve(freqs[f:f + 1])
+        wall.append(time.perf_counter() - t0)
+        dev_ms.append(sum(p.stats()["phase_ms"].values()))
+        del p, r
     gs = solver.GpuSolver(be, precond=N.PREC_JACOBI,
                           params=solver.bicgstab_params(tolerance=1e-10, max_iterations=10_000))
     sample = []
-    for c, f in picks[:3]:
+    for c, f in picks[:2]:
         m = mnas[c]
         cg = T.c_on_g(m)
         a = ir.CsrMatrix(m.size, m.size, m.g.values + 2j * np.pi * freqs[f] * cg,
@@ -324,16 +456,12 @@ def rlc_ac(be, N, ir, solver, n_circuits):
         sample.append({"unknowns": m.size, "freq_hz": float(freqs[f]), "converged": conv,
                        "iterations": gs.last_stats.iterations,
                        "seconds": time.perf_counter() - t1})
-    out = {"circuits": n_circuits, "frequencies": int(freqs.shape[0]), "systems": n_sys,
-           "unknowns_total": int(sum(m.size for m in mnas)) * int(freqs.shape[0]),
-           "build_s": t_build, "batched_s": t_batched, "batched_device_ms": res.device_ms,
-           "batched_solves_per_s": n_sys / t_batched, "worst_residual": worst,
-           "plain_direct_s_per_system": t_plain_per,
-           "plain_solves_per_s": 1 / t_plain_per, "speedup": t_plain_per * n_sys / t_batched,
-           "plain_bicgstab_sample": sample}
-    log(f"rlc AC: {n_sys} systems in {t_batched:.2f} s ({n_sys / t_batched:.0f} solves/s), worst "
-        f"residual {worst:.1e}, plain {1 / t_plain_per:.1f} solves/s")
-    return out
+    per_sys = [d / m.size for d, (c, _) in zip(dev_ms, picks) for m in [mnas[c]]]
+    return {"sample": [{"circuit": c, "frequency": f, "unknowns": mnas[c].size,
+                        "device_ms": d, "wall_ms": 1e3 * w}
+                       for (c, f), d, w in zip(picks, dev_ms, wall)],
+            "device_ms_per_unknown": float(np.mean(per_sys)),
+            "plain_bicgstab_sample": sample}
 
 
 def adjoint(be, N, ir, solver):
@@ -359,23 +487,38 @@ def adjoint(be, N, ir, solver):
     return out
 
 
-def roofline_only(be, N, ir, solver) -> int:
-    """The dominant kernel alone (k_spmv<double> on the diode-mesh matrix)
-    and a plain copy kernel of known size, for rocprofv3 counter runs."""
+def roofline_only(args, be, N, ir, solver) -> int:
+    """The headline's kernels alone, for rocprofv3 counter runs: two sweeps
+    of the AC workload at the given rows per part (OHM_AC_PART_ROWS; the
+    autotuner is off), the stand-alone SpMV on the diode-mesh matrix, and a
+    plain copy kernel of known size (64 MiB read + 64 MiB written per
+    launch) as the counter calibration."""
+    be.set_autotune(False)
+    ckts, freqs = ir.rlc_sweep(n_circuits=args.ac_circuits, seed=4)
+    mnas = [solver.compile(c) for c in ckts]
+    del ckts
+    plan = solver.AcPlan(be, mnas)
+    for _ in range(2):
+        plan.solve(freqs)
+    st = plan.stats()
+    by = ac_bytes([m.size for m in mnas], int(freqs.shape[0]))
+    del plan
     m = solver.compile(ir.diode_mesh(256, 256, seed=2))
     a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
-    bx, by = be.new_buffer(m.size), be.new_buffer(m.size)
+    bx, by_ = be.new_buffer(m.size), be.new_buffer(m.size)
     be.upload_vec(np.ones(m.size), bx)
     ms = ctypes.c_double()
-    N.check(N.lib().ohm_spmv_timed(be.h, a.h, bx.h, by.h, 200, 0, ctypes.byref(ms)))
+    N.check(N.lib().ohm_spmv_timed(be.h, a.h, bx.h, by_.h, 200, 0, ctypes.byref(ms)))
     nc = 8 * 1024 * 1024  # 64 MiB of doubles
     cx, cy = be.new_buffer(nc), be.new_buffer(nc)
     for _ in range(20):
         be.copy(cx, cy)
     be.sync()
     b = spmv_bytes(m.size, m.g.nnz, 8)
-    print(json.dumps({"kernel": "k_spmv<double>", "bytes_per_launch": b,
-                      "avg_launch_us": ms.value * 1e3, "achieved_gbs": b / (ms.value * 1e-3) / 1e9,
+    print(json.dumps({"ac": {"stats": st, "bytes_per_launch": by},
+                      "spmv": {"kernel": "k_spmv<double>", "bytes_per_launch": b,
+                               "avg_launch_us": ms.value * 1e3,
+                               "achieved_gbs": b / (ms.value * 1e-3) / 1e9},
                       "copy_kernel": "k_copy", "copy_bytes_per_launch": 2 * 8 * nc}))
     return 0
 
@@ -427,17 +570,19 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["headline", "all", "ac"], default="all")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the oracle run (parity of the headline then unchecked)")
+    ap.add_argument("--only", choices=["headline", "all", "ac"], default="all",
+                    help="headline/ac: the AC sweep only")
     ap.add_argument("--tran-steps", type=int, default=1000)
     ap.add_argument("--mc-instances", type=int, default=8)
     ap.add_argument("--mc-steps", type=int, default=50)
     ap.add_argument("--ac-circuits", type=int, default=256)
     ap.add_argument("--profile", action="store_true",
-                    help="rocprofv3 runs: headline only, no CPU work, no hipGraphs")
+                    help="rocprofv3 runs: the diode-mesh workload only, no CPU work, no hipGraphs")
     ap.add_argument("--roofline-only", action="store_true",
-                    help="only the roofline kernel: 200 SpMVs on the mesh matrix and, for "
-                         "counter calibration, 20 copies of a 64 MiB vector")
+                    help="only the roofline kernels: two AC sweeps, 200 SpMVs on the mesh matrix "
+                         "and, for counter calibration, 20 copies of a 64 MiB vector")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the backend has no multi-GPU path (asked for "
@@ -449,62 +594,55 @@ def main() -> int:
     from ohmnivore_amd import solver
 
     be = solver.HipBackend()
-    if args.only == "ac":
-        print(json.dumps(rlc_ac(be, N, ir, solver, args.ac_circuits)))
-        return 0
     if args.roofline_only:
-        return roofline_only(be, N, ir, solver)
+        return roofline_only(args, be, N, ir, solver)
     if args.profile:
         be.set_graphs(False)
-        args.only = "headline"
-    h = headline(args, be, N, ir, solver)
-    forms, errors = h["forms"], h["errors"]
-    # the fastest form whose answer is within the parity bar (fp64 when none is)
-    ok = [p for p in forms if not errors or errors[p]["within_1e-6_rel_1e-9_abs"]]
-    best = max(ok, key=lambda p: forms[p]["solves_per_s"]) if ok else "fp64"
+        args.steps = min(args.steps, 2)
+        print(json.dumps(mesh_newton(args, be, N, ir, solver)))
+        return 0
+    h, mnas, freqs = ac_headline(args, be, N, ir, solver)
     workloads = {}
-    speedups = None
     if args.only == "all":
-        workloads["resistor_ladder_10k"] = ladder(be, N, ir, solver, check=not args.no_cpu_baseline)
+        workloads["rlc_ac_256x101_one_system_per_call"] = ac_plain(be, N, ir, solver, mnas, freqs)
+        plain = workloads["rlc_ac_256x101_one_system_per_call"]
+        plain_ms = plain["device_ms_per_unknown"] * h["unknowns_per_sweep"]
+        h["batched_vs_one_system_per_call_device"] = plain_ms / sum(h["device_ms_per_sweep"].values())
+        del mnas
+        workloads["diode_mesh_256"] = mesh_newton(args, be, N, ir, solver)
+        workloads["resistor_ladder_10k"] = ladder(be, N, ir, solver, check=True)
         workloads["rc_grid_512"] = rc_grid(be, N, ir, solver, args.tran_steps, args.mc_instances,
                                            args.mc_steps)
-        workloads["rlc_ac_256x101"] = rlc_ac(be, N, ir, solver, args.ac_circuits)
         workloads["amplifier_adjoint_4096"] = adjoint(be, N, ir, solver)
         workloads["inverter_chain_netlists"] = inverter_chains(be)
-        speedups = {"rc_grid_512_monte_carlo": workloads["rc_grid_512"]["monte_carlo"]["speedup"],
-                    "rlc_ac_256x101": workloads["rlc_ac_256x101"]["speedup"]}
+    checked = h["parity"] is not None
     line = {
         "metric": "solves/sec at 1e-6 rel err vs FP64 LU; SpMV HBM GB/s; batched-vs-plain speedup",
-        "value": forms[best]["solves_per_s"],
+        "value": h["value"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": forms[best]["ms_per_solve"],
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32+f64-refinement" if best == "fp32_ir" else "f64",
+        "dtype": "complex128",
         "data": "synthetic (seeded SplitMix64 circuits)",
         "config": {
-            "workload": "nonlinear DC, 256x256 resistor mesh + 8,192 diodes: 65,540 unknowns, "
-                        "326,664 nnz; Newton + ISAI(1)-BiCGSTAB from x=0",
-            "form": best,
-            "newton_abs_tol": h["newton_kw"]["abs_tol"],
-            "bicgstab_tolerance_rel_to_newton_residual": h["newton_kw"]["bicgstab_tolerance"],
-            "bicgstab_floor_rel_to_b": h["newton_kw"]["bicgstab_floor"],
-            "forcing": ("fixed" if best == "fp64_fixed_tol" else
-                        "Eisenstat-Walker choice 1, safeguard at every iteration, "
-                        f"eta in [{h['newton_kw']['bicgstab_tolerance']:g}, "
-                        f"{h['newton_kw']['forcing_max']:g}]"),
+            "workload": f"AC sweep: {h['circuits']} uneven RLC ladders (1k-64k sections) x "
+                        f"{h['frequencies']} log-spaced frequencies 1 Hz-1 GHz = "
+                        f"{h['systems']:,} complex systems, {h['unknowns_per_sweep']:,} unknowns "
+                        f"per sweep; banded LU with partial pivoting after RCM, symbolic plan "
+                        f"resident in HBM",
+            "rows_per_part": h["rows_per_part"],
+            "parity_checked": checked,
         },
         "roofline": h["roofline"],
-        "cpu_baseline": h["cpu"],
-        "forms": {p: {k: v for k, v in f.items() if k != "x"} for p, f in forms.items()},
-        "error_vs_fp64_lu": errors or None,
-        "spmv_large_working_set": h["spmv_large_working_set"],
-        "hbm_measured": h["hbm_measured"],
-        "batched_vs_plain_speedup": speedups,
+        "cpu_baseline": h["cpu_baseline"],
+        "parity": h["parity"],
+        "headline_detail": {k: v for k, v in h.items()
+                            if k not in ("roofline", "cpu_baseline", "parity", "value")},
         "workloads": workloads or None,
     }
     print(json.dumps(line))
