@@ -85,19 +85,30 @@ def within(x, ref, rel=1e-6, abs_=1e-9):
 
 
 # ------------------------------------------------------------- headline --
-def ac_bytes(sizes, n_freqs: int):
+def ac_bytes(sizes, n_freqs: int, parts: int):
     """Algorithmic bytes of one sweep's kernels. Waves hold up to 64
     frequencies of one circuit; the band rows of G and C and the permuted
-    b (3 + 3 + 2 doubles per row at half bandwidth 1) are read once per wave
-    and direction; per (system, row) the part solve writes the U row with
-    fill (3 complex) and the eliminated right-hand side, reads both back and
-    writes x (144 bytes)."""
+    b (3 + 3 + 2 doubles = 64 bytes per row at half bandwidth 1) are read by
+    every wave once per pass: twice by the part solve (the elimination and
+    the residual rows of the back substitution), once per direction by the
+    relations. Per (system, row) the part solve writes the pivot row
+    divided by its pivot (3 complex: the eliminated right-hand side and the
+    two entries right of the diagonal), reads it back and writes x
+    (112 bytes). The residual pass reads, per part and system, x at the
+    part's first and last rows and their neighbours (6 complex) and the
+    part's two partial sums."""
     waves = -(-n_freqs // 64)
     rows = float(sum(sizes))
     band = 64.0
-    return {"part_solve": rows * (144.0 * n_freqs + band * waves),
+    return {"part_solve": rows * (112.0 * n_freqs + 2 * band * waves),
             "relations": rows * band * 2 * waves,
-            "residual": rows * (16.0 * n_freqs + band * waves)}
+            "residual": parts * n_freqs * (6 * 16.0 + 16.0) + parts * 2 * band * waves}
+
+
+def n_parts(sizes, rows_per_part: int) -> int:
+    """Parts of a partitioned sweep (one per rows_per_part rows of every
+    circuit; 0 for the one-lane solve)."""
+    return sum(-(-n // rows_per_part) for n in sizes) if rows_per_part else 0
 
 
 def check_all_systems(plan, mnas, freqs, threads: int):
@@ -175,12 +186,12 @@ def ac_headline(args, be, N, ir, solver):
     be.sync()
     dt = time.perf_counter() - t0
     st = plan.stats()
-    worst = max(s.residual_norm for sts in res.stats for s in sts)
-    failed = sum(1 for sts in res.stats for s in sts if not s.converged)
+    worst = float(res.residual_norms.max())
+    failed = int((~res.converged).sum())
     ms = st["phase_ms"]
-    by = ac_bytes(sizes, nf)
+    by = ac_bytes(sizes, nf, n_parts(sizes, st["rows_per_part"]))
     kernel = {"part_solve": "k_part_solve" if st["rows_per_part"] else "k_band_lu<1>",
-              "relations": "k_part_rel", "residual": "k_band_resid<1> + k_band_resid_sum"}
+              "relations": "k_part_rel", "residual": "k_part_resid_sum"}
     dom = max(("part_solve", "relations", "residual"), key=lambda k: ms[k])
     achieved = by[dom] / (ms[dom] * 1e-3) / 1e9
     log(f"AC sweep: {1e3 * dt / args.steps:.1f} ms/sweep ({n_sys * args.steps / dt:,.0f} "
@@ -501,7 +512,8 @@ def roofline_only(args, be, N, ir, solver) -> int:
     for _ in range(2):
         plan.solve(freqs)
     st = plan.stats()
-    by = ac_bytes([m.size for m in mnas], int(freqs.shape[0]))
+    sizes = [m.size for m in mnas]
+    by = ac_bytes(sizes, int(freqs.shape[0]), n_parts(sizes, st["rows_per_part"]))
     del plan
     m = solver.compile(ir.diode_mesh(256, 256, seed=2))
     a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)

@@ -12,11 +12,13 @@
 //     registers, the band rows of A = G + j omega C are formed on the fly
 //     from the circuit's shared G and C band arrays, and the right-hand
 //     side is eliminated alongside, so L is never stored;
-//   * the rows of U (2B+1 entries with fill) go to HBM interleaved by lane,
-//     so each wave's stores and loads are coalesced; back substitution
-//     reads them in reverse and overwrites the eliminated right-hand side
-//     with x;
-//   * a residual pass recomputes ||b - A x|| / ||b|| per system in FP64;
+//   * each pivot row goes to HBM divided by its pivot (the eliminated
+//     right-hand side and the 2B entries right of the diagonal, with fill:
+//     2B+1 complex values), interleaved by lane, so each wave's stores and
+//     loads are coalesced; back substitution reads them in reverse and
+//     writes x;
+//   * the back substitution also sums ||b - A x||^2 and ||b||^2 per system
+//     in FP64 (a row is complete once the x it touches are known);
 //   * an unpermute pass scatters x back to the circuit's original order.
 // Waves hold 64 systems sorted by size, so lanes of a wave run for similar
 // lengths.
@@ -87,11 +89,36 @@ __device__ __forceinline__ void band_tile_load(const BandWave &q, int64_t row0, 
   }
 }
 
-// forward elimination + back substitution for the lanes of one wave
+// Band row i of A = G + j omega C (entries for columns i-B .. i+B) and b_i:
+// the same address in every lane of a wave, so one fetch serves the wave.
+template <int W>
+__device__ __forceinline__ void band_row(const BandWave &q, double om, int64_t i, cz (&a)[W],
+                                         cz &bi) {
+#pragma unroll
+  for (int d = 0; d < W; d++) a[d] = cz{q.gb[i * W + d], om * q.cb[i * W + d]};
+  bi = cz{q.b[2 * i], q.b[2 * i + 1]};
+}
+
+// |b_i - sum_d a_d x_d|^2 and |b_i|^2 into rr, bb
+template <int W>
+__device__ __forceinline__ void resid_row(const cz (&a)[W], cz bi, const cz (&xc)[W], double &rr,
+                                          double &bb) {
+  cz acc = bi;
+#pragma unroll
+  for (int d = 0; d < W; d++) acc = acc - a[d] * xc[d];
+  rr += cabs2(acc);
+  bb += cabs2(bi);
+}
+
+// Forward elimination + back substitution for the lanes of one wave, the
+// FP64 residual ||b - A x||^2 and ||b||^2 of every lane's system summed in
+// the back substitution (row i = j + B is complete once x_j is known) and
+// written to res[2 (sys + lane)].
 template <int B>
 __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ waves,
-                                                cz *__restrict__ U, cz *__restrict__ Y,
-                                                int *__restrict__ singular) {
+                                                cz *__restrict__ U, cz *__restrict__ X,
+                                                int *__restrict__ singular,
+                                                double *__restrict__ res) {
   constexpr int W = 2 * B + 1;  // band width in, U row width with fill
   __shared__ double sg[kBandTile][W], sc[kBandTile][W], sb[kBandTile][2];
   const int lane = threadIdx.x;
@@ -100,7 +127,7 @@ __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ wav
   const double om = live ? q.omega[lane] : 0.0;
   const int64_t n = q.n;
   cz *Ub = U + q.store_off * W;
-  cz *Yb = Y + q.store_off;
+  cz *Xb = X + q.store_off;
   const int S = q.nf;  // lane stride of the wave's rows
   cz win[B + 1][W], y[B + 1];
 #pragma unroll
@@ -166,10 +193,10 @@ __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ wav
         for (int c = 1; c < W; c++) win[r][c] = win[r][c] - l * win[0][c];
         y[r] = y[r] - l * y[0];
       }
-      if (live) {
+      if (live) {  // the pivot row divided by its pivot: x_j = e_0 - sum_c e_c x_{j+c}
+        Ub[(j * W) * S + lane] = y[0] * piv_inv;
 #pragma unroll
-        for (int c = 0; c < W; c++) Ub[(j * W + c) * S + lane] = win[0][c];
-        Yb[j * S + lane] = y[0];
+        for (int c = 1; c < W; c++) Ub[(j * W + c) * S + lane] = win[0][c] * piv_inv;
       }
 #pragma unroll
       for (int r = 0; r < B; r++) {
@@ -191,102 +218,79 @@ __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ wav
       }
     }
   }
-  if (!live) return;
-  // back substitution, x overwrites Y; rows fetched 8 at a time
-  constexpr int G = 8;
-  cz xw[W - 1];
+  // back substitution into X, 64 rows per tile; the band rows of the tile's
+  // residual rows (row j+B for row j of the tile) are staged in LDS at slot
+  // hi - j
+  constexpr int G = B == 1 ? 8 : B == 2 ? 4 : 2;
+  cz xw[W - 1];  // x_{j+1} .. x_{j+2B}
 #pragma unroll
   for (int k = 0; k < W - 1; k++) xw[k] = cz{0.0, 0.0};
-  for (int64_t hi = n - 1; hi >= 0; hi -= G) {
-    cz u[G][W], yy[G];
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      const int64_t j = hi - g;
-      if (j >= 0) {
-#pragma unroll
-        for (int c = 0; c < W; c++) u[g][c] = Ub[(j * W + c) * S + lane];
-        yy[g] = Yb[j * S + lane];
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      const int64_t j = hi - g;
-      if (j >= 0) {
-        cz acc = yy[g];
-#pragma unroll
-        for (int c = 1; c < W; c++) acc = acc - u[g][c] * xw[c - 1];
-        const cz xj = acc * cinv(u[g][0]);
-        Yb[j * S + lane] = xj;
-#pragma unroll
-        for (int k = W - 2; k > 0; k--) xw[k] = xw[k - 1];
-        xw[0] = xj;
-      }
-    }
-  }
-  if (bad) singular[q.sys + lane] = 1;
-}
-
-// Per-row residual terms, summed per (system, row chunk): fully parallel
-// over rows. partial[(wave * 64 + lane) * kResChunks + chunk] = (rr, bb).
-constexpr int kResChunks = 64;
-
-template <int B>
-__global__ __launch_bounds__(256) void k_band_resid(const BandWave *__restrict__ waves,
-                                                    const cz *__restrict__ Y,
-                                                    double *__restrict__ partial) {
-  constexpr int W = 2 * B + 1;
-  const int lane = threadIdx.x & 63, slot = threadIdx.x >> 6;
-  const int wave = blockIdx.x / kResChunks, chunk = blockIdx.x % kResChunks;
-  const BandWave q = waves[wave];
-  const bool live = lane < q.nf;
-  const double om = live ? q.omega[lane] : 0.0;
-  const cz *Yb = Y + q.store_off;
-  const int S = q.nf;
   double rr = 0.0, bb = 0.0;
-  if (live)
-    for (int64_t i = (int64_t)chunk * 4 + slot; i < q.n; i += (int64_t)kResChunks * 4) {
-      cz acc = {q.b[2 * i], q.b[2 * i + 1]};
-      bb += cabs2(acc);
+  for (int64_t hi = n - 1; hi >= 0; hi -= kBandTile) {
+    const int64_t lo = hi - kBandTile + 1 > 0 ? hi - kBandTile + 1 : 0;
+    const int64_t ir = hi + B - lane;
+    const bool in = ir < n && ir - B >= lo;
+    __syncthreads();
 #pragma unroll
-      for (int d = 0; d < W; d++) {
-        const int64_t col = i - B + d;
-        if (col >= 0 && col < q.n) {
-          const cz a = {q.gb[i * W + d], om * q.cb[i * W + d]};
-          acc = acc - a * Yb[col * S + lane];
+    for (int d = 0; d < W; d++) {
+      sg[lane][d] = in ? q.gb[ir * W + d] : 0.0;
+      sc[lane][d] = in ? q.cb[ir * W + d] : 0.0;
+    }
+    sb[lane][0] = in ? q.b[2 * ir] : 0.0;
+    sb[lane][1] = in ? q.b[2 * ir + 1] : 0.0;
+    __syncthreads();
+    for (int64_t g0 = hi; g0 >= lo; g0 -= G) {
+      cz u[G][W];
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const int64_t j = g0 - g;
+#pragma unroll
+        for (int c = 0; c < W; c++)
+          u[g][c] = j >= lo && live ? Ub[(j * W + c) * S + lane] : cz{0.0, 0.0};
+      }
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const int64_t j = g0 - g;
+        if (j >= lo) {
+          cz xj = u[g][0];
+#pragma unroll
+          for (int c = 1; c < W; c++) xj = xj - u[g][c] * xw[c - 1];
+          if (live) Xb[j * S + lane] = xj;
+          if (j + B < n) {  // row j + B: columns j .. j + 2B
+            const int sl = (int)(hi - j);
+            cz a[W], xc[W];
+#pragma unroll
+            for (int d = 0; d < W; d++) a[d] = cz{sg[sl][d], om * sc[sl][d]};
+            xc[0] = xj;
+#pragma unroll
+            for (int d = 1; d < W; d++) xc[d] = xw[d - 1];
+            resid_row<W>(a, cz{sb[sl][0], sb[sl][1]}, xc, rr, bb);
+          }
+#pragma unroll
+          for (int k = W - 2; k > 0; k--) xw[k] = xw[k - 1];
+          xw[0] = xj;
         }
       }
-      rr += cabs2(acc);
     }
-  __shared__ double s_rr[4][64], s_bb[4][64];
-  s_rr[slot][lane] = rr;
-  s_bb[slot][lane] = bb;
-  __syncthreads();
-  if (slot == 0) {
-    double a = 0.0, c = 0.0;
-    for (int k = 0; k < 4; k++) {
-      a += s_rr[k][lane];
-      c += s_bb[k][lane];
+  }
+  if (!live) return;
+  // rows 0 .. B-1 (columns below 0 are absent); xw[k] = x_k now
+#pragma unroll
+  for (int i = 0; i < B; i++) {
+    if (i < n) {
+      cz a[W], bi, xc[W];
+      band_row<W>(q, om, i, a, bi);
+#pragma unroll
+      for (int d = 0; d < W; d++) {
+        const int k = i - B + d;
+        xc[d] = k >= 0 && k < W - 1 ? xw[k] : cz{0.0, 0.0};
+      }
+      resid_row<W>(a, bi, xc, rr, bb);
     }
-    const size_t o = ((size_t)wave * 64 + lane) * kResChunks + chunk;
-    partial[2 * o] = a;
-    partial[2 * o + 1] = c;
   }
-}
-
-__global__ void k_band_resid_sum(const BandWave *__restrict__ waves, int n_waves,
-                                 const double *__restrict__ partial, double *__restrict__ res) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_waves * 64) return;
-  const BandWave &q = waves[t / 64];
-  const int lane = t % 64;
-  if (lane >= q.nf) return;
-  double a = 0.0, c = 0.0;
-  for (int k = 0; k < kResChunks; k++) {
-    a += partial[2 * ((size_t)t * kResChunks + k)];
-    c += partial[2 * ((size_t)t * kResChunks + k) + 1];
-  }
-  res[2 * (q.sys + lane)] = a;
-  res[2 * (q.sys + lane) + 1] = c;
+  res[2 * (q.sys + lane)] = rr;
+  res[2 * (q.sys + lane) + 1] = bb;
+  if (bad) singular[q.sys + lane] = 1;
 }
 
 // x back to the original unknown order: out[out_off + lane n + perm[i]] = x_i
@@ -307,7 +311,13 @@ __global__ void k_band_unpermute(const BandWave *__restrict__ waves, const cz *_
 // Partitioned solve of systems of half bandwidth 1 (the RLC ladders after
 // RCM). One lane per system leaves the AC workload with about 400 waves for
 // 25,856 systems, each walking up to 190k dependent rows; here the rows of
-// every system are cut into parts of R rows, eliminated independently:
+// every system are cut into parts of about R rows, eliminated independently.
+// A part block is solved with its neighbours' boundary unknowns set to zero,
+// so each cut is placed, within 16 rows of every multiple of R, where the
+// coupling of the two rows it separates is best contained in their
+// diagonals (in a ladder: between the two ends of a resistor, so that the
+// block sees that resistor grounded, never across an inductor's branch
+// current, which would leave it floating at low frequency):
 //   phase 1 (k_part_rel): every part eliminated top-down and, in a second
 //     wave, bottom-up (LU with partial pivoting of the row-reversed part),
 //     its couplings to the neighbouring parts' boundary unknowns carried as
@@ -322,15 +332,23 @@ __global__ void k_band_unpermute(const BandWave *__restrict__ waves, const cz *_
 //   phase 3 (k_part_solve): every part eliminated again top-down with the
 //     now known boundary values moved to its right-hand side, its U rows
 //     stored and back-substituted, x written where the one-lane solve
-//     writes it.
+//     writes it; the residual of the part's interior rows is summed on the
+//     way.
+//   k_part_resid_sum: per system, the residual of every part's first and
+//     last row from the x written by both neighbours, plus the parts' sums.
 // Phase 1 reads only the band rows shared by the wave's lanes, so the bytes
 // stored and read per (system, row) are those of the one-lane solve.
-// Systems whose phase-3 residual is not small (a part block close to
-// singular) are solved again by k_band_lu.
+// Systems whose residual is not small (a part block close to singular
+// after all) are solved again by k_band_lu.
 constexpr int kPartW = 3;  // band width at half bandwidth 1
 
 struct PartTile {
   double g[kBandTile][kPartW], c[kBandTile][kPartW], b[kBandTile][2];
+};
+
+// one part: its wave (index into the chunk's wave table) and rows [s, s+m)
+struct PartRef {
+  int item, s, m, pad;
 };
 
 // direction-order rows k0 .. k0+63 of part [s, s+m) (rev: row k is s+m-1-k,
@@ -360,10 +378,11 @@ __device__ __forceinline__ void part_row_load(const BandWave &q, int64_t s, int6
 // the part) and the end coupling (direction row m-1's entry right of it);
 // returns the last pivot row's y and diagonal. NR = 1: xs / xe are the
 // known unknowns beyond the start / end, moved to f; with STORE (top-down
-// only) the U rows and y go to Ub / Yb at their global rows.
+// only) the pivot rows (y and U divided by the pivot) go to Ub at their
+// global rows.
 template <int NR, bool STORE>
 __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int64_t m, bool rev,
-                                               double om, bool live, cz xs, cz xe, cz *Ub, cz *Yb,
+                                               double om, bool live, cz xs, cz xe, cz *Ub,
                                                int S, PartTile &t, cz (&ly)[NR], cz &lu,
                                                int &bad) {
   constexpr int W = kPartW;
@@ -427,16 +446,17 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
           y[1][k] = u;
         }
       }
-      const cz l = win[1][0] * cinv(win[0][0]);
+      const cz pinv = cinv(win[0][0]);
+      const cz l = win[1][0] * pinv;
 #pragma unroll
       for (int c = 1; c < W; c++) win[1][c] = win[1][c] - l * win[0][c];
 #pragma unroll
       for (int k = 0; k < NR; k++) y[1][k] = y[1][k] - l * y[0][k];
-      if (STORE && live) {
+      if (STORE && live) {  // the pivot row divided by its pivot, as in k_band_lu
         const int64_t i = s + j;
+        Ub[(i * W) * S + lane] = y[0][0] * pinv;
 #pragma unroll
-        for (int c = 0; c < W; c++) Ub[(i * W + c) * S + lane] = win[0][c];
-        Yb[i * S + lane] = y[0][0];
+        for (int c = 1; c < W; c++) Ub[(i * W + c) * S + lane] = win[0][c] * pinv;
       }
       if (j == m - 1) {
         lu = win[0][0];
@@ -475,21 +495,21 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
 
 // Phase 1: blockIdx.x = 2 ip + direction; relation of (part ip, direction)
 // to rel[((ip * 2 + dir) * 3 + k) * 64 + lane], k = alpha, k_prev, k_next.
-__global__ __launch_bounds__(64) void k_part_rel(const BandWave *__restrict__ waves,
-                                                 const int *__restrict__ ip_item, int64_t R,
+__global__ __launch_bounds__(64, 4) void k_part_rel(const BandWave *__restrict__ waves,
+                                                 const PartRef *__restrict__ parts,
                                                  cz *__restrict__ rel, int *__restrict__ singular) {
   __shared__ PartTile t;
   const int lane = threadIdx.x;
   const int ip = blockIdx.x >> 1, dir = blockIdx.x & 1;
-  const BandWave q = waves[ip_item[ip]];
-  const int p = ip - q.pbase;
-  const int64_t s = (int64_t)p * R, m = min(R, q.n - s);
+  const PartRef pr = parts[ip];
+  const BandWave q = waves[pr.item];
+  const int64_t s = pr.s, m = pr.m;
   const bool live = lane < q.nf;
   const double om = live ? q.omega[lane] : 0.0;
   cz ly[3], lu = {1.0, 0.0};
   int bad = 0;
   part_eliminate<3, false>(q, s, m, dir == 1, om, live, cz{0.0, 0.0}, cz{0.0, 0.0}, nullptr,
-                           nullptr, q.nf, t, ly, lu, bad);
+                           q.nf, t, ly, lu, bad);
   if (!live) return;
   if (bad || !(cabs2(lu) > 0.0)) {
     singular[q.sys + lane] = 2;
@@ -625,18 +645,22 @@ __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__
 
 // Phase 3: blockIdx.x = ip; the part's top-down elimination with its
 // boundary values known, U rows and y stored, then back substitution.
+// The part's interior rows s+1 .. s+m-2 use only x of this part, so their
+// residual terms are summed here, to pres[(ip * 64 + lane) * 2 + {0, 1}].
 __global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ waves,
-                                                   const int *__restrict__ ip_item, int64_t R,
+                                                   const PartRef *__restrict__ parts,
                                                    const cz *__restrict__ bnd, cz *__restrict__ U,
-                                                   cz *__restrict__ Y,
-                                                   int *__restrict__ singular) {
+                                                   cz *__restrict__ X,
+                                                   int *__restrict__ singular,
+                                                   double *__restrict__ pres) {
   constexpr int W = kPartW;
   __shared__ PartTile t;
   const int lane = threadIdx.x;
   const int ip = blockIdx.x;
-  const BandWave q = waves[ip_item[ip]];
+  const PartRef pr = parts[ip];
+  const BandWave q = waves[pr.item];
   const int p = ip - q.pbase;
-  const int64_t s = (int64_t)p * R, m = min(R, q.n - s);
+  const int64_t s = pr.s, m = pr.m;
   const bool live = lane < q.nf;
   const double om = live ? q.omega[lane] : 0.0;
   const size_t rb = (size_t)2 * q.pbase;
@@ -647,43 +671,102 @@ __global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ 
   }
   const int S = q.nf;
   cz *Ub = U + q.store_off * W;
-  cz *Yb = Y + q.store_off;
+  cz *Xb = X + q.store_off;
   cz ly[1], lu;
   int bad = 0;
-  part_eliminate<1, true>(q, s, m, false, om, live, xs, xe, Ub, Yb, S, t, ly, lu, bad);
-  if (!live) return;
-  // back substitution over the part's rows (columns past it carry no entries)
-  constexpr int G = 8;
-  cz xw[W - 1];
+  part_eliminate<1, true>(q, s, m, false, om, live, xs, xe, Ub, S, t, ly, lu, bad);
+  // back substitution over the part's rows (columns past it carry no
+  // entries), 64 rows per tile; the band rows of the tile's residual rows
+  // (row j+1 for row j of the tile) are staged in LDS at slot hi - j
+  constexpr int G = 4;
+  cz xw[W - 1] = {cz{0.0, 0.0}, cz{0.0, 0.0}};
+  double rr = 0.0, bb = 0.0;
+  for (int64_t hi = s + m - 1; hi >= s; hi -= kBandTile) {
+    const int64_t lo = hi - kBandTile + 1 > s ? hi - kBandTile + 1 : s;
+    const int64_t ir = hi + 1 - lane;
+    const bool in = ir >= s + 1 && ir <= s + m - 2 && ir - 1 >= lo;
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < W - 1; k++) xw[k] = cz{0.0, 0.0};
-  for (int64_t hi = s + m - 1; hi >= s; hi -= G) {
-    cz u[G][W], yy[G];
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      const int64_t j = hi - g;
-      if (j >= s) {
-#pragma unroll
-        for (int c = 0; c < W; c++) u[g][c] = Ub[(j * W + c) * S + lane];
-        yy[g] = Yb[j * S + lane];
-      }
+    for (int d = 0; d < W; d++) {
+      t.g[lane][d] = in ? q.gb[ir * W + d] : 0.0;
+      t.c[lane][d] = in ? q.cb[ir * W + d] : 0.0;
     }
+    t.b[lane][0] = in ? q.b[2 * ir] : 0.0;
+    t.b[lane][1] = in ? q.b[2 * ir + 1] : 0.0;
+    __syncthreads();
+    for (int64_t g0 = hi; g0 >= lo; g0 -= G) {
+      cz u[G][W];
 #pragma unroll
-    for (int g = 0; g < G; g++) {
-      const int64_t j = hi - g;
-      if (j >= s) {
-        cz acc = yy[g];
+      for (int g = 0; g < G; g++) {
+        const int64_t j = g0 - g;
 #pragma unroll
-        for (int c = 1; c < W; c++) acc = acc - u[g][c] * xw[c - 1];
-        const cz xj = acc * cinv(u[g][0]);
-        Yb[j * S + lane] = xj;
+        for (int c = 0; c < W; c++)
+          u[g][c] = j >= lo && live ? Ub[(j * W + c) * S + lane] : cz{0.0, 0.0};
+      }
 #pragma unroll
-        for (int k = W - 2; k > 0; k--) xw[k] = xw[k - 1];
-        xw[0] = xj;
+      for (int g = 0; g < G; g++) {
+        const int64_t j = g0 - g;
+        if (j >= lo) {
+          cz xj = u[g][0];
+#pragma unroll
+          for (int c = 1; c < W; c++) xj = xj - u[g][c] * xw[c - 1];
+          if (live) Xb[j * S + lane] = xj;
+          if (j + 1 <= s + m - 2) {  // interior row j + 1: columns j .. j + 2
+            const int sl = (int)(hi - j);
+            cz a[W];
+#pragma unroll
+            for (int d = 0; d < W; d++) a[d] = cz{t.g[sl][d], om * t.c[sl][d]};
+            const cz xc[W] = {xj, xw[0], xw[1]};
+            resid_row<W>(a, cz{t.b[sl][0], t.b[sl][1]}, xc, rr, bb);
+          }
+          xw[1] = xw[0];
+          xw[0] = xj;
+        }
       }
     }
   }
+  if (!live) return;
+  pres[((size_t)ip * 64 + lane) * 2] = rr;
+  pres[((size_t)ip * 64 + lane) * 2 + 1] = bb;
   if (bad) singular[q.sys + lane] = 2;
+}
+
+// One thread per (wave, lane): the residual terms of every part's first and
+// last row (columns in the neighbouring parts, read from the written x) and
+// the parts' interior sums, in part order, to res[2 (sys + lane)].
+__global__ void k_part_resid_sum(const BandWave *__restrict__ waves, int n_waves,
+                                 const PartRef *__restrict__ parts,
+                                 const double *__restrict__ pres, const cz *__restrict__ X,
+                                 double *__restrict__ res) {
+  constexpr int W = kPartW;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_waves * 64) return;
+  const BandWave q = waves[t / 64];
+  const int lane = t % 64;
+  if (lane >= q.nf) return;
+  const double om = q.omega[lane];
+  const cz *Xb = X + q.store_off;
+  const int S = q.nf;
+  double rr = 0.0, bb = 0.0;
+  for (int ip = q.pbase; ip < q.pbase + q.nparts; ip++) {
+    rr += pres[((size_t)ip * 64 + lane) * 2];
+    bb += pres[((size_t)ip * 64 + lane) * 2 + 1];
+    const PartRef pr = parts[ip];
+    for (int e = 0; e < 2; e++) {
+      if (e == 1 && pr.m == 1) break;  // one row: first and last
+      const int64_t i = e == 0 ? pr.s : (int64_t)pr.s + pr.m - 1;
+      cz a[W], bi, xc[W];
+      band_row<W>(q, om, i, a, bi);
+#pragma unroll
+      for (int d = 0; d < W; d++) {
+        const int64_t col = i - 1 + d;
+        xc[d] = col >= 0 && col < q.n ? Xb[col * S + lane] : cz{0.0, 0.0};
+      }
+      resid_row<W>(a, bi, xc, rr, bb);
+    }
+  }
+  res[2 * (q.sys + lane)] = rr;
+  res[2 * (q.sys + lane) + 1] = bb;
 }
 
 }  // namespace ohm
@@ -842,6 +925,9 @@ struct ohm_ac_plan {
   void *d_band = nullptr;
   std::vector<const double *> gb, cb, bp;
   std::vector<const uint32_t *> perm;
+  // half bandwidth 1: per circuit, cut_score[i] rates a part boundary
+  // between RCM rows i and i+1 (lower is better)
+  std::vector<std::vector<float>> cut_score;
   // last solve
   int64_t nf = 0;
   double *d_omega = nullptr;
@@ -863,6 +949,49 @@ struct ohm_ac_plan {
 };
 
 namespace {
+
+// |off| / |diag|: 0 for a missing coupling, infinite for a zero diagonal
+float coupling_ratio(double off, double diag) {
+  off = fabs(off);
+  diag = fabs(diag);
+  if (off == 0.0) return 0.0f;
+  if (diag == 0.0) return INFINITY;
+  return (float)(off / diag);
+}
+
+// Rating of a part boundary between rows i and i+1 of a tridiagonal band
+// (gb: [n][3] = entries at columns i-1, i, i+1 of the DC matrix G): the
+// larger of |a(i,i+1)| / |a(i,i)| and |a(i+1,i)| / |a(i+1,i+1)|. A part is
+// solved with the unknown past its boundary set to zero; where that coupling
+// also sits in the diagonal (a conductance to the neighbour) the block keeps
+// it as a conductance to ground, where it does not (a branch current of an
+// inductor or source) the block loses a path and can be left floating.
+std::vector<float> cut_scores(const std::vector<double> &gb, int64_t n) {
+  std::vector<float> s(std::max<int64_t>(n - 1, 0));
+  for (int64_t i = 0; i + 1 < n; i++)
+    s[i] = std::max(coupling_ratio(gb[i * 3 + 2], gb[i * 3 + 1]),
+                    coupling_ratio(gb[(i + 1) * 3], gb[(i + 1) * 3 + 1]));
+  return s;
+}
+
+constexpr int64_t kCutWindow = 16;
+
+// Part boundaries [0, e_1, ..., n) of an n-row system for about R rows per
+// part: e_k is the best-rated cut within kCutWindow rows of k R.
+std::vector<int> part_cuts(const std::vector<float> &score, int64_t n, int64_t R) {
+  std::vector<int> cut{0};
+  for (int64_t e0 = R; e0 < n; e0 += R) {
+    const int64_t lo = std::max<int64_t>(e0 - kCutWindow, cut.back() + 1);
+    const int64_t hi = std::min<int64_t>(e0 + kCutWindow, n - 1);
+    if (lo > hi) continue;
+    int64_t best = lo;
+    for (int64_t e = lo + 1; e <= hi; e++)
+      if (score[e - 1] < score[best - 1]) best = e;
+    cut.push_back((int)best);
+  }
+  cut.push_back((int)n);
+  return cut;
+}
 
 void plan_release(ohm_ac_plan *p) {
   if (!p) return;
@@ -910,6 +1039,7 @@ ohm_status plan_build(ohm_backend *b, int64_t n_circuits, const ohm_ac_circuit *
   p->cb.assign(n_circuits, nullptr);
   p->bp.assign(n_circuits, nullptr);
   p->perm.assign(n_circuits, nullptr);
+  p->cut_score.assign(n_circuits, {});
   // band arrays in the RCM order, built on host threads 32 circuits at a
   // time and uploaded from this thread
   struct BandHost {
@@ -937,6 +1067,7 @@ ohm_status plan_build(ohm_backend *b, int64_t n_circuits, const ohm_ac_circuit *
         h.bp[2 * i] = circ[c].b_ac[2 * r];
         h.bp[2 * i + 1] = circ[c].b_ac[2 * r + 1];
       }
+      if (B == 1) p->cut_score[c] = cut_scores(h.gb, n);
     });
     for (int64_t t = 0; t < cn; t++) {
       const int64_t c = c0 + t;
@@ -1019,17 +1150,25 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
   size_t pos = 0;
   int chunks = 0;
   auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-  auto parts_of = [&](int64_t n) { return part ? (int)((n + R - 1) / R) : 0; };
+  // part boundaries per circuit (the same for all of its frequencies)
+  std::vector<std::vector<int>> cuts(part ? nc : 0);
+  if (part)
+    parallel_for(nc, [&](int64_t c) {
+      if (p->n[c] > 0) cuts[c] = part_cuts(p->cut_score[c], p->n[c], R);
+    });
+  auto parts_of = [&](int64_t c) { return part ? (int)cuts[c].size() - 1 : 0; };
   while (pos < items.size()) {
     size_t end = pos;
     double bytes = 0.0;
     while (end < items.size()) {
       const double n = (double)p->n[items[end].c];
       const double nfw = items[end].nf;
-      const double np = parts_of(p->n[items[end].c]);
+      const double np = parts_of(items[end].c);
+      // U rows and x per lane-row; per part: its reference, relations,
+      // reduced rows, boundary values and residual partials per lane
       const double wave_b = n * nfw * 16.0 * (W + 1) + (want_x ? n * nfw * 16.0 : 0.0) +
-                            64.0 * kResChunks * 16.0 + sizeof(BandWave) +
-                            np * (4.0 + 64.0 * 16.0 * (6.0 + 12.0 + 2.0));
+                            2.0 * sizeof(BandWave) +
+                            np * (sizeof(PartRef) + 64.0 * 16.0 * (6.0 + 12.0 + 2.0 + 1.0));
       if (end > pos && bytes + wave_b > budget) break;
       bytes += wave_b;
       end++;
@@ -1056,7 +1195,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       out_total += p->n[it.c] * it.nf;
     }
     std::vector<BandWave> hw(n_w);
-    std::vector<int> ip_item;
+    std::vector<PartRef> prefs;
     int64_t store = 0;  // lane-rows
     for (int k = 0; k < n_w; k++) {
       const WaveItem &it = items[pos + k];
@@ -1071,24 +1210,27 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       w.out_off = out_off[k];
       w.nf = it.nf;
       w.sys = (int)(it.c * n_freqs + it.f0);
-      w.pbase = (int)ip_item.size();
-      w.nparts = parts_of(w.n);
-      for (int q = 0; q < w.nparts; q++) ip_item.push_back(k);
+      w.pbase = (int)prefs.size();
+      w.nparts = parts_of(it.c);
+      for (int q = 0; q < w.nparts; q++)
+        prefs.push_back({k, cuts[it.c][q], cuts[it.c][q + 1] - cuts[it.c][q], 0});
       store += w.n * it.nf;
       hw[k] = w;
     }
-    const size_t n_ip = ip_item.size();
+    const size_t n_ip = prefs.size();
     // chunk buffers carved from the backend's workspace (kept across sweeps:
     // re-allocating tens of GB per chunk costs seconds)
     const size_t bw_b = al(sizeof(BandWave) * n_w), u_b = al(sizeof(cz) * store * W),
                  y_b = al(sizeof(cz) * store),
                  x_b = want_x ? al(sizeof(cz) * std::max<int64_t>(out_total, 1)) : 0,
-                 p_b = al(sizeof(double) * 2 * (size_t)n_w * 64 * kResChunks),
-                 ip_b = al(sizeof(int) * std::max<size_t>(n_ip, 1)),
+                 ip_b = al(sizeof(PartRef) * std::max<size_t>(n_ip, 1)),
                  rel_b = al(sizeof(cz) * n_ip * 6 * 64),
                  red_b = al(sizeof(cz) * n_ip * 12 * 64),
-                 bnd_b = al(sizeof(cz) * n_ip * 2 * 64);
-    const size_t need = bw_b + u_b + y_b + x_b + p_b + (part ? ip_b + rel_b + red_b + bnd_b : 0);
+                 bnd_b = al(sizeof(cz) * n_ip * 2 * 64),
+                 pres_b = al(sizeof(double) * n_ip * 2 * 64);
+    // (partitioned: a second wave table for the one-lane redo after pres)
+    const size_t need =
+        bw_b + u_b + y_b + x_b + (part ? ip_b + rel_b + red_b + bnd_b + pres_b + bw_b : 0);
     if (b->ac_ws_bytes < need) {
       if (b->ac_ws) OHM_HIP(hipFree(b->ac_ws));
       b->ac_ws = nullptr;
@@ -1100,20 +1242,32 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     char *wsp = (char *)b->ac_ws;
     BandWave *d_w = (BandWave *)wsp;
     cz *dU = (cz *)(wsp + bw_b);
-    cz *dY = (cz *)(wsp + bw_b + u_b);
+    cz *dY = (cz *)(wsp + bw_b + u_b);  // x in RCM order, lanes interleaved
     cz *dX = want_x ? (cz *)(wsp + bw_b + u_b + y_b) : nullptr;
-    double *d_part = (double *)(wsp + bw_b + u_b + y_b + x_b);
-    char *pw = wsp + bw_b + u_b + y_b + x_b + p_b;
-    int *d_ip = (int *)pw;
+    char *pw = wsp + bw_b + u_b + y_b + x_b;
+    PartRef *d_parts = (PartRef *)pw;
     cz *d_rel = (cz *)(pw + ip_b), *d_red = (cz *)(pw + ip_b + rel_b),
        *d_bnd = (cz *)(pw + ip_b + rel_b + red_b);
+    double *d_pres = (double *)(pw + ip_b + rel_b + red_b + bnd_b);
     OHM_HIP(hipMemcpyAsync(d_w, hw.data(), sizeof(BandWave) * n_w, hipMemcpyHostToDevice, st));
     if (part)
-      OHM_HIP(hipMemcpyAsync(d_ip, ip_item.data(), sizeof(int) * n_ip, hipMemcpyHostToDevice, st));
+      OHM_HIP(hipMemcpyAsync(d_parts, prefs.data(), sizeof(PartRef) * n_ip,
+                             hipMemcpyHostToDevice, st));
+    auto one_lane = [&](int nw, const BandWave *dw) -> ohm_status {
+      if (B == 1)
+        hipLaunchKernelGGL(k_band_lu<1>, dim3(nw), dim3(64), 0, st, dw, dU, dY, p->d_sing, p->d_res);
+      else if (B == 2)
+        hipLaunchKernelGGL(k_band_lu<2>, dim3(nw), dim3(64), 0, st, dw, dU, dY, p->d_sing, p->d_res);
+      else
+        hipLaunchKernelGGL(k_band_lu<4>, dim3(nw), dim3(64), 0, st, dw, dU, dY, p->d_sing, p->d_res);
+      OHM_HIP(hipGetLastError());
+      b->dispatches++;
+      return OHM_OK;
+    };
     hipEvent_t *ev = p->ev;
     OHM_HIP(hipEventRecord(ev[0], st));
     if (part) {
-      hipLaunchKernelGGL(k_part_rel, dim3((unsigned)(2 * n_ip)), dim3(64), 0, st, d_w, d_ip, R,
+      hipLaunchKernelGGL(k_part_rel, dim3((unsigned)(2 * n_ip)), dim3(64), 0, st, d_w, d_parts,
                          d_rel, p->d_sing);
       OHM_HIP(hipGetLastError());
       OHM_HIP(hipEventRecord(ev[1], st));
@@ -1121,38 +1275,20 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
                          p->d_sing);
       OHM_HIP(hipGetLastError());
       OHM_HIP(hipEventRecord(ev[2], st));
-      hipLaunchKernelGGL(k_part_solve, dim3((unsigned)n_ip), dim3(64), 0, st, d_w, d_ip, R, d_bnd,
-                         dU, dY, p->d_sing);
+      hipLaunchKernelGGL(k_part_solve, dim3((unsigned)n_ip), dim3(64), 0, st, d_w, d_parts,
+                         d_bnd, dU, dY, p->d_sing, d_pres);
       OHM_HIP(hipGetLastError());
-      b->dispatches += 3;
+      OHM_HIP(hipEventRecord(ev[3], st));
+      hipLaunchKernelGGL(k_part_resid_sum, dim3((unsigned)((n_w * 64 + 255) / 256)), dim3(256), 0,
+                         st, d_w, n_w, d_parts, d_pres, dY, p->d_res);
+      OHM_HIP(hipGetLastError());
+      b->dispatches += 4;
     } else {
       OHM_HIP(hipEventRecord(ev[1], st));
       OHM_HIP(hipEventRecord(ev[2], st));
-      if (B == 1)
-        hipLaunchKernelGGL(k_band_lu<1>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, p->d_sing);
-      else if (B == 2)
-        hipLaunchKernelGGL(k_band_lu<2>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, p->d_sing);
-      else
-        hipLaunchKernelGGL(k_band_lu<4>, dim3(n_w), dim3(64), 0, st, d_w, dU, dY, p->d_sing);
-      OHM_HIP(hipGetLastError());
-      b->dispatches++;
+      OHM_TRY(one_lane(n_w, d_w));
+      OHM_HIP(hipEventRecord(ev[3], st));
     }
-    OHM_HIP(hipEventRecord(ev[3], st));
-    auto resid = [&](int nw, const BandWave *dw) -> ohm_status {
-      if (B == 1)
-        hipLaunchKernelGGL(k_band_resid<1>, dim3(nw * kResChunks), dim3(256), 0, st, dw, dY, d_part);
-      else if (B == 2)
-        hipLaunchKernelGGL(k_band_resid<2>, dim3(nw * kResChunks), dim3(256), 0, st, dw, dY, d_part);
-      else
-        hipLaunchKernelGGL(k_band_resid<4>, dim3(nw * kResChunks), dim3(256), 0, st, dw, dY, d_part);
-      OHM_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_band_resid_sum, dim3((nw * 64 + 255) / 256), dim3(256), 0, st, dw, nw,
-                         d_part, p->d_res);
-      OHM_HIP(hipGetLastError());
-      b->dispatches += 2;
-      return OHM_OK;
-    };
-    OHM_TRY(resid(n_w, d_w));
     OHM_HIP(hipEventRecord(ev[4], st));
     OHM_HIP(hipEventSynchronize(ev[4]));
     float ms[4] = {0.f, 0.f, 0.f, 0.f};
@@ -1185,18 +1321,13 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       }
       if (!redo.empty()) {
         OHM_HIP(hipMemcpyAsync(p->d_sing, sing.data(), sizeof(int) * n_sys, hipMemcpyHostToDevice, st));
-        // wave table of the redone items after the chunk's (the chunk's table stays for get_x)
-        BandWave *d_redo = nullptr;
-        OHM_HIP(hipMalloc((void **)&d_redo, sizeof(BandWave) * redo.size()));
-        std::unique_ptr<BandWave, void (*)(BandWave *)> g(d_redo, [](BandWave *q) { (void)hipFree(q); });
+        // wave table of the redone items in its own region (the chunk's
+        // table stays for get_x)
+        BandWave *d_redo = (BandWave *)(pw + ip_b + rel_b + red_b + bnd_b + pres_b);
         OHM_HIP(hipMemcpyAsync(d_redo, redo.data(), sizeof(BandWave) * redo.size(),
                                hipMemcpyHostToDevice, st));
         OHM_HIP(hipEventRecord(ev[0], st));
-        hipLaunchKernelGGL(k_band_lu<1>, dim3((unsigned)redo.size()), dim3(64), 0, st, d_redo, dU,
-                           dY, p->d_sing);
-        OHM_HIP(hipGetLastError());
-        b->dispatches++;
-        OHM_TRY(resid((int)redo.size(), d_redo));
+        OHM_TRY(one_lane((int)redo.size(), d_redo));
         OHM_HIP(hipEventRecord(ev[1], st));
         OHM_HIP(hipEventSynchronize(ev[1]));
         float fm = 0.f;

@@ -462,11 +462,35 @@ def ac_sweep(backend: HipBackend, mna: ir.MnaSystem, freqs, batch: int = 0,
 AC_METHODS = {"auto": N.AC_AUTO, "iterative": N.AC_ITERATIVE, "direct": N.AC_DIRECT}
 
 
-@dataclass
 class AcSweepResult:
-    x: list                      # per circuit: [n_freqs, n] complex (None when not kept)
-    stats: list                  # per circuit: list of BatchStats per frequency
-    device_ms: float
+    """Solutions and per-system figures of a multi-circuit AC sweep.
+
+    x: per circuit [n_freqs, n] complex (None when not kept). records:
+    [n_circuits, n_freqs] structured array of ohm_batch_result (iterations,
+    refinements, residual_norm, status, reason). stats: the same records as
+    per-circuit lists of BatchStats, built on first access."""
+
+    def __init__(self, x: list, records: np.ndarray, device_ms: float):
+        self.x = x
+        self.records = records
+        self.device_ms = device_ms
+        self._stats = None
+
+    @property
+    def stats(self) -> list:
+        if self._stats is None:
+            self._stats = [[BatchStats(int(r["iterations"]), int(r["refinements"]),
+                                       float(r["residual_norm"]), int(r["status"]),
+                                       int(r["reason"])) for r in row] for row in self.records]
+        return self._stats
+
+    @property
+    def residual_norms(self) -> np.ndarray:
+        return self.records["residual_norm"]
+
+    @property
+    def converged(self) -> np.ndarray:
+        return self.records["status"] == 0
 
 
 def _ac_circuits(mnas, nf: int, keep_x: bool):
@@ -493,9 +517,10 @@ def _ac_circuits(mnas, nf: int, keep_x: bool):
     return cs, keep, xs
 
 
-def _stats(res, k: int, nf: int):
-    return [[BatchStats(r.iterations, r.refinements, r.residual_norm, r.status, r.reason)
-             for r in res[i * nf:(i + 1) * nf]] for i in range(k)]
+def _records(res, k: int, nf: int) -> np.ndarray:
+    """The ohm_batch_result array as a [k, nf] structured numpy array (a copy)."""
+    a = np.ctypeslib.as_array(res)
+    return a[: k * nf].reshape(k, nf).copy()
 
 
 def ac_sweep_circuits(backend: HipBackend, mnas, freqs, method: str = "auto",
@@ -516,7 +541,7 @@ def ac_sweep_circuits(backend: HipBackend, mnas, freqs, method: str = "auto",
     del keep
     if rc != 0 and raise_on_failure:
         check(rc)
-    return AcSweepResult(xs, _stats(res, k, nf), ms.value)
+    return AcSweepResult(xs, _records(res, k, nf), ms.value)
 
 
 class AcPlan:
@@ -551,7 +576,7 @@ class AcPlan:
         self.n_freqs = nf
         if rc != 0 and raise_on_failure:
             check(rc)
-        return AcSweepResult(xs if keep_x else [None] * k, _stats(res, k, nf), ms.value)
+        return AcSweepResult(xs if keep_x else [None] * k, _records(res, k, nf), ms.value)
 
     def stats(self) -> dict:
         """ohm_ac_plan_stats of the last solve: device ms per phase, rows per
@@ -569,7 +594,9 @@ class AcPlan:
         """Resident solutions of one circuit: [f_count, n] complex."""
         if f_count is None:
             f_count = self.n_freqs - f_first
-        out = np.zeros((f_count, self.sizes[circuit]), dtype=np.complex128)
+        # an out-of-range circuit is reported by ohm_ac_plan_get_x itself
+        n = self.sizes[circuit] if 0 <= circuit < len(self.sizes) else 0
+        out = np.zeros((max(f_count, 0), n), dtype=np.complex128)
         check(lib().ohm_ac_plan_get_x(self.h, ctypes.c_int64(circuit), ctypes.c_int64(f_first),
                                       ctypes.c_int64(f_count), out.ctypes.data_as(N.c_f64p)))
         return out

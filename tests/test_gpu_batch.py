@@ -330,6 +330,8 @@ def test_ac_plan_partitioned_vs_lu(be, rows, monkeypatch):
     res = plan.solve(freqs, keep_x=True)
     st = plan.stats()
     assert st["rows_per_part"] == int(rows) and st["chunks"] == 1
+    if int(rows) >= 64:  # cuts placed at resistors: no part block left floating
+        assert st["redone_systems"] == 0
     for k, (m, x, sts) in enumerate(zip(mnas, res.x, res.stats)):
         assert all(s.converged and s.residual_norm < 1e-12 for s in sts)
         cg = c_on_g_matrix(m)
