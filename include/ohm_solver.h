@@ -162,6 +162,21 @@ ohm_status ohm_bicgstab(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *r
 ohm_status ohm_bicgstab_transpose(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *rhs,
                                   ohm_buffer *x, const ohm_precond *p,
                                   const ohm_bicgstab_params *params, ohm_bicgstab_result *result);
+/* ohm_bicgstab without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The caller passes the matrix, preconditioner, rhs and x as for
+ * ohm_bicgstab (same results, bit for bit); the solve's own vectors, state
+ * word and partial sums live in a 64 MiB static device array that each GPU
+ * the library runs on holds until the process ends, so systems up to about
+ * 700,000 unknowns fit (a larger one is refused with OHM_ERR_INVALID). FP64
+ * only (precision FP32_IR is refused), every chunk launched eagerly. All
+ * calls on one GPU share that GPU's array: calls that overlap on one GPU
+ * (several host threads, or backends on different streams of the same
+ * device) race and must be serialised by the caller. */
+ohm_status ohm_bicgstab_noalloc(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *rhs,
+                                ohm_buffer *x, const ohm_precond *p,
+                                const ohm_bicgstab_params *params, ohm_bicgstab_result *result);
 
 /* ---- LinearSolver::solve_real (src/solver/mod.rs:30): host slices in and
  * out, preconditioner chosen by `precond` (OHM_PREC_AUTO = the reference's

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cmath>
+#include <mutex>
 
 #include "ohm_precond.h"
 #include "ohm_spmv.h"
@@ -557,20 +558,19 @@ struct Workspace {
   T *base = nullptr;
   SolveState *st = nullptr;
   BicgVecs<T> v{};
+  double *partials = nullptr;  // null: the backend's partials buffer
 };
 
 static Workspace<double> g_ws64;  // one cached workspace per type (single-threaded backend)
 static Workspace<float> g_ws32;
 
+constexpr int kWsVectors = 11;  // vectors of one solve
+
+static size_t ws_stride(int n) { return ((size_t)std::max(n, 1) + 63) & ~size_t(63); }
+
+// The eleven solve vectors, `stride` elements apart from q.
 template <typename T>
-static ohm_status ws_get(Workspace<T> &w, int n) {
-  if (w.base && w.n >= n) return OHM_OK;
-  if (w.base) OHM_HIP(hipFree(w.base));
-  if (w.st) OHM_HIP(hipFree(w.st));
-  size_t stride = ((size_t)std::max(n, 1) + 63) & ~size_t(63);
-  OHM_HIP(hipMalloc(&w.base, sizeof(T) * stride * 11));
-  OHM_HIP(hipMalloc(&w.st, sizeof(SolveState)));
-  T *q = w.base;
+static void ws_views(Workspace<T> &w, T *q, size_t stride) {
   w.v.x = q;
   w.v.r = q + stride;
   w.v.rh = q + 2 * stride;
@@ -582,6 +582,20 @@ static ohm_status ws_get(Workspace<T> &w, int n) {
   w.v.sh = q + 8 * stride;
   w.v.tmp = q + 9 * stride;
   w.v.p2 = q + 10 * stride;
+}
+
+template <typename T>
+static ohm_status ws_get(Workspace<T> &w, int n) {
+  if (w.base && w.n >= n) return OHM_OK;
+  if (w.base) OHM_HIP(hipFree(w.base));
+  if (w.st) OHM_HIP(hipFree(w.st));
+  w.base = nullptr;
+  w.st = nullptr;
+  w.n = 0;
+  const size_t stride = ws_stride(n);
+  OHM_HIP(hipMalloc(&w.base, sizeof(T) * stride * kWsVectors));
+  OHM_HIP(hipMalloc(&w.st, sizeof(SolveState)));
+  ws_views(w, w.base, stride);
   w.n = n;
   return OHM_OK;
 }
@@ -619,12 +633,24 @@ struct Ops<float> {
 // transpose view's first factor).
 static const DevCsr &isai_first(const PrecondDev &p) { return p.mm.rowptr ? p.mm : p.ml; }
 
-// Partial regions and consumer widths for one solve; sizes the backend's
-// partials buffer. A consumer re-sums its producer's partials unless that
-// means consumer workgroups x producer workgroups beyond kDeferMaxReads;
-// then a one-workgroup k_fold launch reduces them first.
+// Widest producing grid of one solve (vector kernels, SpMV, first ISAI SpMV)
+// and the doubles its partial regions take.
+static size_t parts_stride(const DevCsr &a, const PrecondDev &p, int n) {
+  const int64_t gv = grid_for(n, 4), ga = std::max(a.n_blocks, 1);
+  const int64_t gm = p.kind == PK_ISAI ? std::max(isai_first(p).n_blocks, 1) : gv;
+  return (size_t)std::max(std::max(gv, ga), gm);
+}
+static size_t parts_doubles(const DevCsr &a, const PrecondDev &p, int n) {
+  return std::max<size_t>(parts_stride(a, p, n) * 11, (size_t)kMaxReduceBlocks * 4);
+}
+
+// Partial regions and consumer widths for one solve, in `base` (at least
+// parts_doubles() doubles) or, when base is null, in the backend's partials
+// buffer (grown to fit). A consumer re-sums its producer's partials unless
+// that means consumer workgroups x producer workgroups beyond
+// kDeferMaxReads; then a one-workgroup k_fold launch reduces them first.
 static ohm_status parts_for(ohm_backend *b, const DevCsr &a, const PrecondDev &p, int n,
-                            Parts &pp) {
+                            Parts &pp, double *base = nullptr) {
   const int64_t gv = grid_for(n, 4), ga = std::max(a.n_blocks, 1);
   const bool isai = p.kind == PK_ISAI;
   const int64_t gm = isai ? std::max(isai_first(p).n_blocks, 1) : gv;
@@ -632,9 +658,9 @@ static ohm_status parts_for(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   auto fold = [](int64_t producer, int64_t consumer) {
     return producer * consumer > kDeferMaxReads;
   };
-  const size_t S = (size_t)std::max(std::max(gv, ga), g3);
-  OHM_TRY(ensure_partials(b, std::max<size_t>(S * 11, (size_t)kMaxReduceBlocks * 4)));
-  double *q = b->d_partials;
+  const size_t S = parts_stride(a, p, n);
+  if (!base) OHM_TRY(ensure_partials(b, parts_doubles(a, p, n)));
+  double *q = base ? base : b->d_partials;
   pp.init = q;
   pp.rv = q + 4 * S;
   pp.ss = q + 6 * S;
@@ -746,7 +772,7 @@ static ohm_status replay_chunk(ohm_backend *b, const DevCsr &a, const PrecondDev
       (uintptr_t)Ops<T>::vals(p.mu), (uintptr_t)p.mu.blocks, (uintptr_t)p.mu.n_blocks,
       (uintptr_t)p.mm.rowptr, (uintptr_t)Ops<T>::vals(p.mm), (uintptr_t)p.mm.blocks,
       (uintptr_t)p.mm.n_blocks,
-      (uintptr_t)v.x, (uintptr_t)v.tmp, (uintptr_t)st, (uintptr_t)b->d_partials,
+      (uintptr_t)v.x, (uintptr_t)v.tmp, (uintptr_t)st, (uintptr_t)pp.init,
       (uintptr_t)iters};
   hipGraphExec_t exec = nullptr;
   for (auto &g : b->graphs)
@@ -794,7 +820,7 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   SolveState *st = w.st;
   BicgVecs<T> &v = w.v;
   Parts pp;
-  OHM_TRY(parts_for(b, a, p, n, pp));
+  OHM_TRY(parts_for(b, a, p, n, pp, w.partials));
   hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, b->stream, st);
   if (o.warm) Ops<T>::spmv(b, a, v.x, v.t, nullptr, 0);
   hipLaunchKernelGGL(k_init<T>, dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream, n, rhs, v.t, v.x,
@@ -825,7 +851,7 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
                     ? o.check_interval
                     : (int)std::min<int64_t>(16, std::max<int64_t>(4, done / 2));
     chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, o.max_iters - enqueued));
-    if (b->use_graphs) {
+    if (b->use_graphs && !o.eager) {
       // replay captured chunks; surplus iterations are no-ops once the
       // state leaves RUNNING
       const bool newton_sub = o.tol_mode == OHM_TOL_INITIAL || o.forcing.kind != 0;
@@ -964,6 +990,9 @@ static int g_ir_n = 0;
 static double *g_ir_x0 = nullptr;  // the caller's starting iterate (FP64 rerun)
 static int g_ir_x0_n = 0;
 
+static ohm_status residual_norm(ohm_backend *b, const DevCsr &a, const double *rhs,
+                                const double *x, double *r, double *partials, double &rel);
+
 ohm_status true_residual(ohm_backend *b, const DevCsr &a, const double *rhs, const double *x,
                          double &rel) {
   if (a.n == 0) {
@@ -978,8 +1007,16 @@ ohm_status true_residual(ohm_backend *b, const DevCsr &a, const double *rhs, con
     g_ir_n = a.n;
   }
   OHM_TRY(ensure_partials(b, (size_t)std::max(a.n_blocks, kMaxReduceBlocks) * 4));
+  return residual_norm(b, a, rhs, x, g_ir_r, b->d_partials, rel);
+}
+
+// ||b - A x|| / ||b|| with r and the partial sums (max(n_blocks, 2048) * 4
+// doubles) in the caller's buffers (synchronous).
+static ohm_status residual_norm(ohm_backend *b, const DevCsr &a, const double *rhs,
+                                const double *x, double *r, double *partials, double &rel) {
   hipLaunchKernelGGL(k_residual, dim3(a.n_blocks), dim3(kBlock), 0, b->stream, a.rowptr, a.col,
-                     a.val, a.blocks, x, rhs, g_ir_r, b->d_partials, b->d_counter, b->d_scalars);
+                     a.val, a.blocks, x, rhs, r, partials, b->d_counter, b->d_scalars);
+  OHM_HIP(hipGetLastError());
   b->dispatches++;
   OHM_HIP(hipMemcpyAsync(b->h_pinned, b->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost,
                          b->stream));
@@ -988,6 +1025,79 @@ ohm_status true_residual(ohm_backend *b, const DevCsr &a, const double *rhs, con
   double rn = sqrt(b->h_pinned[0]), bn = sqrt(b->h_pinned[1]);
   rel = bn > 0 ? rn / bn : rn;
   return OHM_OK;
+}
+
+// ------------------------------------------------ allocation-free solve --
+// Scratch of the allocation-free solve: the eleven solve vectors, the state
+// word and the partial sums of one FP64 solve, carved from one file-scope
+// device array that each GPU holds for the life of the process. On the host
+// the array's name is a stand-in symbol, not a device address: each device's
+// address is looked up once (hipGetSymbolAddress, with that device current)
+// and only that address is passed to kernels and copies. Every solve on a
+// GPU shares that GPU's array, so solves that overlap on one GPU race.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+static void *g_scratch_addr[kMaxScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+ohm_status static_scratch_init(ohm_backend *b) {
+  if (b->device < 0 || b->device >= kMaxScratchDevices)
+    return fail(OHM_ERR_INVALID, "device %d is beyond the scratch table (%d devices)", b->device,
+                kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  void *&addr = g_scratch_addr[b->device];
+  if (!addr) {
+    OHM_HIP(hipSetDevice(b->device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    addr = p;
+  }
+  b->scratch = addr;
+  return OHM_OK;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+ohm_status bicg_solve_f64_noalloc(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
+                                  const double *rhs, double *x, BicgOpts o, BicgOut &out,
+                                  double &rel) {
+  OHM_TRY(check_params(o));
+  if (!b->scratch) return fail(OHM_ERR_INVALID, "the backend's scratch address is not set");
+  const int n = a.n;
+  rel = 0.0;
+  if (n == 0) {
+    out.status = ST_CONVERGED;
+    return OHM_OK;
+  }
+  // vectors | state word | partial sums (the solve's regions, then the
+  // residual's max(n_blocks, 2048) * 4 doubles, which they cover)
+  const size_t stride = ws_stride(n);
+  const size_t st_off = align256(sizeof(double) * stride * kWsVectors);
+  const size_t part_off = st_off + align256(sizeof(SolveState));
+  const size_t need = part_off + sizeof(double) * parts_doubles(a, p, n);
+  if (need > kScratchBytes)
+    return fail(OHM_ERR_INVALID,
+                "ohm_bicgstab_noalloc: %d unknowns need %zu bytes of scratch, more than its "
+                "%zu bytes (about 700,000 unknowns)",
+                n, need, kScratchBytes);
+  char *base = (char *)b->scratch;
+  Workspace<double> w;
+  ws_views(w, (double *)base, stride);
+  w.st = (SolveState *)(base + st_off);
+  w.partials = (double *)(base + part_off);
+  w.n = n;
+  o.eager = true;
+  if (o.warm)
+    OHM_HIP(hipMemcpyAsync(w.v.x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, b->stream));
+  OHM_TRY(bicg_core<double>(b, a, p, rhs, w, o, out));
+  OHM_HIP(hipMemcpyAsync(x, w.v.x, sizeof(double) * n, hipMemcpyDeviceToDevice, b->stream));
+  // the solve is over: its t vector and partial regions hold the residual
+  ohm_status s = residual_norm(b, a, rhs, x, w.v.t, w.partials, rel);
+  if (s != OHM_OK) return s;
+  return status_to_error(out.status, out.iters, o.max_iters);
 }
 
 ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double *rhs, double *x,

@@ -83,6 +83,7 @@ ohm_status ohm_backend_create(int device, ohm_backend **out) {
   OHM_HIP(hipEventCreate(&b->ev0));
   OHM_HIP(hipEventCreate(&b->ev1));
   ohm_status s = ensure_partials(b, (size_t)kMaxReduceBlocks * 4);
+  if (s == OHM_OK) s = static_scratch_init(b);
   if (s != OHM_OK) return s;
   *out = b;
   g_live_backends.push_back(b);
@@ -511,6 +512,52 @@ ohm_status ohm_bicgstab(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *r
   PrecondDev &pd = p ? const_cast<ohm_precond *>(p)->dev : none;
   return run_bicgstab(b, const_cast<ohm_matrix *>(a)->csr, pd, rhs->d, x->d, params, result,
                       false);
+}
+
+ohm_status ohm_bicgstab_noalloc(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *rhs,
+                                ohm_buffer *x, const ohm_precond *p,
+                                const ohm_bicgstab_params *params, ohm_bicgstab_result *result) {
+  if (!b || !a) return fail(OHM_ERR_INVALID, "null handle");
+  OHM_TRY(check_len(rhs, a->csr.n, "rhs"));
+  OHM_TRY(check_len(x, a->csr.n, "x"));
+  ohm_bicgstab_params prm;
+  if (params)
+    prm = *params;
+  else
+    ohm_bicgstab_default_params(&prm);
+  if (prm.precision != OHM_PRECISION_FP64)
+    return fail(OHM_ERR_INVALID, "ohm_bicgstab_noalloc solves in FP64 only");
+  PrecondDev none;
+  none.n = a->csr.n;
+  const PrecondDev &pd = p ? p->dev : none;
+  BicgOpts o;
+  o.max_iters = prm.max_iterations;
+  o.tol = prm.tolerance;
+  o.shadow = prm.shadow;
+  o.warm = prm.warm_start != 0;
+  o.check_interval = prm.check_interval;
+  o.scaled_breakdown = prm.breakdown != OHM_BREAKDOWN_ABSOLUTE;
+  o.tol_mode = prm.tol_mode;
+  o.tol_floor = prm.tol_floor;
+  BicgOut out;
+  double rel = NAN;
+  const int64_t d0 = b->dispatches;
+  OHM_HIP(hipEventRecord(b->ev0, b->stream));
+  ohm_status s = bicg_solve_f64_noalloc(b, a->csr, pd, rhs->d, x->d, o, out, rel);
+  OHM_HIP(hipEventRecord(b->ev1, b->stream));
+  OHM_HIP(hipEventSynchronize(b->ev1));
+  if (result) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    result->iterations = out.iters;
+    result->refinements = 0;
+    result->fp64_fallback = 0;
+    result->device_ms = ms;
+    result->dispatches = b->dispatches - d0;
+    result->readbacks = out.polls;
+    result->residual_norm = (s == OHM_OK || s == OHM_ERR_SOLVE) ? rel : NAN;
+  }
+  return s;
 }
 
 ohm_status ohm_bicgstab_transpose(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *rhs,

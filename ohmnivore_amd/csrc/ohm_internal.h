@@ -123,6 +123,8 @@ struct ohm_backend {
   uint64_t ac_ws_gen = 0;            // bumped by every chunk written into ac_ws
   void *dense_ws = nullptr;          // dense LU: n*n matrix + rhs + flag (grow-only)
   size_t dense_ws_bytes = 0;
+  void *scratch = nullptr;           // this device's address of the allocation-free
+                                     // solve's static scratch array (never freed)
   int64_t dispatches = 0, readbacks = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

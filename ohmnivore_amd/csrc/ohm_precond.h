@@ -59,6 +59,7 @@ struct BicgOpts {
   // (choice 1); tol is then ignored and the forcing term becomes the
   // relative tolerance (OHM_TOL_INITIAL).
   Forcing forcing;
+  bool eager = false;  // launch every chunk eagerly even where graphs are on
 };
 struct BicgOut {
   bool fp64_fallback = false;  // FP32_IR finished in FP64
@@ -86,6 +87,16 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
 // ||b - A x|| / ||b|| in FP64 (synchronous)
 ohm_status true_residual(ohm_backend *b, const DevCsr &a, const double *rhs, const double *x,
                          double &rel);
+// FP64 solve whose vectors, state word and partial sums live in the
+// backend's scratch array (no device allocation; eager launches, no graph);
+// rel = ||b - A x|| / ||b|| of the result. Systems whose working set exceeds
+// the array are refused with OHM_ERR_INVALID.
+ohm_status bicg_solve_f64_noalloc(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
+                                  const double *rhs, double *x, BicgOpts o, BicgOut &out,
+                                  double &rel);
+// Looks up this device's address of the scratch array (once per device,
+// with the backend's device current) and keeps it in b->scratch.
+ohm_status static_scratch_init(ohm_backend *b);
 ohm_status status_to_error(int status, int64_t iters, int64_t max_iters);
 
 }  // namespace ohm

@@ -209,6 +209,18 @@ class HipBackend:
                                  ctypes.byref(res)))
         return SolveStats.from_c(res)
 
+    def bicgstab_noalloc(self, a: Matrix, b: Buffer, x: Buffer,
+                         precond: Preconditioner | None = None,
+                         params: N.BicgstabParams | None = None) -> SolveStats:
+        """bicgstab() without device allocation (ohm_bicgstab_noalloc): the
+        solve's own vectors live in the library's static scratch array;
+        FP64 only, systems up to about 700,000 unknowns."""
+        res = N.BicgstabResult()
+        check(lib().ohm_bicgstab_noalloc(self.h, a.h, b.h, x.h, precond.h if precond else None,
+                                         ctypes.byref(params) if params is not None else None,
+                                         ctypes.byref(res)))
+        return SolveStats.from_c(res)
+
     def bicgstab_transpose(self, a: Matrix, b: Buffer, x: Buffer,
                            precond: Preconditioner | None = None,
                            params: N.BicgstabParams | None = None) -> SolveStats:

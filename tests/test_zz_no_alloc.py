@@ -1,0 +1,93 @@
+"""ohm_bicgstab_noalloc (the allocation-free variant of ohm_bicgstab, its
+vectors in the library's static scratch array) against ohm_bicgstab: the
+same x bit for bit and the same iteration count, on two calls in a row, the
+second with a smaller system, and through a second backend on the same
+device; sizes beyond the scratch array and FP32 refinement are refused."""
+import numpy as np
+import pytest
+
+import oracle
+from ohmnivore_amd import _native as N
+from ohmnivore_amd import circuit as ir
+from ohmnivore_amd import solver
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def be():
+    return solver.HipBackend()
+
+
+def solve_both(be, m, rhs, precond_kind, prm, x0=None):
+    a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
+    p = be.preconditioner(a, precond_kind, 1)
+    b = be.new_buffer(m.size)
+    be.upload_vec(rhs, b)
+    out = []
+    for call in (be.bicgstab, be.bicgstab_noalloc):
+        x = be.new_buffer(m.size)
+        be.upload_vec(np.zeros(m.size) if x0 is None else x0, x)
+        st = call(a, b, x, p, prm)
+        out.append((be.download_vec(x), st))
+    return out
+
+
+def test_noalloc_matches_bicgstab_two_calls(be):
+    # first call: the 96x96 diode mesh's linear part with ISAI(1)
+    big = solver.compile(ir.diode_mesh(96, 96, seed=9))
+    prm = solver.bicgstab_params(tolerance=1e-12, max_iterations=20_000)
+    (x1, s1), (y1, t1) = solve_both(be, big, big.b_dc, N.PREC_ISAI, prm)
+    assert y1.tobytes() == x1.tobytes()
+    assert t1.iterations == s1.iterations > 8
+    assert t1.residual_norm == s1.residual_norm < 1e-11
+    ref = oracle.sparse_lu_solve(big.g, big.b_dc)
+    err = np.abs(y1 - ref)
+    assert np.all(err <= np.maximum(1e-6 * np.abs(ref), 1e-9)), err.max()
+    # second call, smaller system, Jacobi, warm-started from a nonzero x0
+    small = solver.compile(ir.resistor_ladder(300, seed=1))
+    prm2 = solver.bicgstab_params(tolerance=1e-13, max_iterations=20_000, warm_start=True)
+    x0 = np.linspace(0.0, 0.3, small.size)
+    (x2, s2), (y2, t2) = solve_both(be, small, small.b_dc, N.PREC_JACOBI, prm2, x0=x0)
+    assert y2.tobytes() == x2.tobytes()
+    assert t2.iterations == s2.iterations > 0
+    ref2 = oracle.sparse_lu_solve(small.g, small.b_dc)
+    err2 = np.abs(y2 - ref2)
+    assert np.all(err2 <= np.maximum(1e-6 * np.abs(ref2), 1e-9)), err2.max()
+
+
+def test_noalloc_second_backend_same_device(be):
+    m = solver.compile(ir.diode_mesh(40, 40, seed=3))
+    prm = solver.bicgstab_params(tolerance=1e-12)
+    (x, s), (y, t) = solve_both(be, m, m.b_dc, N.PREC_ISAI, prm)
+    be2 = solver.HipBackend()
+    (x2, s2), (y2, t2) = solve_both(be2, m, m.b_dc, N.PREC_ISAI, prm)
+    assert y.tobytes() == x.tobytes() == y2.tobytes() == x2.tobytes()
+    assert t.iterations == s.iterations == t2.iterations
+
+
+def test_noalloc_refuses_oversize_and_fp32(be):
+    # a 1-D chain of 800,000 unknowns: 11 vectors alone exceed 64 MiB
+    n = 800_000
+    rows = np.repeat(np.arange(n), 3)
+    cols = np.clip(rows + np.tile([-1, 0, 1], n), 0, n - 1)
+    vals = np.tile([-1.0, 2.5, -1.0], n)
+    keep = np.ones(rows.shape[0], dtype=bool)
+    keep[0] = keep[-1] = False  # the clipped duplicates at both ends
+    g = oracle.from_triplet_arrays(n, n, rows[keep], cols[keep], vals[keep])
+    a = be.upload_matrix(g.values, g.col_indices, g.row_pointers, n)
+    b, x = be.new_buffer(n), be.new_buffer(n)
+    be.upload_vec(np.ones(n), b)
+    with pytest.raises(N.OhmnivoreError, match="scratch") as e:
+        be.bicgstab_noalloc(a, b, x, None, solver.bicgstab_params())
+    assert e.value.kind == "Invalid"
+    m = solver.compile(ir.resistor_ladder(50, seed=1))
+    a2 = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
+    b2, x2 = be.new_buffer(m.size), be.new_buffer(m.size)
+    be.upload_vec(m.b_dc, b2)
+    with pytest.raises(N.OhmnivoreError, match="FP64 only"):
+        be.bicgstab_noalloc(a2, b2, x2, None, solver.bicgstab_params(precision="fp32_ir"))
+    # the refused calls left the scratch usable
+    st = be.bicgstab_noalloc(a2, b2, x2, be.preconditioner(a2, N.PREC_JACOBI, 1),
+                             solver.bicgstab_params(tolerance=1e-12))
+    assert st.iterations > 0
