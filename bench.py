@@ -49,9 +49,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # copy kernel of known size (65,536 KiB each way): FETCH x 1.9994, WRITE x 1.0.
 # Summary: profiles/spmv_diode_mesh_256_counters.md.
 SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
-# HBM traffic of one launch of the AC sweep's dominant kernel, from the same
-# two counter passes over `bench.py --roofline-only` (None until measured).
-AC_TRAFFIC_BYTES = None
+# HBM traffic of one k_part_solve launch of the AC sweep (2,048 rows per
+# part), from the same two counter passes over `bench.py --roofline-only`:
+# FETCH 62,670,247.5 KiB x 1.99948 + WRITE 158,426,986.4 KiB x 1.0.
+# Summary: profiles/ac_sweep_256x101_counters.md.
+AC_TRAFFIC_BYTES = round((62670247.5 * 65536.0 / 32776.5 + 158426986.4) * 1024)
 
 
 def log(msg: str) -> None:
@@ -180,15 +182,18 @@ def ac_headline(args, be, N, ir, solver):
             first = time.perf_counter() - t0
     be.sync()
     t0 = time.perf_counter()
-    res = None
+    res, acc = None, {}
     for _ in range(args.steps):
         res = plan.solve(freqs)
+        # per-phase device ms of this sweep (HIP events on the backend's stream)
+        for k, v in plan.stats()["phase_ms"].items():
+            acc[k] = acc.get(k, 0.0) + v
     be.sync()
     dt = time.perf_counter() - t0
     st = plan.stats()
     worst = float(res.residual_norms.max())
     failed = int((~res.converged).sum())
-    ms = st["phase_ms"]
+    ms = {k: v / args.steps for k, v in acc.items()}  # average over the timed sweeps
     by = ac_bytes(sizes, nf, n_parts(sizes, st["rows_per_part"]))
     kernel = {"part_solve": "k_part_solve" if st["rows_per_part"] else "k_band_lu<1>",
               "relations": "k_part_rel", "residual": "k_part_resid_sum"}
