@@ -92,17 +92,18 @@ def ac_bytes(sizes, n_freqs: int, parts: int):
     frequencies of one circuit; the band rows of G and C and the permuted
     b (3 + 3 + 2 doubles = 64 bytes per row at half bandwidth 1) are read by
     every wave once per pass: twice by the part solve (the elimination and
-    the residual rows of the back substitution), once per direction by the
-    relations. Per (system, row) the part solve writes the pivot row
-    divided by its pivot (3 complex: the eliminated right-hand side and the
-    two entries right of the diagonal), reads it back and writes x
-    (112 bytes). The residual pass reads, per part and system, x at the
-    part's first and last rows and their neighbours (6 complex) and the
-    part's two partial sums."""
+    the back substitution, which rebuilds pivot rows taken from the next
+    row and sums the residual), once per direction by the relations. Per
+    (system, row) the part solve writes the pivot row divided by its pivot
+    (2 complex: the eliminated right-hand side and the entry right of the
+    diagonal; a pivot row that is the next row as given is a marker there),
+    reads it back and writes x (80 bytes). The residual pass reads, per part
+    and system, x at the part's first and last rows and their neighbours
+    (6 complex) and the part's two partial sums."""
     waves = -(-n_freqs // 64)
     rows = float(sum(sizes))
     band = 64.0
-    return {"part_solve": rows * (112.0 * n_freqs + 2 * band * waves),
+    return {"part_solve": rows * (80.0 * n_freqs + 2 * band * waves),
             "relations": rows * band * 2 * waves,
             "residual": parts * n_freqs * (6 * 16.0 + 16.0) + parts * 2 * band * waves}
 

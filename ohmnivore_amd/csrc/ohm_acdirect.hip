@@ -373,50 +373,56 @@ __device__ __forceinline__ void part_row_load(const BandWave &q, int64_t s, int6
   }
 }
 
-// Eliminates the part [s, s+m) in direction order for one lane. NR = 3:
-// right-hand sides f, the start coupling (direction row 0's entry left of
-// the part) and the end coupling (direction row m-1's entry right of it);
-// returns the last pivot row's y and diagonal. NR = 1: xs / xe are the
-// known unknowns beyond the start / end, moved to f; with STORE (top-down
-// only) the pivot rows (y and U divided by the pivot) go to Ub at their
-// global rows.
-template <int NR, bool STORE>
+// Marker stored in place of a pivot row that is row j+1 as given (see
+// part_eliminate): a quiet NaN real part.
+__device__ __forceinline__ cz pivot_marker() { return cz{__builtin_nan(""), 0.0}; }
+__device__ __forceinline__ bool is_pivot_marker(cz e) { return e.re != e.re; }
+
+// Eliminates the part [s, s+m) in direction order for one lane, carrying
+// one remainder row forward. Step j combines the remainder R (row j after
+// the earlier steps: entries at columns j, j+1) with row j+1 as given (N:
+// columns j, j+1, j+2) into
+//     R' = (N_0 R - R_0 N) / max(|Re|, |Im| of R_0 and N_0),
+// which has no entry at column j. Up to a scalar of modulus between 1 and 2
+// this is the remainder partial pivoting leaves (N - (N_0/R_0) R when
+// |R_0| >= |N_0|, else R - (R_0/N_0) N), so no row exchange is needed.
+// REL: right-hand sides f, the start coupling and the end coupling (the
+//   entries of direction rows 0 and m-1 outside the part); ly and lu are
+//   the last remainder's right-hand sides and diagonal.
+// SOLVE (top-down only): the couplings times the known xs / xe are moved
+//   to f; step j's pivot row goes to Ub at global row s+j: when R is the
+//   pivot (|R_0| >= |N_0|), (f_R / R_0, R_1 / R_0); when N is, a marker,
+//   since N is row j+1 as given, which the back substitution reads from the
+//   band rows. The last row's pivot is the last remainder.
+template <bool REL>
 __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int64_t m, bool rev,
                                                double om, bool live, cz xs, cz xe, cz *Ub,
-                                               int S, PartTile &t, cz (&ly)[NR], cz &lu,
+                                               int S, PartTile &t, cz (&ly)[3], cz &lu,
                                                int &bad) {
   constexpr int W = kPartW;
   const int lane = threadIdx.x;
-  cz win[2][W], y[2][NR];
   double rg[W], rc[W], rb[2];
-  // direction rows 0 and 1
+  // direction row 0 is the first remainder: columns -1 (left of the part),
+  // 0 and 1 (right of the part when m == 1)
+  part_row_load(q, s, m, rev, 0, rg, rc, rb);
+  cz r0, r1, y0 = {rb[0], rb[1]}, y1 = {0.0, 0.0}, y2 = {0.0, 0.0};
+  {
+    cz a[W];
 #pragma unroll
-  for (int r = 0; r < 2; r++) {
-    part_row_load(q, s, m, rev, r, rg, rc, rb);
-#pragma unroll
-    for (int c = 0; c < W; c++) win[r][c] = cz{0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < NR; k++) y[r][k] = cz{0.0, 0.0};
-    if (r < m) {
-      y[r][0] = cz{rb[0], rb[1]};
-#pragma unroll
-      for (int d = 0; d < W; d++) {
-        const cz a = {rg[d], om * rc[d]};
-        const int64_t col = r - 1 + d;  // direction column
-        if (col < 0) {
-          if (NR == 3) y[r][1] = a;
-          else y[r][0] = y[r][0] - a * xs;
-        } else if (col >= m) {
-          if (NR == 3) y[r][NR - 1] = a;
-          else y[r][0] = y[r][0] - a * xe;
-        } else {
-          win[r][col] = a;
-        }
-      }
+    for (int d = 0; d < W; d++) a[d] = cz{rg[d], om * rc[d]};
+    r0 = a[1];
+    r1 = m > 1 ? a[2] : cz{0.0, 0.0};
+    const cz cr = m > 1 ? cz{0.0, 0.0} : a[2];
+    if (REL) {
+      y1 = a[0];
+      y2 = cr;
+    } else {
+      y0 = y0 - a[0] * xs - cr * xe;
     }
   }
-  part_row_load(q, s, m, rev, 2 + lane, rg, rc, rb);
-  for (int64_t t0 = 0; t0 < m; t0 += kBandTile) {
+  // tiles of direction rows 1 + t0 .. 64 + t0; step j reads row j+1
+  part_row_load(q, s, m, rev, 1 + lane, rg, rc, rb);
+  for (int64_t t0 = 0; t0 < m - 1; t0 += kBandTile) {
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < W; d++) {
@@ -426,76 +432,58 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
     t.b[lane][0] = rb[0];
     t.b[lane][1] = rb[1];
     __syncthreads();
-    part_row_load(q, s, m, rev, 2 + t0 + kBandTile + lane, rg, rc, rb);  // next tile in flight
-    const int64_t t1 = t0 + kBandTile < m ? t0 + kBandTile : m;
+    part_row_load(q, s, m, rev, 1 + t0 + kBandTile + lane, rg, rc, rb);  // next tile in flight
+    const int64_t t1 = t0 + kBandTile < m - 1 ? t0 + kBandTile : m - 1;
     for (int64_t j = t0; j < t1; j++) {
-      const double a0 = cabs2(win[0][0]), a1 = cabs2(win[1][0]);
-      const bool sw = j + 1 < m && a1 > a0;
-      if (!((sw ? a1 : a0) > 0.0)) bad = 1;
-      if (sw) {
-#pragma unroll
-        for (int c = 0; c < W; c++) {
-          const cz u = win[0][c];
-          win[0][c] = win[1][c];
-          win[1][c] = u;
-        }
-#pragma unroll
-        for (int k = 0; k < NR; k++) {
-          const cz u = y[0][k];
-          y[0][k] = y[1][k];
-          y[1][k] = u;
-        }
-      }
-      const cz pinv = cinv(win[0][0]);
-      const cz l = win[1][0] * pinv;
-#pragma unroll
-      for (int c = 1; c < W; c++) win[1][c] = win[1][c] - l * win[0][c];
-#pragma unroll
-      for (int k = 0; k < NR; k++) y[1][k] = y[1][k] - l * y[0][k];
-      if (STORE && live) {  // the pivot row divided by its pivot, as in k_band_lu
-        const int64_t i = s + j;
-        Ub[(i * W) * S + lane] = y[0][0] * pinv;
-#pragma unroll
-        for (int c = 1; c < W; c++) Ub[(i * W + c) * S + lane] = win[0][c] * pinv;
-      }
-      if (j == m - 1) {
-        lu = win[0][0];
-#pragma unroll
-        for (int k = 0; k < NR; k++) ly[k] = y[0][k];
-      }
-      // shift; direction row j+2 enters (tile slot j - t0)
-#pragma unroll
-      for (int c = 0; c + 1 < W; c++) win[0][c] = win[1][c + 1];
-      win[0][W - 1] = cz{0.0, 0.0};
-#pragma unroll
-      for (int k = 0; k < NR; k++) {
-        y[0][k] = y[1][k];
-        y[1][k] = cz{0.0, 0.0};
-      }
-      const int64_t kr = j + 2;
       const int sl = (int)(j - t0);
-#pragma unroll
-      for (int c = 0; c < W; c++) win[1][c] = cz{0.0, 0.0};
-      if (kr < m) {
-        y[1][0] = cz{t.b[sl][0], t.b[sl][1]};
-#pragma unroll
-        for (int d = 0; d < W; d++) {
-          const cz a = {t.g[sl][d], om * t.c[sl][d]};
-          if (j + 1 + d >= m) {  // right of the part: only row m-1, d = 2
-            if (NR == 3) y[1][NR - 1] = a;
-            else y[1][0] = y[1][0] - a * xe;
-          } else {
-            win[1][d] = a;
-          }
+      const cz n0 = {t.g[sl][0], om * t.c[sl][0]}, n1 = {t.g[sl][1], om * t.c[sl][1]};
+      cz n2 = {t.g[sl][2], om * t.c[sl][2]}, f = {t.b[sl][0], t.b[sl][1]}, e2 = {0.0, 0.0};
+      if (j + 2 >= m) {  // row m-1: its column-m entry is the end coupling
+        if (REL)
+          e2 = n2;
+        else
+          f = f - n2 * xe;
+        n2 = cz{0.0, 0.0};
+      }
+      if (!REL) {
+        const cz inv = cinv(r0);
+        const bool n_pivot = cabs2(n0) > cabs2(r0);
+        const cz e0 = n_pivot ? pivot_marker() : y0 * inv;
+        const cz e1 = r1 * inv;
+        if (live) {
+          const int64_t i = s + j;
+          Ub[(i * 2) * S + lane] = e0;
+          Ub[(i * 2 + 1) * S + lane] = e1;
         }
+      }
+      const double mx = fmax(fmax(fabs(r0.re), fabs(r0.im)), fmax(fabs(n0.re), fabs(n0.im)));
+      if (!(mx > 0.0)) bad = 1;
+      const double sc = __builtin_amdgcn_rcp(mx);  // any positive scale will do
+      const cz c1 = {n0.re * sc, n0.im * sc}, c2 = {-r0.re * sc, -r0.im * sc};
+      const cz nr0 = c1 * r1 + c2 * n1;
+      r1 = c2 * n2;
+      r0 = nr0;
+      y0 = c1 * y0 + c2 * f;
+      if (REL) {
+        y1 = c1 * y1;
+        y2 = c1 * y2 + c2 * e2;
       }
     }
   }
+  if (!REL && live) {  // the last row: its pivot is the remainder (no entry past the part)
+    const int64_t i = s + m - 1;
+    Ub[(i * 2) * S + lane] = y0 * cinv(r0);
+    Ub[(i * 2 + 1) * S + lane] = cz{0.0, 0.0};
+  }
+  lu = r0;
+  ly[0] = y0;
+  ly[1] = y1;
+  ly[2] = y2;
 }
 
 // Phase 1: blockIdx.x = 2 ip + direction; relation of (part ip, direction)
 // to rel[((ip * 2 + dir) * 3 + k) * 64 + lane], k = alpha, k_prev, k_next.
-__global__ __launch_bounds__(64, 4) void k_part_rel(const BandWave *__restrict__ waves,
+__global__ __launch_bounds__(64) void k_part_rel(const BandWave *__restrict__ waves,
                                                  const PartRef *__restrict__ parts,
                                                  cz *__restrict__ rel, int *__restrict__ singular) {
   __shared__ PartTile t;
@@ -508,8 +496,8 @@ __global__ __launch_bounds__(64, 4) void k_part_rel(const BandWave *__restrict__
   const double om = live ? q.omega[lane] : 0.0;
   cz ly[3], lu = {1.0, 0.0};
   int bad = 0;
-  part_eliminate<3, false>(q, s, m, dir == 1, om, live, cz{0.0, 0.0}, cz{0.0, 0.0}, nullptr,
-                           q.nf, t, ly, lu, bad);
+  part_eliminate<true>(q, s, m, dir == 1, om, live, cz{0.0, 0.0}, cz{0.0, 0.0}, nullptr, q.nf,
+                       t, ly, lu, bad);
   if (!live) return;
   if (bad || !(cabs2(lu) > 0.0)) {
     singular[q.sys + lane] = 2;
@@ -672,19 +660,23 @@ __global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ 
   const int S = q.nf;
   cz *Ub = U + q.store_off * W;
   cz *Xb = X + q.store_off;
-  cz ly[1], lu;
+  cz ly[3], lu;
   int bad = 0;
-  part_eliminate<1, true>(q, s, m, false, om, live, xs, xe, Ub, S, t, ly, lu, bad);
-  // back substitution over the part's rows (columns past it carry no
-  // entries), 64 rows per tile; the band rows of the tile's residual rows
-  // (row j+1 for row j of the tile) are staged in LDS at slot hi - j
+  part_eliminate<false>(q, s, m, false, om, live, xs, xe, Ub, S, t, ly, lu, bad);
+  if (!(cabs2(lu) > 0.0)) bad = 1;
+  // back substitution over the part's rows, 64 rows per tile: row j's pivot
+  // row is (e_0, e_1): x_j = e_0 - e_1 x_{j+1}, or the marker: row j+1 as
+  // given, x_j = (f_{j+1} - a_{j+1,j+1} x_{j+1} - a_{j+1,j+2} x_{j+2}) /
+  // a_{j+1,j} (column s+m is outside the part: its entry times xe is in
+  // f). The band rows j+1 of the tile, which the interior residual rows
+  // also use, are staged in LDS at slot hi - j.
   constexpr int G = 4;
-  cz xw[W - 1] = {cz{0.0, 0.0}, cz{0.0, 0.0}};
+  cz xw[2] = {cz{0.0, 0.0}, cz{0.0, 0.0}};  // x_{j+1}, x_{j+2}
   double rr = 0.0, bb = 0.0;
   for (int64_t hi = s + m - 1; hi >= s; hi -= kBandTile) {
     const int64_t lo = hi - kBandTile + 1 > s ? hi - kBandTile + 1 : s;
     const int64_t ir = hi + 1 - lane;
-    const bool in = ir >= s + 1 && ir <= s + m - 2 && ir - 1 >= lo;
+    const bool in = ir >= s + 1 && ir <= s + m - 1 && ir - 1 >= lo;
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < W; d++) {
@@ -695,29 +687,34 @@ __global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ 
     t.b[lane][1] = in ? q.b[2 * ir + 1] : 0.0;
     __syncthreads();
     for (int64_t g0 = hi; g0 >= lo; g0 -= G) {
-      cz u[G][W];
+      cz u[G][2];
 #pragma unroll
       for (int g = 0; g < G; g++) {
         const int64_t j = g0 - g;
 #pragma unroll
-        for (int c = 0; c < W; c++)
-          u[g][c] = j >= lo && live ? Ub[(j * W + c) * S + lane] : cz{0.0, 0.0};
+        for (int c = 0; c < 2; c++)
+          u[g][c] = j >= lo && live ? Ub[(j * 2 + c) * S + lane] : cz{0.0, 0.0};
       }
 #pragma unroll
       for (int g = 0; g < G; g++) {
         const int64_t j = g0 - g;
         if (j >= lo) {
-          cz xj = u[g][0];
+          const int sl = (int)(hi - j);
+          cz a[W];  // row j+1 (columns j .. j+2), when inside the part
 #pragma unroll
-          for (int c = 1; c < W; c++) xj = xj - u[g][c] * xw[c - 1];
+          for (int d = 0; d < W; d++) a[d] = cz{t.g[sl][d], om * t.c[sl][d]};
+          const cz bj = {t.b[sl][0], t.b[sl][1]};
+          cz xj = u[g][0] - u[g][1] * xw[0];
+          if (is_pivot_marker(u[g][0])) {
+            const bool last = j + 2 >= s + m;  // row j+1 is the part's last row
+            const cz f = last ? bj - a[2] * xe : bj;
+            const cz a2 = last ? cz{0.0, 0.0} : a[2];
+            xj = (f - a[1] * xw[0] - a2 * xw[1]) * cinv(a[0]);
+          }
           if (live) Xb[j * S + lane] = xj;
           if (j + 1 <= s + m - 2) {  // interior row j + 1: columns j .. j + 2
-            const int sl = (int)(hi - j);
-            cz a[W];
-#pragma unroll
-            for (int d = 0; d < W; d++) a[d] = cz{t.g[sl][d], om * t.c[sl][d]};
             const cz xc[W] = {xj, xw[0], xw[1]};
-            resid_row<W>(a, cz{t.b[sl][0], t.b[sl][1]}, xc, rr, bb);
+            resid_row<W>(a, bj, xc, rr, bb);
           }
           xw[1] = xw[0];
           xw[0] = xj;
