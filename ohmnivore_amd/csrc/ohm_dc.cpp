@@ -9,6 +9,7 @@
 // so a step costs one values/RHS upload plus the Newton iterations.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "ohm_internal.h"
@@ -180,10 +181,16 @@ struct Cascade {
         target = last * std::pow(10.0, -(double)sub);
       }
     }
+    // the final true-GMIN step runs NewtonParams::default()'s iteration cap
+    // (OHM_DC_FINAL_MAX_ITERATIONS overrides it: a cap too small to converge
+    // sends the cascade on to pseudo-transient continuation)
     ohm_newton_params dflt;
     ohm_newton_default_params(&dflt);
-    s = step(OHM_DC_GMIN_FINAL, 0.0, m->b_dc, 0.0, have ? x_prev.data() : nullptr,
-             dflt.max_iterations, dflt.bicgstab_max_iterations, 0);
+    int64_t final_it = dflt.max_iterations;
+    if (const char *env = getenv("OHM_DC_FINAL_MAX_ITERATIONS"))
+      final_it = std::max<int64_t>(0, atoll(env));
+    s = step(OHM_DC_GMIN_FINAL, 0.0, m->b_dc, 0.0, have ? x_prev.data() : nullptr, final_it,
+             dflt.bicgstab_max_iterations, 0);
     if (s == OHM_OK || !recoverable(s) || !have) return s;
     return pseudo_transient();
   }

@@ -346,6 +346,30 @@ def test_ac_plan_partitioned_vs_lu(be, rows, monkeypatch):
         assert plan.x(k, 60, 5).tobytes() == res.x[k][60:65].tobytes()
 
 
+@pytest.mark.parametrize("rows", ["0", "4"])
+def test_ac_plan_many_tiny_circuits_in_chunks(be, rows, monkeypatch):
+    """3,000 ladders of 1-5 sections at 7 frequencies with a 0.5 MB device
+    budget per chunk: the sweep is cut into many chunks (the wave tables,
+    part tables and workspace of each rebuilt), and every one of the 21,000
+    systems matches the oracle's LU; a multi-chunk sweep leaves nothing
+    resident."""
+    monkeypatch.setenv("OHM_AC_PART_ROWS", rows)
+    monkeypatch.setenv("OHM_AC_DIRECT_BUDGET_GB", "0.0005")
+    ckts, _ = ir.rlc_sweep(n_circuits=3000, lo=1, hi=5, seed=11)
+    freqs = 10.0 ** np.linspace(0.0, 9.0, 7)
+    mnas = [solver.compile(c) for c in ckts]
+    plan = solver.AcPlan(be, mnas)
+    res = plan.solve(freqs, keep_x=True)
+    st = plan.stats()
+    assert st["chunks"] > 10, st
+    assert bool(res.converged.all()) and float(res.residual_norms.max()) < 1e-12
+    for m, x in zip(mnas, res.x):
+        ok, err = close(x, oracle.ac_sweep(m, freqs))
+        assert ok, (m.size, err)
+    with pytest.raises(N.OhmnivoreError, match="not resident"):
+        plan.x(0)
+
+
 def test_ac_plan_errors(be):
     ckts, freqs = ir.rlc_sweep(n_circuits=2, lo=10, hi=40, seed=4)
     plan = solver.AcPlan(be, [solver.compile(c) for c in ckts])

@@ -1,6 +1,7 @@
 """dc::run on the GPU (src/analysis/dc.rs:25-650): linear operating points,
 the direct Newton attempt and the continuation cascade (source stepping,
-GMIN stepping), each against the oracle's LU answers."""
+GMIN stepping, pseudo-transient continuation), each against the oracle's LU
+answers."""
 import numpy as np
 import pytest
 
@@ -57,6 +58,28 @@ def test_gmin_stepping_path(be):
     labels = [p[0] for p in path]
     assert labels[0] == "direct" and not path[0][2] and "gmin" in labels and labels[-1] == "final"
     ok, err = close(x, oracle_op(mna))
+    assert ok, err
+
+
+def test_pseudo_transient_path(be, monkeypatch):
+    """The final true-GMIN Newton capped at zero iterations (forced from the
+    host) fails after GMIN stepping, so the cascade runs pseudo-transient
+    continuation from the last GMIN point (src/analysis/dc.rs:274-290,
+    341-500): its damped steps, then the final alpha = 0 solve, landing on
+    the oracle's LU operating point."""
+    monkeypatch.setenv("OHM_DC_FINAL_MAX_ITERATIONS", "0")
+    mna = solver.compile(ir.diode_mesh(24, 24, seed=2))
+    res = dc.run(be, mna, max_iterations=1, **TIGHT)
+    labels = [p[0] for p in res.path]
+    assert labels[0] == "direct" and not res.path[0][2] and "gmin" in labels
+    finals = [p for p in res.path if p[0] == "final"]
+    assert len(finals) == 1 and not finals[0][2] and finals[0][3] == 0
+    assert "ptc" in labels and labels[-1] == "ptc-final" and res.path[-1][2]
+    ptc_levels = [p[1] for p in res.path if p[0] == "ptc" and p[2]]
+    assert ptc_levels and ptc_levels == sorted(ptc_levels, reverse=True) and ptc_levels[0] <= 0.1
+    assert res.stage == "ptc-final"
+    assert res.newton_iterations == sum(p[3] for p in res.path)
+    ok, err = close(res.x, oracle_op(mna))
     assert ok, err
 
 
