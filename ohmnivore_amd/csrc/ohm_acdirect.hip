@@ -434,6 +434,7 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
     __syncthreads();
     part_row_load(q, s, m, rev, 1 + t0 + kBandTile + lane, rg, rc, rb);  // next tile in flight
     const int64_t t1 = t0 + kBandTile < m - 1 ? t0 + kBandTile : m - 1;
+#pragma unroll 2
     for (int64_t j = t0; j < t1; j++) {
       const int sl = (int)(j - t0);
       const cz n0 = {t.g[sl][0], om * t.c[sl][0]}, n1 = {t.g[sl][1], om * t.c[sl][1]};

@@ -297,12 +297,6 @@ struct NewtonWord {
   int pad;
 };
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
 __global__ __launch_bounds__(kBlock) void k_converge(int n, const double *__restrict__ xo,
                                                      const double *__restrict__ xn,
                                                      double *partials, unsigned *counter,

@@ -36,10 +36,46 @@ struct Acc<float> {
   __device__ __forceinline__ double get() const { return (double)hi + (double)lo; }
 };
 
+// op(v, the value a DPP lane permutation brings to this lane), moved as the
+// two 32-bit halves of the double; lanes of rows outside kRowMask combine
+// with `ident` (the identity of op) instead.
+template <int kCtrl, int kRowMask, typename Op>
+__device__ __forceinline__ double dpp_op(double v, double ident, Op op) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v), e = __builtin_bit_cast(uint64_t, ident);
+  const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)u, kCtrl, kRowMask,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32), (int)(uint32_t)(u >> 32),
+                                             kCtrl, kRowMask, 0xf, false);
+  return op(v, __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// op over the 64 lanes of a wave (every lane active) with DPP lane
+// permutations, no LDS traffic: pairs and quads (quad_perm), 8-lane halves
+// (row_half_mirror), 16-lane rows (row_mirror), then row 0 into row 1 and
+// row 2 into row 3 (row_bcast:15) and rows 0-1 into rows 2-3
+// (row_bcast:31). Lane 63 ends with the total, which every lane gets back.
+// The combination order is fixed, so sums are reproducible.
+template <typename Op>
+__device__ __forceinline__ double wave_reduce(double v, double ident, Op op) {
+  v = dpp_op<0xb1, 0xf>(v, ident, op);   // quad_perm [1,0,3,2]
+  v = dpp_op<0x4e, 0xf>(v, ident, op);   // quad_perm [2,3,0,1]
+  v = dpp_op<0x141, 0xf>(v, ident, op);  // row_half_mirror
+  v = dpp_op<0x140, 0xf>(v, ident, op);  // row_mirror
+  v = dpp_op<0x142, 0xa>(v, ident, op);  // row_bcast:15 into rows 1 and 3
+  v = dpp_op<0x143, 0xc>(v, ident, op);  // row_bcast:31 into rows 2 and 3
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  return wave_reduce(v, 0.0, [](double a, double b) { return a + b; });
+}
+
+// fmax over the wave (a NaN lane is ignored, as fmax does)
+__device__ __forceinline__ double wave_max(double v) {
+  return wave_reduce(v, -INFINITY, [](double a, double b) { return fmax(a, b); });
 }
 
 // Sum K values over the workgroup; totals land in thread 0's v[].

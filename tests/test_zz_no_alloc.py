@@ -10,6 +10,7 @@ import oracle
 from ohmnivore_amd import _native as N
 from ohmnivore_amd import circuit as ir
 from ohmnivore_amd import solver
+from ohmnivore_amd.transient import c_on_g
 
 pytestmark = pytest.mark.gpu
 
@@ -19,8 +20,9 @@ def be():
     return solver.HipBackend()
 
 
-def solve_both(be, m, rhs, precond_kind, prm, x0=None):
-    a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)
+def solve_both(be, m, rhs, precond_kind, prm, x0=None, values=None):
+    vals = m.g.values if values is None else values
+    a = be.upload_matrix(vals, m.g.col_indices, m.g.row_pointers, m.size)
     p = be.preconditioner(a, precond_kind, 1)
     b = be.new_buffer(m.size)
     be.upload_vec(rhs, b)
@@ -34,14 +36,17 @@ def solve_both(be, m, rhs, precond_kind, prm, x0=None):
 
 
 def test_noalloc_matches_bicgstab_two_calls(be):
-    # first call: the 96x96 diode mesh's linear part with ISAI(1)
-    big = solver.compile(ir.diode_mesh(96, 96, seed=9))
+    # first call: a 96x96 RC grid's backward-Euler companion matrix G + C/h
+    # (h = 1 ps) with ISAI(1)
+    big = solver.compile(ir.rc_grid(96, 96, 64, seed=3))
+    vals = big.g.values + c_on_g(big) / 1e-12
     prm = solver.bicgstab_params(tolerance=1e-12, max_iterations=20_000)
-    (x1, s1), (y1, t1) = solve_both(be, big, big.b_dc, N.PREC_ISAI, prm)
+    (x1, s1), (y1, t1) = solve_both(be, big, big.b_dc, N.PREC_ISAI, prm, values=vals)
     assert y1.tobytes() == x1.tobytes()
     assert t1.iterations == s1.iterations > 8
     assert t1.residual_norm == s1.residual_norm < 1e-11
-    ref = oracle.sparse_lu_solve(big.g, big.b_dc)
+    a = ir.CsrMatrix(big.size, big.size, vals, big.g.col_indices, big.g.row_pointers)
+    ref = oracle.sparse_lu_solve(a, big.b_dc)
     err = np.abs(y1 - ref)
     assert np.all(err <= np.maximum(1e-6 * np.abs(ref), 1e-9)), err.max()
     # second call, smaller system, Jacobi, warm-started from a nonzero x0
