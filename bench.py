@@ -51,9 +51,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
 # HBM traffic of one k_part_solve launch of the AC sweep (2,048 rows per
 # part), from the same two counter passes over `bench.py --roofline-only`:
-# FETCH 62,670,247.5 KiB x 1.99948 + WRITE 158,426,986.4 KiB x 1.0.
+# FETCH 43,184,550.2 KiB x 1.99939 + WRITE 118,645,976.0 KiB x 1.0.
 # Summary: profiles/ac_sweep_256x101_counters.md.
-AC_TRAFFIC_BYTES = round((62670247.5 * 65536.0 / 32776.5 + 158426986.4) * 1024)
+AC_TRAFFIC_BYTES = round((43184550.2 * 65536.0 / 32778.0 + 118645976.0) * 1024)
 
 
 def log(msg: str) -> None:
