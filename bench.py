@@ -279,7 +279,7 @@ def mesh_newton(args, be, N, ir, solver):
             "bicgstab_iterations": stats.bicgstab_iterations,
             "fp64_fallbacks": stats.fp64_fallbacks,
             "max_abs_err_vs_lu_fixture": e, "within_1e-6_rel_1e-9_abs": ok,
-            "us_per_bicgstab_iteration": 1e3 * dt / steps / max(stats.bicgstab_iterations, 1)}
+            "us_per_bicgstab_iteration": 1e6 * dt / steps / max(stats.bicgstab_iterations, 1)}
         log(f"diode mesh {prec}: {forms[prec]['ms_per_solve']:.2f} ms/solve, "
             f"{stats.newton_iterations} Newton, {stats.bicgstab_iterations} BiCGSTAB its, "
             f"err {e:.1e}")
