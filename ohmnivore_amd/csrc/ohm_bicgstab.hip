@@ -527,6 +527,203 @@ __global__ __launch_bounds__(kBlock) void k_xr_update(int n, T *__restrict__ x,
   deposit<2>(vals, pp.rr);
 }
 
+// ------------------------------------------------------ one-wave solve --
+// The whole solve of a system of at most 64 unknowns in one 64-lane
+// workgroup (the reference's bicgstab_fused, src/solver/ds_shaders.rs:
+// 340-728, dispatched for tiny Newton systems at src/solver/newton.rs:
+// 216-252): lane r owns row r and keeps its entries of every vector in
+// registers; a matrix-vector product gathers from the one vector staged in
+// LDS; dot products are DPP wave sums. The recurrences, stopping tests,
+// breakdown tests, shadow residual and forcing are those of the multi-kernel
+// pipeline above (K1-K5 and k_init), in the same order; the loop is bounded
+// by max_iters, so the workgroup always exits.
+constexpr int kFusedRows = 64;
+constexpr uint32_t kFusedLdsBytes = 60 * 1024;  // staged matrices, beside the vector
+
+struct FusedCsr {
+  const uint32_t *rowptr, *col;
+  const double *val;
+};
+
+// A lane's row of one matrix: its bounds in registers, the column indices
+// and values in LDS (STAGE) or global memory.
+struct FusedRow {
+  uint32_t beg, end;
+  const uint32_t *col;
+  const double *val;
+};
+
+// Bytes of LDS the staged column indices and values of a matrix take.
+__host__ __device__ inline uint32_t fused_stage_bytes(uint32_t nnz) {
+  return 8 * nnz + ((4 * nnz + 7) & ~7u);
+}
+
+// Binds lane r to its row of m; with STAGE the whole matrix's columns and
+// values are copied into LDS at lds (advanced past them). The caller
+// synchronises before the first use.
+template <bool STAGE>
+__device__ __forceinline__ FusedRow fused_bind(const FusedCsr &m, int r, int n,
+                                               unsigned char *&lds) {
+  FusedRow f{0, 0, m.col, m.val};
+  if (STAGE) {  // LDS pointers on every path, so the loads compile to ds_read
+    f.col = reinterpret_cast<const uint32_t *>(lds);
+    f.val = reinterpret_cast<const double *>(lds);
+  }
+  if (!m.rowptr) return f;
+  if (r < n) {
+    f.beg = m.rowptr[r];
+    f.end = m.rowptr[r + 1];
+  }
+  if (STAGE) {
+    const uint32_t nnz = m.rowptr[n];
+    double *v = reinterpret_cast<double *>(lds);
+    uint32_t *c = reinterpret_cast<uint32_t *>(lds + 8 * nnz);
+    for (uint32_t k = threadIdx.x; k < nnz; k += kFusedRows) {
+      v[k] = m.val[k];
+      c[k] = m.col[k];
+    }
+    lds += fused_stage_bytes(nnz);
+    f.col = c;
+    f.val = v;
+  }
+  return f;
+}
+
+// A lane's row times src: each product rounded, then summed in column order
+// from 0, the rounding sequence of the pipeline's row-block SpMV (products
+// staged in LDS, summed per row), so the two paths agree to the bit.
+__device__ __forceinline__ double fused_row(const FusedRow &m, const double *__restrict__ src) {
+#pragma clang fp contract(off)
+  double acc = 0.0;
+  for (uint32_t k = m.beg; k < m.end; k++) {
+    const double pr = m.val[k] * src[m.col[k]];
+    acc = acc + pr;
+  }
+  return acc;
+}
+
+// y_r = (A v)_r with v staged in LDS (every lane writes its v_r first)
+__device__ __forceinline__ double fused_spmv(const FusedRow &m, double v, double *sv) {
+  __syncthreads();
+  sv[threadIdx.x] = v;
+  __syncthreads();
+  return fused_row(m, sv);
+}
+
+template <int PK>
+__device__ __forceinline__ double fused_prec(const FusedRow &ml, const FusedRow &mu, bool two,
+                                             double inv, double v, double *sv) {
+  if (PK == PK_JACOBI) return inv * v;
+  if (PK == PK_ISAI) {
+    const double y = fused_spmv(ml, v, sv);
+    return two ? fused_spmv(mu, y, sv) : y;
+  }
+  return v;
+}
+
+template <int PK, bool STAGE>
+__global__ __launch_bounds__(kFusedRows) void k_bicg_fused(
+    int n, FusedCsr ag, FusedCsr mlg, FusedCsr mug, int two, const double *__restrict__ dinv,
+    const double *__restrict__ b, double *__restrict__ x, int shadow, int warm, double tol,
+    double floor_rel, int tol_mode, int max_iters, int scaled, Forcing fc, SolveState *st) {
+  __shared__ double sv[kFusedRows];
+  extern __shared__ __align__(16) unsigned char fused_lds[];
+  const int r = threadIdx.x;
+  const bool in = r < n;
+  unsigned char *lds = fused_lds;
+  const FusedRow a = fused_bind<STAGE>(ag, r, n, lds);
+  const FusedRow ml = fused_bind<STAGE>(mlg, r, n, lds);
+  const FusedRow mu = fused_bind<STAGE>(mug, r, n, lds);
+  if (STAGE) __syncthreads();
+  const double inv = (PK == PK_JACOBI && in) ? dinv[r] : 1.0;
+  // dot products: lanes past n add +0, as threads without a row do in the
+  // pipeline's block sums
+  const auto sum = [in](double v) { return wave_sum(in ? v : 0.0); };
+  // k_init: r = b - A x0 (warm) or b, r_hat, p = v = 0
+  const double bi = in ? b[r] : 0.0;
+  double xi = (in && warm) ? x[r] : 0.0;
+  double ri = bi;
+  if (warm) ri = bi - fused_spmv(a, xi, sv);
+  if (!in) ri = 0.0;
+  const double rh = in ? (shadow ? shadow_value((uint64_t)r) : ri) : 0.0;
+  double p = 0.0, v = 0.0;
+  const double bb = sum(bi * bi), hh = sum(rh * rh);
+  double rr = sum(ri * ri), rhr = sum(rh * ri);
+  const double r0 = sqrt(rr);
+  double eta = 0.0;
+  if (fc.kind) {
+    tol = forcing_eta(fc, r0);
+    eta = tol;
+  }
+  const double bnorm = sqrt(bb);
+  const double abs_tol = tol_mode == OHM_TOL_INITIAL ? fmax(tol * r0, floor_rel * bnorm) : tol * bnorm;
+  double rho = 1.0, alpha = 1.0, omega = 1.0, res2 = rr;
+  int status = ST_RUNNING, iters = 0, s_conv = 0;
+  if (bnorm < 1e-30) {  // b == 0: x = 0 (src/solver/bicgstab.rs:93-97)
+    status = ST_CONVERGED;
+  } else if (warm && r0 < abs_tol) {
+    status = ST_CONVERGED;
+  } else if (rho_breakdown(rhr, hh, rr, scaled)) {
+    status = ST_BRK_RHO;
+  }
+  while (status == ST_RUNNING) {
+    // K1 head (decide_rho) and p update
+    const double rnorm = sqrt(rr);
+    if (s_conv) { status = ST_CONVERGED; break; }
+    if (std::isnan(rnorm) || std::isinf(rnorm)) { status = ST_NAN; break; }
+    if (rnorm < abs_tol) { status = ST_CONVERGED; break; }
+    if (fabs(omega) < 1e-30) { status = ST_BRK_OMEGA; break; }
+    if (iters >= max_iters) { status = ST_MAXITER; break; }
+    if (rho_breakdown(rhr, hh, rr, scaled)) { status = ST_BRK_RHO; break; }
+    const double beta = (rhr / rho) * (alpha / omega);
+    const double rho_new = rhr;
+    p = p_next<double>(ri, p, v, beta, omega);
+    const double ph = fused_prec<PK>(ml, mu, two, inv, p, sv);
+    // K2: v = A p_hat; K3 head: alpha
+    v = fused_spmv(a, ph, sv);
+    if (!in) v = 0.0;
+    const double rv = sum(rh * v), vv = sum(v * v);
+    const bool brk = scaled ? fabs(rv) <= 1e-30 * (sqrt(hh) * sqrt(vv)) : fabs(rv) < 1e-30;
+    if (brk) { status = ST_BRK_RV; break; }
+    alpha = rho_new / rv;
+    rho = rho_new;
+    // K3: s = r - alpha v; K4 head: ||s|| exit
+    const double s = fma(-alpha, v, ri);
+    const double ss = sum(s * s);
+    if (sqrt(ss) < abs_tol) {
+      xi = xi + alpha * ph;
+      iters++;
+      s_conv = 1;
+      res2 = ss;
+      continue;  // the next head turns s_conv into CONVERGED
+    }
+    const double sh = fused_prec<PK>(ml, mu, two, inv, s, sv);
+    // K4: t = A s_hat; K5: omega, x and r updates
+    double t = fused_spmv(a, sh, sv);
+    if (!in) t = 0.0;
+    const double ts = sum(t * s), tt = sum(t * t);
+    if (scaled ? tt <= 1e-30 * ss : fabs(tt) < 1e-30) { status = ST_BRK_TT; break; }
+    omega = ts / tt;
+    iters++;
+    xi = (xi + alpha * ph) + omega * sh;
+    ri = s - omega * t;
+    rr = sum(ri * ri);
+    rhr = sum(rh * ri);
+    res2 = rr;
+  }
+  if (in) x[r] = xi;
+  if (r == 0) {
+    st->status = status;
+    st->s_conv = s_conv;
+    st->iters = iters;
+    st->max_iters = max_iters;
+    st->r0 = r0;
+    st->res2 = res2;
+    st->eta = eta;
+    st->abs_tol = abs_tol;
+  }
+}
+
 // Reduces a producer's np partials (K each) to partials[0..K) for a
 // consumer whose workgroups would otherwise each re-read all of them.
 template <int K>
@@ -909,6 +1106,65 @@ static ohm_status check_params(const BicgOpts &o) {
   return OHM_OK;
 }
 
+// Systems of at most 64 unknowns run the one-wave solve (one launch and one
+// status readback per solve) unless OHM_FUSED_SMALL=0.
+static bool fused_small(int n) {
+  if (n > kFusedRows) return false;
+  const char *e = getenv("OHM_FUSED_SMALL");
+  return !e || atoi(e) != 0;
+}
+
+static ohm_status bicg_solve_fused(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
+                                   const double *rhs, double *x, const BicgOpts &o,
+                                   BicgOut &out) {
+  OHM_TRY(ws_get(g_ws64, a.n));
+  SolveState *st = g_ws64.st;
+  const FusedCsr A{a.rowptr, a.col, a.val};
+  FusedCsr L{nullptr, nullptr, nullptr}, U{nullptr, nullptr, nullptr};
+  int two = 0;
+  uint32_t lds = fused_stage_bytes((uint32_t)a.nnz);
+  if (p.kind == PK_ISAI) {
+    const DevCsr &m1 = isai_first(p);
+    L = FusedCsr{m1.rowptr, m1.col, m1.val};
+    lds += fused_stage_bytes((uint32_t)m1.nnz);
+    two = p.mm.rowptr ? 0 : 1;
+    if (two) {
+      U = FusedCsr{p.mu.rowptr, p.mu.col, p.mu.val};
+      lds += fused_stage_bytes((uint32_t)p.mu.nnz);
+    }
+  }
+  // the matrices go to LDS when they fit beside the vector staging
+  const bool stage = lds <= kFusedLdsBytes;
+  const int shadow = o.shadow == OHM_SHADOW_HASHED ? 1 : 0, warm = o.warm ? 1 : 0;
+  const int max_iters = (int)std::min<int64_t>(o.max_iters, 0x7fffffff);
+  const int scaled = o.scaled_breakdown ? 1 : 0;
+  const dim3 g(1), blk(kFusedRows);
+  const unsigned shm = stage ? lds : 0;
+  auto launch = [&](auto kern) {
+    hipLaunchKernelGGL(kern, g, blk, shm, b->stream, a.n, A, L, U, two, p.inv, rhs, x, shadow,
+                       warm, o.tol, o.tol_floor, o.tol_mode, max_iters, scaled, o.forcing, st);
+  };
+  if (p.kind == PK_JACOBI)
+    stage ? launch(k_bicg_fused<PK_JACOBI, true>) : launch(k_bicg_fused<PK_JACOBI, false>);
+  else if (p.kind == PK_ISAI)
+    stage ? launch(k_bicg_fused<PK_ISAI, true>) : launch(k_bicg_fused<PK_ISAI, false>);
+  else
+    stage ? launch(k_bicg_fused<PK_NONE, true>) : launch(k_bicg_fused<PK_NONE, false>);
+  OHM_HIP(hipGetLastError());
+  b->dispatches++;
+  StatusWord *h = reinterpret_cast<StatusWord *>(b->h_pinned);
+  OHM_HIP(hipMemcpyAsync(h, &st->status, sizeof(StatusWord), hipMemcpyDeviceToHost, b->stream));
+  OHM_HIP(hipStreamSynchronize(b->stream));
+  b->readbacks++;
+  out.polls++;
+  out.status = h->status;
+  out.iters = h->iters;
+  out.r0 = h->r0;
+  out.res = sqrt(h->res2);
+  out.eta = h->eta;
+  return status_to_error(out.status, out.iters, o.max_iters);
+}
+
 ohm_status bicg_solve_f64(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const double *rhs,
                           double *x, const BicgOpts &o, BicgOut &out) {
   OHM_TRY(check_params(o));
@@ -916,6 +1172,7 @@ ohm_status bicg_solve_f64(ohm_backend *b, const DevCsr &a, const PrecondDev &p, 
     out.status = ST_CONVERGED;
     return OHM_OK;
   }
+  if (fused_small(a.n)) return bicg_solve_fused(b, a, p, rhs, x, o, out);
   OHM_TRY(ws_get(g_ws64, a.n));
   if (o.warm) OHM_HIP(hipMemcpyAsync(g_ws64.v.x, x, sizeof(double) * a.n, hipMemcpyDeviceToDevice, b->stream));
   OHM_TRY(bicg_core<double>(b, a, p, rhs, g_ws64, o, out));
