@@ -1862,10 +1862,33 @@ int64_t ohm_generate_frequencies(int32_t sweep, int64_t n_points, double f_start
 // (x_n is already in the iterate) -> probe gather.
 namespace ohm {
 
+// A HIP call's error as an ohm_status (for code that must clean up or
+// record the status instead of returning at once).
+#define OHM_HIP_STATUS(call) ohm::hip_status_of((call), #call, __FILE__, __LINE__)
+static inline ohm_status hip_status_of(hipError_t e, const char *what, const char *file, int line) {
+  return e == hipSuccess ? OHM_OK : hip_fail(e, what, file, line);
+}
+
 struct TRhsArgs {
   double t_next, t_prev;
   double sig_c;   // alpha / h
   int trap;       // TRAP: subtract G x and add b(t_prev)
+};
+
+// Device state of a graph-captured segment of fixed-size backward-Euler
+// steps. Each step's closing kernel accepts the step (time, counters, probe
+// record) when its solve converged; otherwise it sets `stop`, and every
+// later kernel of the segment exits at its head: 1 = the step's solve is
+// still running (the host finishes it), 2 = it failed with `status`.
+struct SegWord {
+  int stop;
+  int status;
+  int steps;       // steps accepted in this segment
+  int n_rec;       // probe records written (the run's running count)
+  long long iters; // BiCGSTAB iterations of the accepted steps
+  int max_iters;   // most iterations one accepted step took
+  int pad;
+  double t;        // time of the last accepted step
 };
 
 // b_eff for every row of every instance; sources are gathered per row in
@@ -1879,7 +1902,13 @@ __global__ __launch_bounds__(kBlock) void k_tran_rhs(const BBlock *__restrict__ 
                                                      const int *__restrict__ sent,
                                                      const int *__restrict__ sslot,
                                                      const TSrc *__restrict__ src,
-                                                     const double *__restrict__ pwl, TRhsArgs a) {
+                                                     const double *__restrict__ pwl, TRhsArgs a,
+                                                     const SegWord *seg, double h) {
+  if (seg) {  // a captured step: t from the segment word, nothing once it stopped
+    if (seg->stop) return;
+    a.t_prev = seg->t;
+    a.t_next = seg->t + h;
+  }
   const BBlock bk = blocks[blockIdx.x];
   const BGeo q = geo[bk.sys];
   const double *xs = x + q.off;
@@ -1908,6 +1937,37 @@ __global__ void k_tran_probe(int K, int np, const uint32_t *__restrict__ probes,
   if (i >= K * np) return;
   const int k = i / np, j = i % np;
   out[i] = x[geo[k].off + probes[j]];
+}
+
+// Closing kernel of a captured step (one wave): accept the step when its
+// solve converged (t += h as the host loop adds it, probe record while
+// t >= tstart and records remain), else stop the segment.
+__global__ __launch_bounds__(64) void k_seg_close(SegWord *seg, const SolveState *st, double h,
+                                                  double tstart, int max_records, int np,
+                                                  const uint32_t *__restrict__ probes,
+                                                  const double *__restrict__ x,
+                                                  double *__restrict__ rec) {
+  const SegWord w = *seg;  // every lane reads before lane 0 writes (one wave)
+  if (w.stop) return;
+  const int status = st->status;
+  if (status != ST_CONVERGED) {
+    if (threadIdx.x == 0) {
+      seg->stop = status == ST_RUNNING ? 1 : 2;
+      seg->status = status;
+    }
+    return;
+  }
+  const double t_next = w.t + h;
+  const bool keep = t_next >= tstart && w.n_rec < max_records;
+  if (keep)
+    for (int j = threadIdx.x; j < np; j += 64) rec[(size_t)w.n_rec * np + j] = x[probes[j]];
+  if (threadIdx.x == 0) {
+    seg->t = t_next;
+    seg->steps = w.steps + 1;
+    seg->iters = w.iters + st->iters;
+    seg->max_iters = max(w.max_iters, st->iters);
+    if (keep) seg->n_rec = w.n_rec + 1;
+  }
 }
 
 // max |a - b| over n entries into *out (bit pattern of a non-negative
@@ -1961,6 +2021,17 @@ struct ohm_transient {
   PrecondDev P1;
   int p1_kind = -1;
   double a1_sigma = NAN;
+  // graph-captured segments of fixed BE steps (one instance, FP64): the
+  // solve's own workspace, the segment word and the captured segments of
+  // this run, by (steps, iterations per step)
+  CapturedSolve *cs = nullptr;
+  SegWord *d_seg = nullptr;
+  struct SegGraph {
+    int steps, iters;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<SegGraph> segs;
 };
 
 namespace {
@@ -1973,7 +2044,7 @@ ohm_status tran_rhs(ohm_transient *tr, double t_next, double t_prev, double sig_
   TRhsArgs a{t_next, t_prev, sig_c, trap ? 1 : 0};
   hipLaunchKernelGGL(k_tran_rhs, dim3(bt->h_blocks.size()), dim3(kBlock), 0, bt->owner->stream,
                      bt->d_full, bt->d_src, x, b, tr->d_srp, tr->d_sent, tr->d_sslot, tr->d_src,
-                     tr->d_pwl, a);
+                     tr->d_pwl, a, (const SegWord *)nullptr, 0.0);
   bt->owner->dispatches++;
   return OHM_OK;
 }
@@ -2116,6 +2187,12 @@ void ohm_transient_free(ohm_transient *tr) {
     for (void *p : ds)
       if (p) (void)hipFree(p);
     graph_cache_clear(tr->owner);
+    for (auto &g : tr->segs) {
+      (void)hipGraphExecDestroy(g.exec);
+      (void)hipGraphDestroy(g.graph);
+    }
+    captured_free(tr->cs);
+    if (tr->d_seg) (void)hipFree(tr->d_seg);
     precond_free(tr->P1);
     csr_free(tr->A1);
   }
@@ -2222,17 +2299,149 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
   if (tstart <= 0.0) OHM_TRY(record(0.0));
   double t = 0.0, h = std::min(tstep, h_max);
   ohm_status rc = OHM_OK;
+  // Graph-captured timesteps (fixed BE, one instance of more than 64
+  // unknowns, FP64, graphs on), enabled by OHM_TRAN_SEGMENT_STEPS = S > 0:
+  // after the first step has formed A = G + C/h and its preconditioner,
+  // runs of up to S full steps are replayed as one graph (per step:
+  // right-hand side, warm-started solve with a fixed number of iterations,
+  // closing kernel) and the host reads one segment word per run instead of
+  // polling every solve. A step whose solve needs more iterations pauses
+  // the segment; the host finishes that solve and the next segment starts
+  // after it. Off by default: a graph has no conditional nodes here, so the
+  // captured iterations a step does not need, and every step after a pause,
+  // still launch and exit at their heads (about 2 us per kernel), which costs
+  // more than the polls it saves (RC grid 512 x 512: 3.27-4.19 ms per step
+  // against 2.95-3.03 ms eager, identical results).
+  int seg_steps = 0, seg_fixed = 0;
+  if (const char *e = getenv("OHM_TRAN_SEGMENT_STEPS")) seg_steps = std::max(0, atoi(e));
+  // OHM_TRAN_SEGMENT_ITERS pins the iterations captured per step (rounded
+  // up to a multiple of 8) instead of following the steps' own counts
+  if (const char *e = getenv("OHM_TRAN_SEGMENT_ITERS")) seg_fixed = (std::max(0, atoi(e)) + 7) / 8 * 8;
+  const bool seg_on = prm->method == OHM_TRAN_FIXED_BE && single && tr->n > 64 &&
+                      sp.precision == OHM_PRECISION_FP64 && b->use_graphs && seg_steps > 0 &&
+                      so.check_interval <= 0;
+  for (auto &g : tr->segs) {  // captured for the previous run's options
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  tr->segs.clear();
+  int seg_iters = 0;  // iterations captured per step (multiple of 8)
+  auto seg_budget = [seg_fixed](int most) {
+    if (seg_fixed > 0) return seg_fixed;
+    const int want = most + std::max(4, most / 8);
+    return std::max(8, (want + 7) / 8 * 8);
+  };
+  // Runs one segment of the full steps ahead; returns false when the eager
+  // loop should take the next step (nothing to capture, or rc set).
+  auto run_segment = [&]() -> bool {
+    // the full steps ahead, with t accumulated as the eager loop adds it
+    std::vector<double> ts;
+    for (double tt = t; (int)ts.size() < seg_steps && tt < tstop && !(tt + h > tstop);) {
+      tt = tt + h;
+      ts.push_back(tt);
+    }
+    if (ts.size() < 2 || seg_iters > 256) return false;
+    const int S = (int)ts.size();
+    if (!tr->cs) {
+      if ((rc = captured_create(b, tr->A1, tr->P1, wx, &tr->cs)) != OHM_OK) return false;
+      if (hipMalloc(&tr->d_seg, sizeof(SegWord)) != hipSuccess) {
+        rc = fail(OHM_ERR_INVALID, "hipMalloc of the segment word failed");
+        return false;
+      }
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : tr->segs)
+      if (g.steps == S && g.iters == seg_iters) exec = g.exec;
+    if (!exec) {
+      const int64_t saved = b->dispatches;
+      ohm_transient::SegGraph g{S, seg_iters, nullptr, nullptr};
+      double *rhs = wx + 10 * stride;
+      const int np = (int)n_probes;
+      if ((rc = OHM_HIP_STATUS(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed))) != OHM_OK)
+        return false;
+      for (int j = 0; j < S && rc == OHM_OK; j++) {
+        hipLaunchKernelGGL(k_tran_rhs, dim3(bt->h_blocks.size()), dim3(kBlock), 0, stream,
+                           bt->d_full, bt->d_src, wx, rhs, tr->d_srp, tr->d_sent, tr->d_sslot,
+                           tr->d_src, tr->d_pwl, TRhsArgs{0.0, 0.0, 1.0 / h, 0},
+                           (const SegWord *)tr->d_seg, h);
+        rc = captured_enqueue(b, tr->cs, tr->A1, tr->P1, rhs, so, seg_iters, &tr->d_seg->stop);
+        hipLaunchKernelGGL(k_seg_close, dim3(1), dim3(64), 0, stream, tr->d_seg,
+                           captured_state(tr->cs), h, tstart, (int)std::max<int64_t>(max_records, 0),
+                           np, tr->d_probes, wx + bt->h_geo[0].off, tr->d_rec);
+      }
+      const ohm_status ce = OHM_HIP_STATUS(hipStreamEndCapture(stream, &g.graph));
+      if (rc == OHM_OK) rc = ce;
+      if (rc == OHM_OK) rc = OHM_HIP_STATUS(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+      b->dispatches = saved;
+      if (rc != OHM_OK) {
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        return false;
+      }
+      tr->segs.push_back(g);
+      exec = g.exec;
+    }
+    SegWord *hw = reinterpret_cast<SegWord *>(b->h_pinned);
+    *hw = SegWord{0, 0, 0, (int)n_rec, 0, 0, 0, t};
+    if ((rc = OHM_HIP_STATUS(hipMemcpyAsync(tr->d_seg, hw, sizeof(SegWord), hipMemcpyHostToDevice,
+                                       stream))) != OHM_OK ||
+        (rc = OHM_HIP_STATUS(hipGraphLaunch(exec, stream))) != OHM_OK ||
+        (rc = OHM_HIP_STATUS(hipMemcpyAsync(hw, tr->d_seg, sizeof(SegWord), hipMemcpyDeviceToHost,
+                                       stream))) != OHM_OK ||
+        (rc = OHM_HIP_STATUS(hipStreamSynchronize(stream))) != OHM_OK)
+      return false;
+    b->readbacks++;
+    const SegWord w = *hw;
+    for (int j = 0; j < w.steps; j++) {  // accepted steps: the host's own clock
+      t = ts[j];
+      st.steps_accepted++;
+      st.linear_solves++;
+      if (t >= tstart && n_rec < max_records) {
+        rec_t.push_back(t);
+        n_rec++;
+      }
+    }
+    st.bicgstab_iterations += w.iters;
+    if (w.n_rec != n_rec || (w.steps > 0 && !(w.t == t))) {
+      rc = fail(OHM_ERR_INVALID, "transient segment bookkeeping mismatch");
+      return false;
+    }
+    int most = w.max_iters;
+    if (w.stop == 1) {  // the step after the accepted ones needs more iterations
+      BicgOut out;
+      st.linear_solves++;
+      rc = captured_resume(b, tr->cs, tr->A1, tr->P1, so, seg_iters, out);
+      st.bicgstab_iterations += out.iters;
+      if (rc == OHM_OK) rc = status_to_error(out.status, out.iters, so.max_iters);
+      if (rc != OHM_OK) return false;
+      st.steps_accepted++;
+      t = ts[w.steps];
+      if (t >= tstart && (rc = record(t)) != OHM_OK) return false;
+      most = std::max<int>(most, (int)out.iters);
+    } else if (w.stop == 2) {
+      st.linear_solves++;
+      rc = status_to_error(w.status, so.max_iters, so.max_iters);
+      return false;
+    }
+    seg_iters = seg_budget(most);
+    return true;
+  };
   if (prm->method == OHM_TRAN_FIXED_BE) {
     // backward Euler at fixed h (the stepping rule of :84-93 without LTE control)
     while (t < tstop) {
       if (t + h > tstop) h = tstop - t;
       if (h < h_min * 0.5) break;
+      if (seg_on && st.steps_accepted > 0 && seg_iters > 0) {
+        if (run_segment()) continue;
+        if (rc != OHM_OK) break;
+      }
       const double t_next = t + h;
       OHM_TRY(tran_set_sigma(tr, 1.0 / h));
       cur_sigma = 1.0 / h;
       OHM_TRY(tran_rhs(tr, t_next, t, 1.0 / h, false));
+      const int64_t it0 = st.bicgstab_iterations;
       rc = solve();
       if (rc != OHM_OK) break;
+      if (seg_on) seg_iters = std::max(seg_iters, seg_budget((int)(st.bicgstab_iterations - it0)));
       st.steps_accepted++;
       t = t_next;
       if (t >= tstart) OHM_TRY(record(t));

@@ -35,8 +35,16 @@
 
 namespace ohm {
 
-__global__ void k_state_reset(SolveState *st) {
+// stop (captured solves): when *stop is set the solve is masked off: the
+// status word leaves RUNNING (ST_SKIPPED) and nothing else is written, so
+// every kernel of the solve exits at its head and the iteration state of a
+// solve paused before it stays as it was (k_unskip reopens it).
+__global__ void k_state_reset(SolveState *st, const int *stop) {
   if (threadIdx.x == 0) {
+    if (stop && *stop) {
+      st->status = ST_SKIPPED;
+      return;
+    }
     SolveState z;
     z.rho = z.alpha = z.omega = z.beta = z.rho_new = 1.0;
     z.abs_tol = z.bnorm2 = z.rr = z.ss = z.rv = z.ts = z.tt = z.rhrh = 0.0;
@@ -193,7 +201,9 @@ __global__ __launch_bounds__(kBlock) void k_init(int n, const T *__restrict__ b,
                                                  T *__restrict__ p, T *__restrict__ v, int shadow,
                                                  int warm, double tol, double floor_rel,
                                                  int tol_mode, int max_iters, int scaled,
-                                                 Forcing fc, SolveState *st, Parts pp) {
+                                                 Forcing fc, SolveState *st, Parts pp,
+                                                 int masked) {
+  if (masked && st->status != ST_RUNNING) return;  // a masked-off captured solve
   Acc<T> bb, rr, rhr, hh;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     T bi = b[i];
@@ -724,6 +734,12 @@ __global__ __launch_bounds__(kFusedRows) void k_bicg_fused(
   }
 }
 
+// Reopens a paused captured solve whose status a later masked-off solve on
+// the same state set to ST_SKIPPED.
+__global__ void k_unskip(SolveState *st) {
+  if (threadIdx.x == 0 && st->status == ST_SKIPPED) st->status = ST_RUNNING;
+}
+
 // Reduces a producer's np partials (K each) to partials[0..K) for a
 // consumer whose workgroups would otherwise each re-read all of them.
 template <int K>
@@ -1008,32 +1024,42 @@ static_assert(offsetof(SolveState, r0) - offsetof(SolveState, status) ==
                   sizeof(SolveState) - offsetof(SolveState, status) == sizeof(StatusWord),
               "StatusWord mirrors the tail of SolveState");
 
-// Runs BiCGSTAB on the workspace vectors; rhs in ws.r on entry? No: rhs is a
-// separate device pointer; x0 (warm) is read from xin.
+// Enqueues the start of a solve on the workspace vectors: state reset, the
+// warm-start product A x0 and k_init (rhs is a separate device vector; x0 is
+// read from w.v.x). With stop set (captured solves) the solve is masked off
+// while *stop is non-zero.
 template <typename T>
-static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const T *rhs,
-                            Workspace<T> &w, const BicgOpts &o, BicgOut &out) {
+static ohm_status bicg_start(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const T *rhs,
+                             Workspace<T> &w, const BicgOpts &o, Parts &pp,
+                             const int *stop = nullptr) {
   const int n = a.n;
   SolveState *st = w.st;
   BicgVecs<T> &v = w.v;
-  Parts pp;
   OHM_TRY(parts_for(b, a, p, n, pp, w.partials));
-  hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, b->stream, st);
-  if (o.warm) Ops<T>::spmv(b, a, v.x, v.t, nullptr, 0);
+  hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, b->stream, st, stop);
+  if (o.warm) Ops<T>::spmv(b, a, v.x, v.t, stop ? st : nullptr, stop ? 1 : 0);
   hipLaunchKernelGGL(k_init<T>, dim3(grid_for(n, 4)), dim3(kBlock), 0, b->stream, n, rhs, v.t, v.x,
                      v.r, v.rh, v.p, v.v, o.shadow == OHM_SHADOW_HASHED ? 1 : 0, o.warm ? 1 : 0,
                      o.tol, o.tol_floor, o.tol_mode,
                      (int)std::min<int64_t>(o.max_iters, 0x7fffffff), o.scaled_breakdown ? 1 : 0,
-                     o.forcing, st, pp);
+                     o.forcing, st, pp, stop ? 1 : 0);
   b->dispatches += 2;
+  return OHM_OK;
+}
+
+// Drives a started solve to its end: chunks of iterations (graph replays or
+// eager launches), each followed by a poll of the status word. `enqueued`
+// counts iterations already enqueued (a resumed captured solve); `poll`
+// says whether to read the status before the first chunk.
+template <typename T>
+static ohm_status bicg_drive(ohm_backend *b, const DevCsr &a, const PrecondDev &p,
+                             Workspace<T> &w, const BicgOpts &o, const Parts &pp, bool poll,
+                             int64_t enqueued, BicgOut &out) {
+  const int n = a.n;
+  SolveState *st = w.st;
+  BicgVecs<T> &v = w.v;
   StatusWord *h = reinterpret_cast<StatusWord *>(b->h_pinned);
-  int64_t enqueued = 0;
-  // The first chunk goes out behind k_init without a poll: the rare solve
-  // that k_init ends (b = 0, rho breakdown; under OHM_TOL_INITIAL a warm
-  // start only once ||r0|| is below the floor) turns every kernel of the
-  // chunk into a no-op at its head. A warm start against tol * ||b|| polls
-  // first, since k_init ends it whenever x already meets that tolerance.
-  for (bool poll = o.warm && o.tol_mode != OHM_TOL_INITIAL;; poll = true) {
+  for (;; poll = true) {
     int64_t done = 0;
     if (poll) {
       OHM_HIP(hipMemcpyAsync(h, &st->status, sizeof(StatusWord), hipMemcpyDeviceToHost,
@@ -1077,6 +1103,85 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   out.eta = h->eta;
   OHM_HIP(hipGetLastError());
   return OHM_OK;
+}
+
+// Runs BiCGSTAB on the workspace vectors (rhs a separate device vector, x0
+// read from w.v.x when warm).
+template <typename T>
+static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const T *rhs,
+                            Workspace<T> &w, const BicgOpts &o, BicgOut &out) {
+  Parts pp;
+  OHM_TRY(bicg_start<T>(b, a, p, rhs, w, o, pp));
+  // The first chunk goes out behind k_init without a poll: the rare solve
+  // that k_init ends (b = 0, rho breakdown; under OHM_TOL_INITIAL a warm
+  // start only once ||r0|| is below the floor) turns every kernel of the
+  // chunk into a no-op at its head. A warm start against tol * ||b|| polls
+  // first, since k_init ends it whenever x already meets that tolerance.
+  return bicg_drive<T>(b, a, p, w, o, pp, o.warm && o.tol_mode != OHM_TOL_INITIAL, 0, out);
+}
+
+// ----------------------------------------------------- captured solves --
+// FP64 solves enqueued inside a caller's graph capture: a workspace of their
+// own with x aliased to the caller's vector, so that the captured kernels'
+// pointers stay valid for every replay.
+struct CapturedSolve {
+  Workspace<double> w;
+  double *vecs = nullptr, *partials = nullptr;
+  SolveState *st = nullptr;
+  Parts pp{};
+  int n = 0;
+};
+
+ohm_status captured_create(ohm_backend *b, const DevCsr &a, const PrecondDev &p, double *x,
+                           CapturedSolve **out) {
+  (void)b;
+  CapturedSolve *c = new CapturedSolve();
+  const int n = a.n;
+  const size_t stride = ws_stride(n);
+  hipError_t e = hipMalloc(&c->vecs, sizeof(double) * stride * kWsVectors);
+  if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(SolveState));
+  if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(double) * parts_doubles(a, p, n));
+  if (e != hipSuccess) {
+    captured_free(c);
+    return hip_fail(e, "hipMalloc (captured solve workspace)", __FILE__, __LINE__);
+  }
+  ws_views(c->w, c->vecs, stride);
+  c->w.v.x = x;
+  c->w.st = c->st;
+  c->w.partials = c->partials;
+  c->w.n = n;
+  c->n = n;
+  *out = c;
+  return OHM_OK;
+}
+
+void captured_free(CapturedSolve *c) {
+  if (!c) return;
+  if (c->vecs) (void)hipFree(c->vecs);
+  if (c->st) (void)hipFree(c->st);
+  if (c->partials) (void)hipFree(c->partials);
+  delete c;
+}
+
+const SolveState *captured_state(const CapturedSolve *c) { return c->st; }
+
+ohm_status captured_enqueue(ohm_backend *b, CapturedSolve *c, const DevCsr &a,
+                            const PrecondDev &p, const double *rhs, const BicgOpts &o, int iters,
+                            const int *stop) {
+  if (a.n != c->n) return fail(OHM_ERR_INVALID, "captured solve: dimension mismatch");
+  OHM_TRY(bicg_start<double>(b, a, p, rhs, c->w, o, c->pp, stop));
+  for (int k = 0; k < iters; k++) enqueue_any<double>(b, a, p, c->w.v, c->st, c->n, c->pp);
+  enqueue_check(b, c->st, c->pp);
+  return OHM_OK;
+}
+
+ohm_status captured_resume(ohm_backend *b, CapturedSolve *c, const DevCsr &a,
+                           const PrecondDev &p, const BicgOpts &o, int64_t enqueued,
+                           BicgOut &out) {
+  hipLaunchKernelGGL(k_unskip, dim3(1), dim3(64), 0, b->stream, c->st);
+  OHM_HIP(hipGetLastError());
+  b->dispatches++;
+  return bicg_drive<double>(b, a, p, c->w, o, c->pp, true, enqueued, out);
 }
 
 ohm_status status_to_error(int status, int64_t iters, int64_t max_iters) {

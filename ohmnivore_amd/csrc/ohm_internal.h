@@ -52,6 +52,7 @@ enum SolveStatus : int {
   ST_NAN = 5,
   ST_BRK_OMEGA = 6,
   ST_MAXITER = 7,
+  ST_SKIPPED = 8,  // a captured solve masked off before it started
 };
 
 struct SolveState {

@@ -97,6 +97,28 @@ ohm_status bicg_solve_f64_noalloc(ohm_backend *b, const DevCsr &a, const Precond
 // Looks up this device's address of the scratch array (once per device,
 // with the backend's device current) and keeps it in b->scratch.
 ohm_status static_scratch_init(ohm_backend *b);
+// FP64 solves for graph capture (timestep and Newton segments): a
+// workspace of their own, x aliased to the caller's vector x.
+struct CapturedSolve;
+ohm_status captured_create(ohm_backend *b, const DevCsr &a, const PrecondDev &p, double *x,
+                           CapturedSolve **out);
+void captured_free(CapturedSolve *c);
+const SolveState *captured_state(const CapturedSolve *c);
+// Enqueues (no poll, no host decision) a solve of a x = rhs warm-started
+// from x: state reset, A x0, k_init, `iters` iterations and a closing check.
+// While *stop is non-zero (stop may be null) the whole solve is a no-op and
+// its state reads ST_SKIPPED. A solve still RUNNING after `iters`
+// iterations is finished by captured_resume.
+ohm_status captured_enqueue(ohm_backend *b, CapturedSolve *c, const DevCsr &a,
+                            const PrecondDev &p, const double *rhs, const BicgOpts &o, int iters,
+                            const int *stop);
+// Drives an enqueued solve that was still RUNNING after its captured
+// iterations to its end (polls between chunks, as a plain solve does), also
+// after later masked-off solves on the same state; `enqueued` iterations
+// are already done.
+ohm_status captured_resume(ohm_backend *b, CapturedSolve *c, const DevCsr &a,
+                           const PrecondDev &p, const BicgOpts &o, int64_t enqueued,
+                           BicgOut &out);
 ohm_status status_to_error(int status, int64_t iters, int64_t max_iters);
 
 }  // namespace ohm

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
+from ohmnivore_amd import _native as N
 from ohmnivore_amd import circuit as ir
 from ohmnivore_amd import solver
 from ohmnivore_amd import transient as T
@@ -109,6 +110,38 @@ def test_rc_grid_be_vs_oracle(be, precision):
     assert ok, err
     ok, err = close(res.x_final[0], xf)
     assert ok, err
+
+
+@pytest.mark.parametrize("precond", [N.PREC_ISAI, N.PREC_JACOBI], ids=["isai", "jacobi"])
+def test_captured_segments_match_eager(be, precond, monkeypatch):
+    """Graph-captured BE timesteps (segments of up to 16 steps replayed as
+    one graph, one segment-word read per segment) give bitwise the eager
+    loop's samples, final state and iteration counts; so do 3-step segments
+    whose 8 captured iterations per step are too few, so that every step
+    pauses and the host finishes its solve. Recording starts at tstart and
+    stops at max_records inside the segments."""
+    ckt = ir.rc_grid(24, 24, 12, seed=3)
+    mna = solver.compile(ckt)
+    src, _ = T.source_table(ckt, mna)
+    x0 = oracle.sparse_lu_solve(mna.g, mna.b_dc)
+    prm = solver.bicgstab_params(tolerance=1e-12, max_iterations=20000)
+    probes = np.array([0, 5, 100, mna.size - 1])
+    out = {}
+    for name, steps, iters in (("eager", "0", None), ("segments", "16", None),
+                               ("paused", "3", "8")):
+        monkeypatch.setenv("OHM_TRAN_SEGMENT_STEPS", steps)
+        if iters:
+            monkeypatch.setenv("OHM_TRAN_SEGMENT_ITERS", iters)
+        else:
+            monkeypatch.delenv("OHM_TRAN_SEGMENT_ITERS", raising=False)
+        tr = T.Transient(be, mna, src)
+        r = tr.run(1e-12, 120e-12, tstart=20e-12, method="be", x0=x0, params=prm,
+                   precond=precond, probes=probes, max_records=90)
+        out[name] = (r.times.tobytes(), r.waves.tobytes(), r.x_final.tobytes(),
+                     r.steps_accepted, r.linear_solves, r.bicgstab_iterations)
+    assert out["segments"] == out["eager"]
+    assert out["paused"] == out["eager"]
+    assert out["eager"][3] == 120 and len(np.frombuffer(out["eager"][0])) == 90
 
 
 def test_monte_carlo_instances_vs_oracle(be):
