@@ -16,7 +16,10 @@ from HIP events on the backend's stream.
 The other workloads are measured once each (they are not the headline):
   diode_mesh_256       nonlinear DC, 65,540 unknowns: GPU Newton with
                        ISAI(1)-BiCGSTAB in FP64 and FP32+FP64 refinement,
-                       against the committed FP64 LU fixture
+                       against the committed FP64 LU fixture; the SpMV of
+                       its matrix alone and of 64 block-diagonal copies
+                       (335 MB, past the Infinity Cache) beside the
+                       stream-read probe
   resistor_ladder_10k  linear DC, 10,002 unknowns, Jacobi-BiCGSTAB to 1e-10
   rc_grid_512          transient, 262,148 unknowns, 1,000 backward-Euler
                        steps of 1 ps, warm-started, against the committed
@@ -301,6 +304,37 @@ def mesh_newton(args, be, N, ir, solver):
                    "achieved_gbs": b / (msv.value * 1e-3) / 1e9,
                    "frac": b / (msv.value * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "traffic_bytes_rocprof": SPMV_TRAFFIC_BYTES}
+    out["spmv_large_working_set"] = spmv_large(be, N, m.g)
+    return out
+
+
+def spmv_large(be, N, g, copies=64):
+    """The mesh matrix replicated block-diagonally 64 times (a 335 MB working
+    set, past the 256 MB Infinity Cache): FP64 and FP32 SpMV with the
+    library's default row-block shape, HIP events over 100 back-to-back
+    launches, against the 8 TB/s peak and against the stream-read probe over
+    a buffer of the same size (32-byte loads)."""
+    val, col, rp, n = replicated(g, copies)
+    a = be.upload_matrix(val, col, rp, n)
+    bx, by = be.new_buffer(n), be.new_buffer(n)
+    be.upload_vec(np.ones(n), bx)
+    out = {"copies": copies, "unknowns": n, "nnz": int(val.shape[0])}
+    for name, prec, v in (("fp64", 0, 8), ("fp32", N.FP32_IR, 4)):
+        ms = ctypes.c_double()
+        N.check(N.lib().ohm_spmv_timed(be.h, a.h, bx.h, by.h, 100, prec, ctypes.byref(ms)))
+        b = spmv_bytes(n, int(val.shape[0]), v)
+        out[name] = {"bytes_per_launch": b, "avg_launch_us": ms.value * 1e3,
+                     "achieved_gbs": b / (ms.value * 1e-3) / 1e9,
+                     "frac": b / (ms.value * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    ms = ctypes.c_double()
+    nbytes = out["fp64"]["bytes_per_launch"]
+    N.check(N.lib().ohm_stream_timed(be.h, ctypes.c_int64(nbytes), 0, 50, ctypes.byref(ms)))
+    rd = nbytes / (ms.value * 1e-3) / 1e9
+    out["stream_read_gbs"] = rd
+    out["stream_read_frac"] = rd / HBM_PEAK_GBS
+    log(f"SpMV 335 MB: fp64 {out['fp64']['achieved_gbs']:.0f} GB/s "
+        f"({out['fp64']['frac']:.3f}), fp32 {out['fp32']['achieved_gbs']:.0f} GB/s "
+        f"({out['fp32']['frac']:.3f}); stream read {rd:.0f} GB/s ({rd / HBM_PEAK_GBS:.3f})")
     return out
 
 
@@ -541,6 +575,22 @@ def roofline_only(args, be, N, ir, solver) -> int:
     return 0
 
 
+def spmv_large_only(args, be, N, ir, solver) -> int:
+    """The 335 MB SpMV alone (FP64 and FP32 on the 64-copy mesh matrix, the
+    stream-read probe over a buffer of the same size) and the copy kernel
+    calibration, for rocprofv3 kernel-trace and counter runs."""
+    m = solver.compile(ir.diode_mesh(256, 256, seed=2))
+    r = spmv_large(be, N, m.g)
+    nc = 8 * 1024 * 1024  # 64 MiB of doubles
+    cx, cy = be.new_buffer(nc), be.new_buffer(nc)
+    for _ in range(20):
+        be.copy(cx, cy)
+    be.sync()
+    r.update({"copy_kernel": "k_copy", "copy_bytes_per_launch": 2 * 8 * nc})
+    print(json.dumps(r))
+    return 0
+
+
 def chain_netlist(stages, analysis):
     """The reference's own benchmark circuit family (bench/generate_chain.py,
     run by bench/run_benchmark.sh for 10-200 stages): a CMOS inverter chain
@@ -601,6 +651,9 @@ def main() -> int:
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline kernels: two AC sweeps, 200 SpMVs on the mesh matrix "
                          "and, for counter calibration, 20 copies of a 64 MiB vector")
+    ap.add_argument("--spmv-large-only", action="store_true",
+                    help="only the 335 MB SpMV (FP64, FP32), its stream-read probe and the copy "
+                         "calibration, for rocprofv3 runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the backend has no multi-GPU path (asked for "
@@ -614,6 +667,8 @@ def main() -> int:
     be = solver.HipBackend()
     if args.roofline_only:
         return roofline_only(args, be, N, ir, solver)
+    if args.spmv_large_only:
+        return spmv_large_only(args, be, N, ir, solver)
     if args.profile:
         be.set_graphs(False)
         args.steps = min(args.steps, 2)

@@ -75,19 +75,22 @@ ohm_status ohm_spmv_transpose(ohm_backend *b, const ohm_matrix *a, const ohm_buf
 ohm_status ohm_spmv_timed(ohm_backend *b, const ohm_matrix *a, const ohm_buffer *x, ohm_buffer *y,
                           int reps, int precision, double *avg_ms);
 /* HBM stream probe (no reference counterpart): `reps` launches streaming a
- * `bytes`-sized device buffer (mode 0: read only, 16-byte loads; mode 1:
- * copy into a second buffer of the same size), average ms per launch. With a
- * buffer beyond the 256 MB Infinity Cache this is the practical HBM ceiling
- * the SpMV bandwidth is read against. */
+ * `bytes`-sized device buffer (mode 0: read only, 32-byte loads, four in
+ * flight per thread; mode 1: copy into a second buffer of the same size),
+ * average ms per launch. With a buffer beyond the 256 MB Infinity Cache this
+ * is the practical HBM ceiling the SpMV bandwidth is read against. */
 ohm_status ohm_stream_timed(ohm_backend *b, int64_t bytes, int mode, int reps, double *avg_ms);
 /* SpMV autotuner (no reference counterpart): times the matrix's SpMV under
  * each candidate row-block shape (rows per workgroup 32-256 x staged
- * products per workgroup 512-2048) and keeps the fastest; the chosen shape
- * is returned. Results do not depend on the shape. The preconditioner
- * matrices the library builds itself (ISAI) are tuned automatically unless
- * ohm_backend_set_autotune(b, 0); system matrices are not, because their
- * block table also sets the grid of the fused SpMV + reduction kernels,
- * whose cost grows with the number of workgroups. */
+ * entries per workgroup 256-2048, i.e. 1-8 entries per thread) and keeps
+ * the fastest; the chosen shape is returned. Every matrix starts with a
+ * shape fitted to its rows (256-row blocks, entries per thread covering
+ * nine in ten of them). The shape changes no result except for rows longer
+ * than a block's staged entries, which are summed in four interleaved
+ * chains. The preconditioner matrices the library builds itself (ISAI) are
+ * tuned automatically unless ohm_backend_set_autotune(b, 0); system matrices
+ * are not, because their block table also sets the grid of the fused SpMV +
+ * reduction kernels, whose cost grows with the number of workgroups. */
 ohm_status ohm_matrix_autotune(ohm_backend *b, ohm_matrix *m, int32_t *rows_per_block,
                                int32_t *nnz_per_block);
 ohm_status ohm_backend_set_autotune(ohm_backend *b, int enable);

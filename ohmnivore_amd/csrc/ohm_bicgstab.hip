@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int n, const T *__restrict_
 // q = p_prev - omega v_prev was written by the previous K5 (zeroed by k_init,
 // so the first p is r). The two roundings are p_next's, split across the
 // two kernels; p is written for the rows of this block.
-template <typename T>
+template <typename T, int PER>
 __global__ __launch_bounds__(kBlock) void k_p_spmv(const uint32_t *__restrict__ rowptr,
                                                    const uint32_t *__restrict__ col,
                                                    const T *__restrict__ val,
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_p_spmv(const uint32_t *__restrict__ 
                                                    T *__restrict__ p, T *__restrict__ y,
                                                    SolveState *st, Parts pp) {
   T beta = T(0);
-  spmv_block_h<T, 2>(
+  spmv_block_h<T, 2, PER>(
       rowptr, col, val, blocks,
       [&](uint32_t j) {
         Opnd<T, 2> o;
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_p_spmv(const uint32_t *__restrict__ 
 }
 
 // K2: v = A p_hat; partials r_hat.v, v.v.
-template <typename T>
+template <typename T, int PER>
 __global__ __launch_bounds__(kBlock) void k_spmv_alpha(const uint32_t *__restrict__ rowptr,
                                                        const uint32_t *__restrict__ col,
                                                        const T *__restrict__ val,
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_alpha(const uint32_t *__restric
                                                        const T *__restrict__ rh, SolveState *st,
                                                        Parts pp) {
   Acc<T> acc, vv;
-  const bool ran = spmv_block_h<T, 1>(
+  const bool ran = spmv_block_h<T, 1, PER>(
       rowptr, col, val, blocks,
       [&](uint32_t j) {
         Opnd<T, 1> o;
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_s_update(int n, const T *__restrict_
 
 // K3 (ISAI): the s update fused into the first preconditioner SpMV
 // (y = M1 s with s_j = r_j - alpha v_j formed on the fly); partial s.s.
-template <typename T>
+template <typename T, int PER>
 __global__ __launch_bounds__(kBlock) void k_s_spmv(const uint32_t *__restrict__ rowptr,
                                                    const uint32_t *__restrict__ col,
                                                    const T *__restrict__ val,
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_s_spmv(const uint32_t *__restrict__ 
                                                    T *__restrict__ y, SolveState *st, Parts pp) {
   T alpha = T(0);
   Acc<T> acc;
-  const bool ran = spmv_block_h<T, 2>(
+  const bool ran = spmv_block_h<T, 2, PER>(
       rowptr, col, val, blocks,
       [&](uint32_t j) {
         Opnd<T, 2> o;
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_s_spmv(const uint32_t *__restrict__ 
 }
 
 // K4: the ||s|| early-exit test, then t = A s_hat; partials t.s, t.t.
-template <typename T>
+template <typename T, int PER>
 __global__ __launch_bounds__(kBlock) void k_spmv_omega(const uint32_t *__restrict__ rowptr,
                                                        const uint32_t *__restrict__ col,
                                                        const T *__restrict__ val,
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_omega(const uint32_t *__restric
                                                        const T *__restrict__ s, SolveState *st,
                                                        Parts pp) {
   Acc<T> ts, tt;
-  const bool ran = spmv_block_h<T, 1>(
+  const bool ran = spmv_block_h<T, 1, PER>(
       rowptr, col, val, blocks,
       [&](uint32_t j) {
         Opnd<T, 1> o;
@@ -896,6 +896,15 @@ static void fold_launch(ohm_backend *b, double *partials, int np) {
   b->dispatches++;
 }
 
+template <typename T>
+static void launch_spmv_alpha(ohm_backend *b, const DevCsr &a, const T *av, BicgVecs<T> &v,
+                              SolveState *st, const Parts &pp) {
+  with_per(a.per, [&](auto P) {
+    hipLaunchKernelGGL((k_spmv_alpha<T, decltype(P)::value>), dim3(a.n_blocks), dim3(kBlock), 0,
+                       b->stream, a.rowptr, a.col, av, a.blocks, v.ph, v.v, v.rh, st, pp);
+  });
+}
+
 // One iteration: K1 .. K5 (ISAI: the p and s updates ride on the first
 // preconditioner SpMV; a two-factor apply adds its second SpMV after each).
 template <typename T, int PK>
@@ -910,29 +919,35 @@ static void enqueue_iteration(ohm_backend *b, const DevCsr &a, const PrecondDev 
     // q (= p - omega v, written by K5) lives in v.p, which k_init zeroes;
     // p itself in v.p2
     T *y1 = two ? v.tmp : v.ph;
-    hipLaunchKernelGGL(k_p_spmv<T>, dim3(m1.n_blocks), blk, 0, b->stream, m1.rowptr, m1.col,
-                       Ops<T>::vals(m1), m1.blocks, v.r, v.p, v.p2, y1, st, pp);
+    with_per(m1.per, [&](auto P) {
+      hipLaunchKernelGGL((k_p_spmv<T, decltype(P)::value>), dim3(m1.n_blocks), blk, 0, b->stream,
+                         m1.rowptr, m1.col, Ops<T>::vals(m1), m1.blocks, v.r, v.p, v.p2, y1, st,
+                         pp);
+    });
     if (two) Ops<T>::spmv(b, p.mu, v.tmp, v.ph, st, 1);
-    hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks,
-                       v.ph, v.v, v.rh, st, pp);
+    launch_spmv_alpha<T>(b, a, av, v, st, pp);
     if (pp.f_rv) fold_launch<2>(b, pp.rv, a.n_blocks);
     T *y3 = two ? v.tmp : v.sh;
-    hipLaunchKernelGGL(k_s_spmv<T>, dim3(m1.n_blocks), blk, 0, b->stream, m1.rowptr, m1.col,
-                       Ops<T>::vals(m1), m1.blocks, v.r, v.v, v.s, y3, st, pp);
+    with_per(m1.per, [&](auto P) {
+      hipLaunchKernelGGL((k_s_spmv<T, decltype(P)::value>), dim3(m1.n_blocks), blk, 0, b->stream,
+                         m1.rowptr, m1.col, Ops<T>::vals(m1), m1.blocks, v.r, v.v, v.s, y3, st,
+                         pp);
+    });
     if (pp.f_ss) fold_launch<1>(b, pp.ss, m1.n_blocks);
     if (two) Ops<T>::spmv(b, p.mu, v.tmp, v.sh, st, 1);
   } else {
     hipLaunchKernelGGL((k_p_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.p, v.v, v.ph, inv,
                        st, pp);
-    hipLaunchKernelGGL(k_spmv_alpha<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks,
-                       v.ph, v.v, v.rh, st, pp);
+    launch_spmv_alpha<T>(b, a, av, v, st, pp);
     if (pp.f_rv) fold_launch<2>(b, pp.rv, a.n_blocks);
     hipLaunchKernelGGL((k_s_update<T, PK>), gv, blk, 0, b->stream, n, v.r, v.v, v.s, v.sh, inv,
                        st, pp);
     if (pp.f_ss) fold_launch<1>(b, pp.ss, (int)gv.x);
   }
-  hipLaunchKernelGGL(k_spmv_omega<T>, ga, blk, 0, b->stream, a.rowptr, a.col, av, a.blocks, v.sh,
-                     v.t, v.s, st, pp);
+  with_per(a.per, [&](auto P) {
+    hipLaunchKernelGGL((k_spmv_omega<T, decltype(P)::value>), ga, blk, 0, b->stream, a.rowptr,
+                       a.col, av, a.blocks, v.sh, v.t, v.s, st, pp);
+  });
   if (pp.f_ts) fold_launch<2>(b, pp.ts, a.n_blocks);
   hipLaunchKernelGGL(k_xr_update<T>, gv, blk, 0, b->stream, n, v.x, v.r, v.rh, v.ph, v.sh, v.s,
                      v.t, v.p2, v.v, PK == PK_ISAI ? v.p : nullptr, st, pp);
@@ -1310,6 +1325,7 @@ ohm_status bicg_solve_transpose(ohm_backend *b, DevCsr &a, PrecondDev &p, const 
 // ------------------------------------------------- iterative refinement --
 // r = b - A x (FP64) with ||r||^2 and ||b||^2 into d_scalars[0..1]; the
 // scaled residual r / ||r|| is written in FP32 by the next kernel.
+template <int PER>
 __global__ __launch_bounds__(kBlock) void k_residual(const uint32_t *__restrict__ rowptr,
                                                      const uint32_t *__restrict__ col,
                                                      const double *__restrict__ val,
@@ -1319,7 +1335,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const uint32_t *__restrict_
                                                      double *__restrict__ r, double *partials,
                                                      unsigned *counter, double *out) {
   Acc<double> rr, bb;
-  spmv_block<double>(rowptr, col, val, blocks, x, [&](uint32_t i, double y) {
+  spmv_block<double, PER>(rowptr, col, val, blocks, x, [&](uint32_t i, double y) {
     double ri = rhs[i] - y;
     r[i] = ri;
     rr.add_prod(ri, ri);
@@ -1376,8 +1392,11 @@ ohm_status true_residual(ohm_backend *b, const DevCsr &a, const double *rhs, con
 // doubles) in the caller's buffers (synchronous).
 static ohm_status residual_norm(ohm_backend *b, const DevCsr &a, const double *rhs,
                                 const double *x, double *r, double *partials, double &rel) {
-  hipLaunchKernelGGL(k_residual, dim3(a.n_blocks), dim3(kBlock), 0, b->stream, a.rowptr, a.col,
-                     a.val, a.blocks, x, rhs, r, partials, b->d_counter, b->d_scalars);
+  with_per(a.per, [&](auto P) {
+    hipLaunchKernelGGL((k_residual<decltype(P)::value>), dim3(a.n_blocks), dim3(kBlock), 0,
+                       b->stream, a.rowptr, a.col, a.val, a.blocks, x, rhs, r, partials,
+                       b->d_counter, b->d_scalars);
+  });
   OHM_HIP(hipGetLastError());
   b->dispatches++;
   OHM_HIP(hipMemcpyAsync(b->h_pinned, b->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost,

@@ -10,7 +10,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ohm_solver.h"
@@ -40,6 +42,27 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kSpmvNnzPerBlock = 2048;   // LDS-staged products per row block
 constexpr int kSpmvRowsPerBlock = kBlock;
 constexpr int kMaxReduceBlocks = 2048;   // cap on grid size of reducing vector kernels
+
+// LDS-staged SpMV: each thread loads `per` matrix entries of its row block
+// (the block holds at most per * kBlock entries), so per is chosen to fit
+// the blocks of a matrix; loads past a block's entries cost as much as the
+// entries themselves. Supported values and the launch dispatcher:
+inline int spmv_per_for(int nnz_cap) {
+  const int need = (std::max(nnz_cap, 1) + kBlock - 1) / kBlock;
+  return need <= 6 ? need : 8;
+}
+template <typename F>
+inline void with_per(int per, F &&f) {
+  switch (per) {
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 3: f(std::integral_constant<int, 3>()); break;
+    case 4: f(std::integral_constant<int, 4>()); break;
+    case 5: f(std::integral_constant<int, 5>()); break;
+    case 6: f(std::integral_constant<int, 6>()); break;
+    default: f(std::integral_constant<int, 8>()); break;
+  }
+}
 constexpr int64_t kDeferMaxReads = int64_t(1) << 21;  // consumer x producer workgroups re-summed
 
 // ----------------------------------------------------------- solve state --
@@ -90,6 +113,7 @@ struct DevCsr {
   int t_n_blocks = 0;
   double *t_val = nullptr;         // values of A^T in its CSR order (gathered via t_src)
   int tuned_rows = kSpmvRowsPerBlock, tuned_nnz = kSpmvNnzPerBlock;  // row-block shape
+  int per = kSpmvNnzPerBlock / kBlock;  // entries per thread: tuned_nnz / kBlock
   std::vector<uint32_t> h_rowptr, h_col; // host copy of the pattern
 };
 
@@ -151,8 +175,11 @@ namespace ohm {
 // longer row gets a workgroup of its own.
 void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blocks,
                        int rows_cap = kSpmvRowsPerBlock, int nnz_cap = kSpmvNnzPerBlock);
+int spmv_default_per(const uint32_t *rowptr, int n);
 // Pick the fastest (rows_cap, nnz_cap) for m by timing each candidate.
-ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out);
+// fused: m is applied by the fused BiCGSTAB kernels (keep 256-row blocks)
+ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out,
+                        bool fused = false);
 ohm_status stream_timed(ohm_backend *b, int64_t bytes, int mode, int reps, double &avg_ms);
 ohm_status csr_upload(ohm_backend *b, DevCsr &m, const double *val, const uint32_t *col,
                       const uint32_t *rowptr, int n, int64_t nnz);

@@ -617,10 +617,10 @@ ohm_status precond_build(ohm_backend *b, const DevCsr &a, int kind, int level, P
   OHM_TRY(precond_refresh(b, a, p));
   if (b->autotune && p.kind == PK_ISAI) {
     if (p.mm.rowptr) {
-      OHM_TRY(csr_autotune(b, p.mm, nullptr, nullptr));
+      OHM_TRY(csr_autotune(b, p.mm, nullptr, nullptr, true));
     } else {
-      OHM_TRY(csr_autotune(b, p.ml, nullptr, nullptr));
-      OHM_TRY(csr_autotune(b, p.mu, nullptr, nullptr));
+      OHM_TRY(csr_autotune(b, p.ml, nullptr, nullptr, true));
+      OHM_TRY(csr_autotune(b, p.mu, nullptr, nullptr, true));
     }
   }
   return OHM_OK;

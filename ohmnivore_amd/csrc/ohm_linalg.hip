@@ -10,6 +10,7 @@
 // a workgroup of its own that strides over it and reduces across lanes.
 // Every launch is a bandwidth-bound pass: bytes = nnz*(V+4) + 4(n+1) + 2nV.
 #include <algorithm>
+#include <cstdlib>
 
 #include "ohm_spmv.h"
 
@@ -38,6 +39,26 @@ void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blo
   }
 }
 
+// Entries per thread of a matrix's default row blocks (of at most 256
+// rows): the mean entries per row over rows of at most 256 entries, rounded
+// down, so that blocks fill their per * 256 staged entries a little before
+// 256 rows instead of leaving slots empty. Fitting per to the blocks matters
+// on large matrices: loads past a block's entries take issue slots and
+// cache bandwidth like real ones (a 335 MB five-point matrix: 4.95 TB/s with
+// 8 per thread, 5.83 with 4, 6.04 with 5).
+int spmv_default_per(const uint32_t *rowptr, int n) {
+  uint64_t entries = 0, rows = 0;
+  for (int r = 0; r < n; r++) {
+    const uint32_t c = rowptr[r + 1] - rowptr[r];
+    if (c <= (uint32_t)kBlock) {
+      entries += c;
+      rows++;
+    }
+  }
+  const int mean = rows ? (int)(entries / rows) : 1;
+  return spmv_per_for(std::max(mean, 1) * kBlock);
+}
+
 // Device block table: (first row, first nonzero) of every block boundary, so
 // a workgroup reads its row and nonzero ranges with one 16-byte scalar load.
 std::vector<uint32_t> spmv_block_pairs(const uint32_t *rowptr, const std::vector<uint32_t> &bounds) {
@@ -49,7 +70,7 @@ std::vector<uint32_t> spmv_block_pairs(const uint32_t *rowptr, const std::vector
   return t;
 }
 
-template <typename T>
+template <typename T, int PER>
 __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t *__restrict__ rowptr,
                                                  const uint32_t *__restrict__ col,
                                                  const T *__restrict__ val,
@@ -57,22 +78,26 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t *__restrict__ ro
                                                  const T *__restrict__ x, T *__restrict__ y,
                                                  const SolveState *st, int skip) {
   if (solve_stopped(st, skip)) return;
-  spmv_block<T>(rowptr, col, val, blocks, x, [&](uint32_t r, T v) { y[r] = v; });
+  spmv_block<T, PER>(rowptr, col, val, blocks, x, [&](uint32_t r, T v) { y[r] = v; });
 }
 
 void launch_spmv(ohm_backend *b, const DevCsr &a, const double *x, double *y,
                  const SolveState *state, int skip) {
   if (a.n == 0) return;
-  hipLaunchKernelGGL(k_spmv<double>, dim3(a.n_blocks), dim3(kBlock), 0, b->stream, a.rowptr,
-                     a.col, a.val, a.blocks, x, y, state, skip);
+  with_per(a.per, [&](auto P) {
+    hipLaunchKernelGGL((k_spmv<double, decltype(P)::value>), dim3(a.n_blocks), dim3(kBlock), 0,
+                       b->stream, a.rowptr, a.col, a.val, a.blocks, x, y, state, skip);
+  });
   b->dispatches++;
 }
 
 void launch_spmv_f32(ohm_backend *b, const DevCsr &a, const float *x, float *y,
                      const SolveState *state, int skip) {
   if (a.n == 0) return;
-  hipLaunchKernelGGL(k_spmv<float>, dim3(a.n_blocks), dim3(kBlock), 0, b->stream, a.rowptr,
-                     a.col, a.val32, a.blocks, x, y, state, skip);
+  with_per(a.per, [&](auto P) {
+    hipLaunchKernelGGL((k_spmv<float, decltype(P)::value>), dim3(a.n_blocks), dim3(kBlock), 0,
+                       b->stream, a.rowptr, a.col, a.val32, a.blocks, x, y, state, skip);
+  });
   b->dispatches++;
 }
 
@@ -243,7 +268,9 @@ ohm_status csr_upload(ohm_backend *b, DevCsr &m, const double *val, const uint32
   m.h_rowptr.assign(rowptr, rowptr + n + 1);
   m.h_col.assign(col, col + nnz);
   std::vector<uint32_t> bounds;
-  build_spmv_blocks(rowptr, n, bounds);
+  m.per = spmv_default_per(rowptr, n);
+  m.tuned_nnz = m.per * kBlock;
+  build_spmv_blocks(rowptr, n, bounds, kSpmvRowsPerBlock, m.tuned_nnz);
   m.n_blocks = (int)bounds.size() - 1;
   const std::vector<uint32_t> blocks = spmv_block_pairs(rowptr, bounds);
   std::vector<int32_t> diag(n > 0 ? n : 1, -1);
@@ -333,7 +360,7 @@ ohm_status csr_transpose_view(ohm_backend *b, DevCsr &m, DevCsr &view) {
 // products per workgroup) is a valid block table for the same kernel, so
 // the choice only changes speed. Each candidate is timed with HIP events over
 // back-to-back launches on the backend stream and the fastest kept.
-ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) {
+ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out, bool fused) {
   if (m.n < 4096 || m.h_rowptr.empty()) {
     if (rows_out) *rows_out = m.tuned_rows;
     if (nnz_out) *nnz_out = m.tuned_nnz;
@@ -342,7 +369,7 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
   // a pattern tuned before keeps its shape (no timing)
   uint64_t h = 1469598103934665603ull;  // FNV-1a over the row pointers
   for (uint32_t v : m.h_rowptr) h = (h ^ v) * 1099511628211ull;
-  const std::vector<uint64_t> key = {(uint64_t)m.n, (uint64_t)m.nnz, h};
+  const std::vector<uint64_t> key = {(uint64_t)m.n, (uint64_t)m.nnz, h, (uint64_t)fused};
   for (auto &e : b->tune_cache)
     if (e.first == key) {
       std::vector<uint32_t> bounds;
@@ -356,12 +383,21 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
       m.n_blocks = (int)bounds.size() - 1;
       m.tuned_rows = e.second.first;
       m.tuned_nnz = e.second.second;
+      m.per = spmv_per_for(m.tuned_nnz);
       if (rows_out) *rows_out = m.tuned_rows;
       if (nnz_out) *nnz_out = m.tuned_nnz;
       return OHM_OK;
     }
-  static const int kRows[] = {32, 64, 128, 256};
-  static const int kNnz[] = {512, 1024, 2048};
+  // Matrices applied by the fused BiCGSTAB kernels keep 256-row blocks
+  // (their block count sets how many partial sums each consumer's head
+  // re-sums); others also try smaller blocks. OHM_TUNE_ALL_ROWS=1 tries
+  // every row count for all.
+  static const int kRowsAll[] = {32, 64, 128, 256}, kRowsFused[] = {256};
+  const char *all_env = getenv("OHM_TUNE_ALL_ROWS");
+  const bool all_rows = !fused || (all_env && atoi(all_env) != 0);
+  const int *kRows = all_rows ? kRowsAll : kRowsFused;
+  const int n_rows = all_rows ? 4 : 1;
+  static const int kPer[] = {1, 2, 3, 4, 5, 6, 8};
   double *x = nullptr, *y = nullptr;
   OHM_HIP(hipMalloc(&x, sizeof(double) * m.n));
   OHM_HIP(hipMalloc(&y, sizeof(double) * m.n));
@@ -373,8 +409,10 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
   uint32_t *d_blk = nullptr;
   size_t d_cap = 0;
   ohm_status rc = OHM_OK;
-  for (int rows : kRows)
-    for (int nz : kNnz) {
+  for (int ri = 0; ri < n_rows; ri++)
+    for (int per : kPer) {
+      const int rows = kRows[ri];
+      const int nz = per * kBlock;
       build_spmv_blocks(m.h_rowptr.data(), m.n, blocks, rows, nz);
       blocks = spmv_block_pairs(m.h_rowptr.data(), blocks);
       if (blocks.size() > d_cap) {
@@ -391,6 +429,8 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
         DevCsr v = m;  // shallow view with the candidate table
         v.blocks = d_blk;
         v.n_blocks = (int)blocks.size() / 2 - 1;
+        v.tuned_nnz = nz;
+        v.per = per;
         v.h_rowptr.clear();
         v.h_col.clear();
         for (int w = 0; w < 3; w++) launch_spmv(b, v, x, y, nullptr, 0);
@@ -422,6 +462,7 @@ ohm_status csr_autotune(ohm_backend *b, DevCsr &m, int *rows_out, int *nnz_out) 
     m.n_blocks = (int)best_blocks.size() / 2 - 1;
     m.tuned_rows = best_r;
     m.tuned_nnz = best_z;
+    m.per = spmv_per_for(best_z);
     graph_cache_clear(b);
     if (b->tune_cache.size() >= 64) b->tune_cache.erase(b->tune_cache.begin());
     b->tune_cache.push_back({key, {best_r, best_z}});
@@ -446,22 +487,26 @@ void csr_free(DevCsr &m) {
 }
 
 // ---------------------------------------------------- HBM stream probes --
-// Streaming read (16-byte loads, four in flight per thread) and copy over a
-// buffer larger than the Infinity Cache: the practical HBM ceilings that the
-// SpMV's bandwidth is read against.
-__global__ __launch_bounds__(kBlock) void k_stream_read(int64_t n2, const double2 *__restrict__ x,
+// Streaming read (32-byte loads, four in flight per thread over a grid of
+// 8,192 workgroups) and copy over a buffer larger than the Infinity Cache:
+// the practical HBM ceilings that the SpMV's bandwidth is read against.
+constexpr int kStreamGrid = 8192;
+
+__global__ __launch_bounds__(kBlock) void k_stream_read(int64_t n4, const double4 *__restrict__ x,
                                                        double *__restrict__ sink) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   double s = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n2; i += 4 * stride) {
-    double2 v[4];
+  int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    double4 v[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int64_t j = i + u * stride;
-      v[u] = j < n2 ? x[j] : double2{0.0, 0.0};
-    }
+    for (int u = 0; u < 4; u++) v[u] = x[i + u * stride];
 #pragma unroll
-    for (int u = 0; u < 4; u++) s += v[u].x + v[u].y;
+    for (int u = 0; u < 4; u++) s += (v[u].x + v[u].y) + (v[u].z + v[u].w);
+  }
+  for (; i < n4; i += stride) {
+    const double4 v = x[i];
+    s += (v.x + v.y) + (v.z + v.w);
   }
   if (s == -1.0) sink[0] = s;  // data-dependent store keeps the loads
 }
@@ -473,19 +518,20 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(int64_t n2, const double
 }
 
 ohm_status stream_timed(ohm_backend *b, int64_t bytes, int mode, int reps, double &avg_ms) {
-  const int64_t n2 = std::max<int64_t>(bytes / 16, 1);
+  const int64_t n2 = std::max<int64_t>(bytes / 32 * 2, 2);  // whole 32-byte groups
   double2 *x = nullptr, *y = nullptr;
   OHM_HIP(hipMalloc(&x, sizeof(double2) * n2));
   if (hipMalloc(&y, mode == 1 ? sizeof(double2) * n2 : sizeof(double2)) != hipSuccess) {
     (void)hipFree(x);
     return fail(OHM_ERR_SOLVE, "Solve error: out of device memory for the stream probe");
   }
-  const dim3 grid(32768), blk(kBlock);
+  const dim3 blk(kBlock);
   auto one = [&]() {
     if (mode == 1)
-      hipLaunchKernelGGL(k_stream_copy, grid, blk, 0, b->stream, n2, x, y);
+      hipLaunchKernelGGL(k_stream_copy, dim3(32768), blk, 0, b->stream, n2, x, y);
     else
-      hipLaunchKernelGGL(k_stream_read, grid, blk, 0, b->stream, n2, x, (double *)y);
+      hipLaunchKernelGGL(k_stream_read, dim3(kStreamGrid), blk, 0, b->stream, n2 / 2,
+                         reinterpret_cast<const double4 *>(x), (double *)y);
   };
   ohm_status rc = OHM_OK;
   if (hipMemsetAsync(x, 0, sizeof(double2) * n2, b->stream) != hipSuccess) rc = fail(OHM_ERR_SOLVE, "memset");

@@ -32,18 +32,19 @@ struct Opnd {
 // workgroup synchronisation overlap them. head() returning false exits the
 // workgroup (every thread gets the same answer); it may synchronise.
 // Returns whether the block ran.
-template <typename T, int NV, typename G, typename H, typename C, typename F>
+template <typename T, int NV, int PER, typename G, typename H, typename C, typename F>
 __device__ __forceinline__ bool spmv_block_h(const uint32_t *__restrict__ rowptr,
                                              const uint32_t *__restrict__ col,
                                              const T *__restrict__ val,
                                              const uint32_t *__restrict__ blocks, G gather,
                                              H head, C comb, F f) {
-  __shared__ T prod[kSpmvNnzPerBlock];
+  constexpr int kCap = PER * kBlock;  // entries a block stages (its matrix's nnz cap)
+  __shared__ T prod[kCap];
   const uint32_t r0 = blocks[2 * blockIdx.x], k0 = blocks[2 * blockIdx.x + 1];
   const uint32_t r1 = blocks[2 * blockIdx.x + 2], k1 = blocks[2 * blockIdx.x + 3];
   const int tid = threadIdx.x;
   const uint32_t cnt = k1 - k0;
-  if (cnt > (uint32_t)kSpmvNnzPerBlock) {
+  if (cnt > (uint32_t)kCap) {
     if (!head()) return false;
     // four independent chains keep four gathers in flight per thread
     T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
@@ -60,7 +61,6 @@ __device__ __forceinline__ bool spmv_block_h(const uint32_t *__restrict__ rowptr
     if (tid == 0) f(r0, (T)v[0]);
     return true;
   }
-  constexpr int PER = kSpmvNnzPerBlock / kBlock;
   // Clamped indices keep every load valid and unconditional (so they issue
   // back to back); entries past cnt are discarded below. An empty block
   // (cnt == 0: rows without entries) loads nothing.
@@ -100,13 +100,13 @@ __device__ __forceinline__ bool spmv_block_h(const uint32_t *__restrict__ rowptr
 }
 
 // y = A x body without a head.
-template <typename T, typename F>
+template <typename T, int PER, typename F>
 __device__ __forceinline__ void spmv_block(const uint32_t *__restrict__ rowptr,
                                            const uint32_t *__restrict__ col,
                                            const T *__restrict__ val,
                                            const uint32_t *__restrict__ blocks,
                                            const T *__restrict__ x, F f) {
-  spmv_block_h<T, 1>(
+  spmv_block_h<T, 1, PER>(
       rowptr, col, val, blocks,
       [&](uint32_t j) {
         Opnd<T, 1> o;
