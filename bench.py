@@ -54,9 +54,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
 # HBM traffic of one k_part_solve launch of the AC sweep (2,048 rows per
 # part), from the same two counter passes over `bench.py --roofline-only`:
-# FETCH 43,184,550.2 KiB x 1.99939 + WRITE 118,645,976.0 KiB x 1.0.
+# FETCH 11,823,809.2 KiB x 1.99948 + WRITE 39,450,830.5 KiB x 1.0.
 # Summary: profiles/ac_sweep_256x101_counters.md.
-AC_TRAFFIC_BYTES = round((43184550.2 * 65536.0 / 32778.0 + 118645976.0) * 1024)
+AC_TRAFFIC_BYTES = round((11823809.2 * 65536.0 / 32776.5 + 39450830.5) * 1024)
 
 
 def log(msg: str) -> None:
@@ -90,31 +90,32 @@ def within(x, ref, rel=1e-6, abs_=1e-9):
 
 
 # ------------------------------------------------------------- headline --
-def ac_bytes(sizes, n_freqs: int, parts: int):
+def ac_bytes(sizes, n_freqs: int, rows_per_part: int):
     """Algorithmic bytes of one sweep's kernels. Waves hold up to 64
     frequencies of one circuit; the band rows of G and C and the permuted
     b (3 + 3 + 2 doubles = 64 bytes per row at half bandwidth 1) are read by
-    every wave once per pass: twice by the part solve (the elimination and
-    the back substitution, which rebuilds pivot rows taken from the next
-    row and sums the residual), once per direction by the relations. Per
-    (system, row) the part solve writes the pivot row divided by its pivot
-    (2 complex: the eliminated right-hand side and the entry right of the
-    diagonal; a pivot row that is the next row as given is a marker there),
-    reads it back and writes x (80 bytes). The residual pass reads, per part
-    and system, x at the part's first and last rows and their neighbours
-    (6 complex) and the part's two partial sums."""
+    every wave once per pass: once per direction by the relations and once by
+    the part solve. The relations' top-down pass stores the elimination state
+    every 8 steps (4 complex = 64 bytes per system: ceil((m - 1) / 8) of them
+    in a part of m rows), which the part solve reads back to recompute each
+    chunk's pivot rows in registers before writing x (16 bytes per (system,
+    row)). The residual pass reads, per part and system, x at the part's
+    first and last rows and their neighbours (6 complex) and the part's two
+    partial sums."""
     waves = -(-n_freqs // 64)
     rows = float(sum(sizes))
     band = 64.0
-    return {"part_solve": rows * (80.0 * n_freqs + 2 * band * waves),
-            "relations": rows * band * 2 * waves,
+    if not rows_per_part:  # one lane per system (k_band_lu<1>): U rows of 3 complex
+        return {"part_solve": rows * (112.0 * n_freqs + band * waves), "relations": 0.0,
+                "residual": 0.0}
+    parts, ck = 0, 0
+    for n in sizes:
+        p = -(-n // rows_per_part)
+        parts += p
+        ck += p * -(-(-(-n // p) - 1) // 8)
+    return {"part_solve": rows * 16.0 * n_freqs + ck * 64.0 * n_freqs + rows * band * waves,
+            "relations": rows * band * 2 * waves + ck * 64.0 * n_freqs,
             "residual": parts * n_freqs * (6 * 16.0 + 16.0) + parts * 2 * band * waves}
-
-
-def n_parts(sizes, rows_per_part: int) -> int:
-    """Parts of a partitioned sweep (one per rows_per_part rows of every
-    circuit; 0 for the one-lane solve)."""
-    return sum(-(-n // rows_per_part) for n in sizes) if rows_per_part else 0
 
 
 def check_all_systems(plan, mnas, freqs, threads: int):
@@ -198,7 +199,7 @@ def ac_headline(args, be, N, ir, solver):
     worst = float(res.residual_norms.max())
     failed = int((~res.converged).sum())
     ms = {k: v / args.steps for k, v in acc.items()}  # average over the timed sweeps
-    by = ac_bytes(sizes, nf, n_parts(sizes, st["rows_per_part"]))
+    by = ac_bytes(sizes, nf, st["rows_per_part"])
     kernel = {"part_solve": "k_part_solve" if st["rows_per_part"] else "k_band_lu<1>",
               "relations": "k_part_rel", "residual": "k_part_resid_sum"}
     dom = max(("part_solve", "relations", "residual"), key=lambda k: ms[k])
@@ -553,7 +554,7 @@ def roofline_only(args, be, N, ir, solver) -> int:
         plan.solve(freqs)
     st = plan.stats()
     sizes = [m.size for m in mnas]
-    by = ac_bytes(sizes, int(freqs.shape[0]), n_parts(sizes, st["rows_per_part"]))
+    by = ac_bytes(sizes, int(freqs.shape[0]), st["rows_per_part"])
     del plan
     m = solver.compile(ir.diode_mesh(256, 256, seed=2))
     a = be.upload_matrix(m.g.values, m.g.col_indices, m.g.row_pointers, m.size)

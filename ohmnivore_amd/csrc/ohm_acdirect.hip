@@ -46,9 +46,16 @@ __device__ __forceinline__ cz operator*(cz a, cz b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 __device__ __forceinline__ double cabs2(cz a) { return a.re * a.re + a.im * a.im; }
+// 1 / a = conj(a) / |a|^2 with the reciprocal of |a|^2 from the hardware
+// estimate and two Newton-Raphson steps r <- r + r (1 - d r), each of which
+// squares the relative error (to within an ulp of the quotient), instead of
+// two IEEE divisions (about nine dependent instructions each).
 __device__ __forceinline__ cz cinv(cz a) {
   const double d = cabs2(a);
-  return {a.re / d, -a.im / d};
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return {a.re * r, -a.im * r};
 }
 
 // One wave = up to 64 frequency points of one circuit (lane l: frequency
@@ -329,15 +336,18 @@ __global__ void k_band_unpermute(const BandWave *__restrict__ waves, const cz *_
 //   phase 2 (k_part_reduce): per system, the 2P boundary unknowns from those
 //     relations: a banded system of half bandwidth 2, LU with partial
 //     pivoting, one lane per system.
-//   phase 3 (k_part_solve): every part eliminated again top-down with the
-//     now known boundary values moved to its right-hand side, its U rows
-//     stored and back-substituted, x written where the one-lane solve
-//     writes it; the residual of the part's interior rows is summed on the
-//     way.
+//   phase 3 (k_part_solve): every part's top-down elimination restarted,
+//     chunk by chunk from the last, from states phase 1 stored every kCk
+//     steps, with the now known boundary values moved to its right-hand
+//     side; each chunk's pivot rows stay in registers and are
+//     back-substituted at once, x written where the one-lane solve writes
+//     it; the residual of the part's interior rows is summed on the way.
 //   k_part_resid_sum: per system, the residual of every part's first and
 //     last row from the x written by both neighbours, plus the parts' sums.
-// Phase 1 reads only the band rows shared by the wave's lanes, so the bytes
-// stored and read per (system, row) are those of the one-lane solve.
+// Phase 1 reads only the band rows shared by the wave's lanes and stores the
+// checkpoints (64 B per lane every kCk rows); phase 3 reads them and writes
+// x, so a (system, row) costs 24 B of HBM traffic where the one-lane solve,
+// which stores its U rows and reads them back, moves 80 B or more.
 // Systems whose residual is not small (a part block close to singular
 // after all) are solved again by k_band_lu.
 constexpr int kPartW = 3;  // band width at half bandwidth 1
@@ -346,10 +356,14 @@ struct PartTile {
   double g[kBandTile][kPartW], c[kBandTile][kPartW], b[kBandTile][2];
 };
 
-// one part: its wave (index into the chunk's wave table) and rows [s, s+m)
+// one part: its wave (index into the chunk's wave table), rows [s, s+m) and
+// the index of its first checkpoint of the top-down elimination
 struct PartRef {
-  int item, s, m, pad;
+  int item, s, m, ck;
 };
+
+// Steps of the top-down elimination between two checkpoints of its state.
+constexpr int kCk = 8;
 
 // direction-order rows k0 .. k0+63 of part [s, s+m) (rev: row k is s+m-1-k,
 // band entries mirrored) into this lane's registers
@@ -373,8 +387,9 @@ __device__ __forceinline__ void part_row_load(const BandWave &q, int64_t s, int6
   }
 }
 
-// Marker stored in place of a pivot row that is row j+1 as given (see
-// part_eliminate): a quiet NaN real part.
+// Marker kept in place of a pivot row when the pivot of step j is row j+1
+// as given (|N_0| > |R_0|; the back substitution then reads that row from
+// the band rows): a quiet NaN real part.
 __device__ __forceinline__ cz pivot_marker() { return cz{__builtin_nan(""), 0.0}; }
 __device__ __forceinline__ bool is_pivot_marker(cz e) { return e.re != e.re; }
 
@@ -386,39 +401,29 @@ __device__ __forceinline__ bool is_pivot_marker(cz e) { return e.re != e.re; }
 // which has no entry at column j. Up to a scalar of modulus between 1 and 2
 // this is the remainder partial pivoting leaves (N - (N_0/R_0) R when
 // |R_0| >= |N_0|, else R - (R_0/N_0) N), so no row exchange is needed.
-// REL: right-hand sides f, the start coupling and the end coupling (the
-//   entries of direction rows 0 and m-1 outside the part); ly and lu are
-//   the last remainder's right-hand sides and diagonal.
-// SOLVE (top-down only): the couplings times the known xs / xe are moved
-//   to f; step j's pivot row goes to Ub at global row s+j: when R is the
-//   pivot (|R_0| >= |N_0|), (f_R / R_0, R_1 / R_0); when N is, a marker,
-//   since N is row j+1 as given, which the back substitution reads from the
-//   band rows. The last row's pivot is the last remainder.
-template <bool REL>
+// The right-hand sides are f, the start coupling and the end coupling (the
+// entries of direction rows 0 and m-1 outside the part); ly and lu are the
+// last remainder's right-hand sides and diagonal. With ck set, the state
+// before every kCk-th step (R_0, R_1, f, start coupling; the end coupling is
+// still zero there) is stored at ck[(j / kCk * 4 + k) * 64].
 __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int64_t m, bool rev,
-                                               double om, bool live, cz xs, cz xe, cz *Ub,
-                                               int S, PartTile &t, cz (&ly)[3], cz &lu,
-                                               int &bad) {
+                                               double om, PartTile &t, cz (&ly)[3], cz &lu,
+                                               int &bad, cz *ck) {
   constexpr int W = kPartW;
   const int lane = threadIdx.x;
   double rg[W], rc[W], rb[2];
   // direction row 0 is the first remainder: columns -1 (left of the part),
   // 0 and 1 (right of the part when m == 1)
   part_row_load(q, s, m, rev, 0, rg, rc, rb);
-  cz r0, r1, y0 = {rb[0], rb[1]}, y1 = {0.0, 0.0}, y2 = {0.0, 0.0};
+  cz r0, r1, y0 = {rb[0], rb[1]}, y1, y2;
   {
     cz a[W];
 #pragma unroll
     for (int d = 0; d < W; d++) a[d] = cz{rg[d], om * rc[d]};
     r0 = a[1];
     r1 = m > 1 ? a[2] : cz{0.0, 0.0};
-    const cz cr = m > 1 ? cz{0.0, 0.0} : a[2];
-    if (REL) {
-      y1 = a[0];
-      y2 = cr;
-    } else {
-      y0 = y0 - a[0] * xs - cr * xe;
-    }
+    y1 = a[0];
+    y2 = m > 1 ? cz{0.0, 0.0} : a[2];
   }
   // tiles of direction rows 1 + t0 .. 64 + t0; step j reads row j+1
   part_row_load(q, s, m, rev, 1 + lane, rg, rc, rb);
@@ -436,26 +441,19 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
     const int64_t t1 = t0 + kBandTile < m - 1 ? t0 + kBandTile : m - 1;
 #pragma unroll 2
     for (int64_t j = t0; j < t1; j++) {
+      if (ck && (j & (kCk - 1)) == 0) {
+        cz *o = ck + (size_t)(j / kCk) * 4 * 64;
+        o[0] = r0;
+        o[64] = r1;
+        o[128] = y0;
+        o[192] = y1;
+      }
       const int sl = (int)(j - t0);
       const cz n0 = {t.g[sl][0], om * t.c[sl][0]}, n1 = {t.g[sl][1], om * t.c[sl][1]};
       cz n2 = {t.g[sl][2], om * t.c[sl][2]}, f = {t.b[sl][0], t.b[sl][1]}, e2 = {0.0, 0.0};
       if (j + 2 >= m) {  // row m-1: its column-m entry is the end coupling
-        if (REL)
-          e2 = n2;
-        else
-          f = f - n2 * xe;
+        e2 = n2;
         n2 = cz{0.0, 0.0};
-      }
-      if (!REL) {
-        const cz inv = cinv(r0);
-        const bool n_pivot = cabs2(n0) > cabs2(r0);
-        const cz e0 = n_pivot ? pivot_marker() : y0 * inv;
-        const cz e1 = r1 * inv;
-        if (live) {
-          const int64_t i = s + j;
-          Ub[(i * 2) * S + lane] = e0;
-          Ub[(i * 2 + 1) * S + lane] = e1;
-        }
       }
       const double mx = fmax(fmax(fabs(r0.re), fabs(r0.im)), fmax(fabs(n0.re), fabs(n0.im)));
       if (!(mx > 0.0)) bad = 1;
@@ -465,16 +463,11 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
       r1 = c2 * n2;
       r0 = nr0;
       y0 = c1 * y0 + c2 * f;
-      if (REL) {
-        y1 = c1 * y1;
-        y2 = c1 * y2 + c2 * e2;
-      }
+      y1 = c1 * y1;
+      // y2 (the end coupling's column) is zero until the last row of a part
+      // of m > 1 rows brings e2, so it is formed there alone
+      if (j + 2 >= m) y2 = c1 * y2 + c2 * e2;
     }
-  }
-  if (!REL && live) {  // the last row: its pivot is the remainder (no entry past the part)
-    const int64_t i = s + m - 1;
-    Ub[(i * 2) * S + lane] = y0 * cinv(r0);
-    Ub[(i * 2 + 1) * S + lane] = cz{0.0, 0.0};
   }
   lu = r0;
   ly[0] = y0;
@@ -484,9 +477,13 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
 
 // Phase 1: blockIdx.x = 2 ip + direction; relation of (part ip, direction)
 // to rel[((ip * 2 + dir) * 3 + k) * 64 + lane], k = alpha, k_prev, k_next.
+// The top-down wave also leaves, every kCk steps, the state (R_0, R_1, and
+// the right-hand sides f and start coupling) at
+// ckpt[((ck + c) * 4 + k) * 64 + lane] for the solve to restart from.
 __global__ __launch_bounds__(64) void k_part_rel(const BandWave *__restrict__ waves,
                                                  const PartRef *__restrict__ parts,
-                                                 cz *__restrict__ rel, int *__restrict__ singular) {
+                                                 cz *__restrict__ rel, int *__restrict__ singular,
+                                                 cz *__restrict__ ckpt) {
   __shared__ PartTile t;
   const int lane = threadIdx.x;
   const int ip = blockIdx.x >> 1, dir = blockIdx.x & 1;
@@ -497,8 +494,8 @@ __global__ __launch_bounds__(64) void k_part_rel(const BandWave *__restrict__ wa
   const double om = live ? q.omega[lane] : 0.0;
   cz ly[3], lu = {1.0, 0.0};
   int bad = 0;
-  part_eliminate<true>(q, s, m, dir == 1, om, live, cz{0.0, 0.0}, cz{0.0, 0.0}, nullptr, q.nf,
-                       t, ly, lu, bad);
+  cz *ck = ckpt && dir == 0 && live ? ckpt + (size_t)pr.ck * 4 * 64 + lane : nullptr;
+  part_eliminate(q, s, m, dir == 1, om, t, ly, lu, bad, ck);
   if (!live) return;
   if (bad || !(cabs2(lu) > 0.0)) {
     singular[q.sys + lane] = 2;
@@ -632,16 +629,25 @@ __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__
   if (bad) singular[q.sys + lane] = 2;
 }
 
-// Phase 3: blockIdx.x = ip; the part's top-down elimination with its
-// boundary values known, U rows and y stored, then back substitution.
-// The part's interior rows s+1 .. s+m-2 use only x of this part, so their
-// residual terms are summed here, to pres[(ip * 64 + lane) * 2 + {0, 1}].
+// Phase 3: blockIdx.x = ip. The part's top-down elimination is recomputed
+// chunk by chunk, last chunk first, from the states phase 1 left every kCk
+// steps (the start coupling column times the now known xs is taken off the
+// right-hand side; the end coupling times xe enters at the last row), and
+// each chunk's pivot rows stay in registers and are back-substituted at
+// once: row j's pivot row is (e_0, e_1), x_j = e_0 - e_1 x_{j+1}, or the
+// marker: row j+1 as given was the pivot, x_j = (f_{j+1} - a_{j+1,j+1}
+// x_{j+1} - a_{j+1,j+2} x_{j+2}) / a_{j+1,j}. Per (system, row) this reads
+// the checkpoints (64 B per kCk rows) and writes x (16 B), where storing the
+// pivot rows and reading them back took 80 B. The part's interior rows
+// s+1 .. s+m-2 use only x of this part, so their residual terms are summed
+// here, to pres[(ip * 64 + lane) * 2 + {0, 1}].
 __global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ waves,
-                                                   const PartRef *__restrict__ parts,
-                                                   const cz *__restrict__ bnd, cz *__restrict__ U,
-                                                   cz *__restrict__ X,
-                                                   int *__restrict__ singular,
-                                                   double *__restrict__ pres) {
+                                                      const PartRef *__restrict__ parts,
+                                                      const cz *__restrict__ bnd,
+                                                      const cz *__restrict__ ckpt,
+                                                      cz *__restrict__ X,
+                                                      int *__restrict__ singular,
+                                                      double *__restrict__ pres) {
   constexpr int W = kPartW;
   __shared__ PartTile t;
   const int lane = threadIdx.x;
@@ -659,66 +665,106 @@ __global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ 
     if (p + 1 < q.nparts) xe = bnd[(rb + 2 * (p + 1)) * 64 + lane];
   }
   const int S = q.nf;
-  cz *Ub = U + q.store_off * W;
   cz *Xb = X + q.store_off;
-  cz ly[3], lu;
-  int bad = 0;
-  part_eliminate<false>(q, s, m, false, om, live, xs, xe, Ub, S, t, ly, lu, bad);
-  if (!(cabs2(lu) > 0.0)) bad = 1;
-  // back substitution over the part's rows, 64 rows per tile: row j's pivot
-  // row is (e_0, e_1): x_j = e_0 - e_1 x_{j+1}, or the marker: row j+1 as
-  // given, x_j = (f_{j+1} - a_{j+1,j+1} x_{j+1} - a_{j+1,j+2} x_{j+2}) /
-  // a_{j+1,j} (column s+m is outside the part: its entry times xe is in
-  // f). The band rows j+1 of the tile, which the interior residual rows
-  // also use, are staged in LDS at slot hi - j.
-  constexpr int G = 4;
-  cz xw[2] = {cz{0.0, 0.0}, cz{0.0, 0.0}};  // x_{j+1}, x_{j+2}
   double rr = 0.0, bb = 0.0;
-  for (int64_t hi = s + m - 1; hi >= s; hi -= kBandTile) {
-    const int64_t lo = hi - kBandTile + 1 > s ? hi - kBandTile + 1 : s;
-    const int64_t ir = hi + 1 - lane;
-    const bool in = ir >= s + 1 && ir <= s + m - 1 && ir - 1 >= lo;
-    __syncthreads();
+  int bad = 0;
+  const int64_t ns = m - 1;  // elimination steps; step j pivots x_{s+j} against row s+j+1
+  if (ns == 0) {             // one row: a_0 xs + a_1 x + a_2 xe = b
+    cz a[W], bi;
+    band_row<W>(q, om, s, a, bi);
+    const cz f = bi - a[0] * xs - a[2] * xe;
+    if (!(cabs2(a[1]) > 0.0)) bad = 1;
+    if (live) Xb[s * S + lane] = f * cinv(a[1]);
+  } else {
+    const cz *ckb = ckpt + (size_t)pr.ck * 4 * 64 + lane;
+    cz xw0 = {0.0, 0.0}, xw1 = {0.0, 0.0};  // x_{s+j+1}, x_{s+j+2}
+    // the next chunk's checkpoint and the next tile's band rows are loaded
+    // one chunk / one tile ahead
+    const int64_t j_last = (ns - 1) / kCk * kCk;
+    cz pf[4] = {};  // lanes past nf load nothing (their results are not stored)
+    if (live) {
 #pragma unroll
-    for (int d = 0; d < W; d++) {
-      t.g[lane][d] = in ? q.gb[ir * W + d] : 0.0;
-      t.c[lane][d] = in ? q.cb[ir * W + d] : 0.0;
+      for (int k = 0; k < 4; k++) pf[k] = ckb[((size_t)(j_last / kCk) * 4 + k) * 64];
     }
-    t.b[lane][0] = in ? q.b[2 * ir] : 0.0;
-    t.b[lane][1] = in ? q.b[2 * ir + 1] : 0.0;
-    __syncthreads();
-    for (int64_t g0 = hi; g0 >= lo; g0 -= G) {
-      cz u[G][2];
+    int64_t t0 = (ns - 1) / kBandTile * kBandTile;
+    double rg[W], rc[W], rbv[2];
+    part_row_load(q, s, m, false, 1 + t0 + lane, rg, rc, rbv);
+    // tiles of 64 steps, last first; step j's row s+j+1 at slot j - t0
+    for (; t0 >= 0; t0 -= kBandTile) {
+      const int64_t t1 = t0 + kBandTile < ns ? t0 + kBandTile : ns;
+      __syncthreads();
 #pragma unroll
-      for (int g = 0; g < G; g++) {
-        const int64_t j = g0 - g;
-#pragma unroll
-        for (int c = 0; c < 2; c++)
-          u[g][c] = j >= lo && live ? Ub[(j * 2 + c) * S + lane] : cz{0.0, 0.0};
+      for (int d = 0; d < W; d++) {
+        t.g[lane][d] = rg[d];
+        t.c[lane][d] = rc[d];
       }
+      t.b[lane][0] = rbv[0];
+      t.b[lane][1] = rbv[1];
+      __syncthreads();
+      if (t0 > 0) part_row_load(q, s, m, false, 1 + t0 - kBandTile + lane, rg, rc, rbv);
+      for (int64_t j0 = t0 + (t1 - 1 - t0) / kCk * kCk; j0 >= t0; j0 -= kCk) {
+        const int64_t j1 = j0 + kCk < t1 ? j0 + kCk : t1;
+        cz r0 = pf[0], r1 = pf[1];
+        cz y = pf[2] - pf[3] * xs;
+        if (j0 > 0 && live) {
 #pragma unroll
-      for (int g = 0; g < G; g++) {
-        const int64_t j = g0 - g;
-        if (j >= lo) {
-          const int sl = (int)(hi - j);
-          cz a[W];  // row j+1 (columns j .. j+2), when inside the part
+          for (int k = 0; k < 4; k++) pf[k] = ckb[((size_t)(j0 / kCk - 1) * 4 + k) * 64];
+        }
+        cz e0[kCk], e1[kCk];
 #pragma unroll
-          for (int d = 0; d < W; d++) a[d] = cz{t.g[sl][d], om * t.c[sl][d]};
-          const cz bj = {t.b[sl][0], t.b[sl][1]};
-          cz xj = u[g][0] - u[g][1] * xw[0];
-          if (is_pivot_marker(u[g][0])) {
-            const bool last = j + 2 >= s + m;  // row j+1 is the part's last row
-            const cz f = last ? bj - a[2] * xe : bj;
-            const cz a2 = last ? cz{0.0, 0.0} : a[2];
-            xj = (f - a[1] * xw[0] - a2 * xw[1]) * cinv(a[0]);
+        for (int k = 0; k < kCk; k++) {
+          const int64_t j = j0 + k;
+          if (j < j1) {
+            const int sl = (int)(j - t0);
+            const cz n0 = {t.g[sl][0], om * t.c[sl][0]}, n1 = {t.g[sl][1], om * t.c[sl][1]};
+            cz n2 = {t.g[sl][2], om * t.c[sl][2]}, f = {t.b[sl][0], t.b[sl][1]};
+            if (j + 2 >= m) {  // row m-1: its column-m entry times xe
+              f = f - n2 * xe;
+              n2 = cz{0.0, 0.0};
+            }
+            const cz inv = cinv(r0);
+            e0[k] = cabs2(n0) > cabs2(r0) ? pivot_marker() : y * inv;
+            e1[k] = r1 * inv;
+            const double mx =
+                fmax(fmax(fabs(r0.re), fabs(r0.im)), fmax(fabs(n0.re), fabs(n0.im)));
+            if (!(mx > 0.0)) bad = 1;
+            const double sc = __builtin_amdgcn_rcp(mx);
+            const cz c1 = {n0.re * sc, n0.im * sc}, c2 = {-r0.re * sc, -r0.im * sc};
+            const cz nr0 = c1 * r1 + c2 * n1;
+            r1 = c2 * n2;
+            r0 = nr0;
+            y = c1 * y + c2 * f;
           }
-          if (live) Xb[j * S + lane] = xj;
-          if (j + 1 <= s + m - 2) {  // interior row j + 1: columns j .. j + 2
-            const cz xc[W] = {xj, xw[0], xw[1]};
-            resid_row<W>(a, bj, xc, rr, bb);
+        }
+        if (j1 == ns) {  // the last row: its pivot is the final remainder
+          if (!(cabs2(r0) > 0.0)) bad = 1;
+          xw0 = y * cinv(r0);
+          if (live) Xb[(s + m - 1) * S + lane] = xw0;
+        }
+#pragma unroll
+        for (int k = kCk - 1; k >= 0; k--) {
+          const int64_t j = j0 + k;
+          if (j < j1) {
+            const int sl = (int)(j - t0);
+            cz a[W];  // row s+j+1 (columns s+j .. s+j+2)
+#pragma unroll
+            for (int d = 0; d < W; d++) a[d] = cz{t.g[sl][d], om * t.c[sl][d]};
+            const cz bj = {t.b[sl][0], t.b[sl][1]};
+            cz xj = e0[k] - e1[k] * xw0;
+            if (is_pivot_marker(e0[k])) {  // row s+j+1 as given was the pivot
+              const bool last = j + 2 >= m;
+              const cz f = last ? bj - a[2] * xe : bj;
+              const cz a2 = last ? cz{0.0, 0.0} : a[2];
+              xj = (f - a[1] * xw0 - a2 * xw1) * cinv(a[0]);
+            }
+            if (live) Xb[(s + j) * S + lane] = xj;
+            if (j + 1 <= m - 2) {  // interior row s+j+1
+              const cz xc[W] = {xj, xw0, xw1};
+              resid_row<W>(a, bj, xc, rr, bb);
+            }
+            xw1 = xw0;
+            xw0 = xj;
           }
-          xw[1] = xw[0];
-          xw[0] = xj;
         }
       }
     }
@@ -1164,9 +1210,11 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       const double np = parts_of(items[end].c);
       // U rows and x per lane-row; per part: its reference, relations,
       // reduced rows, boundary values and residual partials per lane
+      // (and the checkpoints of the top-down elimination: 4 per kCk rows)
       const double wave_b = n * nfw * 16.0 * (W + 1) + (want_x ? n * nfw * 16.0 : 0.0) +
                             2.0 * sizeof(BandWave) +
-                            np * (sizeof(PartRef) + 64.0 * 16.0 * (6.0 + 12.0 + 2.0 + 1.0));
+                            np * (sizeof(PartRef) + 64.0 * 16.0 * (6.0 + 12.0 + 2.0 + 1.0)) +
+                            (part ? (n / kCk + np) * 4.0 * 64.0 * 16.0 : 0.0);
       if (end > pos && bytes + wave_b > budget) break;
       bytes += wave_b;
       end++;
@@ -1195,6 +1243,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     std::vector<BandWave> hw(n_w);
     std::vector<PartRef> prefs;
     int64_t store = 0;  // lane-rows
+    int64_t n_ck = 0;   // checkpoints of the top-down elimination
     for (int k = 0; k < n_w; k++) {
       const WaveItem &it = items[pos + k];
       BandWave w{};
@@ -1210,8 +1259,11 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       w.sys = (int)(it.c * n_freqs + it.f0);
       w.pbase = (int)prefs.size();
       w.nparts = parts_of(it.c);
-      for (int q = 0; q < w.nparts; q++)
-        prefs.push_back({k, cuts[it.c][q], cuts[it.c][q + 1] - cuts[it.c][q], 0});
+      for (int q = 0; q < w.nparts; q++) {
+        const int mq = cuts[it.c][q + 1] - cuts[it.c][q];
+        prefs.push_back({k, cuts[it.c][q], mq, (int)n_ck});
+        n_ck += (mq - 1 + kCk - 1) / kCk;
+      }
       store += w.n * it.nf;
       hw[k] = w;
     }
@@ -1225,10 +1277,11 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
                  rel_b = al(sizeof(cz) * n_ip * 6 * 64),
                  red_b = al(sizeof(cz) * n_ip * 12 * 64),
                  bnd_b = al(sizeof(cz) * n_ip * 2 * 64),
-                 pres_b = al(sizeof(double) * n_ip * 2 * 64);
+                 pres_b = al(sizeof(double) * n_ip * 2 * 64),
+                 ck_b = part ? al(sizeof(cz) * std::max<int64_t>(n_ck, 1) * 4 * 64) : 0;
     // (partitioned: a second wave table for the one-lane redo after pres)
     const size_t need =
-        bw_b + u_b + y_b + x_b + (part ? ip_b + rel_b + red_b + bnd_b + pres_b + bw_b : 0);
+        bw_b + u_b + y_b + x_b + (part ? ip_b + rel_b + red_b + bnd_b + pres_b + bw_b + ck_b : 0);
     if (b->ac_ws_bytes < need) {
       if (b->ac_ws) OHM_HIP(hipFree(b->ac_ws));
       b->ac_ws = nullptr;
@@ -1247,6 +1300,8 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     cz *d_rel = (cz *)(pw + ip_b), *d_red = (cz *)(pw + ip_b + rel_b),
        *d_bnd = (cz *)(pw + ip_b + rel_b + red_b);
     double *d_pres = (double *)(pw + ip_b + rel_b + red_b + bnd_b);
+    // after the redo wave table (ip_b + ... + pres_b + bw_b)
+    cz *d_ck = part ? (cz *)(pw + ip_b + rel_b + red_b + bnd_b + pres_b + bw_b) : nullptr;
     OHM_HIP(hipMemcpyAsync(d_w, hw.data(), sizeof(BandWave) * n_w, hipMemcpyHostToDevice, st));
     if (part)
       OHM_HIP(hipMemcpyAsync(d_parts, prefs.data(), sizeof(PartRef) * n_ip,
@@ -1266,7 +1321,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     OHM_HIP(hipEventRecord(ev[0], st));
     if (part) {
       hipLaunchKernelGGL(k_part_rel, dim3((unsigned)(2 * n_ip)), dim3(64), 0, st, d_w, d_parts,
-                         d_rel, p->d_sing);
+                         d_rel, p->d_sing, d_ck);
       OHM_HIP(hipGetLastError());
       OHM_HIP(hipEventRecord(ev[1], st));
       hipLaunchKernelGGL(k_part_reduce, dim3(n_w), dim3(64), 0, st, d_w, d_rel, d_red, d_bnd,
@@ -1274,7 +1329,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       OHM_HIP(hipGetLastError());
       OHM_HIP(hipEventRecord(ev[2], st));
       hipLaunchKernelGGL(k_part_solve, dim3((unsigned)n_ip), dim3(64), 0, st, d_w, d_parts,
-                         d_bnd, dU, dY, p->d_sing, d_pres);
+                         d_bnd, d_ck, dY, p->d_sing, d_pres);
       OHM_HIP(hipGetLastError());
       OHM_HIP(hipEventRecord(ev[3], st));
       hipLaunchKernelGGL(k_part_resid_sum, dim3((unsigned)((n_w * 64 + 255) / 256)), dim3(256), 0,
