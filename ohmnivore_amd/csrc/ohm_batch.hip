@@ -1029,6 +1029,7 @@ struct ohm_batch {
   // workspaces: 10 vectors each (x r rh p v s t ph sh inv) + rhs/ax scratch
   void *ws64 = nullptr, *ws32 = nullptr;
   int64_t dispatches = 0;
+  int tuned_shape = -1;  // the row-block shape the autotuner picked (index of its candidates; -1: default)
 };
 
 namespace {
@@ -1455,6 +1456,71 @@ ohm_status batch_solve_t(ohm_batch *bt, const BatchOpts &o, int precision, doubl
 // ================================================================ C ABI ==
 extern "C" {
 
+// Batches of at least kBatchTuneRows rows pick their row-block shape by
+// timing the batched SpMV (the body every batched BiCGSTAB kernel shares)
+// over each candidate table: 3 launches to warm, 10 timed with HIP events.
+// The shape only changes how rows are grouped into workgroups (and so the
+// order in which a system's per-block dot partials are added).
+constexpr int64_t kBatchTuneRows = 65536;
+
+static ohm_status batch_tune_blocks(ohm_batch *bt, const std::vector<std::vector<BBlock>> &tables) {
+  ohm_backend *b = bt->owner;
+  hipStream_t st = b->stream;
+  const size_t eb = elem_bytes(bt->cplx, false);
+  const int64_t stride = (bt->N + 63) & ~int64_t(63);
+  char *base = reinterpret_cast<char *>(bt->ws64);
+  void *x = base, *y = base + stride * eb;
+  float best = 1e30f;
+  size_t pick = tables.size() - 1;
+  for (size_t c = 0; c < tables.size(); c++) {
+    const std::vector<BBlock> &tab = tables[c];
+    if (tab.empty()) continue;
+    OHM_HIP(hipMemcpyAsync(bt->d_full, tab.data(), sizeof(BBlock) * tab.size(),
+                           hipMemcpyHostToDevice, st));
+    auto launch = [&]() {
+      if (bt->cplx)
+        hipLaunchKernelGGL(k_bspmv<cx<double>>, dim3(tab.size()), dim3(kBlock), 0, st,
+                           bt->d_full, bt->d_geo, (const cx<double> *)x, (cx<double> *)y);
+      else
+        hipLaunchKernelGGL(k_bspmv<double>, dim3(tab.size()), dim3(kBlock), 0, st, bt->d_full,
+                           bt->d_geo, (const double *)x, (double *)y);
+    };
+    for (int w = 0; w < 3; w++) launch();
+    OHM_HIP(hipEventRecord(b->ev0, st));
+    for (int r = 0; r < 10; r++) launch();
+    OHM_HIP(hipEventRecord(b->ev1, st));
+    OHM_HIP(hipEventSynchronize(b->ev1));
+    OHM_HIP(hipGetLastError());
+    bt->dispatches += 13;
+    float ms = 0.f;
+    OHM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (ms < best) {
+      best = ms;
+      pick = c;
+    }
+  }
+  bt->h_blocks = tables[pick];
+  for (int k = 0; k < bt->K; k++) bt->h_geo[k].first_block = bt->h_geo[k].n_blocks = 0;
+  for (const BBlock &bk : bt->h_blocks) {
+    BGeo &g = bt->h_geo[bk.sys];
+    if (g.n_blocks == 0) g.first_block = bk.slot;
+    g.n_blocks++;
+  }
+  for (int k = 0; k < bt->K; k++) {
+    bt->h_src[k].first_block = bt->h_geo[k].first_block;
+    bt->h_src[k].n_blocks = bt->h_geo[k].n_blocks;
+  }
+  bt->tuned_shape = (int)pick;
+  OHM_HIP(hipMemcpyAsync(bt->d_full, bt->h_blocks.data(), sizeof(BBlock) * bt->h_blocks.size(),
+                         hipMemcpyHostToDevice, st));
+  OHM_HIP(hipMemcpyAsync(bt->d_geo, bt->h_geo.data(), sizeof(BGeo) * bt->K, hipMemcpyHostToDevice,
+                         st));
+  OHM_HIP(hipMemcpyAsync(bt->d_src, bt->h_src.data(), sizeof(BGeo) * bt->K, hipMemcpyHostToDevice,
+                         st));
+  OHM_HIP(hipStreamSynchronize(st));
+  return OHM_OK;
+}
+
 ohm_status ohm_batch_create(ohm_backend *b, int32_t is_complex, int64_t k,
                             const ohm_batch_system *systems, ohm_batch **out) {
   if (!b || !out || (k > 0 && !systems)) return fail(OHM_ERR_INVALID, "null argument");
@@ -1470,6 +1536,7 @@ ohm_status ohm_batch_create(ohm_backend *b, int32_t is_complex, int64_t k,
   };
   // dedupe patterns and value arrays by host pointer
   std::map<std::tuple<const void *, const void *, int64_t>, std::pair<uint32_t *, uint32_t *>> pats;
+  std::vector<const uint32_t *> host_rowptr;  // per system, for the block-shape autotuner
   std::map<std::pair<const void *, int64_t>, double *> vals;
   int64_t off = 0;
   for (int64_t s = 0; s < k; s++) {
@@ -1541,11 +1608,31 @@ ohm_status ohm_batch_create(ohm_backend *b, int32_t is_complex, int64_t k,
     g.n_blocks = (int)bt->h_blocks.size() - g.first_block;
     bt->h_geo.push_back(g);
     bt->h_nnz.push_back(nnz);
+    host_rowptr.push_back(q.rowptr);
     bt->max_n = std::max(bt->max_n, n);
     off += n;
   }
   bt->N = off;
-  const size_t nb = std::max<size_t>(bt->h_blocks.size(), 1);
+  // Row-block shapes the autotuner below may pick (rows per workgroup; the
+  // LDS stages at most 2,048 entries per block): tables for every system,
+  // device arrays sized for the largest.
+  std::vector<int> tune_rows;
+  if (b->autotune && bt->N >= kBatchTuneRows) tune_rows = {64, 128, kSpmvRowsPerBlock};
+  std::vector<std::vector<BBlock>> tables;
+  size_t nb = std::max<size_t>(bt->h_blocks.size(), 1);
+  for (int rows : tune_rows) {
+    std::vector<BBlock> tab;
+    for (int64_t s2 = 0; s2 < k; s2++) {
+      const int n = bt->h_geo[s2].n;
+      if (n <= 0) continue;
+      std::vector<uint32_t> bounds;
+      build_spmv_blocks(host_rowptr[s2], n, bounds, rows, kSpmvNnzPerBlock);
+      for (size_t i = 0; i + 1 < bounds.size(); i++)
+        tab.push_back(BBlock{(int)s2, bounds[i], bounds[i + 1], (int)tab.size()});
+    }
+    nb = std::max(nb, tab.size());
+    tables.push_back(std::move(tab));
+  }
   bt->h_src = bt->h_geo;
   bt->formed.assign(k, nullptr);
   if (hipMalloc(&bt->d_geo, sizeof(BGeo) * k) != hipSuccess ||
@@ -1575,6 +1662,10 @@ ohm_status ohm_batch_create(ohm_backend *b, int32_t is_complex, int64_t k,
   hipLaunchKernelGGL(k_bsys_reset, dim3((k + 255) / 256), dim3(256), 0, b->stream, (int)k,
                      bt->d_sys, nullptr, 1, 1);
   OHM_HIP(hipStreamSynchronize(b->stream));
+  if (!tables.empty()) {
+    ohm_status ts = batch_tune_blocks(bt, tables);
+    if (ts != OHM_OK) return bail(ts);
+  }
   *out = bt;
   return OHM_OK;
 }

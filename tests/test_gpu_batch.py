@@ -108,6 +108,49 @@ def test_batch_uneven_real_vs_lu(be, sizes):
         assert st.residual_norm < 1e-12
 
 
+@pytest.mark.parametrize("complex_", [False, True], ids=["real", "complex"])
+def test_batch_tuned_row_blocks_vs_lu(be, complex_):
+    """Batches of 65,536 rows or more pick their row-block shape by timing
+    the batched SpMV over 64-, 128- and 256-row blocks at creation; the
+    solutions match each system's LU and the untuned batch's (which may add
+    a system's block partials in another order) to rounding."""
+    rng = np.random.default_rng(21)
+    if complex_:
+        n = 6000
+        a = dd_complex(n, 3, rng)
+        rows = np.repeat(np.arange(n), np.diff(a.row_pointers.astype(np.int64)))
+        g = a.values.real.copy()
+        g[rows == a.col_indices] *= 4.0
+        c = np.abs(a.values.imag) * 1e-10
+        omegas = 2 * np.pi * np.logspace(0, 9, 12)
+        systems = [(a.row_pointers, a.col_indices, g, c, 1j * w) for w in omegas]
+        b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        rhs = np.tile(b, len(omegas))
+        refs = [oracle.sparse_lu_solve(ir.CsrMatrix(n, n, g + 1j * w * c, a.col_indices,
+                                                    a.row_pointers), b) for w in omegas]
+    else:
+        mnas = [ladder_mna(s, 30 + i) for i, s in enumerate((40000, 30000))]
+        systems = [(m.g.row_pointers, m.g.col_indices, m.g.values, None, 0.0) for m in mnas]
+        rhs = np.concatenate([m.b_dc for m in mnas])
+        refs = [oracle.sparse_lu_solve(m.g, m.b_dc) for m in mnas]
+    prm = solver.bicgstab_params(tolerance=1e-13, max_iterations=200000)
+    out = []
+    for tune in (True, False):
+        be.set_autotune(tune)
+        try:
+            bt = solver.Batch(be, systems, complex_=complex_)
+            assert bt.rows >= 65536
+            x = bt.solve(rhs, params=prm)
+        finally:
+            be.set_autotune(True)
+        for xi, ref, st in zip(bt.split(x), refs, bt.last_results):
+            ok, err = close(xi, ref)
+            assert ok and st.converged, (tune, err, st)
+        out.append(x)
+    scale = np.abs(out[1]).max()
+    assert np.allclose(out[0], out[1], rtol=1e-8, atol=1e-12 * scale)
+
+
 def test_batch_monte_carlo_shared_pattern(be):
     """Monte Carlo: one pattern, per-instance values (RC companion A = G + C/h)."""
     base = solver.compile(ir.rc_grid(24, 24, 8, seed=3))
