@@ -439,19 +439,14 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
     __syncthreads();
     part_row_load(q, s, m, rev, 1 + t0 + kBandTile + lane, rg, rc, rb);  // next tile in flight
     const int64_t t1 = t0 + kBandTile < m - 1 ? t0 + kBandTile : m - 1;
-#pragma unroll 2
-    for (int64_t j = t0; j < t1; j++) {
-      if (ck && (j & (kCk - 1)) == 0) {
-        cz *o = ck + (size_t)(j / kCk) * 4 * 64;
-        o[0] = r0;
-        o[64] = r1;
-        o[128] = y0;
-        o[192] = y1;
-      }
-      const int sl = (int)(j - t0);
+    // one elimination step on direction row j + 1 (tile slot sl); LAST: the
+    // part's last row, whose column-m entry is the end coupling
+    auto step = [&](int sl, auto last) {
       const cz n0 = {t.g[sl][0], om * t.c[sl][0]}, n1 = {t.g[sl][1], om * t.c[sl][1]};
-      cz n2 = {t.g[sl][2], om * t.c[sl][2]}, f = {t.b[sl][0], t.b[sl][1]}, e2 = {0.0, 0.0};
-      if (j + 2 >= m) {  // row m-1: its column-m entry is the end coupling
+      cz n2 = {t.g[sl][2], om * t.c[sl][2]};
+      const cz f = {t.b[sl][0], t.b[sl][1]};
+      cz e2 = {0.0, 0.0};
+      if (decltype(last)::value) {
         e2 = n2;
         n2 = cz{0.0, 0.0};
       }
@@ -466,7 +461,30 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
       y1 = c1 * y1;
       // y2 (the end coupling's column) is zero until the last row of a part
       // of m > 1 rows brings e2, so it is formed there alone
-      if (j + 2 >= m) y2 = c1 * y2 + c2 * e2;
+      if (decltype(last)::value) y2 = c1 * y2 + c2 * e2;
+    };
+    auto checkpoint = [&](int64_t j) {  // state before step j (j a multiple of kCk)
+      if (ck) {
+        cz *o = ck + (size_t)(j / kCk) * 4 * 64;
+        o[0] = r0;
+        o[64] = r1;
+        o[128] = y0;
+        o[192] = y1;
+      }
+    };
+    int64_t j = t0;
+    // groups of kCk steps that stop short of the last row: no per-step tests
+    for (; j + kCk <= t1 && j + kCk < m - 1; j += kCk) {
+      checkpoint(j);
+#pragma unroll
+      for (int k = 0; k < kCk; k++) step((int)(j - t0) + k, std::false_type());
+    }
+    for (; j < t1; j++) {
+      if ((j & (kCk - 1)) == 0) checkpoint(j);
+      if (j + 2 >= m)
+        step((int)(j - t0), std::true_type());
+      else
+        step((int)(j - t0), std::false_type());
     }
   }
   lu = r0;
