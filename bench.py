@@ -45,6 +45,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64
 
 # HBM traffic of one k_spmv<double> launch on the mesh matrix, from the two
 # rocprofv3 counter passes over `bench.py --roofline-only` (FETCH_SIZE, then
@@ -116,6 +117,21 @@ def ac_bytes(sizes, n_freqs: int, rows_per_part: int):
     return {"part_solve": rows * 16.0 * n_freqs + ck * 64.0 * n_freqs + rows * band * waves,
             "relations": rows * band * 2 * waves + ck * 64.0 * n_freqs,
             "residual": parts * n_freqs * (6 * 16.0 + 16.0) + parts * 2 * band * waves}
+
+
+def ac_flops(sizes, n_freqs: int, rows_per_part: int):
+    """FP64 flops of one sweep's two main kernels (an FMA counted as 2, a
+    complex multiply as 6). The part solve, per (system, row): the forward
+    step recomputed from the checkpoints (omega c products 3, the complex
+    reciprocal of the carried pivot 14, the pivot test 3, the stored pivot row
+    12, the scale 5, the carried row's update 34: 71) and the back
+    substitution with the row's residual (43): 114. The relations, per
+    (system, row) and direction: the scale 5, omega c 3 and the carried row
+    with its two right-hand-side columns (40): 48."""
+    rows = float(sum(sizes)) * n_freqs
+    if not rows_per_part:
+        return {"part_solve": 0.0, "relations": 0.0, "residual": 0.0}
+    return {"part_solve": 114.0 * rows, "relations": 2 * 48.0 * rows, "residual": 0.0}
 
 
 def check_all_systems(plan, mnas, freqs, threads: int):
@@ -200,6 +216,7 @@ def ac_headline(args, be, N, ir, solver):
     failed = int((~res.converged).sum())
     ms = {k: v / args.steps for k, v in acc.items()}  # average over the timed sweeps
     by = ac_bytes(sizes, nf, st["rows_per_part"])
+    fl = ac_flops(sizes, nf, st["rows_per_part"])
     kernel = {"part_solve": "k_part_solve" if st["rows_per_part"] else "k_band_lu<1>",
               "relations": "k_part_rel", "residual": "k_part_resid_sum"}
     dom = max(("part_solve", "relations", "residual"), key=lambda k: ms[k])
@@ -235,7 +252,13 @@ def ac_headline(args, be, N, ir, solver):
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                      "copy-calibrated)" if AC_TRAFFIC_BYTES else None,
                      "bytes_per_launch": by[dom], "avg_launch_ms": ms[dom],
-                     "bytes_per_launch_all": by},
+                     "bytes_per_launch_all": by,
+                     # the kernel is limited by its FP64 dependency chain: its
+                     # flop rate against the 78.6 TFLOP/s FP64 peak
+                     "fp64": {"flops_per_launch": fl[dom],
+                              "achieved_tflops": fl[dom] / (ms[dom] * 1e-3) / 1e12,
+                              "peak_tflops": FP64_PEAK_TFLOPS,
+                              "frac": fl[dom] / (ms[dom] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}},
         "parity": parity, "cpu_baseline": cpu,
     }
     return out, mnas, freqs
