@@ -659,13 +659,12 @@ __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__
 // pivot rows and reading them back took 80 B. The part's interior rows
 // s+1 .. s+m-2 use only x of this part, so their residual terms are summed
 // here, to pres[(ip * 64 + lane) * 2 + {0, 1}].
-__global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ waves,
-                                                      const PartRef *__restrict__ parts,
-                                                      const cz *__restrict__ bnd,
-                                                      const cz *__restrict__ ckpt,
-                                                      cz *__restrict__ X,
-                                                      int *__restrict__ singular,
-                                                      double *__restrict__ pres) {
+// At 128 VGPRs four waves share a SIMD; loading the next chunk's checkpoint
+// and the next tile ahead (three waves at 164 VGPRs) measured 2 % slower.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_part_solve(
+    const BandWave *__restrict__ waves, const PartRef *__restrict__ parts,
+    const cz *__restrict__ bnd, const cz *__restrict__ ckpt, cz *__restrict__ X,
+    int *__restrict__ singular, double *__restrict__ pres) {
   constexpr int W = kPartW;
   __shared__ PartTile t;
   const int lane = threadIdx.x;
@@ -696,20 +695,11 @@ __global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ 
   } else {
     const cz *ckb = ckpt + (size_t)pr.ck * 4 * 64 + lane;
     cz xw0 = {0.0, 0.0}, xw1 = {0.0, 0.0};  // x_{s+j+1}, x_{s+j+2}
-    // the next chunk's checkpoint and the next tile's band rows are loaded
-    // one chunk / one tile ahead
-    const int64_t j_last = (ns - 1) / kCk * kCk;
-    cz pf[4] = {};  // lanes past nf load nothing (their results are not stored)
-    if (live) {
-#pragma unroll
-      for (int k = 0; k < 4; k++) pf[k] = ckb[((size_t)(j_last / kCk) * 4 + k) * 64];
-    }
-    int64_t t0 = (ns - 1) / kBandTile * kBandTile;
-    double rg[W], rc[W], rbv[2];
-    part_row_load(q, s, m, false, 1 + t0 + lane, rg, rc, rbv);
     // tiles of 64 steps, last first; step j's row s+j+1 at slot j - t0
-    for (; t0 >= 0; t0 -= kBandTile) {
+    for (int64_t t0 = (ns - 1) / kBandTile * kBandTile; t0 >= 0; t0 -= kBandTile) {
       const int64_t t1 = t0 + kBandTile < ns ? t0 + kBandTile : ns;
+      double rg[W], rc[W], rbv[2];
+      part_row_load(q, s, m, false, 1 + t0 + lane, rg, rc, rbv);
       __syncthreads();
 #pragma unroll
       for (int d = 0; d < W; d++) {
@@ -719,15 +709,15 @@ __global__ __launch_bounds__(64) void k_part_solve(const BandWave *__restrict__ 
       t.b[lane][0] = rbv[0];
       t.b[lane][1] = rbv[1];
       __syncthreads();
-      if (t0 > 0) part_row_load(q, s, m, false, 1 + t0 - kBandTile + lane, rg, rc, rbv);
       for (int64_t j0 = t0 + (t1 - 1 - t0) / kCk * kCk; j0 >= t0; j0 -= kCk) {
         const int64_t j1 = j0 + kCk < t1 ? j0 + kCk : t1;
-        cz r0 = pf[0], r1 = pf[1];
-        cz y = pf[2] - pf[3] * xs;
-        if (j0 > 0 && live) {
+        cz ck4[4] = {};  // lanes past nf load nothing (their results are not stored)
+        if (live) {
 #pragma unroll
-          for (int k = 0; k < 4; k++) pf[k] = ckb[((size_t)(j0 / kCk - 1) * 4 + k) * 64];
+          for (int k = 0; k < 4; k++) ck4[k] = ckb[((size_t)(j0 / kCk) * 4 + k) * 64];
         }
+        cz r0 = ck4[0], r1 = ck4[1];
+        cz y = ck4[2] - ck4[3] * xs;
         cz e0[kCk], e1[kCk];
 #pragma unroll
         for (int k = 0; k < kCk; k++) {

@@ -2374,6 +2374,7 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
   so.scaled_breakdown = sp.breakdown != OHM_BREAKDOWN_ABSOLUTE;
   so.tol_mode = sp.tol_mode;
   so.tol_floor = sp.tol_floor > 0 ? sp.tol_floor : 1e-15;
+  so.retry_breakdown = true;
   double cur_sigma = 0.0;
   auto solve = [&]() -> ohm_status {
     st.linear_solves += K;

@@ -1285,6 +1285,25 @@ static ohm_status bicg_solve_fused(ohm_backend *b, const DevCsr &a, const Precon
   return status_to_error(out.status, out.iters, o.max_iters);
 }
 
+static bool is_breakdown(int status) {
+  return status == ST_BRK_RHO || status == ST_BRK_RV || status == ST_BRK_TT ||
+         status == ST_BRK_OMEGA;
+}
+
+// The retry of a broken-down solve: the other shadow vector; counts add up.
+static BicgOpts retry_opts(const BicgOpts &o) {
+  BicgOpts r = o;
+  r.shadow = o.shadow == OHM_SHADOW_HASHED ? OHM_SHADOW_RHS : OHM_SHADOW_HASHED;
+  return r;
+}
+static void merge_retry(BicgOut &out, const BicgOut &r) {
+  out.iters += r.iters;
+  out.polls += r.polls;
+  out.status = r.status;
+  out.res = r.res;
+  out.eta = r.eta;
+}
+
 ohm_status bicg_solve_f64(ohm_backend *b, const DevCsr &a, const PrecondDev &p, const double *rhs,
                           double *x, const BicgOpts &o, BicgOut &out) {
   OHM_TRY(check_params(o));
@@ -1292,11 +1311,30 @@ ohm_status bicg_solve_f64(ohm_backend *b, const DevCsr &a, const PrecondDev &p, 
     out.status = ST_CONVERGED;
     return OHM_OK;
   }
-  if (fused_small(a.n)) return bicg_solve_fused(b, a, p, rhs, x, o, out);
+  const size_t xb = sizeof(double) * a.n;
+  if (fused_small(a.n)) {
+    if (!o.retry_breakdown) return bicg_solve_fused(b, a, p, rhs, x, o, out);
+    OHM_TRY(ws_get(g_ws64, a.n));  // the starting iterate, kept for a retry
+    OHM_HIP(hipMemcpyAsync(g_ws64.v.r, x, xb, hipMemcpyDeviceToDevice, b->stream));
+    ohm_status s = bicg_solve_fused(b, a, p, rhs, x, o, out);
+    if (!is_breakdown(out.status)) return s;
+    OHM_HIP(hipMemcpyAsync(x, g_ws64.v.r, xb, hipMemcpyDeviceToDevice, b->stream));
+    BicgOut r;
+    s = bicg_solve_fused(b, a, p, rhs, x, retry_opts(o), r);
+    merge_retry(out, r);
+    return s;
+  }
   OHM_TRY(ws_get(g_ws64, a.n));
-  if (o.warm) OHM_HIP(hipMemcpyAsync(g_ws64.v.x, x, sizeof(double) * a.n, hipMemcpyDeviceToDevice, b->stream));
+  if (o.warm) OHM_HIP(hipMemcpyAsync(g_ws64.v.x, x, xb, hipMemcpyDeviceToDevice, b->stream));
   OHM_TRY(bicg_core<double>(b, a, p, rhs, g_ws64, o, out));
-  OHM_HIP(hipMemcpyAsync(x, g_ws64.v.x, sizeof(double) * a.n, hipMemcpyDeviceToDevice, b->stream));
+  if (o.retry_breakdown && is_breakdown(out.status)) {
+    // x still holds the starting iterate (copied back only below)
+    if (o.warm) OHM_HIP(hipMemcpyAsync(g_ws64.v.x, x, xb, hipMemcpyDeviceToDevice, b->stream));
+    BicgOut r;
+    OHM_TRY(bicg_core<double>(b, a, p, rhs, g_ws64, retry_opts(o), r));
+    merge_retry(out, r);
+  }
+  OHM_HIP(hipMemcpyAsync(x, g_ws64.v.x, xb, hipMemcpyDeviceToDevice, b->stream));
   OHM_HIP(hipStreamSynchronize(b->stream));
   return status_to_error(out.status, out.iters, o.max_iters);
 }

@@ -47,6 +47,10 @@ void build_spmv_blocks(const uint32_t *rowptr, int n, std::vector<uint32_t> &blo
 // cache bandwidth like real ones (a 335 MB five-point matrix: 4.95 TB/s with
 // 8 per thread, 5.83 with 4, 6.04 with 5).
 int spmv_default_per(const uint32_t *rowptr, int n) {
+  if (const char *e = getenv("OHM_SPMV_PER")) {  // fixed entries per thread (1-8)
+    const int v = atoi(e);
+    if (v >= 1 && v <= 8) return spmv_per_for(v * kBlock);
+  }
   uint64_t entries = 0, rows = 0;
   for (int r = 0; r < n; r++) {
     const uint32_t c = rowptr[r + 1] - rowptr[r];

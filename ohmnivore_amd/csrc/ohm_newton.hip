@@ -807,6 +807,7 @@ ohm_status ohm_newton_run(ohm_newton_ctx *c, const ohm_newton_params *params,
   o.warm = true;
   o.tol_mode = OHM_TOL_INITIAL;  // inexact Newton: relative to ||F(x_k)||
   o.tol_floor = prm.bicgstab_floor;
+  o.retry_breakdown = true;
   NewtonWord *h = reinterpret_cast<NewtonWord *>(b->h_pinned);
   int64_t total_bicg = 0, fallbacks = 0;
   double max_diff = 0.0;

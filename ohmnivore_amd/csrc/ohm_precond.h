@@ -60,6 +60,11 @@ struct BicgOpts {
   // relative tolerance (OHM_TOL_INITIAL).
   Forcing forcing;
   bool eager = false;  // launch every chunk eagerly even where graphs are on
+  // A breakdown (rho, r_hat.v, t.t or omega) is retried once from the
+  // solve's starting iterate with the other shadow vector (hashed <-> the
+  // initial residual) before it is reported. The analyses (Newton, transient)
+  // set it; the plain solve reports breakdowns as the reference does.
+  bool retry_breakdown = false;
 };
 struct BicgOut {
   bool fp64_fallback = false;  // FP32_IR finished in FP64

@@ -144,6 +144,34 @@ def test_captured_segments_match_eager(be, precond, monkeypatch):
     assert out["eager"][3] == 120 and len(np.frombuffer(out["eager"][0])) == 90
 
 
+def test_breakdown_retried_inside_transient(be):
+    """The reference's shadow r_hat = b breaks BiCGSTAB down on a resistor
+    ladder (the plain solve reports it, as the reference does); inside the
+    transient analysis a broken-down step is solved again from its starting
+    iterate with the other shadow vector, so the run completes and matches
+    the oracle's LU stepping."""
+    ckt = ir.resistor_ladder(200, seed=1)
+    n_nodes = 201
+    for i in range(1, n_nodes + 1):  # 1 pF to ground at every node: C/h << G at h = 1 ms
+        ckt.capacitor(f"C{i}", str(i), "0", 1e-12)
+    mna = solver.compile(ckt)
+    src, _ = T.source_table(ckt, mna)
+    h = 1e-3
+    prm = solver.bicgstab_params(tolerance=1e-13, max_iterations=20000, shadow="rhs")
+    cg = T.c_on_g(mna)
+    gs = solver.GpuSolver(be, precond=N.PREC_JACOBI, params=prm)
+    a = ir.CsrMatrix(mna.size, mna.size, mna.g.values + cg / h, mna.g.col_indices,
+                     mna.g.row_pointers)
+    with pytest.raises(N.OhmnivoreError, match="BiCGSTAB breakdown"):
+        gs.solve_real(a, mna.b_dc)
+    res = T.Transient(be, mna, src).run(h, 5 * h, method="be", params=prm,
+                                         precond=N.PREC_JACOBI)
+    times, waves, xf, acc, _ = oracle.transient(mna, src, method="be", tstep=h, tstop=5 * h)
+    assert res.steps_accepted == acc == 5
+    ok, err = close(res.waves[:, 0, :], waves)
+    assert ok, err
+
+
 def test_monte_carlo_instances_vs_oracle(be):
     """K instances stepped together, each with its own conductance scale and
     per-node capacitances (stable perturbations); each against its own
