@@ -55,9 +55,9 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64
 SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
 # HBM traffic of one k_part_solve launch of the AC sweep (2,048 rows per
 # part), from the same two counter passes over `bench.py --roofline-only`:
-# FETCH 11,823,809.2 KiB x 1.99948 + WRITE 39,450,830.5 KiB x 1.0.
+# FETCH 11,925,388.0 KiB x 1.99948 + WRITE 39,518,900.0 KiB x 1.0.
 # Summary: profiles/ac_sweep_256x101_counters.md.
-AC_TRAFFIC_BYTES = round((11823809.2 * 65536.0 / 32776.5 + 39450830.5) * 1024)
+AC_TRAFFIC_BYTES = round((11925388.0 * 65536.0 / 32776.5 + 39518900.0) * 1024)
 
 
 def log(msg: str) -> None:
