@@ -46,17 +46,23 @@ __device__ __forceinline__ cz operator*(cz a, cz b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 __device__ __forceinline__ double cabs2(cz a) { return a.re * a.re + a.im * a.im; }
+// a - b c with each part as two FMAs (four instructions where a product
+// formed apart and then subtracted takes six)
+__device__ __forceinline__ cz cfms(cz a, cz b, cz c) {
+  return {fma(-b.re, c.re, fma(b.im, c.im, a.re)), fma(-b.re, c.im, fma(-b.im, c.re, a.im))};
+}
 // 1 / a = conj(a) / |a|^2 with the reciprocal of |a|^2 from the hardware
 // estimate and two Newton-Raphson steps r <- r + r (1 - d r), each of which
 // squares the relative error (to within an ulp of the quotient), instead of
 // two IEEE divisions (about nine dependent instructions each).
-__device__ __forceinline__ cz cinv(cz a) {
-  const double d = cabs2(a);
+// cinv_d takes d = |a|^2 already formed.
+__device__ __forceinline__ cz cinv_d(cz a, double d) {
   double r = __builtin_amdgcn_rcp(d);
   r = fma(fma(-d, r, 1.0), r, r);
   r = fma(fma(-d, r, 1.0), r, r);
   return {a.re * r, -a.im * r};
 }
+__device__ __forceinline__ cz cinv(cz a) { return cinv_d(a, cabs2(a)); }
 
 // One wave = up to 64 frequency points of one circuit (lane l: frequency
 // f0 + l). The circuit's band rows are shared by every lane, so the wave
@@ -73,6 +79,7 @@ struct BandWave {
   int sys;                 // result slot of lane 0
   int pbase;               // partitioned solve: index of the wave's first part
   int nparts;              //   and its number of parts
+  double bnorm2;           // ||b||^2 of the circuit (the same at every frequency)
 };
 
 constexpr int kBandTile = 64;
@@ -106,14 +113,20 @@ __device__ __forceinline__ void band_row(const BandWave &q, double om, int64_t i
   bi = cz{q.b[2 * i], q.b[2 * i + 1]};
 }
 
+// |b_i - sum_d a_d x_d|^2
+template <int W>
+__device__ __forceinline__ double resid2(const cz (&a)[W], cz bi, const cz (&xc)[W]) {
+  cz acc = bi;
+#pragma unroll
+  for (int d = 0; d < W; d++) acc = cfms(acc, a[d], xc[d]);
+  return cabs2(acc);
+}
+
 // |b_i - sum_d a_d x_d|^2 and |b_i|^2 into rr, bb
 template <int W>
 __device__ __forceinline__ void resid_row(const cz (&a)[W], cz bi, const cz (&xc)[W], double &rr,
                                           double &bb) {
-  cz acc = bi;
-#pragma unroll
-  for (int d = 0; d < W; d++) acc = acc - a[d] * xc[d];
-  rr += cabs2(acc);
+  rr += resid2<W>(a, bi, xc);
   bb += cabs2(bi);
 }
 
@@ -400,7 +413,12 @@ __device__ __forceinline__ bool is_pivot_marker(cz e) { return e.re != e.re; }
 //     R' = (N_0 R - R_0 N) / max(|Re|, |Im| of R_0 and N_0),
 // which has no entry at column j. Up to a scalar of modulus between 1 and 2
 // this is the remainder partial pivoting leaves (N - (N_0/R_0) R when
-// |R_0| >= |N_0|, else R - (R_0/N_0) N), so no row exchange is needed.
+// |R_0| >= |N_0|, else R - (R_0/N_0) N), so no row exchange is needed, and
+// the carried row keeps the size partial pivoting gives it: the part solve
+// restarts from the stored states and chooses its pivots by comparing
+// |N_0| with |R_0|, which needs that size (a carried row rescaled to an
+// arbitrary size, by a power of two at each checkpoint, measured 100x
+// larger residuals on 31 of the 25,856 AC-sweep systems).
 // The right-hand sides are f, the start coupling and the end coupling (the
 // entries of direction rows 0 and m-1 outside the part); ly and lu are the
 // last remainder's right-hand sides and diagonal. With ck set, the state
@@ -452,16 +470,18 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
       }
       const double mx = fmax(fmax(fabs(r0.re), fabs(r0.im)), fmax(fabs(n0.re), fabs(n0.im)));
       if (!(mx > 0.0)) bad = 1;
-      const double sc = __builtin_amdgcn_rcp(mx);  // any positive scale will do
-      const cz c1 = {n0.re * sc, n0.im * sc}, c2 = {-r0.re * sc, -r0.im * sc};
-      const cz nr0 = c1 * r1 + c2 * n1;
-      r1 = c2 * n2;
-      r0 = nr0;
-      y0 = c1 * y0 + c2 * f;
+      // (N_0 R - R_0 N) sc = c1 R - cr N
+      const double sc = __builtin_amdgcn_rcp(mx);
+      const cz c1 = {n0.re * sc, n0.im * sc}, cr = {r0.re * sc, r0.im * sc};
+      const cz nr0 = cfms(c1 * r1, cr, n1);
+      const cz crn2 = cr * n2;
+      y0 = cfms(c1 * y0, cr, f);
       y1 = c1 * y1;
       // y2 (the end coupling's column) is zero until the last row of a part
       // of m > 1 rows brings e2, so it is formed there alone
-      if (decltype(last)::value) y2 = c1 * y2 + c2 * e2;
+      if (decltype(last)::value) y2 = cfms(c1 * y2, cr, e2);
+      r0 = nr0;
+      r1 = cz{-crn2.re, -crn2.im};
     };
     auto checkpoint = [&](int64_t j) {  // state before step j (j a multiple of kCk)
       if (ck) {
@@ -653,15 +673,21 @@ __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__
 // right-hand side; the end coupling times xe enters at the last row), and
 // each chunk's pivot rows stay in registers and are back-substituted at
 // once: row j's pivot row is (e_0, e_1), x_j = e_0 - e_1 x_{j+1}, or the
-// marker: row j+1 as given was the pivot, x_j = (f_{j+1} - a_{j+1,j+1}
-// x_{j+1} - a_{j+1,j+2} x_{j+2}) / a_{j+1,j}. Per (system, row) this reads
-// the checkpoints (64 B per kCk rows) and writes x (16 B), where storing the
-// pivot rows and reading them back took 80 B. The part's interior rows
-// s+1 .. s+m-2 use only x of this part, so their residual terms are summed
-// here, to pres[(ip * 64 + lane) * 2 + {0, 1}].
-// At 128 VGPRs four waves share a SIMD; loading the next chunk's checkpoint
-// and the next tile ahead (three waves at 164 VGPRs) measured 2 % slower.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_part_solve(
+// marker with e_1 = 1 / a_{j+1,j}: row j+1 as given was the pivot, x_j =
+// (f_{j+1} - a_{j+1,j+1} x_{j+1} - a_{j+1,j+2} x_{j+2}) e_1. Per (system,
+// row) this reads the checkpoints (64 B per kCk rows) and writes x (16 B),
+// where storing the pivot rows and reading them back took 80 B. The part's
+// interior rows s+1 .. s+m-2 use only x of this part, so their squared
+// residuals are summed here, to pres[ip * 64 + lane].
+// A step whose pivot choice is the same in every live lane (on the RLC
+// ladders about 86 % of them: every third row as given is the pivot at
+// every frequency, and the carried row at most others) eliminates
+// with the pivot's reciprocal, which the pivot row needs anyway:
+// R' = N - (n_0 / r_0) R, or R' = R - (r_0 / n_0) N, both the remainder of
+// partial pivoting; a step where lanes differ uses the scaled combination
+// of part_eliminate.
+// At most 168 VGPRs: three waves share a SIMD.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_part_solve(
     const BandWave *__restrict__ waves, const PartRef *__restrict__ parts,
     const cz *__restrict__ bnd, const cz *__restrict__ ckpt, cz *__restrict__ X,
     int *__restrict__ singular, double *__restrict__ pres) {
@@ -672,8 +698,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const PartRef pr = parts[ip];
   const BandWave q = waves[pr.item];
   const int p = ip - q.pbase;
-  const int64_t s = pr.s, m = pr.m;
+  const int64_t s = pr.s;
+  const int m = pr.m;  // loop bounds in 32 bits keep their tests on the scalar unit
   const bool live = lane < q.nf;
+  const uint64_t live_mask = __ballot(live);
   const double om = live ? q.omega[lane] : 0.0;
   const size_t rb = (size_t)2 * q.pbase;
   cz xs = {0.0, 0.0}, xe = {0.0, 0.0};
@@ -683,10 +711,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   }
   const int S = q.nf;
   cz *Xb = X + q.store_off;
-  double rr = 0.0, bb = 0.0;
+  double rr = 0.0;
   int bad = 0;
-  const int64_t ns = m - 1;  // elimination steps; step j pivots x_{s+j} against row s+j+1
-  if (ns == 0) {             // one row: a_0 xs + a_1 x + a_2 xe = b
+  const int ns = m - 1;  // elimination steps; step j pivots x_{s+j} against row s+j+1
+  if (ns == 0) {         // one row: a_0 xs + a_1 x + a_2 xe = b
     cz a[W], bi;
     band_row<W>(q, om, s, a, bi);
     const cz f = bi - a[0] * xs - a[2] * xe;
@@ -695,9 +723,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   } else {
     const cz *ckb = ckpt + (size_t)pr.ck * 4 * 64 + lane;
     cz xw0 = {0.0, 0.0}, xw1 = {0.0, 0.0};  // x_{s+j+1}, x_{s+j+2}
+    // the checkpoint of the next chunk (chunks run last first) is loaded
+    // while the current one is worked; lanes past nf load nothing (their
+    // results are not stored)
+    cz ckn[4] = {};
+    if (live) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) ckn[k] = ckb[((size_t)((ns - 1) / kCk) * 4 + k) * 64];
+    }
     // tiles of 64 steps, last first; step j's row s+j+1 at slot j - t0
-    for (int64_t t0 = (ns - 1) / kBandTile * kBandTile; t0 >= 0; t0 -= kBandTile) {
-      const int64_t t1 = t0 + kBandTile < ns ? t0 + kBandTile : ns;
+    for (int t0 = (ns - 1) / kBandTile * kBandTile; t0 >= 0; t0 -= kBandTile) {
+      const int t1 = t0 + kBandTile < ns ? t0 + kBandTile : ns;
       double rg[W], rc[W], rbv[2];
       part_row_load(q, s, m, false, 1 + t0 + lane, rg, rc, rbv);
       __syncthreads();
@@ -709,39 +745,66 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       t.b[lane][0] = rbv[0];
       t.b[lane][1] = rbv[1];
       __syncthreads();
-      for (int64_t j0 = t0 + (t1 - 1 - t0) / kCk * kCk; j0 >= t0; j0 -= kCk) {
-        const int64_t j1 = j0 + kCk < t1 ? j0 + kCk : t1;
-        cz ck4[4] = {};  // lanes past nf load nothing (their results are not stored)
-        if (live) {
+      for (int j0 = t0 + (t1 - 1 - t0) / kCk * kCk; j0 >= t0; j0 -= kCk) {
+        const int j1 = j0 + kCk < t1 ? j0 + kCk : t1;
+        cz ck4[4];
 #pragma unroll
-          for (int k = 0; k < 4; k++) ck4[k] = ckb[((size_t)(j0 / kCk) * 4 + k) * 64];
+        for (int k = 0; k < 4; k++) ck4[k] = ckn[k];
+        if (live && j0 >= kCk) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) ckn[k] = ckb[((size_t)(j0 / kCk - 1) * 4 + k) * 64];
         }
         cz r0 = ck4[0], r1 = ck4[1];
-        cz y = ck4[2] - ck4[3] * xs;
+        cz y = cfms(ck4[2], ck4[3], xs);
         cz e0[kCk], e1[kCk];
 #pragma unroll
         for (int k = 0; k < kCk; k++) {
-          const int64_t j = j0 + k;
+          const int j = j0 + k;
           if (j < j1) {
-            const int sl = (int)(j - t0);
+            const int sl = j - t0;
             const cz n0 = {t.g[sl][0], om * t.c[sl][0]}, n1 = {t.g[sl][1], om * t.c[sl][1]};
             cz n2 = {t.g[sl][2], om * t.c[sl][2]}, f = {t.b[sl][0], t.b[sl][1]};
             if (j + 2 >= m) {  // row m-1: its column-m entry times xe
-              f = f - n2 * xe;
+              f = cfms(f, n2, xe);
               n2 = cz{0.0, 0.0};
             }
-            const cz inv = cinv(r0);
-            e0[k] = cabs2(n0) > cabs2(r0) ? pivot_marker() : y * inv;
-            e1[k] = r1 * inv;
-            const double mx =
-                fmax(fmax(fabs(r0.re), fabs(r0.im)), fmax(fabs(n0.re), fabs(n0.im)));
-            if (!(mx > 0.0)) bad = 1;
-            const double sc = __builtin_amdgcn_rcp(mx);
-            const cz c1 = {n0.re * sc, n0.im * sc}, c2 = {-r0.re * sc, -r0.im * sc};
-            const cz nr0 = c1 * r1 + c2 * n1;
-            r1 = c2 * n2;
-            r0 = nr0;
-            y = c1 * y + c2 * f;
+            const double dn = cabs2(n0), dr = cabs2(r0);
+            const bool sw = dn > dr;  // row j+1 as given is the pivot
+            const uint64_t vote = __ballot(sw) & live_mask;
+            if (vote == 0) {  // the carried row is the pivot in every live lane
+              if (!(dr > 0.0)) bad = 1;
+              const cz inv = cinv_d(r0, dr);
+              const cz l = n0 * inv;
+              e0[k] = y * inv;
+              e1[k] = r1 * inv;
+              r0 = cfms(n1, l, r1);
+              r1 = n2;
+              y = cfms(f, l, y);
+            } else if (vote == live_mask) {  // row j+1 is the pivot in every live lane
+              const cz inv = cinv_d(n0, dn);
+              const cz l = r0 * inv;
+              e0[k] = pivot_marker();
+              e1[k] = inv;
+              r0 = cfms(r1, l, n1);
+              const cz ln2 = l * n2;
+              r1 = cz{-ln2.re, -ln2.im};
+              y = cfms(y, l, f);
+            } else {
+              const cz inv = cinv_d(sw ? n0 : r0, sw ? dn : dr);
+              e0[k] = sw ? pivot_marker() : y * inv;
+              e1[k] = sw ? inv : r1 * inv;
+              const double mx =
+                  fmax(fmax(fabs(r0.re), fabs(r0.im)), fmax(fabs(n0.re), fabs(n0.im)));
+              if (!(mx > 0.0)) bad = 1;
+              const double sc = __builtin_amdgcn_rcp(mx);
+              // (N_0 R - R_0 N) sc
+              const cz c1 = {n0.re * sc, n0.im * sc}, cr = {r0.re * sc, r0.im * sc};
+              const cz nr0 = cfms(c1 * r1, cr, n1);
+              const cz crn2 = cr * n2;
+              r1 = cz{-crn2.re, -crn2.im};
+              r0 = nr0;
+              y = cfms(c1 * y, cr, f);
+            }
           }
         }
         if (j1 == ns) {  // the last row: its pivot is the final remainder
@@ -751,24 +814,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
 #pragma unroll
         for (int k = kCk - 1; k >= 0; k--) {
-          const int64_t j = j0 + k;
+          const int j = j0 + k;
           if (j < j1) {
-            const int sl = (int)(j - t0);
+            const int sl = j - t0;
             cz a[W];  // row s+j+1 (columns s+j .. s+j+2)
 #pragma unroll
             for (int d = 0; d < W; d++) a[d] = cz{t.g[sl][d], om * t.c[sl][d]};
             const cz bj = {t.b[sl][0], t.b[sl][1]};
-            cz xj = e0[k] - e1[k] * xw0;
+            cz xj;
             if (is_pivot_marker(e0[k])) {  // row s+j+1 as given was the pivot
               const bool last = j + 2 >= m;
-              const cz f = last ? bj - a[2] * xe : bj;
+              const cz f = last ? cfms(bj, a[2], xe) : bj;
               const cz a2 = last ? cz{0.0, 0.0} : a[2];
-              xj = (f - a[1] * xw0 - a2 * xw1) * cinv(a[0]);
+              xj = cfms(cfms(f, a[1], xw0), a2, xw1) * e1[k];
+            } else {
+              xj = cfms(e0[k], e1[k], xw0);
             }
             if (live) Xb[(s + j) * S + lane] = xj;
             if (j + 1 <= m - 2) {  // interior row s+j+1
               const cz xc[W] = {xj, xw0, xw1};
-              resid_row<W>(a, bj, xc, rr, bb);
+              rr += resid2<W>(a, bj, xc);
             }
             xw1 = xw0;
             xw0 = xj;
@@ -778,14 +843,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
   }
   if (!live) return;
-  pres[((size_t)ip * 64 + lane) * 2] = rr;
-  pres[((size_t)ip * 64 + lane) * 2 + 1] = bb;
+  pres[(size_t)ip * 64 + lane] = rr;
   if (bad) singular[q.sys + lane] = 2;
 }
 
 // One thread per (wave, lane): the residual terms of every part's first and
 // last row (columns in the neighbouring parts, read from the written x) and
-// the parts' interior sums, in part order, to res[2 (sys + lane)].
+// the parts' interior sums, in part order, to res[2 (sys + lane)], beside
+// the circuit's ||b||^2.
 __global__ void k_part_resid_sum(const BandWave *__restrict__ waves, int n_waves,
                                  const PartRef *__restrict__ parts,
                                  const double *__restrict__ pres, const cz *__restrict__ X,
@@ -799,10 +864,9 @@ __global__ void k_part_resid_sum(const BandWave *__restrict__ waves, int n_waves
   const double om = q.omega[lane];
   const cz *Xb = X + q.store_off;
   const int S = q.nf;
-  double rr = 0.0, bb = 0.0;
+  double rr = 0.0;
   for (int ip = q.pbase; ip < q.pbase + q.nparts; ip++) {
-    rr += pres[((size_t)ip * 64 + lane) * 2];
-    bb += pres[((size_t)ip * 64 + lane) * 2 + 1];
+    rr += pres[(size_t)ip * 64 + lane];
     const PartRef pr = parts[ip];
     for (int e = 0; e < 2; e++) {
       if (e == 1 && pr.m == 1) break;  // one row: first and last
@@ -814,11 +878,11 @@ __global__ void k_part_resid_sum(const BandWave *__restrict__ waves, int n_waves
         const int64_t col = i - 1 + d;
         xc[d] = col >= 0 && col < q.n ? Xb[col * S + lane] : cz{0.0, 0.0};
       }
-      resid_row<W>(a, bi, xc, rr, bb);
+      rr += resid2<W>(a, bi, xc);
     }
   }
   res[2 * (q.sys + lane)] = rr;
-  res[2 * (q.sys + lane) + 1] = bb;
+  res[2 * (q.sys + lane) + 1] = q.bnorm2;
 }
 
 }  // namespace ohm
@@ -976,6 +1040,7 @@ struct ohm_ac_plan {
   std::vector<int64_t> n;
   void *d_band = nullptr;
   std::vector<const double *> gb, cb, bp;
+  std::vector<double> bnorm2;  // ||b||^2 per circuit
   std::vector<const uint32_t *> perm;
   // half bandwidth 1: per circuit, cut_score[i] rates a part boundary
   // between RCM rows i and i+1 (lower is better)
@@ -1090,6 +1155,7 @@ ohm_status plan_build(ohm_backend *b, int64_t n_circuits, const ohm_ac_circuit *
   p->gb.assign(n_circuits, nullptr);
   p->cb.assign(n_circuits, nullptr);
   p->bp.assign(n_circuits, nullptr);
+  p->bnorm2.assign(n_circuits, 0.0);
   p->perm.assign(n_circuits, nullptr);
   p->cut_score.assign(n_circuits, {});
   // band arrays in the RCM order, built on host threads 32 circuits at a
@@ -1119,6 +1185,9 @@ ohm_status plan_build(ohm_backend *b, int64_t n_circuits, const ohm_ac_circuit *
         h.bp[2 * i] = circ[c].b_ac[2 * r];
         h.bp[2 * i + 1] = circ[c].b_ac[2 * r + 1];
       }
+      double b2 = 0.0;
+      for (int64_t i = 0; i < 2 * n; i++) b2 += h.bp[i] * h.bp[i];
+      p->bnorm2[c] = b2;
       if (B == 1) p->cut_score[c] = cut_scores(h.gb, n);
     });
     for (int64_t t = 0; t < cn; t++) {
@@ -1267,6 +1336,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       w.sys = (int)(it.c * n_freqs + it.f0);
       w.pbase = (int)prefs.size();
       w.nparts = parts_of(it.c);
+      w.bnorm2 = p->bnorm2[it.c];
       for (int q = 0; q < w.nparts; q++) {
         const int mq = cuts[it.c][q + 1] - cuts[it.c][q];
         prefs.push_back({k, cuts[it.c][q], mq, (int)n_ck});
@@ -1285,7 +1355,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
                  rel_b = al(sizeof(cz) * n_ip * 6 * 64),
                  red_b = al(sizeof(cz) * n_ip * 12 * 64),
                  bnd_b = al(sizeof(cz) * n_ip * 2 * 64),
-                 pres_b = al(sizeof(double) * n_ip * 2 * 64),
+                 pres_b = al(sizeof(double) * n_ip * 64),
                  ck_b = part ? al(sizeof(cz) * std::max<int64_t>(n_ck, 1) * 4 * 64) : 0;
     // (partitioned: a second wave table for the one-lane redo after pres)
     const size_t need =
