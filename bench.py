@@ -55,9 +55,9 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64
 SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
 # HBM traffic of one k_part_solve launch of the AC sweep (2,048 rows per
 # part), from the same two counter passes over `bench.py --roofline-only`:
-# FETCH 11,925,388.0 KiB x 1.99948 + WRITE 39,518,900.0 KiB x 1.0.
+# FETCH 11,830,558.5 KiB x 1.99948 + WRITE 39,514,391.1 KiB x 1.0.
 # Summary: profiles/ac_sweep_256x101_counters.md.
-AC_TRAFFIC_BYTES = round((11925388.0 * 65536.0 / 32776.5 + 39518900.0) * 1024)
+AC_TRAFFIC_BYTES = round((11830558.53 * 65536.0 / 32776.5 + 39514391.13) * 1024)
 
 
 def log(msg: str) -> None:
@@ -121,17 +121,21 @@ def ac_bytes(sizes, n_freqs: int, rows_per_part: int):
 
 def ac_flops(sizes, n_freqs: int, rows_per_part: int):
     """FP64 flops of one sweep's two main kernels (an FMA counted as 2, a
-    complex multiply as 6). The part solve, per (system, row): the forward
-    step recomputed from the checkpoints (omega c products 3, the complex
-    reciprocal of the carried pivot 14, the pivot test 3, the stored pivot row
-    12, the scale 5, the carried row's update 34: 71) and the back
-    substitution with the row's residual (43): 114. The relations, per
-    (system, row) and direction: the scale 5, omega c 3 and the carried row
-    with its two right-hand-side columns (40): 48."""
+    complex multiply as 6, a complex a - b c as 8). The part solve, per
+    (system, row): the forward step recomputed from the checkpoints - on a
+    step whose pivot choice all lanes share (86 % on these ladders): omega c
+    products 3, the two squared pivot moduli 6, the pivot's reciprocal 11,
+    the multiplier 6, the stored pivot row 12 (or the remainder's third
+    entry 6) and the carried row's update 16: 48-54; on the others the
+    scaled two-sided combination: 71; 55 on average - and the back
+    substitution with the row's residual (omega c 3, x 8, or 22 on the rows
+    whose pivot was the next row, the residual 28): 44. 99 in all. The
+    relations, per (system, row) and direction: the scale 5, omega c 3 and
+    the carried row with its two right-hand-side columns (40): 48."""
     rows = float(sum(sizes)) * n_freqs
     if not rows_per_part:
         return {"part_solve": 0.0, "relations": 0.0, "residual": 0.0}
-    return {"part_solve": 114.0 * rows, "relations": 2 * 48.0 * rows, "residual": 0.0}
+    return {"part_solve": 99.0 * rows, "relations": 2 * 48.0 * rows, "residual": 0.0}
 
 
 def check_all_systems(plan, mnas, freqs, threads: int):

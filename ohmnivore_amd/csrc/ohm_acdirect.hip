@@ -365,8 +365,14 @@ __global__ void k_band_unpermute(const BandWave *__restrict__ waves, const cz *_
 // after all) are solved again by k_band_lu.
 constexpr int kPartW = 3;  // band width at half bandwidth 1
 
+// 64 band rows, each row's G, C and b entries together (9 doubles, the
+// ninth padding the 64 lanes' stores over the LDS banks): the 8 values of a
+// row, and of the next rows, sit at small offsets from one address.
 struct PartTile {
-  double g[kBandTile][kPartW], c[kBandTile][kPartW], b[kBandTile][2];
+  double r[kBandTile][9];
+  __device__ double &g(int sl, int d) { return r[sl][d]; }
+  __device__ double &c(int sl, int d) { return r[sl][kPartW + d]; }
+  __device__ double &b(int sl, int k) { return r[sl][2 * kPartW + k]; }
 };
 
 // one part: its wave (index into the chunk's wave table), rows [s, s+m) and
@@ -449,20 +455,20 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < W; d++) {
-      t.g[lane][d] = rg[d];
-      t.c[lane][d] = rc[d];
+      t.g(lane, d) = rg[d];
+      t.c(lane, d) = rc[d];
     }
-    t.b[lane][0] = rb[0];
-    t.b[lane][1] = rb[1];
+    t.b(lane, 0) = rb[0];
+    t.b(lane, 1) = rb[1];
     __syncthreads();
     part_row_load(q, s, m, rev, 1 + t0 + kBandTile + lane, rg, rc, rb);  // next tile in flight
     const int64_t t1 = t0 + kBandTile < m - 1 ? t0 + kBandTile : m - 1;
     // one elimination step on direction row j + 1 (tile slot sl); LAST: the
     // part's last row, whose column-m entry is the end coupling
     auto step = [&](int sl, auto last) {
-      const cz n0 = {t.g[sl][0], om * t.c[sl][0]}, n1 = {t.g[sl][1], om * t.c[sl][1]};
-      cz n2 = {t.g[sl][2], om * t.c[sl][2]};
-      const cz f = {t.b[sl][0], t.b[sl][1]};
+      const cz n0 = {t.g(sl, 0), om * t.c(sl, 0)}, n1 = {t.g(sl, 1), om * t.c(sl, 1)};
+      cz n2 = {t.g(sl, 2), om * t.c(sl, 2)};
+      const cz f = {t.b(sl, 0), t.b(sl, 1)};
       cz e2 = {0.0, 0.0};
       if (decltype(last)::value) {
         e2 = n2;
@@ -739,11 +745,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       __syncthreads();
 #pragma unroll
       for (int d = 0; d < W; d++) {
-        t.g[lane][d] = rg[d];
-        t.c[lane][d] = rc[d];
+        t.g(lane, d) = rg[d];
+        t.c(lane, d) = rc[d];
       }
-      t.b[lane][0] = rbv[0];
-      t.b[lane][1] = rbv[1];
+      t.b(lane, 0) = rbv[0];
+      t.b(lane, 1) = rbv[1];
       __syncthreads();
       for (int j0 = t0 + (t1 - 1 - t0) / kCk * kCk; j0 >= t0; j0 -= kCk) {
         const int j1 = j0 + kCk < t1 ? j0 + kCk : t1;
@@ -762,8 +768,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
           const int j = j0 + k;
           if (j < j1) {
             const int sl = j - t0;
-            const cz n0 = {t.g[sl][0], om * t.c[sl][0]}, n1 = {t.g[sl][1], om * t.c[sl][1]};
-            cz n2 = {t.g[sl][2], om * t.c[sl][2]}, f = {t.b[sl][0], t.b[sl][1]};
+            const cz n0 = {t.g(sl, 0), om * t.c(sl, 0)}, n1 = {t.g(sl, 1), om * t.c(sl, 1)};
+            cz n2 = {t.g(sl, 2), om * t.c(sl, 2)}, f = {t.b(sl, 0), t.b(sl, 1)};
             if (j + 2 >= m) {  // row m-1: its column-m entry times xe
               f = cfms(f, n2, xe);
               n2 = cz{0.0, 0.0};
@@ -819,8 +825,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             const int sl = j - t0;
             cz a[W];  // row s+j+1 (columns s+j .. s+j+2)
 #pragma unroll
-            for (int d = 0; d < W; d++) a[d] = cz{t.g[sl][d], om * t.c[sl][d]};
-            const cz bj = {t.b[sl][0], t.b[sl][1]};
+            for (int d = 0; d < W; d++) a[d] = cz{t.g(sl, d), om * t.c(sl, d)};
+            const cz bj = {t.b(sl, 0), t.b(sl, 1)};
             cz xj;
             if (is_pivot_marker(e0[k])) {  // row s+j+1 as given was the pivot
               const bool last = j + 2 >= m;
