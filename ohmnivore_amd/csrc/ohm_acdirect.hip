@@ -1506,7 +1506,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
 }
 
 // rows per part: OHM_AC_PART_ROWS, else the autotuned value of this plan
-// (the fastest of the candidates, each timed over one whole sweep on the
+// (the fastest of the candidates, each timed over two whole sweeps on the
 // plan's first solve), else 2048
 ohm_status plan_solve(ohm_ac_plan *p, int64_t n_freqs, const double *freqs, double *const *x,
                       ohm_batch_result *results, double *device_ms) {
@@ -1544,14 +1544,20 @@ ohm_status plan_solve(ohm_ac_plan *p, int64_t n_freqs, const double *freqs, doub
   } else if (p->tuned_rows >= 0) {
     R = p->tuned_rows;
   } else if (b->autotune && p->B == 1) {
-    const int64_t cand[] = {0, 1024, 2048, 4096, 8192};
+    // each candidate timed twice (the faster of the two counts), so that a
+    // sweep slowed by its first use of a workspace size does not decide
+    const int64_t cand[] = {0, 1024, 2048, 3072, 4096, 8192};
     double best = 1e300;
     for (int64_t c : cand) {
-      OHM_TRY(plan_solve_once(p, n_freqs, nullptr, c));
-      double t = 0.0;
-      for (double v : p->phase_ms) t += v;
-      if (t < best) {
-        best = t;
+      double tc = 1e300;
+      for (int rep = 0; rep < 2; rep++) {
+        OHM_TRY(plan_solve_once(p, n_freqs, nullptr, c));
+        double t = 0.0;
+        for (double v : p->phase_ms) t += v;
+        tc = std::min(tc, t);
+      }
+      if (tc < best) {
+        best = tc;
         R = c;
       }
     }
