@@ -1134,6 +1134,11 @@ struct BatchOpts {
   int tol_mode = OHM_TOL_RHS;
   double tol_floor = 1e-15;
   int precond = OHM_PREC_JACOBI;
+  // FP64 solves: a copy of every system's starting x (warm; elements of the
+  // solve's scalar type). A system whose solve breaks down is solved once
+  // more from it with the other shadow vector (the analyses' recovery of
+  // the single-system solver).
+  const void *x_start = nullptr;
 };
 
 // Solves the systems flagged in `run` on workspace w (rhs in w.b, x0 in w.x
@@ -1347,9 +1352,30 @@ ohm_status batch_solve_t(ohm_batch *bt, const BatchOpts &o, int precision, doubl
     OHM_TRY(core_solve<S64>(bt, w, run, o));
     OHM_HIP(hipMemcpyAsync(bt->h_sys, bt->d_sys, sizeof(BSys) * K, hipMemcpyDeviceToHost, stream));
     OHM_HIP(hipStreamSynchronize(stream));
+    bool retry = false;
     for (int k = 0; k < K; k++) {
       iters[k] = bt->h_sys[k].iters;
       status[k] = bt->h_sys[k].status;
+      const int s = status[k];
+      run[k] = s == ST_BRK_RHO || s == ST_BRK_RV || s == ST_BRK_TT || s == ST_BRK_OMEGA;
+      retry |= run[k] != 0;
+    }
+    if (retry && o.warm && o.x_start) {
+      for (int k = 0; k < K; k++)
+        if (run[k] && bt->h_geo[k].n > 0)
+          OHM_HIP(hipMemcpyAsync(w.x + bt->h_geo[k].off,
+                                 static_cast<const S64 *>(o.x_start) + bt->h_geo[k].off,
+                                 sizeof(S64) * bt->h_geo[k].n, hipMemcpyDeviceToDevice, stream));
+      BatchOpts o2 = o;
+      o2.shadow = o.shadow == OHM_SHADOW_HASHED ? OHM_SHADOW_RHS : OHM_SHADOW_HASHED;
+      OHM_TRY(core_solve<S64>(bt, w, run, o2));
+      OHM_HIP(hipMemcpyAsync(bt->h_sys, bt->d_sys, sizeof(BSys) * K, hipMemcpyDeviceToHost, stream));
+      OHM_HIP(hipStreamSynchronize(stream));
+      for (int k = 0; k < K; k++)
+        if (run[k]) {
+          iters[k] += bt->h_sys[k].iters;
+          status[k] = bt->h_sys[k].status;
+        }
     }
   } else {
     // FP32 inner solves inside FP64 iterative refinement, per system
@@ -2021,6 +2047,28 @@ __global__ __launch_bounds__(kBlock) void k_tran_rhs(const BBlock *__restrict__ 
   }
 }
 
+// Start of a fixed BE step's solve, extrapolated from the last two
+// solutions: x <- x_n + ratio (x_n - x_{n-1}) with ratio = h_next / h_prev
+// (0 on the first step: the plain warm start from x_n), then x_{n-1} <- x_n.
+// Launched after k_tran_rhs, which reads x_n. start (if set) receives a copy
+// of the new starting x for a second attempt of a broken-down batched
+// solve. In a captured step (seg set) it does nothing once the segment has
+// stopped.
+__global__ void k_tran_predict(int64_t n, double ratio, double *__restrict__ x,
+                               double *__restrict__ prev, double *__restrict__ start,
+                               const SegWord *seg) {
+  if (seg && seg->stop) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double xn = x[i];
+    const double d = ratio != 0.0 ? xn - prev[i] : 0.0;  // prev unset before the first step
+    prev[i] = xn;
+    const double x0 = fma(ratio, d, xn);
+    x[i] = x0;
+    if (start) start[i] = x0;
+  }
+}
+
 __global__ void k_tran_probe(int K, int np, const uint32_t *__restrict__ probes,
                              const BGeo *__restrict__ geo, const double *__restrict__ x,
                              double *__restrict__ out) {
@@ -2102,7 +2150,9 @@ struct ohm_transient {
   int probes_cap = 0;
   double *d_rec = nullptr;      // recorded probe values
   int64_t rec_cap = 0;
-  double *d_save = nullptr, *d_trap = nullptr;  // adaptive mode: x_n and x_trap
+  // adaptive mode: x_n and x_trap; fixed BE: d_save holds x_{n-1} for the
+  // extrapolated start of the next solve
+  double *d_save = nullptr, *d_trap = nullptr;
   unsigned long long *d_max = nullptr;
   // one instance: the companion matrix as a CSR of its own, solved by the
   // single-system BiCGSTAB (ISAI / Jacobi, hipGraph chunks, FP32-IR); its
@@ -2409,6 +2459,13 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
   // OHM_TRAN_SEGMENT_ITERS pins the iterations captured per step (rounded
   // up to a multiple of 8) instead of following the steps' own counts
   if (const char *e = getenv("OHM_TRAN_SEGMENT_ITERS")) seg_fixed = (std::max(0, atoi(e)) + 7) / 8 * 8;
+  // Fixed BE steps start their solve from x_n extrapolated along the last
+  // step (k_tran_predict; OHM_TRAN_PREDICT=0: from x_n, as the adaptive
+  // stepping does)
+  bool predict = prm->method == OHM_TRAN_FIXED_BE;
+  if (const char *e = getenv("OHM_TRAN_PREDICT")) predict = predict && atoi(e) != 0;
+  if (predict && !single) o.x_start = tr->d_trap;  // written by k_tran_predict
+  const int pred_grid = (int)std::min<int64_t>((N + kBlock - 1) / kBlock, 2048);
   const bool seg_on = prm->method == OHM_TRAN_FIXED_BE && single && tr->n > 64 &&
                       sp.precision == OHM_PRECISION_FP64 && b->use_graphs && seg_steps > 0 &&
                       so.check_interval <= 0;
@@ -2456,6 +2513,9 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
                            bt->d_full, bt->d_src, wx, rhs, tr->d_srp, tr->d_sent, tr->d_sslot,
                            tr->d_src, tr->d_pwl, TRhsArgs{0.0, 0.0, 1.0 / h, 0},
                            (const SegWord *)tr->d_seg, h);
+        if (predict)  // full steps of one h: ratio 1
+          hipLaunchKernelGGL(k_tran_predict, dim3(pred_grid), dim3(kBlock), 0, stream, N, 1.0, wx,
+                             tr->d_save, (double *)nullptr, (const SegWord *)tr->d_seg);
         rc = captured_enqueue(b, tr->cs, tr->A1, tr->P1, rhs, so, seg_iters, &tr->d_seg->stop);
         hipLaunchKernelGGL(k_seg_close, dim3(1), dim3(64), 0, stream, tr->d_seg,
                            captured_state(tr->cs), h, tstart, (int)std::max<int64_t>(max_records, 0),
@@ -2519,6 +2579,7 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
   };
   if (prm->method == OHM_TRAN_FIXED_BE) {
     // backward Euler at fixed h (the stepping rule of :84-93 without LTE control)
+    double h_prev = 0.0;
     while (t < tstop) {
       if (t + h > tstop) h = tstop - t;
       if (h < h_min * 0.5) break;
@@ -2530,6 +2591,14 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
       OHM_TRY(tran_set_sigma(tr, 1.0 / h));
       cur_sigma = 1.0 / h;
       OHM_TRY(tran_rhs(tr, t_next, t, 1.0 / h, false));
+      if (predict) {
+        hipLaunchKernelGGL(k_tran_predict, dim3(pred_grid), dim3(kBlock), 0, stream, N,
+                           h_prev > 0.0 ? h / h_prev : 0.0, wx, tr->d_save,
+                           single ? (double *)nullptr : tr->d_trap, (const SegWord *)nullptr);
+        OHM_HIP(hipGetLastError());
+        b->dispatches++;
+      }
+      h_prev = h;
       const int64_t it0 = st.bicgstab_iterations;
       rc = solve();
       if (rc != OHM_OK) break;

@@ -172,6 +172,73 @@ def test_breakdown_retried_inside_transient(be):
     assert ok, err
 
 
+def test_breakdown_retried_inside_batched_transient(be, monkeypatch):
+    """The same ladder as three instances stepped together (scaled G and C):
+    with the reference's shadow r_hat = b each batched step breaks down, and
+    without a saved start (OHM_TRAN_PREDICT=0) the run reports it; by default
+    every fixed BE step keeps a copy of its extrapolated start, a broken-down
+    system is solved again from it with the other shadow vector, and each
+    instance matches the oracle's LU stepping."""
+    ckt = ir.resistor_ladder(200, seed=1)
+    for i in range(1, 202):
+        ckt.capacitor(f"C{i}", str(i), "0", 1e-12)
+    mna = solver.compile(ckt)
+    src, _ = T.source_table(ckt, mna)
+    cg = T.c_on_g(mna)
+    h = 1e-3
+    prm = solver.bicgstab_params(tolerance=1e-13, max_iterations=20000, shadow="rhs")
+    inst, mnas = [], []
+    for sc in (1.0, 1.5, 0.7):
+        inst.append((mna.g.values * sc, cg * sc))
+        mnas.append(ir.MnaSystem(g=ir.CsrMatrix(mna.size, mna.size, mna.g.values * sc,
+                                                mna.g.col_indices, mna.g.row_pointers),
+                                 c=ir.CsrMatrix(mna.size, mna.size, cg * sc, mna.g.col_indices,
+                                                mna.g.row_pointers),
+                                 b_dc=mna.b_dc, b_ac=mna.b_ac, size=mna.size,
+                                 node_names=mna.node_names, branch_names=mna.branch_names,
+                                 diode_descriptors=None, bjt_descriptors=None,
+                                 mosfet_descriptors=None, n_nodes=mna.n_nodes))
+    probes = np.arange(0, mna.size, 5)
+    monkeypatch.setenv("OHM_TRAN_PREDICT", "0")
+    with pytest.raises(N.OhmnivoreError, match="BiCGSTAB breakdown"):
+        T.Transient(be, mna, src, instances=inst).run(h, 5 * h, method="be", params=prm,
+                                                       probes=probes, precond=N.PREC_JACOBI)
+    monkeypatch.delenv("OHM_TRAN_PREDICT")
+    res = T.Transient(be, mna, src, instances=inst).run(h, 5 * h, method="be", params=prm,
+                                                         probes=probes, precond=N.PREC_JACOBI)
+    assert res.steps_accepted == 5
+    for k, m in enumerate(mnas):
+        _, waves, _, acc, _ = oracle.transient(m, src, method="be", tstep=h, tstop=5 * h,
+                                               probes=probes)
+        assert acc == 5
+        ok, err = close(res.waves[:, k, :], waves)
+        assert ok, (k, err)
+
+
+def test_predicted_start_matches_plain_warm_start(be, monkeypatch):
+    """Fixed BE steps start their solve from x_n extrapolated along the last
+    step; with OHM_TRAN_PREDICT=0 they start from x_n. On a 32 x 32 RC grid
+    both runs stay within the parity bar of the oracle's LU stepping, and the
+    extrapolated start needs fewer BiCGSTAB iterations."""
+    ckt = ir.rc_grid(32, 32, 40, seed=3)
+    mna = solver.compile(ckt)
+    src, _ = T.source_table(ckt, mna)
+    x0 = oracle.sparse_lu_solve(mna.g, mna.b_dc)
+    probes = np.arange(0, mna.size, 11)
+    prm = solver.bicgstab_params(tolerance=1e-10, max_iterations=20000)
+    _, ref, _, _, _ = oracle.transient(mna, src, method="be", tstep=1e-12, tstop=200e-12, x0=x0,
+                                       probes=probes)
+    its = {}
+    for pred in ("0", "1"):
+        monkeypatch.setenv("OHM_TRAN_PREDICT", pred)
+        res = T.Transient(be, mna, src).run(1e-12, 200e-12, method="be", x0=x0, probes=probes,
+                                             params=prm, precond=N.PREC_ISAI)
+        ok, err = close(res.waves[:, 0, :], ref)
+        assert ok, (pred, err)
+        its[pred] = res.bicgstab_iterations
+    assert its["1"] < its["0"], its
+
+
 def test_monte_carlo_instances_vs_oracle(be):
     """K instances stepped together, each with its own conductance scale and
     per-node capacitances (stable perturbations); each against its own
