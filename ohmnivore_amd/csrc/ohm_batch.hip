@@ -2047,23 +2047,26 @@ __global__ __launch_bounds__(kBlock) void k_tran_rhs(const BBlock *__restrict__ 
   }
 }
 
-// Start of a fixed BE step's solve, extrapolated from the last two
-// solutions: x <- x_n + ratio (x_n - x_{n-1}) with ratio = h_next / h_prev
-// (0 on the first step: the plain warm start from x_n), then x_{n-1} <- x_n.
-// Launched after k_tran_rhs, which reads x_n. start (if set) receives a copy
-// of the new starting x for a second attempt of a broken-down batched
-// solve. In a captured step (seg set) it does nothing once the segment has
-// stopped.
-__global__ void k_tran_predict(int64_t n, double ratio, double *__restrict__ x,
-                               double *__restrict__ prev, double *__restrict__ start,
-                               const SegWord *seg) {
+// Start of a fixed BE step's solve, extrapolated from the last solutions:
+// x <- x_n + ca (x_n - x_{n-1}) + cb (x_{n-1} - x_{n-2}) (ca = cb = 0 on the
+// first step: the plain warm start from x_n), then x_{n-2} <- x_{n-1},
+// x_{n-1} <- x_n (p2 may be null: linear extrapolation, cb = 0). Launched
+// after k_tran_rhs, which reads x_n. start (if set) receives a copy of the
+// new starting x for a second attempt of a broken-down batched solve. In a
+// captured step (seg set) it does nothing once the segment has stopped.
+__global__ void k_tran_predict(int64_t n, double ca, double cb, double *__restrict__ x,
+                               double *__restrict__ p1, double *__restrict__ p2,
+                               double *__restrict__ start, const SegWord *seg) {
   if (seg && seg->stop) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double xn = x[i];
-    const double d = ratio != 0.0 ? xn - prev[i] : 0.0;  // prev unset before the first step
-    prev[i] = xn;
-    const double x0 = fma(ratio, d, xn);
+    // history not yet written is never read (its coefficient is 0)
+    const double x1 = ca != 0.0 || cb != 0.0 ? p1[i] : xn;
+    const double x2 = cb != 0.0 ? p2[i] : x1;
+    if (p2) p2[i] = x1;
+    p1[i] = xn;
+    const double x0 = fma(cb, x1 - x2, fma(ca, xn - x1, xn));
     x[i] = x0;
     if (start) start[i] = x0;
   }
@@ -2153,6 +2156,7 @@ struct ohm_transient {
   // adaptive mode: x_n and x_trap; fixed BE: d_save holds x_{n-1} for the
   // extrapolated start of the next solve
   double *d_save = nullptr, *d_trap = nullptr;
+  double *d_prev2 = nullptr;  // fixed BE: x_{n-2} (second-order extrapolation)
   unsigned long long *d_max = nullptr;
   // one instance: the companion matrix as a CSR of its own, solved by the
   // single-system BiCGSTAB (ISAI / Jacobi, hipGraph chunks, FP32-IR); its
@@ -2313,6 +2317,7 @@ ohm_status ohm_transient_create(ohm_backend *b, int64_t k, const ohm_batch_syste
     return bail(s);
   const size_t nb = sizeof(double) * std::max<int64_t>(tr->bt->N, 1);
   if (hipMalloc(&tr->d_save, nb) != hipSuccess || hipMalloc(&tr->d_trap, nb) != hipSuccess ||
+      hipMalloc(&tr->d_prev2, nb) != hipSuccess ||
       hipMalloc(&tr->d_max, sizeof(unsigned long long)) != hipSuccess)
     return bail(fail(OHM_ERR_SOLVE, "Solve error: out of device memory for the transient"));
   *out = tr;
@@ -2323,7 +2328,7 @@ void ohm_transient_free(ohm_transient *tr) {
   if (!tr) return;
   if (tr->bt && backend_alive(tr->owner)) {
     void *ds[] = {tr->d_srp, tr->d_sent, tr->d_sslot, tr->d_src, tr->d_pwl, tr->d_probes,
-                  tr->d_rec, tr->d_save, tr->d_trap, tr->d_max};
+                  tr->d_rec, tr->d_save, tr->d_trap, tr->d_prev2, tr->d_max};
     (void)hipStreamSynchronize(tr->owner->stream);
     for (void *p : ds)
       if (p) (void)hipFree(p);
@@ -2459,11 +2464,12 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
   // OHM_TRAN_SEGMENT_ITERS pins the iterations captured per step (rounded
   // up to a multiple of 8) instead of following the steps' own counts
   if (const char *e = getenv("OHM_TRAN_SEGMENT_ITERS")) seg_fixed = (std::max(0, atoi(e)) + 7) / 8 * 8;
-  // Fixed BE steps start their solve from x_n extrapolated along the last
-  // step (k_tran_predict; OHM_TRAN_PREDICT=0: from x_n, as the adaptive
-  // stepping does)
-  bool predict = prm->method == OHM_TRAN_FIXED_BE;
-  if (const char *e = getenv("OHM_TRAN_PREDICT")) predict = predict && atoi(e) != 0;
+  // Fixed BE steps start their solve from the quadratic through the last
+  // three solutions, extrapolated to t_n + h (k_tran_predict;
+  // OHM_TRAN_PREDICT=1: linear, 0: from x_n, as the adaptive stepping does)
+  int order = 2;  // extrapolation order
+  if (const char *e = getenv("OHM_TRAN_PREDICT")) order = std::max(0, std::min(2, atoi(e)));
+  const bool predict = prm->method == OHM_TRAN_FIXED_BE && order > 0;
   if (predict && !single) o.x_start = tr->d_trap;  // written by k_tran_predict
   const int pred_grid = (int)std::min<int64_t>((N + kBlock - 1) / kBlock, 2048);
   const bool seg_on = prm->method == OHM_TRAN_FIXED_BE && single && tr->n > 64 &&
@@ -2513,9 +2519,11 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
                            bt->d_full, bt->d_src, wx, rhs, tr->d_srp, tr->d_sent, tr->d_sslot,
                            tr->d_src, tr->d_pwl, TRhsArgs{0.0, 0.0, 1.0 / h, 0},
                            (const SegWord *)tr->d_seg, h);
-        if (predict)  // full steps of one h: ratio 1
-          hipLaunchKernelGGL(k_tran_predict, dim3(pred_grid), dim3(kBlock), 0, stream, N, 1.0, wx,
-                             tr->d_save, (double *)nullptr, (const SegWord *)tr->d_seg);
+        if (predict)  // full steps of one h
+          hipLaunchKernelGGL(k_tran_predict, dim3(pred_grid), dim3(kBlock), 0, stream, N,
+                             order == 2 ? 2.0 : 1.0, order == 2 ? -1.0 : 0.0, wx, tr->d_save,
+                             order == 2 ? tr->d_prev2 : (double *)nullptr, (double *)nullptr,
+                             (const SegWord *)tr->d_seg);
         rc = captured_enqueue(b, tr->cs, tr->A1, tr->P1, rhs, so, seg_iters, &tr->d_seg->stop);
         hipLaunchKernelGGL(k_seg_close, dim3(1), dim3(64), 0, stream, tr->d_seg,
                            captured_state(tr->cs), h, tstart, (int)std::max<int64_t>(max_records, 0),
@@ -2579,11 +2587,11 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
   };
   if (prm->method == OHM_TRAN_FIXED_BE) {
     // backward Euler at fixed h (the stepping rule of :84-93 without LTE control)
-    double h_prev = 0.0;
+    double h1 = 0.0, h2 = 0.0;  // the last two steps (0: none)
     while (t < tstop) {
       if (t + h > tstop) h = tstop - t;
       if (h < h_min * 0.5) break;
-      if (seg_on && st.steps_accepted > 0 && seg_iters > 0) {
+      if (seg_on && st.steps_accepted > (order == 2 ? 1 : 0) && seg_iters > 0) {
         if (run_segment()) continue;
         if (rc != OHM_OK) break;
       }
@@ -2592,13 +2600,24 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
       cur_sigma = 1.0 / h;
       OHM_TRY(tran_rhs(tr, t_next, t, 1.0 / h, false));
       if (predict) {
-        hipLaunchKernelGGL(k_tran_predict, dim3(pred_grid), dim3(kBlock), 0, stream, N,
-                           h_prev > 0.0 ? h / h_prev : 0.0, wx, tr->d_save,
+        // Newton form of the polynomial through the last order+1 solutions,
+        // in step ratios so that equal steps give exactly 1 (linear) and
+        // 2, -1 (quadratic), the coefficients a captured segment uses
+        double ca = 0.0, cb = 0.0;
+        if (h1 > 0.0) ca = h / h1;
+        if (order == 2 && h1 > 0.0 && h2 > 0.0) {
+          const double u = h / h1, v = h2 / h1, w = u * (u + 1.0) / (1.0 + v);
+          ca = u + w;
+          cb = -w / v;
+        }
+        hipLaunchKernelGGL(k_tran_predict, dim3(pred_grid), dim3(kBlock), 0, stream, N, ca, cb, wx,
+                           tr->d_save, order == 2 ? tr->d_prev2 : (double *)nullptr,
                            single ? (double *)nullptr : tr->d_trap, (const SegWord *)nullptr);
         OHM_HIP(hipGetLastError());
         b->dispatches++;
       }
-      h_prev = h;
+      h2 = h1;
+      h1 = h;
       const int64_t it0 = st.bicgstab_iterations;
       rc = solve();
       if (rc != OHM_OK) break;

@@ -216,10 +216,11 @@ def test_breakdown_retried_inside_batched_transient(be, monkeypatch):
 
 
 def test_predicted_start_matches_plain_warm_start(be, monkeypatch):
-    """Fixed BE steps start their solve from x_n extrapolated along the last
-    step; with OHM_TRAN_PREDICT=0 they start from x_n. On a 32 x 32 RC grid
-    both runs stay within the parity bar of the oracle's LU stepping, and the
-    extrapolated start needs fewer BiCGSTAB iterations."""
+    """Fixed BE steps start their solve from the solution extrapolated from
+    the last three steps (OHM_TRAN_PREDICT=2, the default; 1: from the last
+    two; 0: from x_n). On a 32 x 32 RC grid every run stays within the parity
+    bar of the oracle's LU stepping, and each extrapolated start needs fewer
+    BiCGSTAB iterations than the plain warm start."""
     ckt = ir.rc_grid(32, 32, 40, seed=3)
     mna = solver.compile(ckt)
     src, _ = T.source_table(ckt, mna)
@@ -229,14 +230,14 @@ def test_predicted_start_matches_plain_warm_start(be, monkeypatch):
     _, ref, _, _, _ = oracle.transient(mna, src, method="be", tstep=1e-12, tstop=200e-12, x0=x0,
                                        probes=probes)
     its = {}
-    for pred in ("0", "1"):
+    for pred in ("0", "1", "2"):
         monkeypatch.setenv("OHM_TRAN_PREDICT", pred)
         res = T.Transient(be, mna, src).run(1e-12, 200e-12, method="be", x0=x0, probes=probes,
                                              params=prm, precond=N.PREC_ISAI)
         ok, err = close(res.waves[:, 0, :], ref)
         assert ok, (pred, err)
         its[pred] = res.bicgstab_iterations
-    assert its["1"] < its["0"], its
+    assert its["1"] < its["0"] and its["2"] < its["0"], its
 
 
 def test_monte_carlo_instances_vs_oracle(be):
