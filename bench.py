@@ -58,6 +58,11 @@ SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
 # FETCH 11,830,558.5 KiB x 1.99948 + WRITE 39,514,391.1 KiB x 1.0.
 # Summary: profiles/ac_sweep_256x101_counters.md.
 AC_TRAFFIC_BYTES = round((11830558.53 * 65536.0 / 32776.5 + 39514391.13) * 1024)
+# VALU issue of one k_part_solve launch, from a third counter pass over the
+# same program (SQ_INSTS_VALU; GRBM_GUI_ACTIVE, summed over the 8 XCDs):
+# 4.708e9 wave-level instructions against 1,024 SIMDs x 2.094e8 / 8 cycles,
+# one instruction per SIMD per 4 cycles. Summary: the same file.
+AC_VALU_ISSUE_FRAC = 4.708e9 / (1024 * 2.094e8 / 8 / 4)
 
 
 def log(msg: str) -> None:
@@ -257,8 +262,11 @@ def ac_headline(args, be, N, ir, solver):
                                      "copy-calibrated)" if AC_TRAFFIC_BYTES else None,
                      "bytes_per_launch": by[dom], "avg_launch_ms": ms[dom],
                      "bytes_per_launch_all": by,
-                     # the kernel is limited by its FP64 dependency chain: its
-                     # flop rate against the 78.6 TFLOP/s FP64 peak
+                     # what limits the kernel: the issue rate of its vector
+                     # instructions (counter pass), beside its FP64 flop rate
+                     # against the 78.6 TFLOP/s peak
+                     "valu_issue_frac": AC_VALU_ISSUE_FRAC if dom == "part_solve" and
+                     st["rows_per_part"] else None,
                      "fp64": {"flops_per_launch": fl[dom],
                               "achieved_tflops": fl[dom] / (ms[dom] * 1e-3) / 1e12,
                               "peak_tflops": FP64_PEAK_TFLOPS,
