@@ -1051,6 +1051,15 @@ struct ohm_ac_plan {
   // half bandwidth 1: per circuit, cut_score[i] rates a part boundary
   // between RCM rows i and i+1 (lower is better)
   std::vector<std::vector<float>> cut_score;
+  // part boundaries per circuit for rows-per-part cuts_R (kept between
+  // sweeps: computing them starts a pool of host threads)
+  int64_t cuts_R = -1;
+  std::vector<std::vector<int>> cuts;
+  // host copies of the last sweep's singular flags and residual sums, when a
+  // one-chunk sweep redid no system (they are then the final values)
+  std::vector<int> h_sing;
+  std::vector<double> h_res;
+  bool h_final = false;
   // last solve
   int64_t nf = 0;
   double *d_omega = nullptr;
@@ -1251,6 +1260,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
   const int B = p->B, W = 2 * B + 1;
   const bool part = B == 1 && R > 0;
   p->resident = false;
+  p->h_final = false;
   p->waves.clear();
   p->fallback_systems = 0;
   for (double &t : p->phase_ms) t = 0.0;
@@ -1278,11 +1288,14 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
   int chunks = 0;
   auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
   // part boundaries per circuit (the same for all of its frequencies)
-  std::vector<std::vector<int>> cuts(part ? nc : 0);
-  if (part)
+  if (part && p->cuts_R != R) {
+    p->cuts.assign(nc, {});
     parallel_for(nc, [&](int64_t c) {
-      if (p->n[c] > 0) cuts[c] = part_cuts(p->cut_score[c], p->n[c], R);
+      if (p->n[c] > 0) p->cuts[c] = part_cuts(p->cut_score[c], p->n[c], R);
     });
+    p->cuts_R = R;
+  }
+  const std::vector<std::vector<int>> &cuts = p->cuts;
   auto parts_of = [&](int64_t c) { return part ? (int)cuts[c].size() - 1 : 0; };
   while (pos < items.size()) {
     size_t end = pos;
@@ -1436,10 +1449,13 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     p->phase_ms[3] += ms[3];
     if (part) {
       // systems whose partitioned residual is not small: their waves again, one lane each
-      std::vector<int> sing(n_sys);
-      std::vector<double> res(2 * n_sys);
+      std::vector<int> &sing = p->h_sing;
+      std::vector<double> &res = p->h_res;
+      sing.resize(n_sys);
+      res.resize(2 * n_sys);
       OHM_HIP(hipMemcpy(sing.data(), p->d_sing, sizeof(int) * n_sys, hipMemcpyDeviceToHost));
       OHM_HIP(hipMemcpy(res.data(), p->d_res, sizeof(double) * 2 * n_sys, hipMemcpyDeviceToHost));
+      p->h_final = pos == 0 && end == items.size();
       std::vector<BandWave> redo;
       for (int k = 0; k < n_w; k++) {
         const BandWave &w = hw[k];
@@ -1457,6 +1473,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
         }
       }
       if (!redo.empty()) {
+        p->h_final = false;
         OHM_HIP(hipMemcpyAsync(p->d_sing, sing.data(), sizeof(int) * n_sys, hipMemcpyHostToDevice, st));
         // wave table of the redone items in its own region (the chunk's
         // table stays for get_x)
@@ -1564,10 +1581,14 @@ ohm_status plan_solve(ohm_ac_plan *p, int64_t n_freqs, const double *freqs, doub
     p->tuned_rows = R;
   }
   OHM_TRY(plan_solve_once(p, n_freqs, x, R));
-  std::vector<int> sing(n_sys, 0);
-  std::vector<double> res(2 * n_sys, 0.0);
-  OHM_HIP(hipMemcpy(sing.data(), p->d_sing, sizeof(int) * n_sys, hipMemcpyDeviceToHost));
-  OHM_HIP(hipMemcpy(res.data(), p->d_res, sizeof(double) * 2 * n_sys, hipMemcpyDeviceToHost));
+  if (!p->h_final) {
+    p->h_sing.assign(n_sys, 0);
+    p->h_res.assign(2 * n_sys, 0.0);
+    OHM_HIP(hipMemcpy(p->h_sing.data(), p->d_sing, sizeof(int) * n_sys, hipMemcpyDeviceToHost));
+    OHM_HIP(hipMemcpy(p->h_res.data(), p->d_res, sizeof(double) * 2 * n_sys, hipMemcpyDeviceToHost));
+  }
+  const std::vector<int> &sing = p->h_sing;
+  const std::vector<double> &res = p->h_res;
   ohm_status first = OHM_OK;
   std::string first_msg;
   for (int64_t c = 0; c < nc; c++) {

