@@ -2166,6 +2166,9 @@ struct ohm_transient {
   PrecondDev P1;
   int p1_kind = -1;
   double a1_sigma = NAN;
+  // iterations of the last step's FP64 solve, or of its first inner FP32
+  // solve under FP32_IR (0: none)
+  int64_t last_iters = 0;
   // graph-captured segments of fixed BE steps (one instance, FP64): the
   // solve's own workspace, the segment word and the captured segments of
   // this run, by (steps, iterations per step)
@@ -2223,10 +2226,18 @@ ohm_status tran_single_solve(ohm_transient *tr, double sig, int precond, const B
   BicgOut out;
   ohm_status s;
   if (precision == OHM_PRECISION_FP32_IR) {
+    // the last step's first inner solve sizes this one's first chunks
     int64_t refs = 0;
-    s = bicg_solve_ir(b, tr->A1, tr->P1, rhs, x, o, inner_tol, max_ref, out, refs);
+    BicgOpts oe = o;
+    oe.expect_iters = tr->last_iters;
+    s = bicg_solve_ir(b, tr->A1, tr->P1, rhs, x, oe, inner_tol, max_ref, out, refs);
+    tr->last_iters = s == OHM_OK && !out.fp64_fallback ? out.first_inner : 0;
   } else {
-    s = bicg_solve_f64(b, tr->A1, tr->P1, rhs, x, o, out);
+    // the last step's solve length sizes this one's chunks
+    BicgOpts oe = o;
+    oe.expect_iters = tr->last_iters;
+    s = bicg_solve_f64(b, tr->A1, tr->P1, rhs, x, oe, out);
+    tr->last_iters = s == OHM_OK ? out.iters : 0;
   }
   iters += out.iters;
   return s;
@@ -2409,6 +2420,7 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
     return OHM_OK;
   };
   ohm_tran_stats st{};
+  tr->last_iters = 0;
   std::vector<ohm_batch_result> res(K);
   ohm_bicgstab_params sp = prm->solver;
   sp.warm_start = 1;

@@ -1088,13 +1088,22 @@ static ohm_status bicg_drive(ohm_backend *b, const DevCsr &a, const PrecondDev &
     int chunk = o.check_interval > 0
                     ? o.check_interval
                     : (int)std::min<int64_t>(16, std::max<int64_t>(4, done / 2));
+    // with an expected length: a first chunk ending about 4 short of it,
+    // then chunks of 4 until well past it
+    const bool near = o.check_interval <= 0 && o.expect_iters > 0 &&
+                      (enqueued == 0 || done < o.expect_iters + 8);
+    if (near)
+      chunk = enqueued == 0 ? (int)std::max<int64_t>(kGraphItersShort,
+                                                     (o.expect_iters - 4) / kGraphIters * kGraphIters)
+                            : kGraphItersShort;
     chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, o.max_iters - enqueued));
     if (b->use_graphs && !o.eager) {
       // replay captured chunks; surplus iterations are no-ops once the
       // state leaves RUNNING
       const bool newton_sub = o.tol_mode == OHM_TOL_INITIAL || o.forcing.kind != 0;
-      const int gi =
-          newton_sub && enqueued == 0 && chunk < kGraphIters ? kGraphItersShort : kGraphIters;
+      const int gi = chunk < kGraphIters && (near || (newton_sub && enqueued == 0))
+                         ? kGraphItersShort
+                         : kGraphIters;
       int reps = (chunk + gi - 1) / gi;
       OHM_TRY(replay_chunk<T>(b, a, p, v, st, n, pp, gi, reps));
       chunk = reps * gi;
@@ -1132,7 +1141,9 @@ static ohm_status bicg_core(ohm_backend *b, const DevCsr &a, const PrecondDev &p
   // start only once ||r0|| is below the floor) turns every kernel of the
   // chunk into a no-op at its head. A warm start against tol * ||b|| polls
   // first, since k_init ends it whenever x already meets that tolerance.
-  return bicg_drive<T>(b, a, p, w, o, pp, o.warm && o.tol_mode != OHM_TOL_INITIAL, 0, out);
+  // With an expected length the first chunk goes out without that poll too.
+  return bicg_drive<T>(b, a, p, w, o, pp,
+                       o.warm && o.tol_mode != OHM_TOL_INITIAL && o.expect_iters <= 0, 0, out);
 }
 
 // ----------------------------------------------------- captured solves --
@@ -1579,6 +1590,7 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
     io.warm = false;
     io.tol_mode = OHM_TOL_RHS;
     io.forcing.kind = 0;
+    io.expect_iters = k == 0 ? o.expect_iters : 0;
     io.tol = std::max(inner_tol, 0.5 * target / std::max(rel, 1e-300));
     io.tol = std::min(io.tol, 0.5);
     io.max_iters = std::max<int64_t>(1, std::min<int64_t>(kInnerMaxIters, o.max_iters - total));
@@ -1586,6 +1598,7 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
     OHM_TRY(bicg_core<float>(b, a, p, g_ir_r32, g_ws32, io, iout));
     total += iout.iters;
     out.polls += iout.polls;
+    if (k == 0) out.first_inner = iout.iters;
     last_status = iout.status;
     refinements++;
     // An FP32 breakdown or iteration cap still leaves a useful correction (the
@@ -1607,6 +1620,7 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
   fo.tol_mode = OHM_TOL_RHS;
   fo.tol = target;
   fo.forcing.kind = 0;
+  fo.expect_iters = 0;
   fo.max_iters = std::max<int64_t>(1, o.max_iters - total);
   BicgOut f64;
   ohm_status s = bicg_solve_f64(b, a, p, rhs, x, fo, f64);
@@ -1625,7 +1639,9 @@ ohm_status bicg_solve_ir(ohm_backend *b, DevCsr &a, PrecondDev &p, const double 
     else
       OHM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, b->stream));
     BicgOut re;
-    s = bicg_solve_f64(b, a, p, rhs, x, o, re);
+    BicgOpts ro = o;
+    ro.expect_iters = 0;
+    s = bicg_solve_f64(b, a, p, rhs, x, ro, re);
     out.iters += re.iters;
     out.polls += re.polls;
     out.status = re.status;

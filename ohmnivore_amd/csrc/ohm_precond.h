@@ -65,6 +65,11 @@ struct BicgOpts {
   // initial residual) before it is reported. The analyses (Newton, transient)
   // set it; the plain solve reports breakdowns as the reference does.
   bool retry_breakdown = false;
+  // Iterations an earlier solve of a like system took (0: unknown). The
+  // first chunk then runs to about 4 short of it without a poll, and the
+  // chunks near it are short, so that few iterations run past the stop.
+  // Under FP32_IR it is the expected length of the first inner FP32 solve.
+  int64_t expect_iters = 0;
 };
 struct BicgOut {
   bool fp64_fallback = false;  // FP32_IR finished in FP64
@@ -74,6 +79,7 @@ struct BicgOut {
   double r0 = 0.0;   // ||b - A x0|| (the Newton residual when x0 = x_k)
   double res = 0.0;  // residual norm at exit (recurrence; FP64 true residual under IR)
   double eta = 0.0;  // forcing term the solve used (forcing on)
+  int64_t first_inner = 0;  // FP32_IR: iterations of the first inner FP32 solve
 };
 
 // FP64 solve on device vectors (rhs, x of length a.n).
