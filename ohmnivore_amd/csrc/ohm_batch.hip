@@ -1139,6 +1139,10 @@ struct BatchOpts {
   // more from it with the other shadow vector (the analyses' recovery of
   // the single-system solver).
   const void *x_start = nullptr;
+  // iterations the slowest system of an earlier like batch took (0:
+  // unknown): the first chunk runs to about 4 short of it with no poll
+  // before it, then chunks of 4 until well past it
+  int64_t expect_iters = 0;
 };
 
 // Solves the systems flagged in `run` on workspace w (rhs in w.b, x0 in w.x
@@ -1219,32 +1223,40 @@ ohm_status core_solve(ohm_batch *bt, BW<S> &w, const std::vector<char> &run, con
   b->dispatches += 2;
   std::vector<char> live = run;
   int64_t done = 0;
-  for (;;) {
-    OHM_HIP(hipMemcpyAsync(bt->h_sys, bt->d_sys, sizeof(BSys) * K, hipMemcpyDeviceToHost, stream));
-    OHM_HIP(hipStreamSynchronize(stream));
-    b->readbacks++;
-    int still = 0;
-    bool shrank = false;
-    int64_t min_iters = INT64_MAX;
-    for (int k = 0; k < K; k++) {
-      if (!live[k]) continue;
-      if (bt->h_sys[k].status != ST_RUNNING) {
-        live[k] = 0;
-        shrank = true;
-      } else {
-        still++;
-        min_iters = std::min<int64_t>(min_iters, bt->h_sys[k].iters);
+  for (bool first = true;; first = false) {
+    int chunk = 0;
+    if (first && o.expect_iters > 0 && o.check_interval <= 0) {
+      chunk = (int)std::max<int64_t>(4, (o.expect_iters - 4) / 8 * 8);
+    } else {
+      OHM_HIP(hipMemcpyAsync(bt->h_sys, bt->d_sys, sizeof(BSys) * K, hipMemcpyDeviceToHost,
+                             stream));
+      OHM_HIP(hipStreamSynchronize(stream));
+      b->readbacks++;
+      int still = 0;
+      bool shrank = false;
+      int64_t min_iters = INT64_MAX;
+      for (int k = 0; k < K; k++) {
+        if (!live[k]) continue;
+        if (bt->h_sys[k].status != ST_RUNNING) {
+          live[k] = 0;
+          shrank = true;
+        } else {
+          still++;
+          min_iters = std::min<int64_t>(min_iters, bt->h_sys[k].iters);
+        }
       }
+      if (still == 0) break;
+      if (done > o.max_iters + 64) break;  // every system has hit max_iters by now
+      if (shrank) {
+        tab = active_table(bt, live);
+        OHM_TRY(upload_tables(bt, tab));
+      }
+      chunk = o.check_interval > 0
+                  ? o.check_interval
+                  : (int)std::min<int64_t>(16, std::max<int64_t>(4, min_iters / 2));
+      if (o.check_interval <= 0 && o.expect_iters > 0 && min_iters < o.expect_iters + 8)
+        chunk = 4;
     }
-    if (still == 0) break;
-    if (done > o.max_iters + 64) break;  // every system has hit max_iters by now
-    if (shrank) {
-      tab = active_table(bt, live);
-      OHM_TRY(upload_tables(bt, tab));
-    }
-    int chunk = o.check_interval > 0
-                    ? o.check_interval
-                    : (int)std::min<int64_t>(16, std::max<int64_t>(4, min_iters / 2));
     for (int c = 0; c < chunk; c++) {
       if (pk == PK_JACOBI)
         enqueue_iter<S, PK_JACOBI>(bt, w, (int)tab.size());
@@ -2443,13 +2455,20 @@ ohm_status ohm_transient_run(ohm_transient *tr, const double *x0, const ohm_tran
   so.tol_floor = sp.tol_floor > 0 ? sp.tol_floor : 1e-15;
   so.retry_breakdown = true;
   double cur_sigma = 0.0;
+  int64_t last_batch = 0;  // iterations of the last step's slowest batched system
   auto solve = [&]() -> ohm_status {
     st.linear_solves += K;
     if (single)
       return tran_single_solve(tr, cur_sigma, prm->precond, so, sp.precision, itol, mref,
                                st.bicgstab_iterations);
+    // the last step's slowest system sizes this step's chunks (FP64)
+    o.expect_iters = sp.precision == OHM_PRECISION_FP64 ? last_batch : 0;
     ohm_status s = batch_solve_t<double, float>(bt, o, sp.precision, itol, mref, res.data(), false);
-    for (int k = 0; k < K; k++) st.bicgstab_iterations += res[k].iterations;
+    last_batch = 0;
+    for (int k = 0; k < K; k++) {
+      st.bicgstab_iterations += res[k].iterations;
+      if (s == OHM_OK) last_batch = std::max<int64_t>(last_batch, res[k].iterations);
+    }
     return s;
   };
   OHM_HIP(hipEventRecord(b->ev0, stream));
