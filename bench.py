@@ -674,6 +674,28 @@ def inverter_chains(be):
     return out
 
 
+def rc_profile(args, be, N, ir, solver) -> dict:
+    """The RC-grid transient alone for rocprofv3 kernel traces: ISAI FP64,
+    --tran-steps fixed 1 ps BE steps from the DC operating point (hipGraphs
+    are off in this mode, so every launch is traced)."""
+    from ohmnivore_amd import transient as T
+    ckt = ir.rc_grid(512, 512, 1024, seed=3)
+    mna = solver.compile(ckt)
+    dc = solver.GpuSolver(be, precond=N.PREC_AUTO,
+                          params=solver.bicgstab_params(tolerance=1e-12, max_iterations=100_000))
+    x0 = dc.solve_real(mna.g, mna.b_dc)
+    src, _ = T.source_table(ckt, mna)
+    tr = T.Transient(be, mna, src)
+    prm = solver.bicgstab_params(tolerance=1e-9, max_iterations=20_000)
+    t0 = time.perf_counter()
+    r = tr.run(1e-12, args.tran_steps * 1e-12, method="be", x0=x0, params=prm,
+               precond=N.PREC_ISAI)
+    dt = time.perf_counter() - t0
+    return {"workload": "rc_grid_512 ISAI FP64 (graphs off)", "steps": r.steps_accepted,
+            "ms_per_step": 1e3 * dt / r.steps_accepted,
+            "bicgstab_iters_per_step": r.bicgstab_iterations / r.steps_accepted}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -689,6 +711,9 @@ def main() -> int:
     ap.add_argument("--ac-circuits", type=int, default=256)
     ap.add_argument("--profile", action="store_true",
                     help="rocprofv3 runs: the diode-mesh workload only, no CPU work, no hipGraphs")
+    ap.add_argument("--profile-rc", action="store_true",
+                    help="rocprofv3 runs: the RC-grid transient only (ISAI FP64, --tran-steps "
+                         "steps), no hipGraphs")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline kernels: two AC sweeps, 200 SpMVs on the mesh matrix "
                          "and, for counter calibration, 20 copies of a 64 MiB vector")
@@ -710,6 +735,10 @@ def main() -> int:
         return roofline_only(args, be, N, ir, solver)
     if args.spmv_large_only:
         return spmv_large_only(args, be, N, ir, solver)
+    if args.profile_rc:
+        be.set_graphs(False)
+        print(json.dumps(rc_profile(args, be, N, ir, solver)))
+        return 0
     if args.profile:
         be.set_graphs(False)
         args.steps = min(args.steps, 2)
