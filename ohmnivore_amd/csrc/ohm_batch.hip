@@ -1226,7 +1226,9 @@ ohm_status core_solve(ohm_batch *bt, BW<S> &w, const std::vector<char> &run, con
   for (bool first = true;; first = false) {
     int chunk = 0;
     if (first && o.expect_iters > 0 && o.check_interval <= 0) {
-      chunk = (int)std::max<int64_t>(4, (o.expect_iters - 4) / 8 * 8);
+      // at most 64 iterations, so that a long earlier solve cannot queue a
+      // long run of no-ops
+      chunk = (int)std::min<int64_t>(64, std::max<int64_t>(4, (o.expect_iters - 4) / 8 * 8));
     } else {
       OHM_HIP(hipMemcpyAsync(bt->h_sys, bt->d_sys, sizeof(BSys) * K, hipMemcpyDeviceToHost,
                              stream));
@@ -1254,8 +1256,13 @@ ohm_status core_solve(ohm_batch *bt, BW<S> &w, const std::vector<char> &run, con
       chunk = o.check_interval > 0
                   ? o.check_interval
                   : (int)std::min<int64_t>(16, std::max<int64_t>(4, min_iters / 2));
-      if (o.check_interval <= 0 && o.expect_iters > 0 && min_iters < o.expect_iters + 8)
-        chunk = 4;
+      if (o.check_interval <= 0 && o.expect_iters > 0) {
+        const int64_t e = o.expect_iters;
+        if (min_iters < e - 4)
+          chunk = (int)std::min<int64_t>(16, std::max<int64_t>(4, (e - 4 - min_iters + 3) / 4 * 4));
+        else if (min_iters < e + 8)
+          chunk = 4;
+      }
     }
     for (int c = 0; c < chunk; c++) {
       if (pk == PK_JACOBI)

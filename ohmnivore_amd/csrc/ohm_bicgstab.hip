@@ -1088,22 +1088,33 @@ static ohm_status bicg_drive(ohm_backend *b, const DevCsr &a, const PrecondDev &
     int chunk = o.check_interval > 0
                     ? o.check_interval
                     : (int)std::min<int64_t>(16, std::max<int64_t>(4, done / 2));
-    // with an expected length: a first chunk ending about 4 short of it,
-    // then chunks of 4 until well past it
-    const bool near = o.check_interval <= 0 && o.expect_iters > 0 &&
-                      (enqueued == 0 || done < o.expect_iters + 8);
-    if (near)
-      chunk = enqueued == 0 ? (int)std::max<int64_t>(kGraphItersShort,
-                                                     (o.expect_iters - 4) / kGraphIters * kGraphIters)
-                            : kGraphItersShort;
+    // With an expected length E: a first chunk ending about 4 short of E
+    // (at most 64 iterations, so that a long earlier solve cannot queue a
+    // long run of no-ops), then chunks that close on E - 4 (at most 16) and
+    // chunks of 4 from there to E + 8; past that the usual growth.
+    bool near = false;
+    if (o.check_interval <= 0 && o.expect_iters > 0) {
+      const int64_t e = o.expect_iters;
+      if (enqueued == 0) {
+        chunk = (int)std::min<int64_t>(64, std::max<int64_t>(kGraphItersShort,
+                                                             (e - 4) / kGraphIters * kGraphIters));
+        near = true;
+      } else if (done < e - 4) {
+        chunk = (int)std::min<int64_t>(16, std::max<int64_t>(kGraphItersShort, (e - 4 - done + 3) / 4 * 4));
+        near = true;
+      } else if (done < e + 8) {
+        chunk = kGraphItersShort;
+        near = true;
+      }
+    }
     chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, o.max_iters - enqueued));
     if (b->use_graphs && !o.eager) {
       // replay captured chunks; surplus iterations are no-ops once the
       // state leaves RUNNING
       const bool newton_sub = o.tol_mode == OHM_TOL_INITIAL || o.forcing.kind != 0;
-      const int gi = chunk < kGraphIters && (near || (newton_sub && enqueued == 0))
-                         ? kGraphItersShort
-                         : kGraphIters;
+      const int gi = near ? (chunk % kGraphIters == 0 ? kGraphIters : kGraphItersShort)
+                          : (newton_sub && enqueued == 0 && chunk < kGraphIters ? kGraphItersShort
+                                                                                 : kGraphIters);
       int reps = (chunk + gi - 1) / gi;
       OHM_TRY(replay_chunk<T>(b, a, p, v, st, n, pp, gi, reps));
       chunk = reps * gi;
