@@ -605,6 +605,10 @@ __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__
     }
   }
   int bad = 0;
+  // the row that enters the window after the next step is loaded one step
+  // ahead, so its loads overlap a step's arithmetic
+  cz na[W], nf;
+  row(KB + 1, na, nf);
   for (int j = 0; j < n; j++) {
     int pv = 0;
     double best = cabs2(win[0][0]);
@@ -649,20 +653,31 @@ __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__
       win[r][W - 1] = cz{0.0, 0.0};
       y[r] = y[r + 1];
     }
-    cz a[W], f;
-    row(j + 1 + KB, a, f);
 #pragma unroll
-    for (int d = 0; d < W; d++) win[KB][d] = a[d];
-    y[KB] = f;
+    for (int d = 0; d < W; d++) win[KB][d] = na[d];
+    y[KB] = nf;
+    row(j + 2 + KB, na, nf);
   }
   cz xw[W - 1];
 #pragma unroll
   for (int k = 0; k < W - 1; k++) xw[k] = cz{0.0, 0.0};
-  for (int j = n - 1; j >= 0; j--) {
+  // the next (lower) row of U and its right-hand side loaded one step ahead
+  cz un[W + 1];
+  auto u_load = [&](int j) {
+    if (j < 0) return;
     const cz *u = red + ((rb + j) * (W + 1)) * 64 + lane;
-    cz acc = u[W * 64];
 #pragma unroll
-    for (int c = 1; c < W; c++) acc = acc - u[c * 64] * xw[c - 1];
+    for (int c = 0; c <= W; c++) un[c] = u[c * 64];
+  };
+  u_load(n - 1);
+  for (int j = n - 1; j >= 0; j--) {
+    cz u[W + 1];
+#pragma unroll
+    for (int c = 0; c <= W; c++) u[c] = un[c];
+    u_load(j - 1);
+    cz acc = u[W];
+#pragma unroll
+    for (int c = 1; c < W; c++) acc = acc - u[c] * xw[c - 1];
     const cz d = u[0];
     const cz xj = cabs2(d) > 0.0 ? acc * cinv(d) : cz{0.0, 0.0};
     bnd[(rb + j) * 64 + lane] = xj;
