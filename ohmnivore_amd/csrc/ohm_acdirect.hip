@@ -73,12 +73,13 @@ struct BandWave {
   const double *b;         // permuted right-hand side, (re, im) pairs
   const uint32_t *perm;    // permuted row i holds original unknown perm[i]
   const double *omega;     // omega of lane 0 (consecutive lanes follow)
-  int64_t store_off;       // lane-row offset of this wave's U / Y region (U: W entries per lane-row)
+  int64_t store_off;       // lane-row offset of the U / Y region this wave's lanes sit in
+                           // (U: W entries per lane-row)
   int64_t out_off;         // complex offset of lane 0's x in the output (lanes n apart)
   int nf;                  // active lanes
   int sys;                 // result slot of lane 0
-  int pbase;               // partitioned solve: index of the wave's first part
-  int nparts;              //   and its number of parts
+  int stride;              // lanes of the region (row i of lane l: i stride + lane0 + l);
+  int lane0;               //   nf and 0 unless the wave shares a partitioned wave
   double bnorm2;           // ||b||^2 of the circuit (the same at every frequency)
 };
 
@@ -146,9 +147,9 @@ __global__ __launch_bounds__(64) void k_band_lu(const BandWave *__restrict__ wav
   const bool live = lane < q.nf;
   const double om = live ? q.omega[lane] : 0.0;
   const int64_t n = q.n;
-  cz *Ub = U + q.store_off * W;
-  cz *Xb = X + q.store_off;
-  const int S = q.nf;  // lane stride of the wave's rows
+  cz *Ub = U + q.store_off * W + q.lane0;
+  cz *Xb = X + q.store_off + q.lane0;
+  const int S = q.stride;  // lane stride of the region's rows
   cz win[B + 1][W], y[B + 1];
 #pragma unroll
   for (int r = 0; r <= B; r++) {
@@ -321,10 +322,10 @@ __global__ void k_band_unpermute(const BandWave *__restrict__ waves, const cz *_
   const int wave = blockIdx.x / 64, part = blockIdx.x % 64;
   const BandWave q = waves[wave];
   if (lane >= q.nf) return;
-  const cz *Yb = Y + q.store_off;
+  const cz *Yb = Y + q.store_off + q.lane0;
   const int rows_per = blockDim.x / 64;
   for (int64_t i = part * rows_per + (threadIdx.x >> 6); i < q.n; i += (int64_t)64 * rows_per)
-    out[q.out_off + (int64_t)lane * q.n + q.perm[i]] = Yb[i * q.nf + lane];
+    out[q.out_off + (int64_t)lane * q.n + q.perm[i]] = Yb[i * q.stride + lane];
 }
 
 // ---------------------------------------------------------------------------
@@ -363,34 +364,65 @@ __global__ void k_band_unpermute(const BandWave *__restrict__ waves, const cz *_
 // which stores its U rows and reads them back, moves 80 B or more.
 // Systems whose residual is not small (a part block close to singular
 // after all) are solved again by k_band_lu.
+// A wave of these kernels takes 64 (circuit, frequency) systems in the order
+// circuits longest first, frequencies ascending, so it holds up to two
+// segments (band waves): the last frequencies of one circuit and the first
+// of the next, which is no longer. Both are cut at the same rows (chosen for
+// the two circuits together); the shorter continues with identity rows up to
+// the wave's row count, which couple to nothing and solve to x = 0. With 101
+// frequencies per circuit, one circuit per wave left 27 of every 128 lanes
+// idle.
 constexpr int kPartW = 3;  // band width at half bandwidth 1
+// steps per band tile: the tile holds this many rows of each segment
+constexpr int kPartTile = 32;
 
-// 64 band rows, each row's G, C and b entries together (9 doubles, the
-// ninth padding the 64 lanes' stores over the LDS banks): the 8 values of a
-// row, and of the next rows, sit at small offsets from one address.
+// 32 band rows of each of the wave's two segments, each row's G, C and b
+// entries together (9 doubles, the ninth padding the lanes' stores over the
+// LDS banks): the 8 values of a row, and of the next rows, sit at small
+// offsets from one address. The second segment's rows start one double
+// further on, so that the two addresses a wave reads at once fall in
+// different banks.
 struct PartTile {
-  double r[kBandTile][9];
-  __device__ double &g(int sl, int d) { return r[sl][d]; }
-  __device__ double &c(int sl, int d) { return r[sl][kPartW + d]; }
-  __device__ double &b(int sl, int k) { return r[sl][2 * kPartW + k]; }
+  double r[2][kPartTile * 9 + 1];
+  __device__ double &g(int h, int sl, int d) { return r[h][sl * 9 + d]; }
+  __device__ double &c(int h, int sl, int d) { return r[h][sl * 9 + kPartW + d]; }
+  __device__ double &b(int h, int sl, int k) { return r[h][sl * 9 + 2 * kPartW + k]; }
 };
 
-// one part: its wave (index into the chunk's wave table), rows [s, s+m) and
-// the index of its first checkpoint of the top-down elimination
+// A wave of the partitioned kernels: lanes [0, split) are the lanes of band
+// wave seg_a, lanes [split, nf) those of seg_b (seg_b == seg_a and split ==
+// nf for one segment). Its parts are parts[pbase .. pbase + nparts), over n
+// rows; x of lane L at row i is X[store_off + i nf + L] (the segments' own
+// stride and lane0 name the same places).
+struct PartWave {
+  int seg_a, seg_b, split, nf;
+  int pbase, nparts;
+  int64_t n, store_off;
+};
+
+// one part: its wave (index into the chunk's partitioned-wave table), rows
+// [s, s+m) and the index of its first checkpoint of the top-down elimination
 struct PartRef {
   int item, s, m, ck;
 };
 
+// result slot of lane L of a partitioned wave
+__device__ __forceinline__ int part_sys(const BandWave &qa, const BandWave &qb,
+                                        const PartWave &w, int lane) {
+  return lane >= w.split ? qb.sys + lane - w.split : qa.sys + lane;
+}
+
 // Steps of the top-down elimination between two checkpoints of its state.
 constexpr int kCk = 8;
 
-// direction-order rows k0 .. k0+63 of part [s, s+m) (rev: row k is s+m-1-k,
-// band entries mirrored) into this lane's registers
+// direction-order row k of part [s, s+m) of segment q (rev: row k is
+// s+m-1-k, band entries mirrored) into this lane's registers; rows past the
+// segment's own n are identity rows
 __device__ __forceinline__ void part_row_load(const BandWave &q, int64_t s, int64_t m, bool rev,
                                               int64_t k, double (&rg)[kPartW],
                                               double (&rc)[kPartW], double (&rb)[2]) {
-  if (k < m) {
-    const int64_t i = rev ? s + m - 1 - k : s + k;
+  const int64_t i = rev ? s + m - 1 - k : s + k;
+  if (k < m && i < q.n) {
 #pragma unroll
     for (int d = 0; d < kPartW; d++) {
       const int dd = rev ? kPartW - 1 - d : d;
@@ -399,11 +431,43 @@ __device__ __forceinline__ void part_row_load(const BandWave &q, int64_t s, int6
     }
     rb[0] = q.b[2 * i];
     rb[1] = q.b[2 * i + 1];
+  } else if (k < m) {
+#pragma unroll
+    for (int d = 0; d < kPartW; d++) rg[d] = rc[d] = 0.0;
+    rg[1] = 1.0;
+    rb[0] = rb[1] = 0.0;
   } else {
 #pragma unroll
     for (int d = 0; d < kPartW; d++) rg[d] = rc[d] = 0.0;
     rb[0] = rb[1] = 0.0;
   }
+}
+
+// The tile's rows: direction rows k0 .. k0+31 of segment a (lanes below 32)
+// and of segment b (the others), loaded into this lane's registers, then
+// stored to its tile slot.
+__device__ __forceinline__ void part_tile_load(const BandWave &qa, const BandWave &qb, int64_t s,
+                                               int64_t m, bool rev, int64_t k0,
+                                               double (&rg)[kPartW], double (&rc)[kPartW],
+                                               double (&rb)[2]) {
+  const int lane = threadIdx.x;
+  const int64_t k = k0 + (lane & (kPartTile - 1));
+  if (lane < kPartTile)
+    part_row_load(qa, s, m, rev, k, rg, rc, rb);
+  else
+    part_row_load(qb, s, m, rev, k, rg, rc, rb);
+}
+__device__ __forceinline__ void part_tile_store(PartTile &t, const double (&rg)[kPartW],
+                                                const double (&rc)[kPartW],
+                                                const double (&rb)[2]) {
+  const int h = threadIdx.x / kPartTile, sl = threadIdx.x % kPartTile;
+#pragma unroll
+  for (int d = 0; d < kPartW; d++) {
+    t.g(h, sl, d) = rg[d];
+    t.c(h, sl, d) = rc[d];
+  }
+  t.b(h, sl, 0) = rb[0];
+  t.b(h, sl, 1) = rb[1];
 }
 
 // Marker kept in place of a pivot row when the pivot of step j is row j+1
@@ -429,16 +493,20 @@ __device__ __forceinline__ bool is_pivot_marker(cz e) { return e.re != e.re; }
 // entries of direction rows 0 and m-1 outside the part); ly and lu are the
 // last remainder's right-hand sides and diagonal. With ck set, the state
 // before every kCk-th step (R_0, R_1, f, start coupling; the end coupling is
-// still zero there) is stored at ck[(j / kCk * 4 + k) * 64].
-__device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int64_t m, bool rev,
-                                               double om, PartTile &t, cz (&ly)[3], cz &lu,
-                                               int &bad, cz *ck) {
+// still zero there) is stored at ck[(j / kCk * 4 + k) * 64]. The lane's
+// system is in segment qb when h is 1, else in qa.
+__device__ __forceinline__ void part_eliminate(const BandWave &qa, const BandWave &qb, int h,
+                                               int64_t s, int64_t m, bool rev, double om,
+                                               PartTile &t, cz (&ly)[3], cz &lu, int &bad,
+                                               cz *ck) {
   constexpr int W = kPartW;
-  const int lane = threadIdx.x;
   double rg[W], rc[W], rb[2];
   // direction row 0 is the first remainder: columns -1 (left of the part),
   // 0 and 1 (right of the part when m == 1)
-  part_row_load(q, s, m, rev, 0, rg, rc, rb);
+  if (h)
+    part_row_load(qb, s, m, rev, 0, rg, rc, rb);
+  else
+    part_row_load(qa, s, m, rev, 0, rg, rc, rb);
   cz r0, r1, y0 = {rb[0], rb[1]}, y1, y2;
   {
     cz a[W];
@@ -449,26 +517,21 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
     y1 = a[0];
     y2 = m > 1 ? cz{0.0, 0.0} : a[2];
   }
-  // tiles of direction rows 1 + t0 .. 64 + t0; step j reads row j+1
-  part_row_load(q, s, m, rev, 1 + lane, rg, rc, rb);
-  for (int64_t t0 = 0; t0 < m - 1; t0 += kBandTile) {
+  // tiles of direction rows 1 + t0 .. 32 + t0; step j reads row j+1
+  part_tile_load(qa, qb, s, m, rev, 1, rg, rc, rb);
+  for (int64_t t0 = 0; t0 < m - 1; t0 += kPartTile) {
     __syncthreads();
-#pragma unroll
-    for (int d = 0; d < W; d++) {
-      t.g(lane, d) = rg[d];
-      t.c(lane, d) = rc[d];
-    }
-    t.b(lane, 0) = rb[0];
-    t.b(lane, 1) = rb[1];
+    part_tile_store(t, rg, rc, rb);
     __syncthreads();
-    part_row_load(q, s, m, rev, 1 + t0 + kBandTile + lane, rg, rc, rb);  // next tile in flight
-    const int64_t t1 = t0 + kBandTile < m - 1 ? t0 + kBandTile : m - 1;
+    part_tile_load(qa, qb, s, m, rev, 1 + t0 + kPartTile, rg, rc, rb);  // next tile in flight
+    const int64_t t1 = t0 + kPartTile < m - 1 ? t0 + kPartTile : m - 1;
     // one elimination step on direction row j + 1 (tile slot sl); LAST: the
     // part's last row, whose column-m entry is the end coupling
     auto step = [&](int sl, auto last) {
-      const cz n0 = {t.g(sl, 0), om * t.c(sl, 0)}, n1 = {t.g(sl, 1), om * t.c(sl, 1)};
-      cz n2 = {t.g(sl, 2), om * t.c(sl, 2)};
-      const cz f = {t.b(sl, 0), t.b(sl, 1)};
+      const cz n0 = {t.g(h, sl, 0), om * t.c(h, sl, 0)},
+               n1 = {t.g(h, sl, 1), om * t.c(h, sl, 1)};
+      cz n2 = {t.g(h, sl, 2), om * t.c(h, sl, 2)};
+      const cz f = {t.b(h, sl, 0), t.b(h, sl, 1)};
       cz e2 = {0.0, 0.0};
       if (decltype(last)::value) {
         e2 = n2;
@@ -525,6 +588,7 @@ __device__ __forceinline__ void part_eliminate(const BandWave &q, int64_t s, int
 // the right-hand sides f and start coupling) at
 // ckpt[((ck + c) * 4 + k) * 64 + lane] for the solve to restart from.
 __global__ __launch_bounds__(64) void k_part_rel(const BandWave *__restrict__ waves,
+                                                 const PartWave *__restrict__ pwaves,
                                                  const PartRef *__restrict__ parts,
                                                  cz *__restrict__ rel, int *__restrict__ singular,
                                                  cz *__restrict__ ckpt) {
@@ -532,17 +596,19 @@ __global__ __launch_bounds__(64) void k_part_rel(const BandWave *__restrict__ wa
   const int lane = threadIdx.x;
   const int ip = blockIdx.x >> 1, dir = blockIdx.x & 1;
   const PartRef pr = parts[ip];
-  const BandWave q = waves[pr.item];
+  const PartWave w = pwaves[pr.item];
+  const BandWave qa = waves[w.seg_a], qb = waves[w.seg_b];
   const int64_t s = pr.s, m = pr.m;
-  const bool live = lane < q.nf;
-  const double om = live ? q.omega[lane] : 0.0;
+  const int h = lane >= w.split ? 1 : 0;
+  const bool live = lane < w.nf;
+  const double om = live ? (h ? qb.omega[lane - w.split] : qa.omega[lane]) : 0.0;
   cz ly[3], lu = {1.0, 0.0};
   int bad = 0;
   cz *ck = ckpt && dir == 0 && live ? ckpt + (size_t)pr.ck * 4 * 64 + lane : nullptr;
-  part_eliminate(q, s, m, dir == 1, om, t, ly, lu, bad, ck);
+  part_eliminate(qa, qb, h, s, m, dir == 1, om, t, ly, lu, bad, ck);
   if (!live) return;
   if (bad || !(cabs2(lu) > 0.0)) {
-    singular[q.sys + lane] = 2;
+    singular[part_sys(qa, qb, w, lane)] = 2;
     lu = cz{1.0, 0.0};
   }
   const cz iu = cinv(lu);
@@ -560,13 +626,15 @@ __global__ __launch_bounds__(64) void k_part_rel(const BandWave *__restrict__ wa
 //   L_p + k_prev L_{p-1} + k_next F_{p+1} = alpha   (top-down relation)
 // form a band of half bandwidth 2; solved by LU with partial pivoting (U rows
 // of 5 entries and y to red, x to bnd[(2 pbase + r) * 64 + lane]).
+// blockIdx.x: the partitioned wave.
 __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__ waves,
+                                                    const PartWave *__restrict__ pwaves,
                                                     const cz *__restrict__ rel,
                                                     cz *__restrict__ red, cz *__restrict__ bnd,
                                                     int *__restrict__ singular) {
   constexpr int KB = 2, W = 2 * KB + 1;
   const int lane = threadIdx.x;
-  const BandWave q = waves[blockIdx.x];
+  const PartWave q = pwaves[blockIdx.x];
   if (lane >= q.nf) return;
   const int P = q.nparts, n = 2 * P;
   const size_t rb = (size_t)2 * q.pbase;
@@ -685,7 +753,7 @@ __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__
     for (int k = W - 2; k > 0; k--) xw[k] = xw[k - 1];
     xw[0] = xj;
   }
-  if (bad) singular[q.sys + lane] = 2;
+  if (bad) singular[part_sys(waves[q.seg_a], waves[q.seg_b], q, lane)] = 2;
 }
 
 // Phase 3: blockIdx.x = ip. The part's top-down elimination is recomputed
@@ -709,21 +777,23 @@ __global__ __launch_bounds__(64) void k_part_reduce(const BandWave *__restrict__
 // of part_eliminate.
 // At most 168 VGPRs: three waves share a SIMD.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_part_solve(
-    const BandWave *__restrict__ waves, const PartRef *__restrict__ parts,
-    const cz *__restrict__ bnd, const cz *__restrict__ ckpt, cz *__restrict__ X,
-    int *__restrict__ singular, double *__restrict__ pres) {
+    const BandWave *__restrict__ waves, const PartWave *__restrict__ pwaves,
+    const PartRef *__restrict__ parts, const cz *__restrict__ bnd, const cz *__restrict__ ckpt,
+    cz *__restrict__ X, int *__restrict__ singular, double *__restrict__ pres) {
   constexpr int W = kPartW;
   __shared__ PartTile t;
   const int lane = threadIdx.x;
   const int ip = blockIdx.x;
   const PartRef pr = parts[ip];
-  const BandWave q = waves[pr.item];
+  const PartWave q = pwaves[pr.item];
+  const BandWave qa = waves[q.seg_a], qb = waves[q.seg_b];
   const int p = ip - q.pbase;
   const int64_t s = pr.s;
   const int m = pr.m;  // loop bounds in 32 bits keep their tests on the scalar unit
+  const int h = lane >= q.split ? 1 : 0;
   const bool live = lane < q.nf;
   const uint64_t live_mask = __ballot(live);
-  const double om = live ? q.omega[lane] : 0.0;
+  const double om = live ? (h ? qb.omega[lane - q.split] : qa.omega[lane]) : 0.0;
   const size_t rb = (size_t)2 * q.pbase;
   cz xs = {0.0, 0.0}, xe = {0.0, 0.0};
   if (live) {
@@ -736,8 +806,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   int bad = 0;
   const int ns = m - 1;  // elimination steps; step j pivots x_{s+j} against row s+j+1
   if (ns == 0) {         // one row: a_0 xs + a_1 x + a_2 xe = b
-    cz a[W], bi;
-    band_row<W>(q, om, s, a, bi);
+    double rg[W], rc[W], rbv[2];
+    if (h)
+      part_row_load(qb, s, 1, false, 0, rg, rc, rbv);
+    else
+      part_row_load(qa, s, 1, false, 0, rg, rc, rbv);
+    cz a[W];
+#pragma unroll
+    for (int d = 0; d < W; d++) a[d] = cz{rg[d], om * rc[d]};
+    const cz bi = {rbv[0], rbv[1]};
     const cz f = bi - a[0] * xs - a[2] * xe;
     if (!(cabs2(a[1]) > 0.0)) bad = 1;
     if (live) Xb[s * S + lane] = f * cinv(a[1]);
@@ -752,19 +829,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
       for (int k = 0; k < 4; k++) ckn[k] = ckb[((size_t)((ns - 1) / kCk) * 4 + k) * 64];
     }
-    // tiles of 64 steps, last first; step j's row s+j+1 at slot j - t0
-    for (int t0 = (ns - 1) / kBandTile * kBandTile; t0 >= 0; t0 -= kBandTile) {
-      const int t1 = t0 + kBandTile < ns ? t0 + kBandTile : ns;
+    // tiles of 32 steps, last first; step j's row s+j+1 at slot j - t0
+    for (int t0 = (ns - 1) / kPartTile * kPartTile; t0 >= 0; t0 -= kPartTile) {
+      const int t1 = t0 + kPartTile < ns ? t0 + kPartTile : ns;
       double rg[W], rc[W], rbv[2];
-      part_row_load(q, s, m, false, 1 + t0 + lane, rg, rc, rbv);
+      part_tile_load(qa, qb, s, m, false, 1 + t0, rg, rc, rbv);
       __syncthreads();
-#pragma unroll
-      for (int d = 0; d < W; d++) {
-        t.g(lane, d) = rg[d];
-        t.c(lane, d) = rc[d];
-      }
-      t.b(lane, 0) = rbv[0];
-      t.b(lane, 1) = rbv[1];
+      part_tile_store(t, rg, rc, rbv);
       __syncthreads();
       for (int j0 = t0 + (t1 - 1 - t0) / kCk * kCk; j0 >= t0; j0 -= kCk) {
         const int j1 = j0 + kCk < t1 ? j0 + kCk : t1;
@@ -783,8 +854,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
           const int j = j0 + k;
           if (j < j1) {
             const int sl = j - t0;
-            const cz n0 = {t.g(sl, 0), om * t.c(sl, 0)}, n1 = {t.g(sl, 1), om * t.c(sl, 1)};
-            cz n2 = {t.g(sl, 2), om * t.c(sl, 2)}, f = {t.b(sl, 0), t.b(sl, 1)};
+            const cz n0 = {t.g(h, sl, 0), om * t.c(h, sl, 0)},
+                     n1 = {t.g(h, sl, 1), om * t.c(h, sl, 1)};
+            cz n2 = {t.g(h, sl, 2), om * t.c(h, sl, 2)}, f = {t.b(h, sl, 0), t.b(h, sl, 1)};
             if (j + 2 >= m) {  // row m-1: its column-m entry times xe
               f = cfms(f, n2, xe);
               n2 = cz{0.0, 0.0};
@@ -840,8 +912,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             const int sl = j - t0;
             cz a[W];  // row s+j+1 (columns s+j .. s+j+2)
 #pragma unroll
-            for (int d = 0; d < W; d++) a[d] = cz{t.g(sl, d), om * t.c(sl, d)};
-            const cz bj = {t.b(sl, 0), t.b(sl, 1)};
+            for (int d = 0; d < W; d++) a[d] = cz{t.g(h, sl, d), om * t.c(h, sl, d)};
+            const cz bj = {t.b(h, sl, 0), t.b(h, sl, 1)};
             cz xj;
             if (is_pivot_marker(e0[k])) {  // row s+j+1 as given was the pivot
               const bool last = j + 2 >= m;
@@ -865,33 +937,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
   if (!live) return;
   pres[(size_t)ip * 64 + lane] = rr;
-  if (bad) singular[q.sys + lane] = 2;
+  if (bad) singular[part_sys(qa, qb, q, lane)] = 2;
 }
 
-// One thread per (wave, lane): the residual terms of every part's first and
-// last row (columns in the neighbouring parts, read from the written x) and
-// the parts' interior sums, in part order, to res[2 (sys + lane)], beside
-// the circuit's ||b||^2.
-__global__ void k_part_resid_sum(const BandWave *__restrict__ waves, int n_waves,
+// One thread per (partitioned wave, lane): the residual terms of every
+// part's first and last row (columns in the neighbouring parts, read from
+// the written x) and the parts' interior sums, in part order, to
+// res[2 sys], beside the circuit's ||b||^2. Identity rows past the lane's
+// own segment have zero residual (x is zero there).
+__global__ void k_part_resid_sum(const BandWave *__restrict__ waves,
+                                 const PartWave *__restrict__ pwaves, int n_waves,
                                  const PartRef *__restrict__ parts,
                                  const double *__restrict__ pres, const cz *__restrict__ X,
                                  double *__restrict__ res) {
   constexpr int W = kPartW;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_waves * 64) return;
-  const BandWave q = waves[t / 64];
+  const PartWave w = pwaves[t / 64];
   const int lane = t % 64;
-  if (lane >= q.nf) return;
-  const double om = q.omega[lane];
-  const cz *Xb = X + q.store_off;
-  const int S = q.nf;
+  if (lane >= w.nf) return;
+  const bool second = lane >= w.split;
+  const BandWave q = waves[second ? w.seg_b : w.seg_a];
+  const int l = second ? lane - w.split : lane;
+  const double om = q.omega[l];
+  const cz *Xb = X + w.store_off;
+  const int S = w.nf;
   double rr = 0.0;
-  for (int ip = q.pbase; ip < q.pbase + q.nparts; ip++) {
+  for (int ip = w.pbase; ip < w.pbase + w.nparts; ip++) {
     rr += pres[(size_t)ip * 64 + lane];
     const PartRef pr = parts[ip];
     for (int e = 0; e < 2; e++) {
       if (e == 1 && pr.m == 1) break;  // one row: first and last
       const int64_t i = e == 0 ? pr.s : (int64_t)pr.s + pr.m - 1;
+      if (i >= q.n) continue;
       cz a[W], bi, xc[W];
       band_row<W>(q, om, i, a, bi);
 #pragma unroll
@@ -902,8 +980,8 @@ __global__ void k_part_resid_sum(const BandWave *__restrict__ waves, int n_waves
       rr += resid2<W>(a, bi, xc);
     }
   }
-  res[2 * (q.sys + lane)] = rr;
-  res[2 * (q.sys + lane) + 1] = q.bnorm2;
+  res[2 * (q.sys + l)] = rr;
+  res[2 * (q.sys + l) + 1] = q.bnorm2;
 }
 
 }  // namespace ohm
@@ -1070,6 +1148,10 @@ struct ohm_ac_plan {
   // sweeps: computing them starts a pool of host threads)
   int64_t cuts_R = -1;
   std::vector<std::vector<int>> cuts;
+  // boundaries of the partitioned waves that hold two circuits, by wave
+  // index, for (rows per part, frequencies) pair_key
+  std::pair<int64_t, int64_t> pair_key{-1, -1};
+  std::vector<std::vector<int>> pair_cuts;
   // host copies of the last sweep's singular flags and residual sums, when a
   // one-chunk sweep redid no system (they are then the final values)
   std::vector<int> h_sing;
@@ -1124,20 +1206,46 @@ std::vector<float> cut_scores(const std::vector<double> &gb, int64_t n) {
 constexpr int64_t kCutWindow = 16;
 
 // Part boundaries [0, e_1, ..., n) of an n-row system for about R rows per
-// part: e_k is the best-rated cut within kCutWindow rows of k R.
-std::vector<int> part_cuts(const std::vector<float> &score, int64_t n, int64_t R) {
+// part: e_k is the best-rated cut within kCutWindow rows of k R, rate(i)
+// rating the boundary between rows i and i+1.
+template <class Rate>
+std::vector<int> part_cuts_by(Rate rate, int64_t n, int64_t R) {
   std::vector<int> cut{0};
   for (int64_t e0 = R; e0 < n; e0 += R) {
     const int64_t lo = std::max<int64_t>(e0 - kCutWindow, cut.back() + 1);
     const int64_t hi = std::min<int64_t>(e0 + kCutWindow, n - 1);
     if (lo > hi) continue;
     int64_t best = lo;
-    for (int64_t e = lo + 1; e <= hi; e++)
-      if (score[e - 1] < score[best - 1]) best = e;
+    float bs = rate(lo - 1);
+    for (int64_t e = lo + 1; e <= hi; e++) {
+      const float v = rate(e - 1);
+      if (v < bs) {
+        best = e;
+        bs = v;
+      }
+    }
     cut.push_back((int)best);
   }
   cut.push_back((int)n);
   return cut;
+}
+
+std::vector<int> part_cuts(const std::vector<float> &score, int64_t n, int64_t R) {
+  return part_cuts_by([&](int64_t i) { return score[i]; }, n, R);
+}
+
+// Boundaries shared by two circuits of na and nb rows cut at the same rows
+// (a partitioned wave holding frequencies of both): a cut is rated by the
+// worse of its two ratings; past a circuit's own rows (identity rows, no
+// coupling) its rating is zero.
+std::vector<int> part_cuts_pair(const std::vector<float> &sa, int64_t na,
+                                const std::vector<float> &sb, int64_t nb, int64_t R) {
+  return part_cuts_by(
+      [&](int64_t i) {
+        const float a = i + 1 < na ? sa[i] : 0.0f, b = i + 1 < nb ? sb[i] : 0.0f;
+        return std::max(a, b);
+      },
+      std::max(na, nb), R);
 }
 
 void plan_release(ohm_ac_plan *p) {
@@ -1282,19 +1390,48 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
   const size_t n_sys = (size_t)std::max<int64_t>(nc * n_freqs, 1);
   OHM_HIP(hipMemsetAsync(p->d_sing, 0, sizeof(int) * n_sys, st));
   OHM_HIP(hipMemsetAsync(p->d_res, 0, sizeof(double) * 2 * n_sys, st));
-  // waves: up to 64 frequencies of one circuit; longest circuits first
+  // band waves (segments): up to 64 frequencies of one circuit, longest
+  // circuits first. One lane per system: a wave each. Partitioned: a wave of
+  // 64 lanes takes the next segment of the next circuit once one circuit's
+  // frequencies run out, up to two segments (OHM_AC_PACK=0: one), and the
+  // circuits take their frequencies upwards and downwards in turn, so that
+  // a shared wave holds the same end of the band of both: the part solve's
+  // pivot choice, which its lanes share on most steps when their
+  // frequencies are close, depends on the frequency.
   struct WaveItem {
     int64_t c, f0;
-    int nf;
+    int nf, lane0;  // lane0: the segment's first lane in its wave
   };
-  std::vector<WaveItem> items;
+  struct Pack {
+    size_t first;  // its first segment in items
+    int count, nf;
+  };
+  std::vector<int64_t> order;
   for (int64_t c = 0; c < nc; c++)
-    if (p->n[c] > 0)
-      for (int64_t f0 = 0; f0 < n_freqs; f0 += 64)
-        items.push_back({c, f0, (int)std::min<int64_t>(64, n_freqs - f0)});
-  std::stable_sort(items.begin(), items.end(), [&](const WaveItem &a, const WaveItem &b2) {
-    return p->n[a.c] > p->n[b2.c];
-  });
+    if (p->n[c] > 0) order.push_back(c);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b2) { return p->n[a] > p->n[b2]; });
+  const char *pack_env = getenv("OHM_AC_PACK");
+  const bool pack = part && !(pack_env && atoi(pack_env) == 0);
+  std::vector<WaveItem> items;
+  std::vector<Pack> packs;
+  bool down = false;
+  for (int64_t c : order) {
+    for (int64_t done = 0; done < n_freqs;) {
+      const bool join = pack && !packs.empty() && packs.back().count == 1 && packs.back().nf < 64;
+      const int lane0 = join ? packs.back().nf : 0;
+      const int take = (int)std::min<int64_t>(64 - lane0, n_freqs - done);
+      items.push_back({c, down ? n_freqs - done - take : done, take, lane0});
+      if (join) {
+        packs.back().count++;
+        packs.back().nf += take;
+      } else {
+        packs.push_back({items.size() - 1, 1, take});
+      }
+      done += take;
+    }
+    if (pack) down = !down;
+  }
   const double budget = direct_budget(b);
   bool want_x = false;
   if (x)
@@ -1311,38 +1448,61 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     p->cuts_R = R;
   }
   const std::vector<std::vector<int>> &cuts = p->cuts;
-  auto parts_of = [&](int64_t c) { return part ? (int)cuts[c].size() - 1 : 0; };
-  while (pos < items.size()) {
+  // boundaries of the waves holding two circuits, cut at the same rows
+  if (pack && p->pair_key != std::make_pair(R, n_freqs)) {
+    p->pair_cuts.assign(packs.size(), {});
+    parallel_for((int64_t)packs.size(), [&](int64_t k) {
+      const Pack &pk = packs[k];
+      if (pk.count < 2) return;
+      const int64_t ca = items[pk.first].c, cb2 = items[pk.first + 1].c;
+      p->pair_cuts[k] = part_cuts_pair(p->cut_score[ca], p->n[ca], p->cut_score[cb2], p->n[cb2], R);
+    });
+    p->pair_key = std::make_pair(R, n_freqs);
+  }
+  auto cuts_of = [&](size_t k) -> const std::vector<int> & {
+    const Pack &pk = packs[k];
+    return pk.count == 2 ? p->pair_cuts[k] : cuts[items[pk.first].c];
+  };
+  auto rows_of = [&](size_t k) {
+    const Pack &pk = packs[k];
+    int64_t n = 0;
+    for (int e = 0; e < pk.count; e++) n = std::max(n, p->n[items[pk.first + e].c]);
+    return n;
+  };
+  while (pos < packs.size()) {
     size_t end = pos;
     double bytes = 0.0;
-    while (end < items.size()) {
-      const double n = (double)p->n[items[end].c];
-      const double nfw = items[end].nf;
-      const double np = parts_of(items[end].c);
+    while (end < packs.size()) {
+      const double n = (double)rows_of(end);
+      const double nfw = packs[end].nf;
+      const double np = part ? (double)cuts_of(end).size() - 1 : 0.0;
       // U rows and x per lane-row; per part: its reference, relations,
       // reduced rows, boundary values and residual partials per lane
       // (and the checkpoints of the top-down elimination: 4 per kCk rows)
       const double wave_b = n * nfw * 16.0 * (W + 1) + (want_x ? n * nfw * 16.0 : 0.0) +
-                            2.0 * sizeof(BandWave) +
+                            2.0 * packs[end].count * sizeof(BandWave) + sizeof(PartWave) +
                             np * (sizeof(PartRef) + 64.0 * 16.0 * (6.0 + 12.0 + 2.0 + 1.0)) +
                             (part ? (n / kCk + np) * 4.0 * 64.0 * 16.0 : 0.0);
       if (end > pos && bytes + wave_b > budget) break;
       bytes += wave_b;
       end++;
     }
-    const int n_w = (int)(end - pos);
+    // this chunk: waves [pos, end), segments [s0, s0 + n_w)
+    const size_t s0 = packs[pos].first;
+    const int n_w = (int)(packs[end - 1].first + packs[end - 1].count - s0);
+    const int n_pw = (int)(end - pos);
     // output in (circuit, frequency) order so each circuit's block is contiguous
     std::vector<size_t> by_cf(n_w);
     std::iota(by_cf.begin(), by_cf.end(), 0);
     std::stable_sort(by_cf.begin(), by_cf.end(), [&](size_t a, size_t b2) {
-      const WaveItem &u = items[pos + a], &v = items[pos + b2];
+      const WaveItem &u = items[s0 + a], &v = items[s0 + b2];
       return u.c != v.c ? u.c < v.c : u.f0 < v.f0;
     });
     std::vector<int64_t> out_off(n_w);
     std::vector<int64_t> f_lo(nc, INT64_MAX), f_hi(nc, -1), x_at(nc, 0);
     int64_t out_total = 0;
     for (size_t k : by_cf) {
-      const WaveItem &it = items[pos + k];
+      const WaveItem &it = items[s0 + k];
       out_off[k] = out_total;
       if (it.f0 < f_lo[it.c]) {
         f_lo[it.c] = it.f0;
@@ -1352,32 +1512,52 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       out_total += p->n[it.c] * it.nf;
     }
     std::vector<BandWave> hw(n_w);
+    std::vector<PartWave> hpw(part ? n_pw : 0);
     std::vector<PartRef> prefs;
     int64_t store = 0;  // lane-rows
     int64_t n_ck = 0;   // checkpoints of the top-down elimination
-    for (int k = 0; k < n_w; k++) {
-      const WaveItem &it = items[pos + k];
-      BandWave w{};
-      w.n = p->n[it.c];
-      w.gb = p->gb[it.c];
-      w.cb = p->cb[it.c];
-      w.b = p->bp[it.c];
-      w.perm = p->perm[it.c];
-      w.omega = p->d_omega + it.f0;
-      w.store_off = store;
-      w.out_off = out_off[k];
-      w.nf = it.nf;
-      w.sys = (int)(it.c * n_freqs + it.f0);
-      w.pbase = (int)prefs.size();
-      w.nparts = parts_of(it.c);
-      w.bnorm2 = p->bnorm2[it.c];
-      for (int q = 0; q < w.nparts; q++) {
-        const int mq = cuts[it.c][q + 1] - cuts[it.c][q];
-        prefs.push_back({k, cuts[it.c][q], mq, (int)n_ck});
-        n_ck += (mq - 1 + kCk - 1) / kCk;
+    for (int kw = 0; kw < n_pw; kw++) {
+      const Pack &pk = packs[pos + kw];
+      const int64_t rows = rows_of(pos + kw);
+      const int ka = (int)(pk.first - s0);
+      for (int e = 0; e < pk.count; e++) {
+        const int k = ka + e;
+        const WaveItem &it = items[s0 + k];
+        BandWave w{};
+        w.n = p->n[it.c];
+        w.gb = p->gb[it.c];
+        w.cb = p->cb[it.c];
+        w.b = p->bp[it.c];
+        w.perm = p->perm[it.c];
+        w.omega = p->d_omega + it.f0;
+        w.store_off = store;
+        w.out_off = out_off[k];
+        w.nf = it.nf;
+        w.sys = (int)(it.c * n_freqs + it.f0);
+        w.stride = pk.nf;
+        w.lane0 = it.lane0;
+        w.bnorm2 = p->bnorm2[it.c];
+        hw[k] = w;
       }
-      store += w.n * it.nf;
-      hw[k] = w;
+      if (part) {
+        const std::vector<int> &ct = cuts_of(pos + kw);
+        PartWave v{};
+        v.seg_a = ka;
+        v.seg_b = ka + pk.count - 1;
+        v.split = pk.count == 2 ? items[pk.first].nf : pk.nf;
+        v.nf = pk.nf;
+        v.pbase = (int)prefs.size();
+        v.nparts = (int)ct.size() - 1;
+        v.n = rows;
+        v.store_off = store;
+        for (int q = 0; q < v.nparts; q++) {
+          const int mq = ct[q + 1] - ct[q];
+          prefs.push_back({kw, ct[q], mq, (int)n_ck});
+          n_ck += (mq - 1 + kCk - 1) / kCk;
+        }
+        hpw[kw] = v;
+      }
+      store += rows * pk.nf;
     }
     const size_t n_ip = prefs.size();
     // chunk buffers carved from the backend's workspace (kept across sweeps:
@@ -1390,10 +1570,11 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
                  red_b = al(sizeof(cz) * n_ip * 12 * 64),
                  bnd_b = al(sizeof(cz) * n_ip * 2 * 64),
                  pres_b = al(sizeof(double) * n_ip * 64),
-                 ck_b = part ? al(sizeof(cz) * std::max<int64_t>(n_ck, 1) * 4 * 64) : 0;
+                 ck_b = part ? al(sizeof(cz) * std::max<int64_t>(n_ck, 1) * 4 * 64) : 0,
+                 pw_b = part ? al(sizeof(PartWave) * n_pw) : 0;
     // (partitioned: a second wave table for the one-lane redo after pres)
-    const size_t need =
-        bw_b + u_b + y_b + x_b + (part ? ip_b + rel_b + red_b + bnd_b + pres_b + bw_b + ck_b : 0);
+    const size_t need = bw_b + u_b + y_b + x_b +
+                        (part ? ip_b + rel_b + red_b + bnd_b + pres_b + bw_b + ck_b + pw_b : 0);
     if (b->ac_ws_bytes < need) {
       if (b->ac_ws) OHM_HIP(hipFree(b->ac_ws));
       b->ac_ws = nullptr;
@@ -1414,10 +1595,14 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     double *d_pres = (double *)(pw + ip_b + rel_b + red_b + bnd_b);
     // after the redo wave table (ip_b + ... + pres_b + bw_b)
     cz *d_ck = part ? (cz *)(pw + ip_b + rel_b + red_b + bnd_b + pres_b + bw_b) : nullptr;
+    PartWave *d_pw = part ? (PartWave *)(pw + ip_b + rel_b + red_b + bnd_b + pres_b + bw_b + ck_b)
+                          : nullptr;
     OHM_HIP(hipMemcpyAsync(d_w, hw.data(), sizeof(BandWave) * n_w, hipMemcpyHostToDevice, st));
-    if (part)
+    if (part) {
       OHM_HIP(hipMemcpyAsync(d_parts, prefs.data(), sizeof(PartRef) * n_ip,
                              hipMemcpyHostToDevice, st));
+      OHM_HIP(hipMemcpyAsync(d_pw, hpw.data(), sizeof(PartWave) * n_pw, hipMemcpyHostToDevice, st));
+    }
     auto one_lane = [&](int nw, const BandWave *dw) -> ohm_status {
       if (B == 1)
         hipLaunchKernelGGL(k_band_lu<1>, dim3(nw), dim3(64), 0, st, dw, dU, dY, p->d_sing, p->d_res);
@@ -1432,20 +1617,20 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     hipEvent_t *ev = p->ev;
     OHM_HIP(hipEventRecord(ev[0], st));
     if (part) {
-      hipLaunchKernelGGL(k_part_rel, dim3((unsigned)(2 * n_ip)), dim3(64), 0, st, d_w, d_parts,
-                         d_rel, p->d_sing, d_ck);
+      hipLaunchKernelGGL(k_part_rel, dim3((unsigned)(2 * n_ip)), dim3(64), 0, st, d_w, d_pw,
+                         d_parts, d_rel, p->d_sing, d_ck);
       OHM_HIP(hipGetLastError());
       OHM_HIP(hipEventRecord(ev[1], st));
-      hipLaunchKernelGGL(k_part_reduce, dim3(n_w), dim3(64), 0, st, d_w, d_rel, d_red, d_bnd,
-                         p->d_sing);
+      hipLaunchKernelGGL(k_part_reduce, dim3(n_pw), dim3(64), 0, st, d_w, d_pw, d_rel, d_red,
+                         d_bnd, p->d_sing);
       OHM_HIP(hipGetLastError());
       OHM_HIP(hipEventRecord(ev[2], st));
-      hipLaunchKernelGGL(k_part_solve, dim3((unsigned)n_ip), dim3(64), 0, st, d_w, d_parts,
-                         d_bnd, d_ck, dY, p->d_sing, d_pres);
+      hipLaunchKernelGGL(k_part_solve, dim3((unsigned)n_ip), dim3(64), 0, st, d_w, d_pw,
+                         d_parts, d_bnd, d_ck, dY, p->d_sing, d_pres);
       OHM_HIP(hipGetLastError());
       OHM_HIP(hipEventRecord(ev[3], st));
-      hipLaunchKernelGGL(k_part_resid_sum, dim3((unsigned)((n_w * 64 + 255) / 256)), dim3(256), 0,
-                         st, d_w, n_w, d_parts, d_pres, dY, p->d_res);
+      hipLaunchKernelGGL(k_part_resid_sum, dim3((unsigned)((n_pw * 64 + 255) / 256)), dim3(256),
+                         0, st, d_w, d_pw, n_pw, d_parts, d_pres, dY, p->d_res);
       OHM_HIP(hipGetLastError());
       b->dispatches += 4;
     } else {
@@ -1470,7 +1655,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
       res.resize(2 * n_sys);
       OHM_HIP(hipMemcpy(sing.data(), p->d_sing, sizeof(int) * n_sys, hipMemcpyDeviceToHost));
       OHM_HIP(hipMemcpy(res.data(), p->d_res, sizeof(double) * 2 * n_sys, hipMemcpyDeviceToHost));
-      p->h_final = pos == 0 && end == items.size();
+      p->h_final = pos == 0 && end == packs.size();
       std::vector<BandWave> redo;
       for (int k = 0; k < n_w; k++) {
         const BandWave &w = hw[k];
@@ -1523,7 +1708,7 @@ ohm_status plan_solve_once(ohm_ac_plan *p, int64_t n_freqs, double *const *x, in
     }
     b->readbacks++;
     chunks++;
-    if (end == items.size() && chunks == 1) {
+    if (end == packs.size() && chunks == 1) {
       p->waves = std::move(hw);
       p->y_off = bw_b + u_b;
     }
@@ -1647,7 +1832,7 @@ __global__ void k_band_gather_x(BandWave q, int l0, int l1, const cz *__restrict
   if (t >= q.n * span) return;
   const int64_t i = t / span;
   const int lane = l0 + (int)(t % span);
-  out[(int64_t)(lane - l0) * q.n + q.perm[i]] = Y[q.store_off + i * q.nf + lane];
+  out[(int64_t)(lane - l0) * q.n + q.perm[i]] = Y[q.store_off + q.lane0 + i * q.stride + lane];
 }
 
 }  // namespace

@@ -389,6 +389,34 @@ def test_ac_plan_partitioned_vs_lu(be, rows, monkeypatch):
         assert plan.x(k, 60, 5).tobytes() == res.x[k][60:65].tobytes()
 
 
+@pytest.mark.parametrize("n_freqs", [101, 37, 5])
+def test_ac_plan_packed_waves_match_one_circuit_waves(be, n_freqs, monkeypatch):
+    """Partitioned waves holding the frequencies of two circuits (cut at
+    the same rows, the shorter padded with identity rows) against waves of
+    one circuit each (OHM_AC_PACK=0): every system's residual below 1e-12,
+    the two solutions equal to 1e-10 relative, and the resident solutions
+    of segments that share a wave read back (ohm_ac_plan_get_x) equal to
+    the copied-out ones."""
+    monkeypatch.setenv("OHM_AC_PART_ROWS", "64")
+    ckts, _ = ir.rlc_sweep(n_circuits=9, lo=20, hi=900, seed=7)
+    freqs = 10.0 ** (9.0 * np.arange(n_freqs) / max(n_freqs - 1, 1))
+    mnas = [solver.compile(c) for c in ckts]
+    plan = solver.AcPlan(be, mnas)
+    monkeypatch.setenv("OHM_AC_PACK", "0")
+    one = plan.solve(freqs, keep_x=True)
+    monkeypatch.setenv("OHM_AC_PACK", "1")
+    two = plan.solve(freqs, keep_x=True)
+    assert plan.stats()["chunks"] == 1
+    for res in (one, two):
+        assert bool(res.converged.all()) and float(res.residual_norms.max()) < 1e-12
+    for k, (x1, x2) in enumerate(zip(one.x, two.x)):
+        err = np.abs(x2 - x1).max(axis=1) / np.maximum(np.abs(x1).max(axis=1), 1e-300)
+        assert float(err.max()) < 1e-10, (k, float(err.max()))
+        assert plan.x(k).tobytes() == x2.tobytes()
+    f_mid = n_freqs // 2
+    assert plan.x(4, f_mid, 2).tobytes() == two.x[4][f_mid:f_mid + 2].tobytes()
+
+
 @pytest.mark.parametrize("rows", ["0", "4"])
 def test_ac_plan_many_tiny_circuits_in_chunks(be, rows, monkeypatch):
     """3,000 ladders of 1-5 sections at 7 frequencies with a 0.5 MB device
