@@ -55,14 +55,14 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64
 SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
 # HBM traffic of one k_part_solve launch of the AC sweep (2,048 rows per
 # part), from the same two counter passes over `bench.py --roofline-only`:
-# FETCH 11,830,558.5 KiB x 1.99948 + WRITE 39,514,391.1 KiB x 1.0.
+# FETCH 12,209,692.4 KiB x 1.99948 + WRITE 39,380,344.9 KiB x 1.0.
 # Summary: profiles/ac_sweep_256x101_counters.md.
-AC_TRAFFIC_BYTES = round((11830558.53 * 65536.0 / 32776.5 + 39514391.13) * 1024)
+AC_TRAFFIC_BYTES = round((12209692.41 * 65536.0 / 32776.5 + 39380344.88) * 1024)
 # VALU issue of one k_part_solve launch, from a third counter pass over the
 # same program (SQ_INSTS_VALU; GRBM_GUI_ACTIVE, summed over the 8 XCDs):
-# 4.708e9 wave-level instructions against 1,024 SIMDs x 2.094e8 / 8 cycles,
+# 3.619e9 wave-level instructions against 1,024 SIMDs x 1.857e8 / 8 cycles,
 # one instruction per SIMD per 4 cycles. Summary: the same file.
-AC_VALU_ISSUE_FRAC = 4.708e9 / (1024 * 2.094e8 / 8 / 4)
+AC_VALU_ISSUE_FRAC = 3.619e9 / (1024 * 1.857e8 / 8 / 4)
 
 
 def log(msg: str) -> None:
@@ -96,32 +96,59 @@ def within(x, ref, rel=1e-6, abs_=1e-9):
 
 
 # ------------------------------------------------------------- headline --
+def ac_wave_counts(sizes, n_freqs: int, packed: bool):
+    """Waves of the sweep holding each circuit's frequencies. Circuits
+    longest first; one lane per system: ceil(n_freqs / 64) waves each;
+    partitioned (packed): a wave of 64 lanes takes the next circuit's
+    frequencies once one circuit's run out, up to two circuits per wave."""
+    if not packed:
+        return [-(-n_freqs // 64) if n > 0 else 0 for n in sizes]
+    counts = [0] * len(sizes)
+    fill, segs = 64, 2  # the open wave: lanes used, circuits in it
+    for c in sorted((c for c in range(len(sizes)) if sizes[c] > 0), key=lambda c: -sizes[c]):
+        left = n_freqs
+        while left > 0:
+            if fill < 64 and segs < 2:
+                take = min(64 - fill, left)
+                fill, segs = fill + take, segs + 1
+            else:
+                take = min(64, left)
+                fill, segs = take, 1
+            counts[c] += 1
+            left -= take
+    return counts
+
+
 def ac_bytes(sizes, n_freqs: int, rows_per_part: int):
-    """Algorithmic bytes of one sweep's kernels. Waves hold up to 64
-    frequencies of one circuit; the band rows of G and C and the permuted
-    b (3 + 3 + 2 doubles = 64 bytes per row at half bandwidth 1) are read by
-    every wave once per pass: once per direction by the relations and once by
-    the part solve. The relations' top-down pass stores the elimination state
+    """Algorithmic bytes of one sweep's kernels. A partitioned wave holds 64
+    frequencies of up to two circuits (ac_wave_counts); the band rows of G
+    and C and the permuted b (3 + 3 + 2 doubles = 64 bytes per row at half
+    bandwidth 1) of each of its circuits are read by the wave once per pass:
+    once per direction by the relations and once by the part solve (the
+    identity rows that pad the shorter circuit are not read, and their x and
+    checkpoints are not counted). The relations' top-down pass stores the elimination state
     every 8 steps (4 complex = 64 bytes per system: ceil((m - 1) / 8) of them
     in a part of m rows), which the part solve reads back to recompute each
     chunk's pivot rows in registers before writing x (16 bytes per (system,
     row)). The residual pass reads, per part and system, x at the part's
     first and last rows and their neighbours (6 complex) and the part's two
     partial sums."""
-    waves = -(-n_freqs // 64)
     rows = float(sum(sizes))
     band = 64.0
+    wc = ac_wave_counts(sizes, n_freqs, packed=bool(rows_per_part))
+    band_rows = float(sum(n * w for n, w in zip(sizes, wc)))  # (row, wave) pairs
     if not rows_per_part:  # one lane per system (k_band_lu<1>): U rows of 3 complex
-        return {"part_solve": rows * (112.0 * n_freqs + band * waves), "relations": 0.0,
+        return {"part_solve": rows * 112.0 * n_freqs + band_rows * band, "relations": 0.0,
                 "residual": 0.0}
-    parts, ck = 0, 0
-    for n in sizes:
+    parts, ck, part_waves = 0, 0, 0
+    for n, w in zip(sizes, wc):
         p = -(-n // rows_per_part)
         parts += p
+        part_waves += p * w
         ck += p * -(-(-(-n // p) - 1) // 8)
-    return {"part_solve": rows * 16.0 * n_freqs + ck * 64.0 * n_freqs + rows * band * waves,
-            "relations": rows * band * 2 * waves + ck * 64.0 * n_freqs,
-            "residual": parts * n_freqs * (6 * 16.0 + 16.0) + parts * 2 * band * waves}
+    return {"part_solve": rows * 16.0 * n_freqs + ck * 64.0 * n_freqs + band_rows * band,
+            "relations": band_rows * band * 2 + ck * 64.0 * n_freqs,
+            "residual": parts * n_freqs * (6 * 16.0 + 16.0) + part_waves * 2 * band}
 
 
 def ac_flops(sizes, n_freqs: int, rows_per_part: int):
