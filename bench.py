@@ -55,14 +55,14 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64
 SPMV_TRAFFIC_BYTES = round((2897.01 * 1.9994 + 512.03 * 1.0) * 1024)
 # HBM traffic of one k_part_solve launch of the AC sweep (2,048 rows per
 # part), from the same two counter passes over `bench.py --roofline-only`:
-# FETCH 12,209,692.4 KiB x 1.99948 + WRITE 39,380,344.9 KiB x 1.0.
+# FETCH 11,758,936.7 KiB x 1.99948 + WRITE 39,383,843.2 KiB x 1.0.
 # Summary: profiles/ac_sweep_256x101_counters.md.
-AC_TRAFFIC_BYTES = round((12209692.41 * 65536.0 / 32776.5 + 39380344.88) * 1024)
+AC_TRAFFIC_BYTES = round((11758936.7 * 65536.0 / 32776.5 + 39383843.2) * 1024)
 # VALU issue of one k_part_solve launch, from a third counter pass over the
 # same program (SQ_INSTS_VALU; GRBM_GUI_ACTIVE, summed over the 8 XCDs):
-# 3.619e9 wave-level instructions against 1,024 SIMDs x 1.857e8 / 8 cycles,
+# 3.619e9 wave-level instructions against 1,024 SIMDs x 1.795e8 / 8 cycles,
 # one instruction per SIMD per 4 cycles. Summary: the same file.
-AC_VALU_ISSUE_FRAC = 3.619e9 / (1024 * 1.857e8 / 8 / 4)
+AC_VALU_ISSUE_FRAC = 3.619e9 / (1024 * 1.795e8 / 8 / 4)
 
 
 def log(msg: str) -> None:

@@ -470,6 +470,19 @@ __device__ __forceinline__ void part_tile_store(PartTile &t, const double (&rg)[
   t.b(h, sl, 1) = rb[1];
 }
 
+// x and the checkpoints are written once and read once, far apart (tens of
+// GB per sweep, well past the caches): their stores and the checkpoints'
+// loads are nontemporal (measured: part solve 12.75 -> 12.35 ms, relations
+// 8.3 -> 8.15 ms on the 25,856-system sweep).
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void nt_store(cz *p, cz v) {
+  __builtin_nontemporal_store(d2v{v.re, v.im}, (d2v *)p);
+}
+__device__ __forceinline__ cz nt_load(const cz *p) {
+  const d2v v = __builtin_nontemporal_load((const d2v *)p);
+  return cz{v.x, v.y};
+}
+
 // Marker kept in place of a pivot row when the pivot of step j is row j+1
 // as given (|N_0| > |R_0|; the back substitution then reads that row from
 // the band rows): a quiet NaN real part.
@@ -555,10 +568,10 @@ __device__ __forceinline__ void part_eliminate(const BandWave &qa, const BandWav
     auto checkpoint = [&](int64_t j) {  // state before step j (j a multiple of kCk)
       if (ck) {
         cz *o = ck + (size_t)(j / kCk) * 4 * 64;
-        o[0] = r0;
-        o[64] = r1;
-        o[128] = y0;
-        o[192] = y1;
+        nt_store(&o[0], r0);
+        nt_store(&o[64], r1);
+        nt_store(&o[128], y0);
+        nt_store(&o[192], y1);
       }
     };
     int64_t j = t0;
@@ -827,7 +840,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     cz ckn[4] = {};
     if (live) {
 #pragma unroll
-      for (int k = 0; k < 4; k++) ckn[k] = ckb[((size_t)((ns - 1) / kCk) * 4 + k) * 64];
+      for (int k = 0; k < 4; k++) ckn[k] = nt_load(&ckb[((size_t)((ns - 1) / kCk) * 4 + k) * 64]);
     }
     // tiles of 32 steps, last first; step j's row s+j+1 at slot j - t0
     for (int t0 = (ns - 1) / kPartTile * kPartTile; t0 >= 0; t0 -= kPartTile) {
@@ -844,7 +857,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         for (int k = 0; k < 4; k++) ck4[k] = ckn[k];
         if (live && j0 >= kCk) {
 #pragma unroll
-          for (int k = 0; k < 4; k++) ckn[k] = ckb[((size_t)(j0 / kCk - 1) * 4 + k) * 64];
+          for (int k = 0; k < 4; k++) ckn[k] = nt_load(&ckb[((size_t)(j0 / kCk - 1) * 4 + k) * 64]);
         }
         cz r0 = ck4[0], r1 = ck4[1];
         cz y = cfms(ck4[2], ck4[3], xs);
@@ -903,7 +916,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         if (j1 == ns) {  // the last row: its pivot is the final remainder
           if (!(cabs2(r0) > 0.0)) bad = 1;
           xw0 = y * cinv(r0);
-          if (live) Xb[(s + m - 1) * S + lane] = xw0;
+          if (live) nt_store(&Xb[(s + m - 1) * S + lane], xw0);
         }
 #pragma unroll
         for (int k = kCk - 1; k >= 0; k--) {
@@ -923,7 +936,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             } else {
               xj = cfms(e0[k], e1[k], xw0);
             }
-            if (live) Xb[(s + j) * S + lane] = xj;
+            if (live) nt_store(&Xb[(s + j) * S + lane], xj);
             if (j + 1 <= m - 2) {  // interior row s+j+1
               const cz xc[W] = {xj, xw0, xw1};
               rr += resid2<W>(a, bj, xc);
