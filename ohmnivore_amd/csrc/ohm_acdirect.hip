@@ -64,9 +64,10 @@ __device__ __forceinline__ cz cinv_d(cz a, double d) {
 }
 __device__ __forceinline__ cz cinv(cz a) { return cinv_d(a, cabs2(a)); }
 
-// One wave = up to 64 frequency points of one circuit (lane l: frequency
-// f0 + l). The circuit's band rows are shared by every lane, so the wave
-// loads them cooperatively, 64 rows at a time, into LDS.
+// One band wave (segment) = up to 64 frequency points of one circuit (lane
+// l: frequency f0 + l). The circuit's band rows are shared by every lane, so
+// the one-lane LU's wave loads them cooperatively, 64 rows at a time, into
+// LDS; a partitioned wave (PartWave) holds up to two segments.
 struct BandWave {
   int64_t n;
   const double *gb, *cb;   // band rows of the permuted G and C: [n][2B+1]
